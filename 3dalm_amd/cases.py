@@ -1,0 +1,251 @@
+"""Synthetic case builders for the 3dalm hot path (host-side, numpy only).
+
+They produce the inputs the reference reads from its case directory -- ``mesh.json``
+(``src/io.h:98-164``), ``windturbine.json`` and the ``windturbine array`` entry of
+``setup.json`` (``src/alm.h:35-206``, ``src/solver.h:702-708``) -- for the workloads
+this project measures, plus the seeded initial perturbation the reference lacks.
+
+Choices that the reference leaves open are fixed here once, so that every consumer
+(the HIP library, the C++ driver, the CPU oracle and the reference build used for
+golden vectors) sees bit-identical inputs:
+
+* mesh nodes: x uniform; y, z two-sided tanh,
+  ``s_j = L0 + (L1-L0) * (1 + tanh(beta*(2j/n-1))/tanh(beta)) / 2``;
+  cell centres are node midpoints, ghost centres follow the extrapolation
+  ``x_g = 3 x_{g+1} - 3 x_{g+2} + x_{g+3}`` of ``script/convertmesh.py:28-32``;
+* widths: the reference rule ``src/io.h:115-125`` (central, one-sided 2nd order at
+  the ends);
+* noise: counter-based splitmix64 of (seed, component, interior index), top 53 bits
+  mapped to [-1, 1);
+* polar: every 1 degree on [-180, 180], ``Cl = clip(2 pi alpha, +-1.2)``,
+  ``Cd = 0.01 + 0.05 alpha^2`` (alpha in radians).
+"""
+from __future__ import annotations
+
+import json
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+GC = 2
+
+
+# ----------------------------------------------------------------------------- mesh
+def uniform_nodes(l0: float, l1: float, n: int) -> np.ndarray:
+    return l0 + (l1 - l0) * (np.arange(n + 1, dtype=np.float64) / n)
+
+
+def tanh_nodes(l0: float, l1: float, n: int, beta: float) -> np.ndarray:
+    s = np.arange(n + 1, dtype=np.float64)
+    return l0 + (l1 - l0) * 0.5 * (1.0 + np.tanh(beta * (2.0 * s / n - 1.0)) / math.tanh(beta))
+
+
+def centres_with_ghosts(nodes: np.ndarray, gc: int = GC) -> np.ndarray:
+    """Interior centres at node midpoints, ghosts by cubic extrapolation
+    (script/convertmesh.py:15-34)."""
+    nint = len(nodes) - 1
+    c = np.zeros(nint + 2 * gc)
+    c[gc:gc + nint] = 0.5 * (nodes[:-1] + nodes[1:])
+    for i in range(gc - 1, -1, -1):
+        c[i] = 3 * c[i + 1] - 3 * c[i + 2] + c[i + 3]
+    for i in range(gc + nint, nint + 2 * gc):
+        c[i] = 3 * c[i - 1] - 3 * c[i - 2] + c[i - 3]
+    return c
+
+
+def widths(c: np.ndarray) -> np.ndarray:
+    """Cell widths from centres, src/io.h:115-125."""
+    d = np.empty_like(c)
+    d[1:-1] = 0.5 * (c[2:] - c[:-2])
+    d[0] = -1.5 * c[0] + 2.0 * c[1] - 0.5 * c[2]
+    d[-1] = 1.5 * c[-1] - 2.0 * c[-2] + 0.5 * c[-3]
+    return d
+
+
+@dataclass
+class Mesh:
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    gc: int = GC
+
+    def __post_init__(self):
+        self.x = np.ascontiguousarray(self.x, dtype=np.float64)
+        self.y = np.ascontiguousarray(self.y, dtype=np.float64)
+        self.z = np.ascontiguousarray(self.z, dtype=np.float64)
+        self.dx, self.dy, self.dz = widths(self.x), widths(self.y), widths(self.z)
+
+    @property
+    def size(self):
+        return (len(self.x), len(self.y), len(self.z))
+
+    @property
+    def interior(self):
+        return tuple(n - 2 * self.gc for n in self.size)
+
+    @property
+    def L(self) -> int:
+        a, b, c = self.size
+        return a * b * c
+
+    @property
+    def N(self) -> int:
+        a, b, c = self.interior
+        return a * b * c
+
+    def to_json(self) -> str:
+        """mesh.json text (src/io.h:105-151); repr keeps every double exact."""
+        def arr(v):
+            return "[" + ", ".join(repr(float(t)) for t in v) + "]"
+        return ('{"guide cell": %d, "x": %s, "y": %s, "z": %s}'
+                % (self.gc, arr(self.x), arr(self.y), arr(self.z)))
+
+
+def make_mesh(nx, ny, nz, box, beta_y=None, beta_z=None, gc=GC) -> Mesh:
+    """box = ((x0,x1),(y0,y1),(z0,z1)); beta None -> uniform."""
+    (x0, x1), (y0, y1), (z0, z1) = box
+    xn = uniform_nodes(x0, x1, nx)
+    yn = tanh_nodes(y0, y1, ny, beta_y) if beta_y else uniform_nodes(y0, y1, ny)
+    zn = tanh_nodes(z0, z1, nz, beta_z) if beta_z else uniform_nodes(z0, z1, nz)
+    return Mesh(centres_with_ghosts(xn, gc), centres_with_ghosts(yn, gc),
+                centres_with_ghosts(zn, gc), gc)
+
+
+# ----------------------------------------------------------------------------- noise
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(z: np.ndarray) -> np.ndarray:
+    z = (z + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(_M64)
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(_M64)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(_M64)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform_pm1(seed: int, count: int, stream: int = 0) -> np.ndarray:
+    """count values in [-1, 1), counter-based (seed, stream, index)."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0xD1342543DE82EF95)
+               + np.uint64(stream) * np.uint64(0x9E3779B97F4A7C15)) & np.uint64(_M64)
+        z = splitmix64(np.arange(count, dtype=np.uint64) ^ key)
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -52) - 1.0
+
+
+def initial_velocity(mesh: Mesh, uin, amp: float, seed: int) -> np.ndarray:
+    """[L][3] velocity: Uin everywhere, interior perturbed by amp*xi per component
+    (xi uniform in [-1,1), interior cells in (i,j,k) order)."""
+    U = np.empty((mesh.L, 3))
+    U[:] = np.asarray(uin, dtype=np.float64)
+    if amp:
+        g = mesh.gc
+        nx, ny, nz = mesh.size
+        idx = np.arange(mesh.L).reshape(nx, ny, nz)[g:nx - g, g:ny - g, g:nz - g].ravel()
+        for m in range(3):
+            U[idx, m] = U[idx, m] + amp * uniform_pm1(seed, len(idx), stream=m)
+    return U
+
+
+# ----------------------------------------------------------------------------- turbines
+def polar_table(step_deg: float = 1.0):
+    atk = np.arange(-180.0, 180.0 + 0.5 * step_deg, step_deg)
+    a = np.deg2rad(atk)
+    cl = np.clip(2 * np.pi * a, -1.2, 1.2)
+    cd = 0.01 + 0.05 * a * a
+    return atk, cl, cd
+
+
+@dataclass
+class TurbineModel:
+    """windturbine.json (src/alm.h:39-63, 151-152)."""
+    R: float = 0.5
+    hub_R: float = 0.0
+    blades: int = 3
+    tower: float = 4.0
+    overhang: float = 0.0
+    stations_r: Optional[np.ndarray] = None      # written under "r/R" (absolute r, Q5)
+    stations_chord: Optional[np.ndarray] = None  # written under "chord/R" (absolute)
+    stations_twist: Optional[np.ndarray] = None  # degrees
+    atk: Optional[np.ndarray] = None
+    cl: Optional[np.ndarray] = None              # [stations][atk]
+    cd: Optional[np.ndarray] = None
+
+    def to_json(self) -> str:
+        af = []
+        for s in range(len(self.stations_r)):
+            af.append({"r/R": float(self.stations_r[s]), "chord/R": float(self.stations_chord[s]),
+                       "twist[deg]": float(self.stations_twist[s]),
+                       "Cl": [float(v) for v in self.cl[s]], "Cd": [float(v) for v in self.cd[s]]})
+        return json.dumps({"R": self.R, "hub R": self.hub_R, "number of blades": self.blades,
+                           "tower": self.tower, "overhang": self.overhang,
+                           "blade": {"airfoil": af, "attack angle": [float(a) for a in self.atk]}})
+
+
+def default_turbine(R=0.5, tower=4.0, n_stations=16, chord=0.05, twist_root=13.0, twist_tip=1.0):
+    atk, cl, cd = polar_table()
+    r = R * np.arange(n_stations) / (n_stations - 1)
+    return TurbineModel(R=R, hub_R=0.0, blades=3, tower=tower, overhang=0.0,
+                        stations_r=r, stations_chord=np.full(n_stations, chord),
+                        stations_twist=twist_root + (twist_tip - twist_root) * r / R,
+                        atk=atk, cl=np.tile(cl, (n_stations, 1)), cd=np.tile(cd, (n_stations, 1)))
+
+
+@dataclass
+class TurbineSite:
+    """One entry of setup.json "windturbine array" (src/solver.h:702-708)."""
+    base: tuple
+    rot_speed: float
+    angle: tuple = (0.0, 0.0, 0.0)  # roll, pitch, yaw: number (deg) or dict {amp,T,phase,offset}
+    pitch: float = 0.0              # collective blade pitch, degrees (not in the reference)
+
+    def to_dict(self):
+        return {"base": list(self.base), "angle": list(self.angle), "rotation speed": self.rot_speed}
+
+
+def sites_json(sites: List[TurbineSite]) -> str:
+    return json.dumps([s.to_dict() for s in sites])
+
+
+# ----------------------------------------------------------------------------- cases
+@dataclass
+class Case:
+    name: str
+    mesh: Mesh
+    uin: tuple = (1.0, 0.0, 0.0)
+    Re: float = 1e4
+    Cs: float = 0.1
+    dt: float = 0.005
+    method: str = "SOR"          # "SOR" | "BiCG"
+    pc: str = "Jacobi"           # "Jacobi" | "SOR"
+    omega: float = 1.2
+    pc_omega: float = 1.0
+    pc_it: int = 3
+    max_it: int = 1000
+    tol: float = 1e-6
+    noise: float = 0.05
+    seed: int = 1
+    steps: int = 100
+    turbine: Optional[TurbineModel] = None
+    sites: List[TurbineSite] = field(default_factory=list)
+    ap_per_blade: int = 64
+    width: float = 0.125
+
+    def U0(self) -> np.ndarray:
+        return initial_velocity(self.mesh, self.uin, self.noise * abs(self.uin[0]), self.seed)
+
+
+def case_c1(nx=64, ny=48, nz=48, steps=100) -> Case:
+    """Plain FP64, no turbine: 64x48x48 on 10x4x4, tanh beta=1.2 in y,z, SOR 1.2 to 1e-6."""
+    m = make_mesh(nx, ny, nz, ((0.0, 10.0), (-2.0, 2.0), (0.0, 4.0)), 1.2, 1.2)
+    return Case("c1", m, method="SOR", omega=1.2, tol=1e-6, max_it=1000, steps=steps)
+
+
+def case_c2(nx=256, ny=128, nz=128, steps=1000, ap_per_blade=64) -> Case:
+    """One 3-blade rotor (D=1 at x=3), TSR 6, eps = 2 dx, BiCGStab(Jacobi 3) to 1e-8."""
+    m = make_mesh(nx, ny, nz, ((0.0, 16.0), (-4.0, 4.0), (0.0, 8.0)), 1.5, 1.5)
+    dx = 16.0 / nx
+    t = default_turbine(R=0.5, tower=4.0)
+    return Case("c2", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000,
+                steps=steps, turbine=t, sites=[TurbineSite((3.0, 0.0, 0.0), 12.0)],
+                ap_per_blade=ap_per_blade, width=2 * dx)
