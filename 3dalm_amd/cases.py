@@ -249,3 +249,27 @@ def case_c2(nx=256, ny=128, nz=128, steps=1000, ap_per_blade=64) -> Case:
     return Case("c2", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000,
                 steps=steps, turbine=t, sites=[TurbineSite((3.0, 0.0, 0.0), 12.0)],
                 ap_per_blade=ap_per_blade, width=2 * dx)
+
+
+def turbine_records(turbine: TurbineModel, sites: List[TurbineSite]) -> np.ndarray:
+    """Turbine rows (include/dalm_hip.h layout), src/alm.h:144-206: rotation centre
+    (-overhang, 0, tower); the angle type is the first non-zero or object entry of
+    (roll, pitch, yaw), with formula (0, 0, 0, rad(a)) for a number and
+    (rad(amp), 2 pi / T, rad(phase), rad(offset)) for an object.  All-zero angles
+    give type "none" (the reference leaves that case unset)."""
+    wt = np.zeros((len(sites), 16))
+    for w, s in enumerate(sites):
+        wt[w, 0:3] = s.base
+        wt[w, 3:6] = (-turbine.overhang, 0.0, turbine.tower)
+        wt[w, 6] = 3
+        for j, a in enumerate(s.angle):
+            if isinstance(a, dict) or a != 0:
+                wt[w, 6] = j
+                if isinstance(a, dict):
+                    wt[w, 9:13] = (a["amp"] * math.pi / 180, 2 * math.pi / a["T"],
+                                   a["phase"] * math.pi / 180, a["offset"] * math.pi / 180)
+                else:
+                    wt[w, 9:13] = (0.0, 0.0, 0.0, a * math.pi / 180)
+                break
+        wt[w, 13] = s.rot_speed
+    return wt

@@ -1,0 +1,196 @@
+// common.h -- shared device/host definitions for the 3dalm hot path on gfx950.
+//
+// Layout on the device (HBM): every field is stored structure-of-arrays, one FP64
+// (or FP32 in the precision form) array per component, over the reference's padded
+// box n[0] x n[1] x n[2] (gc ghost layers per side) with the reference's linear
+// index i*n1*n2 + j*n2 + k (z fastest, src/util.h:10-13).  A wavefront covers 64
+// consecutive k of one (i, j) row, so every stencil load is a coalesced 512-byte
+// (FP64) row segment; the j and i neighbours of a block's tile are re-read from
+// L2 / the 256 MB last-level cache rather than HBM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DALM_HD __host__ __device__ __forceinline__
+
+// Block shape used by all cell kernels: 64 lanes along k (one wavefront row) x
+// 4 rows along j = 256 threads = 4 wavefronts.
+constexpr int TK = 64;
+constexpr int TJ = 4;
+constexpr int BLOCK = TK * TJ;
+
+struct Dims {
+    int n0, n1, n2;     // padded extents
+    int gc;             // ghost layers
+    int nx, ny, nz;     // interior counts
+    int si, sj;         // strides of i and j (k stride is 1)
+    int tk, tj;         // tiles along k and j: ceil(nz/TK), ceil(ny/TJ)
+    int ntiles;         // nx * tj * tk  (one tile = 1 x TJ x TK interior cells)
+    DALM_HD int at(int i, int j, int k) const { return i * si + j * sj + k; }
+};
+
+// Grid-stride assignment of interior tiles to blocks.  Blocks are dispatched to
+// the eight XCDs round-robin (block b -> XCD b % 8); the virtual block number
+// below gives each XCD one contiguous range of tiles (i-major order), so halo
+// planes shared by neighbouring tiles are mostly served by the same XCD's L2.
+DALM_HD int virtual_block(int b, int nblocks) {
+    if (nblocks % 8) return b;
+    return (b & 7) * (nblocks >> 3) + (b >> 3);
+}
+
+// Tile t -> base interior coordinates of this thread (may fall outside the
+// interior along j or k on ragged edges; callers test `inside`).
+struct Cell {
+    int i, j, k;
+    bool inside;
+};
+__device__ __forceinline__ Cell tile_cell(const Dims& d, int t) {
+    int kt = t % d.tk;
+    int r = t / d.tk;
+    int jt = r % d.tj;
+    int it = r / d.tj;
+    Cell c;
+    c.k = d.gc + kt * TK + (int)(threadIdx.x % TK);
+    c.j = d.gc + jt * TJ + (int)(threadIdx.x / TK);
+    c.i = d.gc + it;
+    c.inside = (c.k < d.gc + d.nz) && (c.j < d.gc + d.ny);
+    return c;
+}
+
+// Contiguous chunk of tiles for this block.
+struct TileRange {
+    int begin, end;
+};
+__device__ __forceinline__ TileRange block_tiles(const Dims& d) {
+    int vb = virtual_block((int)blockIdx.x, (int)gridDim.x);
+    int per = (d.ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    int b = vb * per;
+    int e = b + per;
+    if (e > d.ntiles) e = d.ntiles;
+    if (b > d.ntiles) b = d.ntiles;
+    TileRange r;
+    r.begin = b;
+    r.end = e;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (wavefront shuffle, then LDS across the 4 wavefronts).
+// Fixed trees: results are bitwise reproducible from run to run.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Sum NV values over the block; the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double red[NV][BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) red[q][wave] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = red[q][0];
+            for (int w = 1; w < BLOCK / 64; ++w) s += red[q][w];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double red[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = wave_max(v);
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; ++w) v = red[w] > v ? red[w] : v;
+    }
+    __syncthreads();
+    return v;
+}
+
+// Device-coherent load/store for cross-block hand-off of partial results.
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void coherent_store(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Last-block-done protocol: every block stores its NV partials, then the block
+// that increments the counter last reduces all partials in block order and
+// returns true (in every thread) so the caller can finish on the device.
+template <int NV>
+__device__ __forceinline__ bool publish_partials(const double (&v)[NV], double* partials,
+                                                 unsigned* counter) {
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NV; ++q) coherent_store(&partials[q * gridDim.x + blockIdx.x], v[q]);
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        if (threadIdx.x == 0) coherent_store(counter, 0u);
+    }
+    return last;
+}
+
+// In the last block: reduce partials[q*nb + b] over b (fixed order).  Result in
+// all threads' out[] after the call.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* partials, int nb, double (&out)[NV]) {
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = 0;
+        for (int b = (int)threadIdx.x; b < nb; b += BLOCK) s += coherent_load(&partials[q * nb + b]);
+        v[q] = s;
+    }
+    block_sum<NV>(v);
+    __shared__ double bc[NV];
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NV; ++q) bc[q] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = bc[q];
+}
+
+__device__ __forceinline__ double reduce_partials_max(const double* partials, int nb) {
+    double v = 0;
+    for (int b = (int)threadIdx.x; b < nb; b += BLOCK) {
+        double w = coherent_load(&partials[b]);
+        v = w > v ? w : v;
+    }
+    v = block_max(v);
+    __shared__ double bc;
+    if (threadIdx.x == 0) bc = v;
+    __syncthreads();
+    return bc;
+}

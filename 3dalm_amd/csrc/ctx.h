@@ -1,0 +1,163 @@
+// ctx.h -- the device context behind the C ABI (include/dalm_hip.h).
+//
+// One context owns one padded box of fields in HBM, the 1-D mesh metrics, the
+// matrix-free Poisson coefficients, the actuator-line tables, reduction scratch
+// and one HIP stream.  The reference keeps the same state in its Cfd / Eq / Mesh /
+// Alm structs (src/solver.h:207-517); here it lives on the GPU for the whole run and
+// only scalars cross to the host each step.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "../../include/dalm_hip.h"
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return dalm_fail(ctx, e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+// Device-resident scalar state of one Poisson solve (read by every kernel of the
+// solve; written only by the last block of a reduction).
+struct SolveState {
+    double rho, rho_prev, alpha, omega, beta;
+    double err;
+    double tol;
+    double root_n;
+    double aux0, aux1;
+    long long it, max_it;
+    int done;       // 1 once the stopping test (or a breakdown) has ended the solve
+    int status;     // DALM_OK, DALM_NONCONVERGED, DALM_BREAKDOWN, DALM_NONFINITE
+    int pad[2];
+};
+
+// Per-step quantities returned to the host in one copy.
+struct StepScalars {
+    double avg_div_sq;  // sum of div^2 over the interior
+    double max_cfl;
+    double torque[DALM_MAX_TURBINES];
+    double thrust[DALM_MAX_TURBINES];
+};
+
+// 1-D Poisson coefficients reproducing build_A (src/eq.h:701-798) bit for bit:
+// the normalised off-diagonals are 1-D in one index each; the diagonal is
+// accumulated per cell from 1-D pieces in the reference's order and divided by
+// max|a_c|, so no 3-D operator array is stored or read.
+struct PoissonCoef {
+    double* ae_n;   // [n0] a_e / max_diag (0 at the last interior i)
+    double* aw_n;   // [n0]
+    double* an_n;   // [n1]
+    double* as_n;   // [n1]
+    double* at_n;   // [n2]
+    double* ab_n;   // [n2]
+    double* acx;    // [n0] (0 - a_e|2/dx^2) - a_w   (unnormalised)
+    double* an_u;   // [n1] unnormalised a_n, a_s, a_t, a_b
+    double* as_u;
+    double* at_u;
+    double* ab_u;
+    double max_diag;
+};
+
+// Device view of the Poisson coefficients passed to kernels by value.
+struct PCoef {
+    const double *ae, *aw, *an, *as, *at, *ab, *acx, *anu, *asu, *atu, *abu;
+    double max_diag;
+};
+
+struct AlmTables {
+    int wt_count = 0, ap_count = 0, blade_per_wt = 0, ap_per_blade = 0, atk_count = 0;
+    double width = 0;
+    double* wt = nullptr;     // [wt][DALM_WT_STRIDE]
+    double* ap = nullptr;     // [ap][DALM_AP_STRIDE]
+    double* cd = nullptr;     // [ap][atk]
+    double* cl = nullptr;     // [ap][atk]
+    double* atk = nullptr;    // [atk]
+    std::vector<double> h_wt; // host copy of turbine records (angles are set on the host)
+    std::vector<double> h_rmax; // largest AP radius per turbine
+    int* box_dev = nullptr;   // [wt][6] i0,i1,j0,j1,k0,k1 (half-open) cells any AP can reach
+};
+
+struct dalm_ctx {
+    Dims d;
+    long long L = 0, N = 0;
+    int precision = DALM_FP64;
+    hipStream_t stream = nullptr;
+    int grid = 0;  // blocks for the cell kernels
+
+    // mesh (device)
+    double *x = nullptr, *y = nullptr, *z = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    std::vector<double> hx, hy, hz, hdx, hdy, hdz;
+
+    // FP64 fields (structure of arrays)
+    double* U[3] = {};
+    double* Uold[3] = {};
+    double* JU[3] = {};
+    double* f[3] = {};
+    double *p = nullptr, *nut = nullptr, *div = nullptr, *b = nullptr, *solid = nullptr;
+    bool has_solid = false;
+    // Krylov work vectors
+    double *r = nullptr, *r0 = nullptr, *kp = nullptr, *pp = nullptr, *q = nullptr, *s = nullptr,
+           *ss = nullptr, *t = nullptr, *tmp = nullptr, *tmp2 = nullptr;
+    long long prev_it[2] = {0, 0};  // iterations of the previous SOR / BiCGStab solve
+
+    PoissonCoef pc{};
+    bool have_A = false;
+
+    // reduction scratch
+    double* partials = nullptr;  // [8][grid]
+    unsigned* counter = nullptr;
+    SolveState* solve = nullptr;   // device
+    SolveState* h_solve = nullptr; // pinned host mirror
+    StepScalars* scal = nullptr;   // device
+    StepScalars* h_scal = nullptr; // pinned host
+
+    // run parameters (dalm_set_params)
+    dalm_params prm{};
+    bool have_prm = false;
+    long long step = 0;
+    dalm_step_stats last{};
+
+    AlmTables alm;
+
+    std::string err;
+    int hip_err = 0;
+
+    PCoef coef() const {
+        PCoef c;
+        c.ae = pc.ae_n; c.aw = pc.aw_n; c.an = pc.an_n; c.as = pc.as_n; c.at = pc.at_n; c.ab = pc.ab_n;
+        c.acx = pc.acx; c.anu = pc.an_u; c.asu = pc.as_u; c.atu = pc.at_u; c.abu = pc.ab_u;
+        c.max_diag = pc.max_diag;
+        return c;
+    }
+};
+
+int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
+int dalm_fail_msg(dalm_ctx* ctx, int status, const std::string& msg);
+
+// kernel launchers (cfd.hip, poisson.hip, alm.hip)
+int launch_predictor(dalm_ctx* ctx, double Re, double dt);
+int launch_copy_uold(dalm_ctx* ctx);
+int launch_ghost_copy(dalm_ctx* ctx);
+int launch_flux(dalm_ctx* ctx, const double Uin[3], bool apply_bc);
+int launch_jubc(dalm_ctx* ctx, const double Uin[3]);
+int launch_rhs(dalm_ctx* ctx, double dt, double scale);
+int launch_pbc(dalm_ctx* ctx);
+int launch_project(dalm_ctx* ctx, double dt, const double* Uin_bc);
+int launch_solid(dalm_ctx* ctx);
+int launch_ubc(dalm_ctx* ctx, const double Uin[3], double dt, int retracted, double floor_x);
+int launch_nut(dalm_ctx* ctx, double Cs);
+int launch_post(dalm_ctx* ctx, double Cs, double dt, bool store_div);
+int launch_divergence(dalm_ctx* ctx);
+int launch_l2(dalm_ctx* ctx, const double* v, double* out_host);
+int launch_max_cfl(dalm_ctx* ctx, double dt, double* out_host);
+int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long* it, double* err);
+int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
+                 long long* it, double* err);
+int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r);
+int launch_Ax(dalm_ctx* ctx, const double* x, double* y);
+int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color);
+int alm_run(dalm_ctx* ctx, double t, bool sync_scalars);
+int sync_stream(dalm_ctx* ctx);
+int zero_device(dalm_ctx* ctx, double* a, long long n);

@@ -1,0 +1,575 @@
+// ctx.hip -- context lifetime, host<->device field copies, matrix-free operator
+// set-up, and the C ABI entry points of include/dalm_hip.h.
+#include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "ctx.h"
+
+int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line) {
+    if (ctx) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+        ctx->err = buf;
+        ctx->hip_err = (int)e;
+    }
+    return DALM_HIP_ERROR;
+}
+
+int dalm_fail_msg(dalm_ctx* ctx, int status, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return status;
+}
+
+int sync_stream(dalm_ctx* ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DALM_OK;
+}
+
+namespace {
+
+template <typename T>
+int dev_alloc(dalm_ctx* ctx, T** p, size_t n) {
+    HIP_TRY(hipMalloc((void**)p, n * sizeof(T) + 256));
+    HIP_TRY(hipMemset(*p, 0, n * sizeof(T) + 256));
+    return DALM_OK;
+}
+
+int upload_vec(dalm_ctx* ctx, double* dst, const std::vector<double>& src) {
+    HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+    return DALM_OK;
+}
+
+// Host computation of the 1-D pieces of build_A (src/eq.h:716-795), single domain.
+int build_operator(dalm_ctx* ctx) {
+    const Dims& d = ctx->d;
+    const int g = d.gc;
+    std::vector<double> ae(d.n0, 0), aw(d.n0, 0), acx(d.n0, 0), an(d.n1, 0), as(d.n1, 0),
+        at(d.n2, 0), ab(d.n2, 0);
+    for (int i = g; i < d.n0 - g; ++i) {
+        double c = 0;
+        if (i < d.n0 - g - 1) {
+            ae[i] = 1 / (ctx->hdx[i] * (ctx->hx[i + 1] - ctx->hx[i]));
+            c -= ae[i];
+        } else {
+            c -= 2 / (ctx->hdx[i] * ctx->hdx[i]);
+        }
+        if (i > g) {
+            aw[i] = 1 / (ctx->hdx[i] * (ctx->hx[i] - ctx->hx[i - 1]));
+            c -= aw[i];
+        }
+        acx[i] = c;
+    }
+    for (int j = g; j < d.n1 - g; ++j) {
+        if (j < d.n1 - g - 1) an[j] = 1 / (ctx->hdy[j] * (ctx->hy[j + 1] - ctx->hy[j]));
+        if (j > g) as[j] = 1 / (ctx->hdy[j] * (ctx->hy[j] - ctx->hy[j - 1]));
+    }
+    for (int k = g; k < d.n2 - g; ++k) {
+        if (k < d.n2 - g - 1) at[k] = 1 / (ctx->hdz[k] * (ctx->hz[k + 1] - ctx->hz[k]));
+        if (k > g) ab[k] = 1 / (ctx->hdz[k] * (ctx->hz[k] - ctx->hz[k - 1]));
+    }
+    // max |a_c| over the interior, a_c accumulated in the reference's order
+    double mx = 0;
+    for (int i = g; i < d.n0 - g; ++i)
+        for (int j = g; j < d.n1 - g; ++j) {
+            double cij = (acx[i] - an[j]) - as[j];
+            for (int k = g; k < d.n2 - g; ++k) {
+                double c = (cij - at[k]) - ab[k];
+                if (fabs(c) > mx) mx = fabs(c);
+            }
+        }
+    auto norm = [&](const std::vector<double>& v) {
+        std::vector<double> o(v.size());
+        for (size_t q = 0; q < v.size(); ++q) o[q] = v[q] / mx;
+        return o;
+    };
+    PoissonCoef& P = ctx->pc;
+    int rc;
+    if ((rc = upload_vec(ctx, P.ae_n, norm(ae))) || (rc = upload_vec(ctx, P.aw_n, norm(aw))) ||
+        (rc = upload_vec(ctx, P.an_n, norm(an))) || (rc = upload_vec(ctx, P.as_n, norm(as))) ||
+        (rc = upload_vec(ctx, P.at_n, norm(at))) || (rc = upload_vec(ctx, P.ab_n, norm(ab))) ||
+        (rc = upload_vec(ctx, P.acx, acx)) || (rc = upload_vec(ctx, P.an_u, an)) ||
+        (rc = upload_vec(ctx, P.as_u, as)) || (rc = upload_vec(ctx, P.at_u, at)) ||
+        (rc = upload_vec(ctx, P.ab_u, ab)))
+        return rc;
+    P.max_diag = mx;
+    ctx->have_A = true;
+    return DALM_OK;
+}
+
+// field id -> device component pointers (FP64 context)
+int field_ptrs(dalm_ctx* ctx, int64_t field, double** c, int* ncomp) {
+    *ncomp = 1;
+    switch (field) {
+    case DALM_F_U: c[0] = ctx->U[0]; c[1] = ctx->U[1]; c[2] = ctx->U[2]; *ncomp = 3; break;
+    case DALM_F_UOLD: c[0] = ctx->Uold[0]; c[1] = ctx->Uold[1]; c[2] = ctx->Uold[2]; *ncomp = 3; break;
+    case DALM_F_JU: c[0] = ctx->JU[0]; c[1] = ctx->JU[1]; c[2] = ctx->JU[2]; *ncomp = 3; break;
+    case DALM_F_F: c[0] = ctx->f[0]; c[1] = ctx->f[1]; c[2] = ctx->f[2]; *ncomp = 3; break;
+    case DALM_F_P: c[0] = ctx->p; break;
+    case DALM_F_NUT: c[0] = ctx->nut; break;
+    case DALM_F_DIV: c[0] = ctx->div; break;
+    case DALM_F_SOLID: c[0] = ctx->solid; break;
+    case DALM_F_B: c[0] = ctx->b; break;
+    default: return DALM_BAD_ARG;
+    }
+    return DALM_OK;
+}
+
+void fill_stats(dalm_ctx* ctx, dalm_step_stats* st, long long it, double err, int status) {
+    std::memset(st, 0, sizeof *st);
+    st->step = ctx->step;
+    st->time = ctx->prm.dt * (double)ctx->step;
+    st->it = it;
+    st->err = err;
+    st->avg_div = sqrt(ctx->h_scal->avg_div_sq) / sqrt((double)ctx->N);
+    st->max_cfl = ctx->h_scal->max_cfl;
+    st->status = status;
+    st->wt_count = ctx->alm.wt_count;
+    for (int w = 0; w < ctx->alm.wt_count && w < DALM_MAX_TURBINES; ++w) {
+        st->torque[w] = ctx->h_scal->torque[w];
+        st->thrust[w] = ctx->h_scal->thrust[w];
+        st->power[w] = ctx->alm.h_wt[DALM_WT_STRIDE * w + 13] * st->torque[w];
+    }
+    ctx->last = *st;
+}
+
+int read_scalars(dalm_ctx* ctx) {
+    HIP_TRY(hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(StepScalars), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DALM_OK;
+}
+
+}  // namespace
+
+#define CHECK_CTX()                             \
+    do {                                        \
+        if (!ctx) return DALM_BAD_ARG;          \
+    } while (0)
+#define TRY(x)              \
+    do {                    \
+        int rc_ = (x);      \
+        if (rc_) return rc_; \
+    } while (0)
+
+extern "C" {
+
+int dalm_set_device(int64_t device) {
+    return hipSetDevice((int)device) == hipSuccess ? DALM_OK : DALM_HIP_ERROR;
+}
+
+int dalm_mesh_widths(const double* c, int64_t n, double* d) {
+    if (!c || !d || n < 3) return DALM_BAD_ARG;
+    for (int64_t i = 1; i < n - 1; ++i) d[i] = 0.5 * (c[i + 1] - c[i - 1]);
+    d[0] = -1.5 * c[0] + 2.0 * c[1] - 0.5 * c[2];
+    d[n - 1] = 1.5 * c[n - 1] - 2.0 * c[n - 2] + 0.5 * c[n - 3];
+    return DALM_OK;
+}
+
+const char* dalm_last_error(const dalm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double* x, const double* y,
+                const double* z, int64_t precision) {
+    if (!out || !size || !x || !y || !z) return DALM_BAD_ARG;
+    *out = nullptr;
+    if (gc < 2) return DALM_BAD_ARG;  // the reference asserts gc >= 2 (src/solver.h:615)
+    for (int a = 0; a < 3; ++a)
+        if (size[a] < 2 * gc + 3) return DALM_BAD_ARG;
+    long long L = size[0] * size[1] * size[2];
+    if (L >= (1LL << 31) - (1LL << 20)) return DALM_BAD_ARG;  // 32-bit cell indices
+    if (precision != DALM_FP64 && precision != DALM_MIXED) return DALM_BAD_ARG;
+
+    dalm_ctx* ctx = new dalm_ctx();
+    Dims& d = ctx->d;
+    d.n0 = (int)size[0];
+    d.n1 = (int)size[1];
+    d.n2 = (int)size[2];
+    d.gc = (int)gc;
+    d.nx = d.n0 - 2 * d.gc;
+    d.ny = d.n1 - 2 * d.gc;
+    d.nz = d.n2 - 2 * d.gc;
+    d.si = d.n1 * d.n2;
+    d.sj = d.n2;
+    d.tk = (d.nz + TK - 1) / TK;
+    d.tj = (d.ny + TJ - 1) / TJ;
+    d.ntiles = d.nx * d.tj * d.tk;
+    ctx->L = L;
+    ctx->N = (long long)d.nx * d.ny * d.nz;
+    ctx->precision = (int)precision;
+    int g = d.ntiles < 2048 ? d.ntiles : 2048;
+    ctx->grid = g >= 64 ? (g / 8) * 8 : g;
+
+    ctx->hx.assign(x, x + d.n0);
+    ctx->hy.assign(y, y + d.n1);
+    ctx->hz.assign(z, z + d.n2);
+    ctx->hdx.resize(d.n0);
+    ctx->hdy.resize(d.n1);
+    ctx->hdz.resize(d.n2);
+    dalm_mesh_widths(x, d.n0, ctx->hdx.data());
+    dalm_mesh_widths(y, d.n1, ctx->hdy.data());
+    dalm_mesh_widths(z, d.n2, ctx->hdz.data());
+
+    int rc = DALM_OK;
+    auto fail = [&](int r) {
+        dalm_destroy(ctx);
+        return r;
+    };
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) return fail(DALM_HIP_ERROR);
+    double** one_d[6] = {&ctx->x, &ctx->y, &ctx->z, &ctx->dx, &ctx->dy, &ctx->dz};
+    const std::vector<double>* src[6] = {&ctx->hx, &ctx->hy, &ctx->hz, &ctx->hdx, &ctx->hdy, &ctx->hdz};
+    for (int q = 0; q < 6; ++q) {
+        if ((rc = dev_alloc(ctx, one_d[q], src[q]->size()))) return fail(rc);
+        if ((rc = upload_vec(ctx, *one_d[q], *src[q]))) return fail(rc);
+    }
+    PoissonCoef& P = ctx->pc;
+    double** pc1[11] = {&P.ae_n, &P.aw_n, &P.an_n, &P.as_n, &P.at_n, &P.ab_n, &P.acx, &P.an_u, &P.as_u,
+                        &P.at_u, &P.ab_u};
+    int pcn[11] = {d.n0, d.n0, d.n1, d.n1, d.n2, d.n2, d.n0, d.n1, d.n1, d.n2, d.n2};
+    for (int q = 0; q < 11; ++q)
+        if ((rc = dev_alloc(ctx, pc1[q], pcn[q]))) return fail(rc);
+
+    for (int q = 0; q < 3; ++q) {
+        if ((rc = dev_alloc(ctx, &ctx->U[q], L)) || (rc = dev_alloc(ctx, &ctx->Uold[q], L)) ||
+            (rc = dev_alloc(ctx, &ctx->JU[q], L)) || (rc = dev_alloc(ctx, &ctx->f[q], L)))
+            return fail(rc);
+    }
+    double** sc[15] = {&ctx->p, &ctx->nut, &ctx->div, &ctx->b, &ctx->solid, &ctx->r, &ctx->r0,
+                       &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t, &ctx->tmp, &ctx->tmp2};
+    for (int q = 0; q < 15; ++q)
+        if ((rc = dev_alloc(ctx, sc[q], L))) return fail(rc);
+    if ((rc = dev_alloc(ctx, &ctx->partials, 8 * (size_t)ctx->grid)) ||
+        (rc = dev_alloc(ctx, &ctx->counter, 16)) || (rc = dev_alloc(ctx, &ctx->solve, 1)) ||
+        (rc = dev_alloc(ctx, &ctx->scal, 1)))
+        return fail(rc);
+    if (hipHostMalloc((void**)&ctx->h_solve, sizeof(SolveState)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_scal, sizeof(StepScalars)) != hipSuccess)
+        return fail(DALM_HIP_ERROR);
+    std::memset(ctx->h_solve, 0, sizeof(SolveState));
+    std::memset(ctx->h_scal, 0, sizeof(StepScalars));
+    if ((rc = build_operator(ctx))) return fail(rc);
+    *out = ctx;
+    return DALM_OK;
+}
+
+int dalm_destroy(dalm_ctx* ctx) {
+    if (!ctx) return DALM_OK;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
+                      ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
+                      ctx->t, ctx->tmp, ctx->tmp2, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
+                      ctx->pc.an_n, ctx->pc.as_n, ctx->pc.at_n, ctx->pc.ab_n, ctx->pc.acx,
+                      ctx->pc.an_u, ctx->pc.as_u, ctx->pc.at_u, ctx->pc.ab_u, ctx->alm.wt, ctx->alm.ap,
+                      ctx->alm.cd, ctx->alm.cl, ctx->alm.atk};
+    for (double* p : ptrs)
+        if (p) hipFree(p);
+    for (int q = 0; q < 3; ++q) {
+        if (ctx->U[q]) hipFree(ctx->U[q]);
+        if (ctx->Uold[q]) hipFree(ctx->Uold[q]);
+        if (ctx->JU[q]) hipFree(ctx->JU[q]);
+        if (ctx->f[q]) hipFree(ctx->f[q]);
+    }
+    if (ctx->alm.box_dev) hipFree(ctx->alm.box_dev);
+    if (ctx->counter) hipFree(ctx->counter);
+    if (ctx->solve) hipFree(ctx->solve);
+    if (ctx->scal) hipFree(ctx->scal);
+    if (ctx->h_solve) hipHostFree(ctx->h_solve);
+    if (ctx->h_scal) hipHostFree(ctx->h_scal);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DALM_OK;
+}
+
+int dalm_upload(dalm_ctx* ctx, int64_t field, const double* host) {
+    CHECK_CTX();
+    if (!host) return DALM_BAD_ARG;
+    double* c[3];
+    int nc;
+    TRY(field_ptrs(ctx, field, c, &nc));
+    std::vector<double> tmp(ctx->L);
+    for (int q = 0; q < nc; ++q) {
+        for (long long e = 0; e < ctx->L; ++e) tmp[e] = host[e * nc + q];
+        HIP_TRY(hipMemcpyAsync(c[q], tmp.data(), ctx->L * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    if (field == DALM_F_SOLID) {
+        ctx->has_solid = false;
+        for (long long e = 0; e < ctx->L; ++e)
+            if (host[e] != 0) {
+                ctx->has_solid = true;
+                break;
+            }
+    }
+    return DALM_OK;
+}
+
+int dalm_download(dalm_ctx* ctx, int64_t field, double* host) {
+    CHECK_CTX();
+    if (!host) return DALM_BAD_ARG;
+    double* c[3];
+    int nc;
+    TRY(field_ptrs(ctx, field, c, &nc));
+    std::vector<double> tmp(ctx->L);
+    for (int q = 0; q < nc; ++q) {
+        HIP_TRY(hipMemcpyAsync(tmp.data(), c[q], ctx->L * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (long long e = 0; e < ctx->L; ++e) host[e * nc + q] = tmp[e];
+    }
+    return DALM_OK;
+}
+
+int dalm_set_params(dalm_ctx* ctx, const dalm_params* prm) {
+    CHECK_CTX();
+    if (!prm || !(prm->dt > 0) || !(prm->Re > 0) || prm->max_it < 1 ||
+        (prm->method != DALM_SOR && prm->method != DALM_BICGSTAB) ||
+        (prm->pc_kind != DALM_PC_JACOBI && prm->pc_kind != DALM_PC_SOR) || prm->pc_it < 0)
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "invalid dalm_params");
+    ctx->prm = *prm;
+    ctx->have_prm = true;
+    return DALM_OK;
+}
+
+// ---- per-kernel entry points --------------------------------------------------
+int dalm_copy_u_to_uold(dalm_ctx* ctx) {
+    CHECK_CTX();
+    TRY(launch_copy_uold(ctx));
+    return sync_stream(ctx);
+}
+int dalm_predictor(dalm_ctx* ctx, double Re, double dt) {
+    CHECK_CTX();
+    TRY(launch_predictor(ctx, Re, dt));
+    return sync_stream(ctx);
+}
+int dalm_interpolate_JU(dalm_ctx* ctx) {
+    CHECK_CTX();
+    TRY(launch_flux(ctx, nullptr, false));
+    return sync_stream(ctx);
+}
+int dalm_apply_JUbc(dalm_ctx* ctx, const double Uin[3]) {
+    CHECK_CTX();
+    if (!Uin) return DALM_BAD_ARG;
+    TRY(launch_jubc(ctx, Uin));
+    return sync_stream(ctx);
+}
+int dalm_poisson_rhs(dalm_ctx* ctx, double dt, double scale) {
+    CHECK_CTX();
+    TRY(launch_rhs(ctx, dt, scale));
+    return sync_stream(ctx);
+}
+int dalm_build_A(dalm_ctx* ctx, double* max_diag) {
+    CHECK_CTX();
+    if (!ctx->have_A) TRY(build_operator(ctx));
+    if (max_diag) *max_diag = ctx->pc.max_diag;
+    return DALM_OK;
+}
+int dalm_run_sor(dalm_ctx* ctx, double omega, int64_t max_it, double tol, int64_t* it, double* err) {
+    CHECK_CTX();
+    if (max_it < 1) return DALM_BAD_ARG;
+    long long i = 0;
+    double e = 0;
+    int rc = run_sor(ctx, omega, max_it, tol, &i, &e);
+    if (it) *it = i;
+    if (err) *err = e;
+    return rc;
+}
+int dalm_run_bicgstab(dalm_ctx* ctx, int64_t pc_kind, double pc_omega, int64_t pc_it, int64_t max_it,
+                      double tol, int64_t* it, double* err) {
+    CHECK_CTX();
+    if (max_it < 1 || pc_it < 0 || (pc_kind != DALM_PC_JACOBI && pc_kind != DALM_PC_SOR))
+        return DALM_BAD_ARG;
+    long long i = 0;
+    double e = 0;
+    int rc = run_bicgstab(ctx, (int)pc_kind, pc_omega, (int)pc_it, max_it, tol, &i, &e);
+    if (it) *it = i;
+    if (err) *err = e;
+    return rc;
+}
+int dalm_residual(dalm_ctx* ctx, double* out) {
+    CHECK_CTX();
+    TRY(launch_residual(ctx, ctx->p, ctx->b, ctx->r));
+    TRY(sync_stream(ctx));
+    if (out) {
+        std::vector<double> tmp(ctx->L);
+        HIP_TRY(hipMemcpy(tmp.data(), ctx->r, ctx->L * sizeof(double), hipMemcpyDeviceToHost));
+        std::memcpy(out, tmp.data(), ctx->L * sizeof(double));
+    }
+    return DALM_OK;
+}
+int dalm_apply_pbc(dalm_ctx* ctx) {
+    CHECK_CTX();
+    TRY(launch_pbc(ctx));
+    return sync_stream(ctx);
+}
+int dalm_project_p(dalm_ctx* ctx, double dt) {
+    CHECK_CTX();
+    TRY(launch_project(ctx, dt, nullptr));
+    return sync_stream(ctx);
+}
+int dalm_set_solid_U(dalm_ctx* ctx) {
+    CHECK_CTX();
+    TRY(launch_solid(ctx));
+    return sync_stream(ctx);
+}
+int dalm_apply_Ubc(dalm_ctx* ctx, const double Uin[3], double dt, int64_t retracted_floor,
+                   double floor_start_x) {
+    CHECK_CTX();
+    if (!Uin) return DALM_BAD_ARG;
+    TRY(launch_ubc(ctx, Uin, dt, (int)retracted_floor, floor_start_x));
+    return sync_stream(ctx);
+}
+int dalm_calc_nut(dalm_ctx* ctx, double Cs) {
+    CHECK_CTX();
+    TRY(launch_nut(ctx, Cs));
+    return sync_stream(ctx);
+}
+int dalm_calc_divergence(dalm_ctx* ctx) {
+    CHECK_CTX();
+    TRY(launch_divergence(ctx));
+    return sync_stream(ctx);
+}
+int dalm_l2_norm(dalm_ctx* ctx, int64_t field, double* out) {
+    CHECK_CTX();
+    double* c[3];
+    int nc;
+    TRY(field_ptrs(ctx, field, c, &nc));
+    if (nc != 1 || !out) return DALM_BAD_ARG;
+    return launch_l2(ctx, c[0], out);
+}
+int dalm_max_cfl(dalm_ctx* ctx, double dt, double* out) {
+    CHECK_CTX();
+    if (!out) return DALM_BAD_ARG;
+    return launch_max_cfl(ctx, dt, out);
+}
+
+// ---- ALM --------------------------------------------------------------------------
+int dalm_alm_setup(dalm_ctx* ctx, const double* wt, int64_t wt_count, const double* ap,
+                   int64_t ap_count, int64_t blade_per_wt, int64_t ap_per_blade, const double* cd,
+                   const double* cl, const double* atk, int64_t atk_count, double width) {
+    CHECK_CTX();
+    if (wt_count < 0 || wt_count > DALM_MAX_TURBINES || ap_count < 0 ||
+        ap_count != wt_count * blade_per_wt * ap_per_blade || (ap_count > 0 && atk_count < 1) ||
+        (ap_count > 0 && !(width > 0)))
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "invalid ALM set-up");
+    AlmTables& A = ctx->alm;
+    A.wt_count = (int)wt_count;
+    A.ap_count = (int)ap_count;
+    A.blade_per_wt = (int)blade_per_wt;
+    A.ap_per_blade = (int)ap_per_blade;
+    A.atk_count = (int)atk_count;
+    A.width = width;
+    if (ap_count == 0) return DALM_OK;
+    A.h_wt.assign(wt, wt + DALM_WT_STRIDE * wt_count);
+    A.h_rmax.assign(wt_count, 0.0);
+    int per_wt = (int)(blade_per_wt * ap_per_blade);
+    for (int64_t a = 0; a < ap_count; ++a) {
+        double r = fabs(ap[DALM_AP_STRIDE * a + 3]);
+        double& m = A.h_rmax[a / per_wt];
+        m = r > m ? r : m;
+    }
+    int rc;
+    if (!A.wt && ((rc = dev_alloc(ctx, &A.wt, DALM_WT_STRIDE * DALM_MAX_TURBINES)) ||
+                  (rc = dev_alloc(ctx, &A.box_dev, 6 * DALM_MAX_TURBINES))))
+        return rc;
+    if (A.ap) { hipFree(A.ap); hipFree(A.cd); hipFree(A.cl); hipFree(A.atk); }
+    if ((rc = dev_alloc(ctx, &A.ap, DALM_AP_STRIDE * ap_count)) ||
+        (rc = dev_alloc(ctx, &A.cd, ap_count * atk_count)) ||
+        (rc = dev_alloc(ctx, &A.cl, ap_count * atk_count)) || (rc = dev_alloc(ctx, &A.atk, atk_count)))
+        return rc;
+    HIP_TRY(hipMemcpy(A.wt, wt, sizeof(double) * DALM_WT_STRIDE * wt_count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A.ap, ap, sizeof(double) * DALM_AP_STRIDE * ap_count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A.cd, cd, sizeof(double) * ap_count * atk_count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A.cl, cl, sizeof(double) * ap_count * atk_count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A.atk, atk, sizeof(double) * atk_count, hipMemcpyHostToDevice));
+    return DALM_OK;
+}
+
+int dalm_run_alm(dalm_ctx* ctx, double t, double* torque, double* thrust) {
+    CHECK_CTX();
+    TRY(alm_run(ctx, t, true));
+    for (int w = 0; w < ctx->alm.wt_count; ++w) {
+        if (torque) torque[w] = ctx->h_scal->torque[w];
+        if (thrust) thrust[w] = ctx->h_scal->thrust[w];
+    }
+    return DALM_OK;
+}
+
+int dalm_alm_download(dalm_ctx* ctx, double* ap, double* wt) {
+    CHECK_CTX();
+    AlmTables& A = ctx->alm;
+    if (A.ap_count == 0) return DALM_OK;
+    if (ap)
+        HIP_TRY(hipMemcpy(ap, A.ap, sizeof(double) * DALM_AP_STRIDE * A.ap_count, hipMemcpyDeviceToHost));
+    if (wt) {
+        HIP_TRY(hipMemcpy(wt, A.wt, sizeof(double) * DALM_WT_STRIDE * A.wt_count, hipMemcpyDeviceToHost));
+        for (int w = 0; w < A.wt_count; ++w) {
+            wt[DALM_WT_STRIDE * w + 14] = ctx->h_scal->torque[w];
+            wt[DALM_WT_STRIDE * w + 15] = ctx->h_scal->thrust[w];
+        }
+    }
+    return DALM_OK;
+}
+
+// ---- whole step -----------------------------------------------------------------------
+int dalm_initialize(dalm_ctx* ctx, dalm_step_stats* stats) {
+    CHECK_CTX();
+    if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
+    const dalm_params& P = ctx->prm;
+    ctx->step = 0;
+    if (!ctx->have_A) TRY(build_operator(ctx));
+    if (ctx->has_solid) TRY(launch_solid(ctx));
+    TRY(launch_ubc(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
+    TRY(launch_flux(ctx, P.Uin, true));
+    TRY(launch_post(ctx, P.Cs, P.dt, true));
+    TRY(alm_run(ctx, 0.0, false));
+    TRY(read_scalars(ctx));
+    dalm_step_stats st;
+    fill_stats(ctx, &st, 0, 0.0, DALM_OK);
+    if (stats) *stats = st;
+    return DALM_OK;
+}
+
+int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
+    CHECK_CTX();
+    if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
+    const dalm_params& P = ctx->prm;
+    ctx->step += 1;
+    const double t = P.dt * (double)ctx->step;
+    // Uold <- U: swap the buffers, then restore U's ghost shell from Uold
+    for (int q = 0; q < 3; ++q) std::swap(ctx->U[q], ctx->Uold[q]);
+    TRY(launch_ghost_copy(ctx));
+    TRY(launch_predictor(ctx, P.Re, P.dt));
+    TRY(launch_flux(ctx, P.Uin, true));
+    TRY(launch_rhs(ctx, P.dt, ctx->pc.max_diag));
+    long long it = 0;
+    double err = 0;
+    int status;
+    if (P.method == DALM_BICGSTAB)
+        status = run_bicgstab(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+    else
+        status = run_sor(ctx, P.omega, P.max_it, P.tol, &it, &err);
+    if (status == DALM_HIP_ERROR) return status;
+    TRY(launch_pbc(ctx));
+    TRY(launch_project(ctx, P.dt, P.Uin));
+    if (ctx->has_solid) TRY(launch_solid(ctx));
+    TRY(launch_ubc(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
+    TRY(launch_post(ctx, P.Cs, P.dt, true));
+    TRY(alm_run(ctx, t, false));
+    TRY(read_scalars(ctx));
+    dalm_step_stats st;
+    fill_stats(ctx, &st, it, err, status);
+    if (stats) *stats = st;
+    return status == DALM_NONCONVERGED ? DALM_OK : status;
+}
+
+int dalm_run(dalm_ctx* ctx, int64_t n, dalm_step_stats* stats) {
+    CHECK_CTX();
+    dalm_step_stats st{};
+    for (int64_t s = 0; s < n; ++s) TRY(dalm_step(ctx, &st));
+    if (stats) *stats = st;
+    return DALM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,453 @@
+// poisson.hip -- FP64 Poisson solvers of the drop-in form: red-black SOR
+// (src/eq.h:145-306) and preconditioned BiCGStab with Jacobi or SOR
+// preconditioning (src/eq.h:6-143, 308-473), on the matrix-free 7-point operator
+// of build_A (src/eq.h:701-798).
+//
+// Design:
+//  * the operator is never stored: off-diagonals are 1-D arrays, the diagonal is
+//    rebuilt per cell from 1-D pieces in the reference's accumulation order, so
+//    each sweep / SpMV reads only the iterate and the right-hand side;
+//  * every scalar of the iteration (rho, alpha, omega, beta, err, it) lives in
+//    device memory and is produced by the last block of the reduction that feeds
+//    it (deterministic fixed-order tree); the stopping test runs on the device;
+//  * the host enqueues iterations in chunks without waiting; every kernel of an
+//    iteration returns at once when the device flag `done` is set, so the solve
+//    stops at exactly the reference's iteration.  The host reads the flag once per
+//    chunk.
+// Compiled with -ffp-contract=off: sweep, residual and SpMV values are the
+// reference's bit for bit; dot products differ only in summation order.
+#include <float.h>
+
+#include "ctx.h"
+
+namespace {
+
+__device__ __forceinline__ double diag_at(const PCoef& A, int i, int j, int k) {
+    return ((((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k]) / A.max_diag;
+}
+
+// a_c x_c + a_e x_e + a_w x_w + a_n x_n + a_s x_s + a_t x_t + a_b x_b, left to
+// right (src/mv.h:87)
+__device__ __forceinline__ double apply7(const Dims& d, const PCoef& A, const double* x, int c,
+                                         int i, int j, int k, double ac) {
+    return ac * x[c] + A.ae[i] * x[c + d.si] + A.aw[i] * x[c - d.si] + A.an[j] * x[c + d.sj] +
+           A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
+}
+
+__device__ __forceinline__ bool finished(const SolveState* st) {
+    return st != nullptr && st->done;
+}
+
+// ---------------------------------------------------------------------------- SOR
+__global__ __launch_bounds__(BLOCK) void k_sor(Dims d, PCoef A, double* __restrict__ x,
+                                               const double* __restrict__ b, double omega,
+                                               int color, const SolveState* st) {
+    if (finished(st)) return;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside || ((cc.i + cc.j + cc.k) & 1) != color) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double ac = diag_at(A, cc.i, cc.j, cc.k);
+        double rel = (b[c] - apply7(d, A, x, c, cc.i, cc.j, cc.k, ac)) / ac;
+        x[c] = x[c] + omega * rel;
+    }
+}
+
+// r = b - A x (stored if r != null) and the stopping test of run_sor
+// (src/eq.h:297-305): err = |r| / sqrt(N); it += 1; stop when !(it < max_it &&
+// err > tol).
+__global__ __launch_bounds__(BLOCK) void k_resnorm(Dims d, PCoef A, const double* __restrict__ x,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ r, double* partials,
+                                                   unsigned* counter, SolveState* st, int count_it) {
+    if (count_it && finished(st)) return;
+    TileRange tr = block_tiles(d);
+    double v[1] = {0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double ac = diag_at(A, cc.i, cc.j, cc.k);
+        double rv = b[c] - apply7(d, A, x, c, cc.i, cc.j, cc.k, ac);
+        if (r) r[c] = rv;
+        v[0] += rv * rv;
+    }
+    block_sum<1>(v);
+    double pv[1] = {v[0]};
+    if (publish_partials<1>(pv, partials, counter)) {
+        double s[1];
+        reduce_partials<1>(partials, gridDim.x, s);
+        if (threadIdx.x == 0) {
+            double err = sqrt(s[0]) / st->root_n;
+            st->err = err;
+            st->aux0 = s[0];
+            if (count_it) {
+                long long it = st->it + 1;
+                st->it = it;
+                if (!(err == err) || isinf(err)) {
+                    st->done = 1;
+                    st->status = DALM_NONFINITE;
+                } else if (!(it < st->max_it && err > st->tol)) {
+                    st->done = 1;
+                    st->status = err > st->tol ? DALM_NONCONVERGED : DALM_OK;
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_state_reset(SolveState* st, double tol, long long max_it, double root_n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->rho = st->rho_prev = st->alpha = st->omega = 1.0;
+        st->beta = 0;
+        st->err = 0;
+        st->tol = tol;
+        st->root_n = root_n;
+        st->it = 0;
+        st->max_it = max_it;
+        st->done = 0;
+        st->status = DALM_OK;
+    }
+}
+
+// -------------------------------------------------------------------- BiCGStab
+// r = b - A x, r0 = r, rho = <r0, r> (src/eq.h:342-353)
+__global__ __launch_bounds__(BLOCK) void k_bi_init(Dims d, PCoef A, const double* __restrict__ x,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ r, double* __restrict__ r0,
+                                                   double* partials, unsigned* counter,
+                                                   SolveState* st) {
+    TileRange tr = block_tiles(d);
+    double v[1] = {0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double ac = diag_at(A, cc.i, cc.j, cc.k);
+        double rv = b[c] - apply7(d, A, x, c, cc.i, cc.j, cc.k, ac);
+        r[c] = rv;
+        r0[c] = rv;
+        v[0] += rv * rv;
+    }
+    block_sum<1>(v);
+    double pv[1] = {v[0]};
+    if (publish_partials<1>(pv, partials, counter)) {
+        double s[1];
+        reduce_partials<1>(partials, gridDim.x, s);
+        if (threadIdx.x == 0) {
+            double rho = s[0];
+            st->rho = rho;
+            if (fabs(rho) < (double)FLT_MIN) {
+                st->err = rho;
+                st->done = 1;
+                st->status = DALM_BREAKDOWN;
+            }
+        }
+    }
+}
+
+// p = r (first iteration) or r + beta (p - omega q); then the first
+// preconditioner sweep from zero: Jacobi gives z = p / a_c pointwise, SOR starts
+// from z = 0 (src/eq.h:355-368, 437-439).
+__global__ __launch_bounds__(BLOCK) void k_bi_p(Dims d, PCoef A, const double* __restrict__ r,
+                                                double* __restrict__ p, const double* __restrict__ q,
+                                                double* __restrict__ z, int pc_first,
+                                                const SolveState* st) {
+    if (st->done) return;
+    const bool first = st->it == 0;
+    const double beta = st->beta, om = st->omega;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double pv = first ? r[c] : r[c] + beta * (p[c] - om * q[c]);
+        p[c] = pv;
+        z[c] = pc_first ? pv / diag_at(A, cc.i, cc.j, cc.k) : 0.0;
+    }
+}
+
+// One Jacobi sweep z_out = z_in + (v - A z_in) / a_c (src/eq.h:6-39)
+__global__ __launch_bounds__(BLOCK) void k_jacobi(Dims d, PCoef A, const double* __restrict__ zin,
+                                                  double* __restrict__ zout,
+                                                  const double* __restrict__ v,
+                                                  const SolveState* st) {
+    if (finished(st)) return;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double ac = diag_at(A, cc.i, cc.j, cc.k);
+        double rel = (v[c] - apply7(d, A, zin, c, cc.i, cc.j, cc.k, ac)) / ac;
+        zout[c] = zin[c] + rel;
+    }
+}
+
+// y = A z and the partial sums of <a1, y> (and <y, a2> if a2): with which = 0 the
+// last block sets alpha = rho / <r0, q>; with which = 1 it sets
+// omega = <t, s> / <t, t> (src/eq.h:369-383).
+template <int WHICH>
+__global__ __launch_bounds__(BLOCK) void k_bi_spmv(Dims d, PCoef A, const double* __restrict__ z,
+                                                   double* __restrict__ y,
+                                                   const double* __restrict__ a1, double* partials,
+                                                   unsigned* counter, SolveState* st) {
+    if (st->done) return;
+    TileRange tr = block_tiles(d);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double ac = diag_at(A, cc.i, cc.j, cc.k);
+        double yv = apply7(d, A, z, c, cc.i, cc.j, cc.k, ac);
+        y[c] = yv;
+        if (WHICH == 0) {
+            v[0] += a1[c] * yv;   // <r0, q>
+        } else {
+            v[0] += yv * a1[c];   // <t, s>
+            v[1] += yv * yv;      // <t, t>
+        }
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    if (publish_partials<2>(pv, partials, counter)) {
+        double s[2];
+        reduce_partials<2>(partials, gridDim.x, s);
+        if (threadIdx.x == 0) {
+            if (WHICH == 0) st->alpha = st->rho / s[0];
+            else st->omega = s[0] / s[1];
+        }
+    }
+}
+
+// s = r - alpha q, then the first preconditioner sweep for ss (src/eq.h:373-380)
+__global__ __launch_bounds__(BLOCK) void k_bi_s(Dims d, PCoef A, const double* __restrict__ r,
+                                                const double* __restrict__ q, double* __restrict__ s,
+                                                double* __restrict__ z, int pc_first,
+                                                const SolveState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double sv = r[c] - alpha * q[c];
+        s[c] = sv;
+        z[c] = pc_first ? sv / diag_at(A, cc.i, cc.j, cc.k) : 0.0;
+    }
+}
+
+// x += alpha pp + omega ss; r = s - omega t; err = |r| / sqrt(N); next rho =
+// <r0, r>; stopping test and breakdown test of the next iteration
+// (src/eq.h:385-401, 349-353).
+__global__ __launch_bounds__(BLOCK) void k_bi_xr(Dims d, double* __restrict__ x,
+                                                 const double* __restrict__ pp,
+                                                 const double* __restrict__ ss,
+                                                 const double* __restrict__ s,
+                                                 const double* __restrict__ t_,
+                                                 double* __restrict__ r,
+                                                 const double* __restrict__ r0, double* partials,
+                                                 unsigned* counter, SolveState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha, om = st->omega;
+    TileRange tr = block_tiles(d);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        x[c] = x[c] + alpha * pp[c] + om * ss[c];
+        double rv = s[c] - om * t_[c];
+        r[c] = rv;
+        v[0] += rv * rv;
+        v[1] += r0[c] * rv;
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    if (publish_partials<2>(pv, partials, counter)) {
+        double sm[2];
+        reduce_partials<2>(partials, gridDim.x, sm);
+        if (threadIdx.x == 0) {
+            double err = sqrt(sm[0]) / st->root_n;
+            long long it = st->it + 1;
+            st->it = it;
+            st->err = err;
+            if (!(err == err) || isinf(err)) {
+                st->done = 1;
+                st->status = DALM_NONFINITE;
+            } else if (it < st->max_it && err > st->tol) {
+                double rho_new = sm[1];
+                if (fabs(rho_new) < (double)FLT_MIN) {
+                    st->err = rho_new;
+                    st->done = 1;
+                    st->status = DALM_BREAKDOWN;
+                } else {
+                    st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
+                    st->rho_prev = st->rho;
+                    st->rho = rho_new;
+                }
+            } else {
+                st->done = 1;
+                st->status = err > st->tol ? DALM_NONCONVERGED : DALM_OK;
+            }
+        }
+    }
+}
+
+__global__ void k_zero(long long n, double* __restrict__ a) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        a[e] = 0.0;
+}
+
+int read_state(dalm_ctx* ctx) {
+    HIP_TRY(hipMemcpyAsync(ctx->h_solve, ctx->solve, sizeof(SolveState), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DALM_OK;
+}
+
+// Iterations enqueued before the host looks at the device flag again.
+long long next_chunk(long long done_it, long long predicted, long long max_it) {
+    long long left = predicted - done_it;
+    long long c = left > 8 ? left : 8;
+    if (c > 256) c = 256;
+    if (done_it + c > max_it) c = max_it - done_it;
+    return c < 1 ? 1 : c;
+}
+
+}  // namespace
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color) {
+    hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x, b,
+                       omega, color, (const SolveState*)nullptr);
+    LAUNCH_CHECK();
+    return DALM_OK;
+}
+
+int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r) {
+    hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x,
+                       b, r, ctx->partials, ctx->counter, ctx->solve, 0);
+    LAUNCH_CHECK();
+    return DALM_OK;
+}
+
+__global__ void k_Ax_plain(Dims d, PCoef A, const double* __restrict__ z, double* __restrict__ y) {
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        y[c] = apply7(d, A, z, c, cc.i, cc.j, cc.k, diag_at(A, cc.i, cc.j, cc.k));
+    }
+}
+
+int launch_Ax(dalm_ctx* ctx, const double* x, double* y) {
+    hipLaunchKernelGGL(k_Ax_plain, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(),
+                       x, y);
+    LAUNCH_CHECK();
+    return DALM_OK;
+}
+
+int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long* it, double* err) {
+    const double root_n = sqrt((double)ctx->N);
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it, root_n);
+    LAUNCH_CHECK();
+    PCoef A = ctx->coef();
+    long long launched = 0;
+    for (;;) {
+        long long chunk = next_chunk(launched, ctx->prev_it[0], max_it);
+        for (long long c = 0; c < chunk; ++c) {
+            hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, ctx->p,
+                               ctx->b, omega, 0, ctx->solve);
+            hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, ctx->p,
+                               ctx->b, omega, 1, ctx->solve);
+            hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A,
+                               ctx->p, ctx->b, ctx->r, ctx->partials, ctx->counter, ctx->solve, 1);
+        }
+        LAUNCH_CHECK();
+        launched += chunk;
+        int rc = read_state(ctx);
+        if (rc) return rc;
+        if (ctx->h_solve->done || launched >= max_it) break;
+    }
+    *it = ctx->h_solve->it;
+    *err = ctx->h_solve->err;
+    ctx->prev_it[0] = *it;
+    return ctx->h_solve->status;
+}
+
+int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
+                 long long* it, double* err) {
+    const double root_n = sqrt((double)ctx->N);
+    PCoef A = ctx->coef();
+    const Dims& d = ctx->d;
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it, root_n);
+    hipLaunchKernelGGL(k_bi_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b,
+                       ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+    LAUNCH_CHECK();
+    const int jac = pc_kind == DALM_PC_JACOBI;
+    const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
+    // preconditioner application for one vector; returns the buffer holding z
+    auto precondition = [&](double* z0, double* spare, const double* v) -> double* {
+        double* cur = z0;
+        if (jac) {
+            double* other = spare;
+            for (int m = 1; m < pc_it; ++m) {
+                hipLaunchKernelGGL(k_jacobi, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
+                                   other, v, (const SolveState*)ctx->solve);
+                double* tsw = cur;
+                cur = other;
+                other = tsw;
+            }
+        } else {
+            for (int m = 0; m < pc_it; ++m) {
+                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
+                                   pc_omega, 0, (const SolveState*)ctx->solve);
+                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
+                                   pc_omega, 1, (const SolveState*)ctx->solve);
+            }
+        }
+        return cur;
+    };
+    long long launched = 0;
+    int rc = read_state(ctx);
+    if (rc) return rc;
+    while (!ctx->h_solve->done && launched < max_it) {
+        long long chunk = next_chunk(launched, ctx->prev_it[1], max_it);
+        for (long long c = 0; c < chunk; ++c) {
+            hipLaunchKernelGGL(k_bi_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
+                               ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
+            double* ppz = precondition(ctx->pp, ctx->tmp, ctx->kp);
+            hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
+                               ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+            hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
+                               ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
+            double* ssz = precondition(ctx->ss, ctx->tmp2, ctx->s);
+            hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
+                               ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
+            hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ppz, ssz,
+                               ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+        }
+        LAUNCH_CHECK();
+        launched += chunk;
+        rc = read_state(ctx);
+        if (rc) return rc;
+    }
+    *it = ctx->h_solve->it;
+    *err = ctx->h_solve->err;
+    ctx->prev_it[1] = *it;
+    return ctx->h_solve->status;
+}
+
+int zero_device(dalm_ctx* ctx, double* a, long long n) {
+    hipLaunchKernelGGL(k_zero, dim3(n > 256 * 4096 ? 4096 : (int)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, n, a);
+    LAUNCH_CHECK();
+    return DALM_OK;
+}

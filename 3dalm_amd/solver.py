@@ -1,0 +1,66 @@
+"""Host-side mirror of the reference's Solver (src/solver.h:519-1310) on the HIP path.
+
+``Solver(case)`` does what ``Solver::initialize`` does after parsing the case
+(allocation, operator, initial fields, ALM set-up, start-up pass) and
+``main_loop_once`` advances one time step on the GPU; ``can_continue`` follows
+``step <= max_step`` (src/solver.h:1308-1310).  Every numerical call goes through
+the C ABI (include/dalm_hip.h); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import hip
+from .cases import Case, turbine_records
+
+
+def params_for(case: Case) -> hip.Params:
+    p = hip.Params()
+    p.Uin[:] = case.uin
+    p.Re, p.Cs, p.dt = case.Re, case.Cs, case.dt
+    p.method = hip.BICGSTAB if case.method == "BiCG" else hip.SOR
+    p.pc_kind = hip.PC_SOR if case.pc == "SOR" else hip.PC_JACOBI
+    p.omega, p.pc_omega, p.pc_it = case.omega, case.pc_omega, case.pc_it
+    p.max_it, p.tol = case.max_it, case.tol
+    p.retracted_floor, p.floor_start_x = 0, 0.0
+    return p
+
+
+class Solver:
+    def __init__(self, case: Case, precision: int = hip.FP64, U0: np.ndarray | None = None):
+        self.case = case
+        self.ctx = hip.Context(case.mesh, precision)
+        L = case.mesh.L
+        U = case.U0() if U0 is None else U0
+        Uold = np.empty((L, 3))
+        Uold[:] = case.uin
+        self.ctx.upload("U", U)
+        self.ctx.upload("Uold", Uold)
+        self.ctx.set_params(params_for(case))
+        self.ap = None
+        if case.turbine is not None and case.sites:
+            nwt = len(case.sites)
+            ap, cd, cl, atk = hip.build_ap_props(case.turbine, nwt, case.ap_per_blade)
+            per_wt = case.turbine.blades * case.ap_per_blade
+            for a in range(len(ap)):
+                ap[a, 13] = case.sites[a // per_wt].pitch
+            wt = turbine_records(case.turbine, case.sites)
+            self.ctx.alm_setup(wt, ap, case.turbine.blades, case.ap_per_blade, cd, cl, atk, case.width)
+            self.ap = ap
+        self.stats = self.ctx.initialize()
+        self.max_step = case.steps
+
+    def main_loop_once(self):
+        self.stats = self.ctx.step()
+        return self.stats
+
+    step = main_loop_once
+
+    def can_continue(self):
+        return self.stats["step"] < self.max_step
+
+    def field(self, name):
+        return self.ctx.download(name)
+
+    def close(self):
+        self.ctx.close()

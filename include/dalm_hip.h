@@ -1,0 +1,209 @@
+/*
+ * dalm_hip.h -- C ABI of the MI355X-native 3dalm per-time-step hot path.
+ *
+ * One opaque context holds one padded grid (reference layout: n[0] x n[1] x n[2]
+ * cells including gc ghost layers per side, z fastest) resident in HBM.  Host
+ * arrays crossing this boundary use the reference's host formats: FP64 values,
+ * int64 sizes, vector fields as AoS double[L][3] (src/type.h:5-6,
+ * src/solver.h:286-317).  Every entry point returns a dalm_status; no exception
+ * crosses the boundary.  A context is not thread-safe; calls are stream-ordered on
+ * the context's own HIP stream and return after the result is complete unless
+ * stated otherwise.
+ *
+ * Each entry point names the reference routine it replaces (path:line in the
+ * reference tree).  The reference's functions are static and void; their solver
+ * status came back through it/err out-parameters (src/eq.h:291-296), which is
+ * kept here, plus explicit status codes for non-convergence, breakdown and
+ * non-finite residuals.
+ */
+#ifndef DALM_HIP_H
+#define DALM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dalm_ctx dalm_ctx;
+
+enum dalm_status {
+    DALM_OK = 0,
+    DALM_NONCONVERGED = 1, /* Poisson solve stopped at max_it with err > tol */
+    DALM_BREAKDOWN = 2,    /* BiCGStab |rho| < FLT_MIN (src/eq.h:350-353) */
+    DALM_NONFINITE = 3,    /* residual norm is NaN or Inf */
+    DALM_HIP_ERROR = 4,    /* a HIP runtime call failed; see dalm_last_error */
+    DALM_BAD_ARG = 5
+};
+
+enum dalm_precision {
+    DALM_FP64 = 0,  /* drop-in: FP64 everywhere, reference arithmetic order */
+    DALM_MIXED = 1  /* FP32 stencils and inner Poisson sweeps, FP64 reductions,
+                       FP64 face fluxes and pressure, FP64 iterative refinement */
+};
+
+enum dalm_field {
+    DALM_F_U = 0,     /* [L][3] cell velocity          (Cfd::U)     */
+    DALM_F_UOLD = 1,  /* [L][3] previous velocity      (Cfd::Uold)  */
+    DALM_F_JU = 2,    /* [L][3] face fluxes e/n/t      (Cfd::JU)    */
+    DALM_F_P = 3,     /* [L]    pressure               (Cfd::p)     */
+    DALM_F_NUT = 4,   /* [L]    eddy viscosity         (Cfd::nut)   */
+    DALM_F_DIV = 5,   /* [L]    divergence             (Cfd::div)   */
+    DALM_F_F = 6,     /* [L][3] ALM body force         (Cfd::f)     */
+    DALM_F_SOLID = 7, /* [L]    solid fraction         (Cfd::solid) */
+    DALM_F_B = 8      /* [L]    Poisson right-hand side (Eq::b)     */
+};
+
+enum dalm_method { DALM_SOR = 0, DALM_BICGSTAB = 1 };
+enum dalm_pc { DALM_PC_JACOBI = 0, DALM_PC_SOR = 1 };
+
+/* Turbine / actuator-point records (flat FP64 rows).
+ * turbine: [0..2] base, [3..5] rotation centre, [6] angle type (0 roll, 1 pitch,
+ *          2 yaw, 3 none), [7] angle, [8] angular rate, [9..12] angle formula
+ *          A,B,C,D (angle = A sin(B t + C) + D), [13] rotor speed, [14] torque,
+ *          [15] thrust                      (WindTurbine, src/alm.h:23-33)
+ * point:   [0..2] position, [3] r, [4] chord, [5] twist (deg), [6] theta, [7] dr,
+ *          [8..10] force, [11] torque, [12] thrust, [13] collective pitch (deg,
+ *          0 in the reference), [14..15] unused (ActuatorPoint, src/alm.h:10-21) */
+#define DALM_WT_STRIDE 16
+#define DALM_AP_STRIDE 16
+#define DALM_MAX_TURBINES 64
+
+/* Per-run parameters (setup.json keys, src/solver.h:589-652). */
+typedef struct {
+    double Uin[3];         /* inflow.value */
+    double Re, Cs;         /* cfd.Re, cfd.Cs */
+    double dt;             /* runtime.dt */
+    int64_t method;        /* eq.method: DALM_SOR | DALM_BICGSTAB */
+    int64_t pc_kind;       /* eq.preconditioner.method */
+    double omega;          /* eq.relaxation rate (SOR) */
+    double pc_omega;       /* eq.preconditioner.relaxation rate */
+    int64_t pc_it;         /* eq.preconditioner.max iteration */
+    int64_t max_it;        /* eq.max iteration */
+    double tol;            /* eq.tolerance */
+    int64_t retracted_floor;
+    double floor_start_x;  /* "floor start" */
+} dalm_params;
+
+/* One line of the reference's per-step log (src/solver.h:1188-1190) plus status
+ * and per-turbine torque / thrust / power (power = rotor speed * torque). */
+typedef struct {
+    int64_t step;
+    int64_t it;
+    double time;
+    double err;
+    double avg_div;
+    double max_cfl;
+    int64_t status;
+    int64_t wt_count;
+    double torque[DALM_MAX_TURBINES];
+    double thrust[DALM_MAX_TURBINES];
+    double power[DALM_MAX_TURBINES];
+} dalm_step_stats;
+
+/* ---- context ---------------------------------------------------------------- */
+
+/* Allocate a context for a padded grid with cell centres x[size[0]], y[size[1]],
+ * z[size[2]] (ghosts included).  Widths follow the reference's mesh rule.
+ * Replaces Mesh/Cfd/Eq allocation and `acc enter data` (src/solver.h:207-404,
+ * src/io.h:98-164). */
+int dalm_create(dalm_ctx **out, const int64_t size[3], int64_t gc, const double *x,
+                const double *y, const double *z, int64_t precision);
+int dalm_destroy(dalm_ctx *ctx);
+/* Human-readable text of the last failure on this context ("" if none). */
+const char *dalm_last_error(const dalm_ctx *ctx);
+/* Device selection for subsequent dalm_create calls (src/solver.h:560-562). */
+int dalm_set_device(int64_t device);
+
+/* Mesh width rule, dx[i] = (x[i+1]-x[i-1])/2 with one-sided 2nd-order ends
+ * (build_mesh_from_json, src/io.h:115-125).  Host only. */
+int dalm_mesh_widths(const double *c, int64_t n, double *d);
+
+/* Host <-> device copies of a whole field in the reference's AoS layout
+ * (`acc update host/device`, src/io.h:89-95).  In DALM_MIXED, FP32 fields are
+ * converted on the way. */
+int dalm_upload(dalm_ctx *ctx, int64_t field, const double *host);
+int dalm_download(dalm_ctx *ctx, int64_t field, double *host);
+
+int dalm_set_params(dalm_ctx *ctx, const dalm_params *prm);
+
+/* ---- per-kernel entry points (FP64 context) --------------------------------- */
+
+/* cpy_array(Uold, U) over the padded length (src/util.h:36-57) */
+int dalm_copy_u_to_uold(dalm_ctx *ctx);
+/* calc_intermediate_U (src/cfd.h:244-392): MUSCL convection + diffusion + force */
+int dalm_predictor(dalm_ctx *ctx, double Re, double dt);
+/* interpolate_JU (src/cfd.h:422-567) */
+int dalm_interpolate_JU(dalm_ctx *ctx);
+/* apply_JUbc (src/bc.h:235-302) */
+int dalm_apply_JUbc(dalm_ctx *ctx, const double Uin[3]);
+/* calc_poisson_rhs (src/cfd.h:569-598) */
+int dalm_poisson_rhs(dalm_ctx *ctx, double dt, double scale);
+/* build_A (src/eq.h:701-798): returns max|a_c|; the operator is kept matrix-free */
+int dalm_build_A(dalm_ctx *ctx, double *max_diag);
+/* run_sor (src/eq.h:291-306) on x = p, b = rhs */
+int dalm_run_sor(dalm_ctx *ctx, double omega, int64_t max_it, double tol, int64_t *it,
+                 double *err);
+/* run_pbicgstab, Jacobi PC (src/eq.h:333-402) or SOR PC (src/eq.h:404-473) */
+int dalm_run_bicgstab(dalm_ctx *ctx, int64_t pc_kind, double pc_omega, int64_t pc_it,
+                      int64_t max_it, double tol, int64_t *it, double *err);
+/* calc_residual r = b - A p (src/mv.h:90-182), result in field B's twin: r is
+ * written to `out` (host, [L]) when out != NULL */
+int dalm_residual(dalm_ctx *ctx, double *out);
+/* apply_pbc (src/bc.h:304-411) */
+int dalm_apply_pbc(dalm_ctx *ctx);
+/* project_p (src/cfd.h:620-783) */
+int dalm_project_p(dalm_ctx *ctx, double dt);
+/* set_solid_U (src/cfd.h:1143-1160) */
+int dalm_set_solid_U(dalm_ctx *ctx);
+/* apply_Ubc (src/bc.h:5-233) */
+int dalm_apply_Ubc(dalm_ctx *ctx, const double Uin[3], double dt, int64_t retracted_floor,
+                   double floor_start_x);
+/* calc_nut (src/cfd.h:827-946) */
+int dalm_calc_nut(dalm_ctx *ctx, double Cs);
+/* calc_divergence (src/cfd.h:1079-1107) */
+int dalm_calc_divergence(dalm_ctx *ctx);
+/* calc_l2_norm (src/mv.h:33-55) of a scalar field */
+int dalm_l2_norm(dalm_ctx *ctx, int64_t field, double *out);
+/* calc_max_cfl (src/cfd.h:1109-1141) */
+int dalm_max_cfl(dalm_ctx *ctx, double dt, double *out);
+
+/* ---- actuator-line model ------------------------------------------------------ */
+
+/* build_ap_props (src/alm.h:35-142), host only: AP records (r, dr, chord, twist)
+ * and per-AP Cd/Cl rows interpolated between airfoil stations by absolute r.
+ * af_cd/af_cl: [af_count][atk_count]; ap: [n][DALM_AP_STRIDE]; cd/cl: [n][atk_count]
+ * with n = wt_count*blades*ap_per_blade. */
+int dalm_build_ap_props(const double *af_r, const double *af_chord, const double *af_twist,
+                        const double *af_cd, const double *af_cl, int64_t af_count,
+                        int64_t atk_count, double R, double hub_R, int64_t blades,
+                        int64_t wt_count, int64_t ap_per_blade, double *ap, double *cd,
+                        double *cl);
+/* Upload turbines, points and tables (Alm::intialize + acc copyin,
+ * src/solver.h:475-493). */
+int dalm_alm_setup(dalm_ctx *ctx, const double *wt, int64_t wt_count, const double *ap,
+                   int64_t ap_count, int64_t blade_per_wt, int64_t ap_per_blade,
+                   const double *cd, const double *cl, const double *atk, int64_t atk_count,
+                   double width);
+/* run_alm (src/alm.h:450-479): AP positions at time t, AP forces from U, Gaussian
+ * projection into f.  torque/thrust per turbine (may be NULL). */
+int dalm_run_alm(dalm_ctx *ctx, double t, double *torque, double *thrust);
+/* Current AP / turbine records (`acc update host(ap_lst)`, src/alm.h:380). */
+int dalm_alm_download(dalm_ctx *ctx, double *ap, double *wt);
+
+/* ---- whole step ----------------------------------------------------------------- */
+
+/* Start-up pass of Solver::initialize after the fields are filled
+ * (src/solver.h:727-833): build_A, set_solid_U, apply_Ubc, interpolate_JU,
+ * apply_JUbc, calc_nut, calc_divergence, run_alm(t = 0), avg_div, max_cfl. */
+int dalm_initialize(dalm_ctx *ctx, dalm_step_stats *stats);
+/* One Solver::main_loop_once (src/solver.h:1034-1190): advances the step counter
+ * and integrates it (first call = step 1, t = dt). */
+int dalm_step(dalm_ctx *ctx, dalm_step_stats *stats);
+/* n steps back to back; stats of the last one. */
+int dalm_run(dalm_ctx *ctx, int64_t n, dalm_step_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -166,25 +166,8 @@ def ap_props_oracle(turbine, wt_count, ap_per_blade):
 
 
 def wt_records(turbine, sites):
-    """Turbine records, src/alm.h:144-206 restated (angle type from the first
-    non-zero/object entry of (roll, pitch, yaw); all-zero -> none)."""
-    import math
-    wt = np.zeros((len(sites), WT_STRIDE))
-    for w, s in enumerate(sites):
-        wt[w, 0:3] = s.base
-        wt[w, 3:6] = (-turbine.overhang, 0.0, turbine.tower)
-        wt[w, 6] = 3
-        for j, a in enumerate(s.angle):
-            if isinstance(a, dict) or (a != 0):
-                wt[w, 6] = j
-                if isinstance(a, dict):
-                    wt[w, 9:13] = (a["amp"] * math.pi / 180, 2 * math.pi / a["T"],
-                                   a["phase"] * math.pi / 180, a["offset"] * math.pi / 180)
-                else:
-                    wt[w, 9:13] = (0, 0, 0, a * math.pi / 180)
-                break
-        wt[w, 13] = s.rot_speed
-    return wt
+    from importlib import import_module
+    return import_module("3dalm_amd.cases").turbine_records(turbine, sites)
 
 
 # ----------------------------------------------------------------------------- solvers

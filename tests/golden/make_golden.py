@@ -1,0 +1,274 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own code.
+
+Run in a container where /root/reference exists and `make -C oracle ref` has built
+oracle/_ref/libref3dalm.so (the reference headers compiled from where they lie).
+The fixtures hold inputs and expected outputs only.
+
+    python tests/golden/make_golden.py [--full-c1]
+
+Fixtures:
+  kernels_tiny.npz  per-kernel inputs/outputs on an 6x5x5 stretched grid
+  alm_tiny.npz      AP positions/forces/projection for roll, pitch, yaw and a
+                    time-varying (amp, T, phase, offset) angle
+  steps_c1s.npz     10 whole steps of a 16x12x12 C1-like case (SOR)
+  steps_c2s.npz     5 whole steps of a 24x12x12 C2-like case (BiCGStab + ALM)
+  trace_c1.npz      (--full-c1) the 64x48x48 C1 case, 100 steps: per-step log,
+                    final field norms and mid-planes
+  wtprop.npz        the reference's test/wtprop.json fixture (expected output of
+                    build_ap_props(test/turbine.json, 2, 20)) with its inputs
+  mesh.npz          build_mesh_from_json on a generated mesh.json
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+cases = importlib.import_module("3dalm_amd.cases")
+po = importlib.import_module("pyoracle")
+from helpers import rand_field, interior_mask  # noqa: E402
+
+REF_TEST = "/root/reference/test"
+
+
+def tiny_mesh():
+    return cases.make_mesh(6, 5, 5, ((0.0, 2.0), (-1.0, 1.0), (0.0, 1.5)), 1.3, 1.3)
+
+
+def kernels_tiny(out):
+    R = po.Kernels("ref")
+    m = tiny_mesh()
+    g = po.make_grid(m)
+    d = {"x": m.x, "y": m.y, "z": m.z}
+    Uold = rand_field(m, 1, 3, 0.3, 1.0)
+    JU = rand_field(m, 2, 3, 0.05)
+    nut = np.abs(rand_field(m, 3, 1, 1e-3))
+    f = rand_field(m, 4, 3, 0.2)
+    U = rand_field(m, 5, 3)
+    d.update(Uold=Uold, JU_in=JU, nut_in=nut, f_in=f)
+    Up = U.copy()
+    R.predictor(g, Up, Uold, JU, nut, f, 1e4, 0.005)
+    d["predictor_U"] = Up
+    Uf = rand_field(m, 7, 3, 0.4, 1.0)
+    JUf = np.zeros((m.L, 3))
+    R.interpolate_JU(g, Uf, JUf)
+    d.update(flux_U=Uf, flux_JU=JUf.copy())
+    R.apply_JUbc(g, JUf, np.array([1.0, 0.0, 0.0]))
+    d["flux_JU_bc"] = JUf.copy()
+    b = np.zeros(m.L)
+    R.poisson_rhs(g, JUf, b, 0.005, 12.5)
+    d["rhs_b"] = b
+    A = np.zeros((m.L, 7))
+    d["max_diag"] = np.array([R.build_A(g, A)])
+    d["A"] = A
+    p0 = rand_field(m, 11, 1, 0.1)
+    bb = rand_field(m, 12, 1, 1.0) * interior_mask(m)
+    d.update(p0=p0, b0=bb)
+    r = np.zeros(m.L)
+    R.residual(g, A, p0, bb, r)
+    d["residual"] = r
+    y = np.zeros(m.L)
+    R.Ax(g, A, p0, y)
+    d["Ax"] = y
+    x = p0.copy()
+    R.sor_sweep(g, A, x, bb, 1.2, 0)
+    R.sor_sweep(g, A, x, bb, 1.2, 1)
+    d["sor_2sweeps"] = x
+    x2 = np.zeros(m.L)
+    R.jacobi_sweep(g, A, x2, p0, bb)
+    d["jacobi"] = x2
+    for kind, name in ((0, "jacobi"), (1, "sor")):
+        x = p0.copy()
+        work = np.zeros(9 * m.L)
+        it, err = po.C.c_int64(), po.C.c_double()
+        po.lib("ref")[0].ref_run_bicgstab(po.C.byref(g), po.ptr(A), po.ptr(x), po.ptr(bb), po.ptr(work),
+                                          kind, 1.1, 3, po.C.byref(it), 200, po.C.byref(err), 1e-12)
+        d[f"bicg_{name}_x"] = x
+        d[f"bicg_{name}_it_err"] = np.array([it.value, err.value])
+    x = p0.copy()
+    rr = np.zeros(m.L)
+    it, err = po.C.c_int64(), po.C.c_double()
+    po.lib("ref")[0].ref_run_sor(po.C.byref(g), po.ptr(A), po.ptr(x), po.ptr(bb), po.ptr(rr), 1.2,
+                                 po.C.byref(it), 30, po.C.byref(err), 1e-9)
+    d["sor_x"] = x
+    d["sor_it_err"] = np.array([it.value, err.value])
+    p = rand_field(m, 31, 1, 0.2)
+    d["pbc_in"] = p.copy()
+    R.apply_pbc(g, p)
+    d["pbc_out"] = p.copy()
+    Upj = rand_field(m, 32, 3, 0.3, 1.0)
+    Uo = rand_field(m, 33, 3, 0.3, 1.0)
+    Jpj = rand_field(m, 34, 3, 0.1)
+    d.update(proj_U_in=Upj.copy(), proj_JU_in=Jpj.copy(), ubc_Uold=Uo)
+    R.project(g, Upj, Jpj, p, 0.005)
+    d.update(proj_U=Upj.copy(), proj_JU=Jpj.copy())
+    for rf in (0, 1):
+        Ub = Upj.copy()
+        R.apply_Ubc(g, Ub, Uo, np.array([1.0, 0.0, 0.0]), 0.005, rf, 0.7)
+        d[f"ubc_U_{rf}"] = Ub
+    nut2 = np.zeros(m.L)
+    R.nut(g, Upj, nut2, 0.1)
+    d["nut"] = nut2
+    div = np.zeros(m.L)
+    R.divergence(g, Jpj, div)
+    d["div"] = div
+    d["max_cfl"] = np.array([R.max_cfl(g, Upj, 0.005)])
+    d["l2_div"] = np.array([R.l2_norm(g, div)])
+    d["dot"] = np.array([R.dot(g, p0, bb)])
+    np.savez_compressed(os.path.join(out, "kernels_tiny.npz"), **d)
+
+
+def alm_tiny(out):
+    R = po.Kernels("ref")
+    case = cases.case_c2(16, 10, 10, steps=1, ap_per_blade=5)
+    case.width = 0.6
+    m = case.mesh
+    g = po.make_grid(m)
+    U = rand_field(m, 51, 3, 0.1, 1.0)
+    U[:, 1:] *= 0.2
+    d = {"x": m.x, "y": m.y, "z": m.z, "U": U, "width": np.array([case.width])}
+    tj = case.turbine.to_json()
+    configs = {
+        "none": (0.0, 0.0, 0.0),
+        "roll": (9.0, 0.0, 0.0),
+        "pitch": (0.0, -6.0, 0.0),
+        "yaw": (0.0, 0.0, 12.0),
+        "yaw_osc": (0.0, 0.0, {"amp": 10.0, "T": 2.0, "phase": 30.0, "offset": 5.0}),
+        "pitch_osc": (0.0, {"amp": 4.0, "T": 1.5, "phase": 0.0, "offset": 2.0}, 0.0),
+        "roll_osc": ({"amp": 3.0, "T": 1.0, "phase": 10.0, "offset": 0.0}, 0.0, 0.0),
+    }
+    n = 3 * case.ap_per_blade
+    ap0 = np.zeros((n, 16))
+    cd = np.zeros((n, len(case.turbine.atk)))
+    cl = np.zeros_like(cd)
+    atk = np.zeros(len(case.turbine.atk))
+    na = po.C.c_int64()
+    po.lib("ref")[0].ref_build_ap_props(tj.encode(), 1, case.ap_per_blade, po.ptr(ap0), po.ptr(cd), po.ptr(cl),
+                                         po.ptr(atk), po.C.byref(na))
+    d.update(ap0=ap0, cd=cd, cl=cl, atk=atk)
+    for name, ang in configs.items():
+        site = cases.TurbineSite((3.0, 0.1, -0.05), 12.0, ang)
+        wt = np.zeros((1, 16))
+        po.lib("ref")[0].ref_build_wt_props(tj.encode(), cases.sites_json([site]).encode(), po.ptr(wt))
+        d[f"{name}_wt0"] = wt.copy()
+        d[f"{name}_sites"] = np.frombuffer(cases.sites_json([site]).encode(), dtype=np.uint8)
+        ap = ap0.copy()
+        t = 0.37
+        R.update_ap_position(wt, 1, ap, n, 3, case.ap_per_blade, t)
+        d[f"{name}_pos"] = ap[:, :8].copy()
+        R.calc_ap_force(g, U, wt, 1, ap, n, 3, case.ap_per_blade, cd, cl, atk, len(atk))
+        d[f"{name}_ap"] = ap.copy()
+        d[f"{name}_wt"] = wt.copy()
+        if name in ("yaw", "yaw_osc"):
+            f = np.zeros((m.L, 3))
+            R.distribute_ap_force(g, f, ap, n, case.width)
+            d[f"{name}_f"] = f
+    np.savez_compressed(os.path.join(out, "alm_tiny.npz"), **d)
+
+
+def run_steps(case, steps, fields):
+    r = po.RefSolver(case)
+    log = []
+    tq = []
+    for _ in range(steps):
+        s = r.step()
+        log.append([s["step"], s["it"], s["err"], s["avg_div"], s["max_cfl"], s["max_diag"]])
+        if "torque" in s:
+            tq.append(np.concatenate([s["torque"], s["thrust"]]))
+    d = {"log": np.array(log)}
+    if tq:
+        d["torque_thrust"] = np.array(tq)
+    for f in fields:
+        d[f] = r.field(f)
+    r.close()
+    return d
+
+
+def steps_small(out):
+    c1 = cases.case_c1(16, 12, 12, steps=10)
+    d = run_steps(c1, 10, ("U", "p"))
+    np.savez_compressed(os.path.join(out, "steps_c1s.npz"), **d)
+    c2 = cases.case_c2(24, 12, 12, steps=5, ap_per_blade=6)
+    d = run_steps(c2, 5, ("U", "p", "f"))
+    np.savez_compressed(os.path.join(out, "steps_c2s.npz"), **d)
+
+
+def full_c1(out, steps=100):
+    c = cases.case_c1(64, 48, 48, steps=steps)
+    r = po.RefSolver(c)
+    log = []
+    for n in range(steps):
+        s = r.step()
+        log.append([s["step"], s["it"], s["err"], s["avg_div"], s["max_cfl"], s["max_diag"]])
+        print(f"c1 step {n + 1} it {s['it']} err {s['err']:.6e}", flush=True)
+    d = {"log": np.array(log)}
+    nx, ny, nz = c.mesh.size
+    for name in ("U", "p", "JU", "nut", "div"):
+        a = r.field(name)
+        d[f"{name}_norm"] = np.array([np.linalg.norm(a[interior_mask(c.mesh)])])
+        a3 = a.reshape(nx, ny, nz, -1)
+        d[f"{name}_plane_i"] = a3[nx // 2].copy()
+    r.close()
+    np.savez_compressed(os.path.join(out, "trace_c1.npz"), **d)
+
+
+def wtprop(out):
+    """The reference's own fixture: test/wtprop.json = build_ap_props(test/turbine.json, 2, 20)."""
+    tin = json.load(open(os.path.join(REF_TEST, "turbine.json")))
+    exp = json.load(open(os.path.join(REF_TEST, "wtprop.json")))
+    af = tin["blade"]["airfoil"]
+    d = {
+        "R": np.array([tin["R"]]), "hub_R": np.array([tin["hub R"]]),
+        "blades": np.array([tin["number of blades"]]),
+        "af_r": np.array([a["r/R"] for a in af]), "af_chord": np.array([a["chord/R"] for a in af]),
+        "af_twist": np.array([a["twist[deg]"] for a in af]),
+        "af_cl": np.array([a["Cl"] for a in af]), "af_cd": np.array([a["Cd"] for a in af]),
+        "atk": np.array(tin["blade"]["attack angle"], dtype=np.float64),
+        "exp_r": np.array([a["r"] for a in exp["ap"]]), "exp_dr": np.array([a["dr"] for a in exp["ap"]]),
+        "exp_chord": np.array([a["chord"] for a in exp["ap"]]),
+        "exp_twist": np.array([a["twist"] for a in exp["ap"]]),
+        "exp_cd": np.array([a["cd"] for a in exp["ap"]]), "exp_cl": np.array([a["cl"] for a in exp["ap"]]),
+        "exp_attack": np.array(exp["attack"], dtype=np.float64),
+        "wt_count": np.array([2]), "ap_per_blade": np.array([20]),
+    }
+    np.savez_compressed(os.path.join(out, "wtprop.npz"), **d)
+
+
+def mesh_fixture(out):
+    m = cases.make_mesh(10, 8, 6, ((0.0, 3.0), (-1.0, 1.0), (0.0, 2.0)), 1.4, 1.1)
+    with tempfile.TemporaryDirectory() as t:
+        path = os.path.join(t, "mesh.json")
+        open(path, "w").write(m.to_json())
+        L = po.lib("ref")[0]
+        arrs = [np.zeros(n) for n in m.size] + [np.zeros(n) for n in m.size]
+        sz = (po.C.c_int64 * 3)()
+        gc = po.C.c_int64()
+        L.ref_build_mesh(path.encode(), *[po.ptr(a) for a in arrs], sz, po.C.byref(gc))
+    np.savez_compressed(os.path.join(out, "mesh.npz"), x=arrs[0], y=arrs[1], z=arrs[2], dx=arrs[3],
+                        dy=arrs[4], dz=arrs[5], size=np.array(list(sz)), gc=np.array([gc.value]))
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--full-c1", action="store_true")
+    ap.add_argument("--only", default="")
+    a = ap.parse_args()
+    todo = {"kernels": kernels_tiny, "alm": alm_tiny, "steps": steps_small, "wtprop": wtprop,
+            "mesh": mesh_fixture}
+    for k, fn in todo.items():
+        if a.only and k not in a.only.split(","):
+            continue
+        fn(HERE)
+        print("wrote", k)
+    if a.full_c1:
+        full_c1(HERE)
+        print("wrote trace_c1")

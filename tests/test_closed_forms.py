@@ -1,0 +1,208 @@
+"""Closed-form and invariant checks that pin the oracle independently of the
+reference build (each would catch a dropped term, a sign or an index slip)."""
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import face_mask, interior_mask, small_mesh
+
+cases = importlib.import_module("3dalm_amd.cases")
+po = importlib.import_module("pyoracle")
+
+
+@pytest.fixture(scope="module")
+def K():
+    return po.Kernels("oracle")
+
+
+def grid3(m):
+    nx, ny, nz = m.size
+    X, Y, Z = np.meshgrid(m.x, m.y, m.z, indexing="ij")
+    return X.reshape(-1), Y.reshape(-1), Z.reshape(-1)
+
+
+def uniform_mesh(n=8):
+    return cases.make_mesh(n, n, n, ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)))
+
+
+def test_predictor_constant_and_linear(K):
+    m = uniform_mesh()
+    g = po.make_grid(m)
+    mi = interior_mask(m)
+    # constant field, zero force: U* = U exactly (convection and diffusion vanish)
+    Uold = np.full((m.L, 3), 0.7)
+    JU = np.full((m.L, 3), 0.3)
+    U = np.zeros((m.L, 3))
+    K.predictor(g, U, Uold, JU, np.zeros(m.L), np.zeros((m.L, 3)), 1e3, 0.01)
+    assert np.allclose(U[mi], 0.7, atol=1e-15, rtol=0)
+    # linear field along x with constant positive face flux J: the limiter returns
+    # the slope, face states are exact, convection = J*slope/(dy*dz)... per volume:
+    # (F_e - F_w)/V with F = J*f_face -> J*slope*dx/V
+    X, Y, Z = grid3(m)
+    s = 2.5
+    Uold = np.stack([s * X, 0 * X, 0 * X], axis=1)
+    J = 0.2
+    JU = np.zeros((m.L, 3))
+    JU[:, 0] = J
+    U = np.zeros((m.L, 3))
+    K.predictor(g, U, Uold, JU, np.zeros(m.L), np.zeros((m.L, 3)), 1e300, 1.0)
+    h = m.dx[3]
+    V = m.dx[3] * m.dy[3] * m.dz[3]
+    conv = J * s * h / V
+    assert np.allclose((Uold[:, 0] - U[:, 0])[mi], conv, rtol=1e-12)
+
+
+def test_diffusion_of_quadratic(K):
+    # f = x^2 on a stretched x: discrete diffusion = 2*nu exactly because
+    # dx = (x[i+1]-x[i-1])/2 (src/io.h:115-117)
+    m = cases.make_mesh(10, 6, 6, ((0.0, 2.0), (0.0, 1.0), (0.0, 1.0)))
+    m2 = cases.Mesh(m.x + 0.08 * m.x * m.x, m.y, m.z)
+    g = po.make_grid(m2)
+    X, _, _ = grid3(m2)
+    Uold = np.stack([X * X, 0 * X, 0 * X], axis=1)
+    U = np.zeros((m2.L, 3))
+    nu = 1 / 50.0
+    K.predictor(g, U, Uold, np.zeros((m2.L, 3)), np.zeros(m2.L), np.zeros((m2.L, 3)), 50.0, 1.0)
+    mi = interior_mask(m2)
+    assert np.allclose((U[:, 0] - Uold[:, 0])[mi], 2 * nu, rtol=1e-9)
+
+
+def test_smagorinsky_shear(K):
+    m = small_mesh(8, 8, 8, 1.2)
+    g = po.make_grid(m)
+    _, Y, _ = grid3(m)
+    a = 0.8
+    U = np.stack([a * Y, 0 * Y, 0 * Y], axis=1)
+    nut = np.zeros(m.L)
+    K.nut(g, U, nut, 0.1)
+    mi = interior_mask(m)
+    nx, ny, nz = m.size
+    V = np.einsum("i,j,k->ijk", m.dx, m.dy, m.dz).reshape(-1)
+    assert np.allclose(nut[mi], (0.1 * np.cbrt(V[mi])) ** 2 * a, rtol=1e-13)
+
+
+def test_operator_symmetry_and_null(K):
+    m = small_mesh(7, 6, 5, 1.4)
+    g = po.make_grid(m)
+    A = np.zeros((m.L, 7))
+    md = K.build_A(g, A)
+    assert md > 0
+    mi = interior_mask(m)
+    idx = np.flatnonzero(mi)
+    pos = {c: n for n, c in enumerate(idx)}
+    nx, ny, nz = m.size
+    si, sj = ny * nz, nz
+    M = np.zeros((len(idx), len(idx)))
+    offs = [0, si, -si, sj, -sj, 1, -1]
+    for n, c in enumerate(idx):
+        for q, o in enumerate(offs):
+            if A[c, q] != 0:
+                M[n, pos[c + o]] += A[c, q]
+    V = np.einsum("i,j,k->ijk", m.dx, m.dy, m.dz).reshape(-1)[idx]
+    VM = V[:, None] * M
+    assert np.allclose(VM, VM.T, rtol=1e-12, atol=1e-14 * np.abs(VM).max())
+    # A*1 vanishes except on the x+ Dirichlet row
+    row = M.sum(axis=1)
+    ii = np.unravel_index(idx, (nx, ny, nz))[0]
+    assert np.allclose(row[ii != nx - 3], 0, atol=1e-13)
+    assert np.all(row[ii == nx - 3] < 0)
+    assert np.isclose(np.abs(np.diag(M)).max(), 1.0)
+
+
+def test_projection_identity(K):
+    """After rhs -> one SOR solve -> pbc -> project, div(JU) = dt*max_diag*(b - A p)."""
+    m = small_mesh(8, 7, 6, 1.3)
+    g = po.make_grid(m)
+    rng = np.random.default_rng(3)
+    U = 1.0 + 0.1 * rng.uniform(-1, 1, (m.L, 3))
+    JU = np.zeros((m.L, 3))
+    K.interpolate_JU(g, U, JU)
+    K.apply_JUbc(g, JU, np.array([1.0, 0.0, 0.0]))
+    A = np.zeros((m.L, 7))
+    md = K.build_A(g, A)
+    b = np.zeros(m.L)
+    dt = 0.01
+    K.poisson_rhs(g, JU, b, dt, md)
+    p = np.zeros(m.L)
+    r = np.zeros(m.L)
+    it, err = po.C.c_int64(), po.C.c_double()
+    po.lib("oracle")[0].or_run_sor(po.C.byref(g), po.ptr(A), po.ptr(p), po.ptr(b), po.ptr(r), 1.5,
+                                   po.C.byref(it), 40, po.C.byref(err), 0.0)
+    K.apply_pbc(g, p)
+    K.project(g, U, JU, p, dt)
+    K.apply_JUbc(g, JU, np.array([1.0, 0.0, 0.0]))
+    div = np.zeros(m.L)
+    K.divergence(g, JU, div)
+    K.residual(g, A, p, b, r)
+    mi = interior_mask(m)
+    assert np.allclose(div[mi], dt * md * r[mi], rtol=1e-7, atol=1e-12)
+
+
+def test_quadratic_extrapolation_exact(K):
+    m = small_mesh(6, 7, 8, 1.5)
+    g = po.make_grid(m)
+    X, Y, Z = grid3(m)
+    # a quadratic in y with u(y_wall) given by the slip rule is reproduced exactly
+    # in the y ghosts when the field is even about the wall (slip: wall value = u_1).
+    U = np.stack([1 + 0 * Y, 0 * Y, 0.3 + 0 * Y], axis=1)
+    Uo = U.copy()
+    U2 = U.copy()
+    K.apply_Ubc(g, U2, Uo, np.array([1.0, 0.0, 0.0]), 0.01, 0, 0.0)
+    nx, ny, nz = m.size
+    U3 = U2.reshape(nx, ny, nz, 3)
+    # constant u, w with slip walls: ghosts stay constant (interior i, k)
+    assert np.allclose(U3[2:-2, :2, 2:-2, 0], 1.0, rtol=1e-13)
+    assert np.allclose(U3[2:-2, :2, 2:-2, 2], 0.3, rtol=1e-13)
+    # z- no-slip: u ghost = quadratic through (z1, 1), (z0, 1), (wall, 0)
+    zg = m.z[1]
+    zw = m.z[2] - 0.5 * m.dz[2]
+    x0, x1, x2 = m.z[3], m.z[2], zw
+    lag = ((zg - x1) * (zg - x2)) / ((x0 - x1) * (x0 - x2)) + ((zg - x0) * (zg - x2)) / ((x1 - x0) * (x1 - x2))
+    assert np.allclose(U3[2:-2, 2:-2, 1, 0], lag, rtol=1e-12)
+    # pressure: x+ image is -p_i * (h_o / h_i)
+    p = np.ones(m.L)
+    K.apply_pbc(g, p)
+    P3 = p.reshape(nx, ny, nz)
+    assert np.allclose(P3[nx - 2, 2:-2, 2:-2], -(m.dx[nx - 2] / m.dx[nx - 3]), rtol=1e-14)
+
+
+def test_single_ap_force_closed_form(K):
+    """One AP in uniform axial flow, no yaw: force from phi, the polar and 1/2 C V^2 c dr."""
+    case = cases.case_c2(20, 10, 10, steps=1, ap_per_blade=1)
+    m = case.mesh
+    g = po.make_grid(m)
+    U = np.zeros((m.L, 3))
+    U[:, 0] = 1.0
+    ap, cd, cl, atk = po.ap_props_oracle(case.turbine, 1, 1)
+    wt = cases.turbine_records(case.turbine, case.sites)
+    n = len(ap)
+    K.update_ap_position(wt, 1, ap, n, 3, 1, 0.0)
+    K.calc_ap_force(g, U, wt, 1, ap, n, 3, 1, cd, cl, atk, len(atk))
+    P = ap[0]
+    r, om = P[3], wt[0, 13]
+    ux, ut = 1.0, r * om - 0.0
+    phi = np.arctan(ux / ut)
+    a = np.degrees(phi) - P[5]
+    c_l = np.interp(a, atk, cl[0])
+    c_d = np.interp(a, atk, cd[0])
+    V2 = ux * ux + ut * ut
+    Fl, Fd = 0.5 * c_l * V2 * P[4] * P[7], 0.5 * c_d * V2 * P[4] * P[7]
+    Fx = Fl * np.cos(phi) + Fd * np.sin(phi)
+    Ft = Fl * np.sin(phi) - Fd * np.cos(phi)
+    assert np.isclose(P[12], Fx, rtol=1e-12)
+    assert np.isclose(P[11], abs(Ft) * r, rtol=1e-12)
+    assert np.isclose(P[8], -Fx, rtol=1e-12)
+
+
+def test_gaussian_weights_integrate_to_one(K):
+    m = cases.make_mesh(40, 40, 40, ((0.0, 4.0), (-2.0, 2.0), (-2.0, 2.0)))
+    g = po.make_grid(m)
+    ap = np.zeros((1, 16))
+    ap[0, :3] = (2.0, 0.0, 0.0)
+    ap[0, 8:11] = (1.0, -2.0, 0.5)
+    f = np.zeros((m.L, 3))
+    K.distribute_ap_force(g, f, ap, 1, 0.35)
+    V = np.einsum("i,j,k->ijk", m.dx, m.dy, m.dz).reshape(-1)
+    tot = (f * V[:, None]).sum(axis=0)
+    assert np.allclose(tot, ap[0, 8:11], rtol=1e-6)
