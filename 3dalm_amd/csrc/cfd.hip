@@ -513,10 +513,12 @@ int grid_for(long long n) {
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
 int launch_predictor(dalm_ctx* ctx, double Re, double dt) {
+    probe_begin(ctx, DALM_K_PREDICTOR);
     hipLaunchKernelGGL(k_predictor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, mesh(ctx),
                        ctx->Uold[0], ctx->Uold[1], ctx->Uold[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
                        ctx->nut, ctx->f[0], ctx->f[1], ctx->f[2], ctx->U[0], ctx->U[1], ctx->U[2],
                        1 / Re, dt);
+    probe_end(ctx, DALM_K_PREDICTOR);
     LAUNCH_CHECK();
     return DALM_OK;
 }

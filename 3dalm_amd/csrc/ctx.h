@@ -79,6 +79,16 @@ struct AlmTables {
     int* box_dev = nullptr;   // [wt][6] i0,i1,j0,j1,k0,k1 (half-open) cells any AP can reach
 };
 
+// Event timing of every launch of one selected kernel on the context's stream
+// (dalm_probe_kernel); pairs are resolved and recycled at host sync points.
+struct KernelProbe {
+    int kernel = 0;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    double total_ms = 0;
+    long long launches = 0;
+};
+
 struct dalm_ctx {
     Dims d;
     long long L = 0, N = 0;
@@ -120,6 +130,7 @@ struct dalm_ctx {
     dalm_step_stats last{};
 
     AlmTables alm;
+    KernelProbe probe;
 
     std::string err;
     int hip_err = 0;
@@ -161,3 +172,6 @@ int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, in
 int alm_run(dalm_ctx* ctx, double t, bool sync_scalars);
 int sync_stream(dalm_ctx* ctx);
 int zero_device(dalm_ctx* ctx, double* a, long long n);
+void probe_begin(dalm_ctx* ctx, int kernel);
+void probe_end(dalm_ctx* ctx, int kernel);
+int probe_collect(dalm_ctx* ctx);

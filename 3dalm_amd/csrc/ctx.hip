@@ -28,6 +28,40 @@ int sync_stream(dalm_ctx* ctx) {
     return DALM_OK;
 }
 
+void probe_begin(dalm_ctx* ctx, int kernel) {
+    KernelProbe& P = ctx->probe;
+    if (P.kernel == 0 || P.kernel != kernel) return;
+    if (P.used + 2 > P.ev.size()) {
+        for (int q = 0; q < 64; ++q) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            P.ev.push_back(e);
+        }
+    }
+    hipEventRecord(P.ev[P.used], ctx->stream);
+}
+
+void probe_end(dalm_ctx* ctx, int kernel) {
+    KernelProbe& P = ctx->probe;
+    if (P.kernel == 0 || P.kernel != kernel || P.used + 2 > P.ev.size()) return;
+    hipEventRecord(P.ev[P.used + 1], ctx->stream);
+    P.used += 2;
+}
+
+int probe_collect(dalm_ctx* ctx) {
+    KernelProbe& P = ctx->probe;
+    if (P.used == 0) return DALM_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t q = 0; q + 1 < P.used; q += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, P.ev[q], P.ev[q + 1]));
+        P.total_ms += ms;
+        P.launches += 1;
+    }
+    P.used = 0;
+    return DALM_OK;
+}
+
 namespace {
 
 template <typename T>
@@ -256,6 +290,7 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
 int dalm_destroy(dalm_ctx* ctx) {
     if (!ctx) return DALM_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->probe.ev) hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
                       ctx->t, ctx->tmp, ctx->tmp2, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
@@ -558,10 +593,29 @@ int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
     TRY(launch_post(ctx, P.Cs, P.dt, true));
     TRY(alm_run(ctx, t, false));
     TRY(read_scalars(ctx));
+    TRY(probe_collect(ctx));
     dalm_step_stats st;
     fill_stats(ctx, &st, it, err, status);
     if (stats) *stats = st;
     return status == DALM_NONCONVERGED ? DALM_OK : status;
+}
+
+int dalm_probe_kernel(dalm_ctx* ctx, int64_t kernel) {
+    CHECK_CTX();
+    if (kernel < 0 || kernel > DALM_K_SUPDATE) return DALM_BAD_ARG;
+    TRY(probe_collect(ctx));
+    ctx->probe.kernel = (int)kernel;
+    ctx->probe.total_ms = 0;
+    ctx->probe.launches = 0;
+    return DALM_OK;
+}
+
+int dalm_probe_read(dalm_ctx* ctx, int64_t* launches, double* total_ms) {
+    CHECK_CTX();
+    TRY(probe_collect(ctx));
+    if (launches) *launches = ctx->probe.launches;
+    if (total_ms) *total_ms = ctx->probe.total_ms;
+    return DALM_OK;
 }
 
 int dalm_run(dalm_ctx* ctx, int64_t n, dalm_step_stats* stats) {

@@ -363,12 +363,16 @@ int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long
     for (;;) {
         long long chunk = next_chunk(launched, ctx->prev_it[0], max_it);
         for (long long c = 0; c < chunk; ++c) {
-            hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, ctx->p,
-                               ctx->b, omega, 0, ctx->solve);
-            hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, ctx->p,
-                               ctx->b, omega, 1, ctx->solve);
+            for (int color = 0; color < 2; ++color) {
+                probe_begin(ctx, DALM_K_SOR);
+                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, ctx->p,
+                                   ctx->b, omega, color, ctx->solve);
+                probe_end(ctx, DALM_K_SOR);
+            }
+            probe_begin(ctx, DALM_K_RESNORM);
             hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A,
                                ctx->p, ctx->b, ctx->r, ctx->partials, ctx->counter, ctx->solve, 1);
+            probe_end(ctx, DALM_K_RESNORM);
         }
         LAUNCH_CHECK();
         launched += chunk;
@@ -399,8 +403,10 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
         if (jac) {
             double* other = spare;
             for (int m = 1; m < pc_it; ++m) {
+                probe_begin(ctx, DALM_K_JACOBI);
                 hipLaunchKernelGGL(k_jacobi, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
                                    other, v, (const SolveState*)ctx->solve);
+                probe_end(ctx, DALM_K_JACOBI);
                 double* tsw = cur;
                 cur = other;
                 other = tsw;
@@ -421,18 +427,28 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
     while (!ctx->h_solve->done && launched < max_it) {
         long long chunk = next_chunk(launched, ctx->prev_it[1], max_it);
         for (long long c = 0; c < chunk; ++c) {
+            probe_begin(ctx, DALM_K_PUPDATE);
             hipLaunchKernelGGL(k_bi_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
                                ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
+            probe_end(ctx, DALM_K_PUPDATE);
             double* ppz = precondition(ctx->pp, ctx->tmp, ctx->kp);
+            probe_begin(ctx, DALM_K_SPMV);
             hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
                                ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+            probe_end(ctx, DALM_K_SPMV);
+            probe_begin(ctx, DALM_K_SUPDATE);
             hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
                                ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
+            probe_end(ctx, DALM_K_SUPDATE);
             double* ssz = precondition(ctx->ss, ctx->tmp2, ctx->s);
+            probe_begin(ctx, DALM_K_SPMV);
             hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
                                ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
+            probe_end(ctx, DALM_K_SPMV);
+            probe_begin(ctx, DALM_K_UPDATE);
             hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ppz, ssz,
                                ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+            probe_end(ctx, DALM_K_UPDATE);
         }
         LAUNCH_CHECK();
         launched += chunk;

@@ -1,0 +1,274 @@
+"""Benchmark of the 3dalm per-time-step hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed]
+                    [--grid 256,128,128] [--no-cpu] [--probe jacobi]
+
+Workload (one "step" = one Solver::main_loop_once): the single-turbine ALM case on
+256x128x128 cells (box 16x8x8, y/z tanh beta=1.5, rotor D=1 at x=3, 3 blades x 64
+actuator points, TSR 6, eps = 2 dx, 5% seeded noise, dt = 0.005), Poisson by
+BiCGStab with 3 Jacobi sweeps as preconditioner to 1e-8.  Synthetic inputs,
+resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0).  value = Mcell-steps/s = N_interior * K / t / 1e6
+of the precision form when it is built, else of the FP64 drop-in; both forms are
+reported under "forms".  roofline: the dominant kernel's algorithmic bytes per
+launch over its mean launch time, timed with HIP events on the library's stream.
+cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) when
+that library is present and loads, else the C restatement in oracle/ (one core),
+on a bounded sample of this workload (see "sample").
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-steps/s mixed vs FP64 at div<=1e-6; adjoint/forward step-cost ratio"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per interior cell for one launch of each probed kernel (FP64);
+# see DESIGN.md for the derivation (compulsory reads + writes, matrix-free A)
+PROBE_BYTES = {
+    "jacobi": 24,     # read z_in, v; write z_out
+    "spmv": 24,       # read z, a1 (r0 or s); write y
+    "update": 64,     # read x, pp, ss, s, t, r0; write x, r
+    "pupdate": 40,    # read r, p, q; write p, z
+    "supdate": 32,    # read r, q; write s, z
+    "predictor": 104,  # read Uold(3), JU(3), nut, f(3); write U(3)
+    "sor": 16,        # half-sweep: read x, b over the colour + neighbours ~ 2w, write half
+    "resnorm": 16,    # read x, b
+}
+PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
+             "supdate": 8}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(grid, steps):
+    cases = importlib.import_module("3dalm_amd.cases")
+    nx, ny, nz = grid
+    return cases.case_c2(nx, ny, nz, steps=steps)
+
+
+def run_form(case, precision, steps, warmup, probe):
+    hip = importlib.import_module("3dalm_amd.hip")
+    solver = importlib.import_module("3dalm_amd.solver")
+    import ctypes as C
+    log(f"[bench] form={'fp64' if precision == hip.FP64 else 'mixed'} grid={case.mesh.interior} "
+        f"warmup={warmup} steps={steps}")
+    s = solver.Solver(case, precision)
+    try:
+        for w in range(warmup):
+            st = s.step()
+            log(f"[bench] warmup step {st['step']} it={st['it']} err={st['err']:.3e} div={st['avg_div']:.3e}")
+        if probe:
+            s.ctx.check(s.ctx.L.dalm_probe_kernel(s.ctx.h, PROBE_IDS[probe]), "probe")
+        sync()
+        t0 = time.perf_counter()
+        its, divs, power = [], [], []
+        for k in range(steps):
+            st = s.step()
+            its.append(st["it"])
+            divs.append(st["avg_div"])
+            if "power" in st:
+                power.append(float(st["power"].sum()))
+        sync()
+        t1 = time.perf_counter()
+        for k in range(0, steps, max(1, steps // 5)):
+            pass
+        log(f"[bench] timed {steps} steps in {t1 - t0:.3f} s; it/step mean {np.mean(its):.1f}; "
+            f"last div {divs[-1]:.3e}")
+        res = {"seconds": t1 - t0, "it_mean": float(np.mean(its)), "it": its,
+               "avg_div_max": float(np.max(divs)), "power_last": power[-1] if power else None}
+        if probe:
+            n, ms = C.c_int64(), C.c_double()
+            s.ctx.check(s.ctx.L.dalm_probe_read(s.ctx.h, C.byref(n), C.byref(ms)), "probe_read")
+            res["probe"] = {"kernel": probe, "launches": n.value, "total_ms": ms.value}
+        res["U"] = None
+        return res, s
+    except Exception:
+        s.close()
+        raise
+
+
+_torch = None
+
+
+def sync():
+    global _torch
+    if _torch is None:
+        try:
+            import torch
+            _torch = torch if torch.cuda.is_available() else False
+        except Exception:
+            _torch = False
+    if _torch:
+        _torch.cuda.synchronize()
+
+
+class _StdoutToStderr:
+    """The reference prints its per-step log with printf; keep bench's stdout to
+    the one JSON line by pointing file descriptor 1 at stderr meanwhile."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def cpu_baseline(case, gpu_it_mean):
+    with _StdoutToStderr():
+        return _cpu_baseline(case, gpu_it_mean)
+
+
+def _cpu_baseline(case, gpu_it_mean):
+    """Reference (or restatement) on the host: one initialisation + one step of this
+    workload with the Poisson solve capped, then a separately timed capped solve;
+    a full step at the GPU run's mean iteration count is extrapolated from the fixed
+    and per-iteration costs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    po = importlib.import_module("pyoracle")
+    kind = "port"
+    try:
+        if po.ref_available():
+            po.lib("ref")
+            kind = "reference"
+    except OSError as e:
+        log(f"[bench] reference build does not load here ({e}); timing the restatement")
+        kind = "port"
+    cap = 4
+    case.max_it = cap
+    case.tol = 0.0
+    t0 = time.perf_counter()
+    S = po.RefSolver(case) if kind == "reference" else po.OracleSolver(case)
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    S.step()
+    t_step_cap = time.perf_counter() - t0
+    # per-iteration cost: a capped solve on the same data
+    K = po.Kernels("ref" if kind == "reference" else "oracle")
+    m = case.mesh
+    g = po.make_grid(m)
+    A = np.zeros((m.L, 7))
+    K.build_A(g, A)
+    x = np.zeros(m.L)
+    b = np.ascontiguousarray(S.field("b"))
+    work = np.zeros(9 * m.L)
+    it, err = po.C.c_int64(), po.C.c_double()
+    fn = po.lib(kind if kind == "oracle" else "ref")[0]
+    pref = "ref_" if kind == "reference" else "or_"
+    t0 = time.perf_counter()
+    getattr(fn, pref + "run_bicgstab")(po.C.byref(g), po.ptr(A), po.ptr(x), po.ptr(b), po.ptr(work), 0, 1.0,
+                                       case.pc_it, po.C.byref(it), cap, po.C.byref(err), 0.0)
+    t_solve = time.perf_counter() - t0
+    t_iter = t_solve / max(it.value, 1)
+    t_fixed = max(t_step_cap - cap * t_iter, 0.0)
+    t_step = t_fixed + gpu_it_mean * t_iter
+    if hasattr(S, "close"):
+        S.close()
+    return {"value": m.N / t_step / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": kind,
+            "sample": (f"{m.interior[0]}x{m.interior[1]}x{m.interior[2]}: init {t_init:.1f}s + 1 step with "
+                       f"the solve capped at {cap} it ({t_step_cap:.1f}s) + a {it.value}-it solve "
+                       f"({t_solve:.1f}s); step cost extrapolated to {gpu_it_mean:.1f} it/step "
+                       f"(fixed {t_fixed:.2f}s + {t_iter:.3f}s/it)"),
+            "seconds_per_step": t_step}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--grid", default="256,128,128")
+    ap.add_argument("--probe", default="auto")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the whole domain lives on one MI355X and this build has "
+              "no multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    hip = importlib.import_module("3dalm_amd.hip")
+    grid = tuple(int(v) for v in a.grid.split(","))
+    forms = [f for f in a.forms.split(",") if f]
+    have_mixed = getattr(hip, "MIXED_BUILT", False)
+    if "mixed" in forms and not have_mixed:
+        log("[bench] precision form not built in this library; measuring the FP64 drop-in only")
+        forms = [f for f in forms if f != "mixed"]
+    results = {}
+    N = None
+    for form in forms:
+        case = make_case(grid, a.steps)
+        N = case.mesh.N
+        probe = "jacobi" if a.probe == "auto" else (a.probe if a.probe != "none" else None)
+        prec = hip.FP64 if form == "fp64" else hip.MIXED
+        res, s = run_form(case, prec, a.steps, a.warmup, probe)
+        s.close()
+        res["mcell_steps_s"] = N * a.steps / res["seconds"] / 1e6
+        res["ms_per_step"] = 1e3 * res["seconds"] / a.steps
+        results[form] = res
+    main_form = "mixed" if "mixed" in results else "fp64"
+    R = results[main_form]
+    roof = None
+    if "probe" in R and R["probe"]["launches"]:
+        pr = R["probe"]
+        mean_s = pr["total_ms"] / pr["launches"] / 1e3
+        bytes_per_launch = PROBE_BYTES[pr["kernel"]] * N * (0.5 if main_form == "mixed" else 1.0)
+        ach = bytes_per_launch / mean_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": pr["kernel"],
+                "launches": pr["launches"], "mean_us": round(mean_s * 1e6, 2),
+                "bytes_per_launch": bytes_per_launch}
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(make_case(grid, 1), R["it_mean"])
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
+    forms_out = {f: {"mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
+                     "it_per_step": round(r["it_mean"], 2), "avg_div_max": r["avg_div_max"],
+                     "power_last": r["power_last"]} for f, r in results.items()}
+    line = {
+        "metric": METRIC,
+        "value": round(R["mcell_steps_s"], 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(R["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_form == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"single 3-blade turbine ALM, {grid[0]}x{grid[1]}x{grid[2]} cells, "
+                                f"BiCGStab(Jacobi x3) to 1e-8, form={main_form}"),
+                   "cells": N, "turbines": 1, "actuator_points": 192, "poisson": "BiCGStab+Jacobi3",
+                   "tol": 1e-8},
+        "forms": forms_out,
+        "mixed_vs_fp64": (round(results["mixed"]["mcell_steps_s"] / results["fp64"]["mcell_steps_s"], 3)
+                          if "mixed" in results and "fp64" in results else None),
+        "adjoint_forward_ratio": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

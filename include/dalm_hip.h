@@ -191,6 +191,26 @@ int dalm_run_alm(dalm_ctx *ctx, double t, double *torque, double *thrust);
 /* Current AP / turbine records (`acc update host(ap_lst)`, src/alm.h:380). */
 int dalm_alm_download(dalm_ctx *ctx, double *ap, double *wt);
 
+/* ---- instrumentation (no counterpart in the reference) ---------------------- */
+
+/* Kernels that can be timed individually. */
+enum dalm_kernel {
+    DALM_K_NONE = 0,
+    DALM_K_JACOBI = 1,    /* one Jacobi preconditioner sweep          */
+    DALM_K_SPMV = 2,      /* BiCGStab SpMV + dot products (q and t)   */
+    DALM_K_UPDATE = 3,    /* BiCGStab x/r update + norms              */
+    DALM_K_PREDICTOR = 4, /* MUSCL/diffusion predictor                */
+    DALM_K_SOR = 5,       /* one red-black SOR colour sweep           */
+    DALM_K_RESNORM = 6,   /* SOR residual + norm                      */
+    DALM_K_PUPDATE = 7,   /* BiCGStab p update + first PC sweep       */
+    DALM_K_SUPDATE = 8    /* BiCGStab s update + first PC sweep       */
+};
+/* Time every launch of `kernel` with HIP events on the context's stream from now
+ * on (DALM_K_NONE stops); resets the totals. */
+int dalm_probe_kernel(dalm_ctx *ctx, int64_t kernel);
+/* Launch count and summed device time (ms) of the probed kernel so far. */
+int dalm_probe_read(dalm_ctx *ctx, int64_t *launches, double *total_ms);
+
 /* ---- whole step ----------------------------------------------------------------- */
 
 /* Start-up pass of Solver::initialize after the fields are filled

@@ -65,3 +65,30 @@ def test_c2_small_bicgstab_sor_pc():
     c = cases.case_c2(24, 12, 12, steps=3, ap_per_blade=6)
     c.pc, c.pc_omega, c.pc_it = "SOR", 1.1, 2
     compare(c, 3)
+
+
+def test_c1_full_against_reference_trace():
+    """The 64x48x48 C1 case, 100 steps, against the reference's own per-step log
+    and final fields (tests/golden/trace_c1.npz)."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trace_c1.npz"))
+    case = cases.case_c1(64, 48, 48, steps=100)
+    print("[case c1 full 64x48x48 SOR 100 steps]", flush=True)
+    s = solver.Solver(case)
+    try:
+        for n in range(100):
+            st = s.step()
+            row = d["log"][n]
+            assert st["step"] == row[0] and st["it"] == row[1], (n, st["it"], row[1])
+            assert abs(st["err"] - row[2]) <= 1e-10 * row[2]
+            assert abs(st["avg_div"] - row[3]) <= 1e-8 * row[3]
+            assert abs(st["max_cfl"] - row[4]) <= 1e-12 * row[4]
+        m = interior_mask(case.mesh)
+        nx = case.mesh.size[0]
+        for name in ("U", "p", "JU", "nut", "div"):
+            a = s.field(name)
+            assert abs(np.linalg.norm(a[m]) - d[f"{name}_norm"][0]) <= TOL * d[f"{name}_norm"][0], name
+            plane = a.reshape(case.mesh.size + (-1,))[nx // 2]
+            assert rel_l2(plane, d[f"{name}_plane_i"]) <= TOL, name
+    finally:
+        s.close()
