@@ -273,3 +273,30 @@ def turbine_records(turbine: TurbineModel, sites: List[TurbineSite]) -> np.ndarr
                 break
         wt[w, 13] = s.rot_speed
     return wt
+
+
+def write_case_dir(case: Case, path: str, time_steps: Optional[int] = None) -> None:
+    """Write setup.json / mesh.json / windturbine.json for a Case in the reference's
+    schemas (src/solver.h:564-708, src/io.h:105-151, src/alm.h:39-63)."""
+    import os
+    os.makedirs(path, exist_ok=True)
+    steps = case.steps if time_steps is None else time_steps
+    eq = {"tolerance": case.tol, "max iteration": case.max_it,
+          "method": "BiCG" if case.method == "BiCG" else "SOR"}
+    if case.method == "BiCG":
+        eq["preconditioner"] = {"method": case.pc, "max iteration": case.pc_it,
+                                "relaxation rate": case.pc_omega}
+    else:
+        eq["relaxation rate"] = case.omega
+    setup = {"runtime": {"dt": case.dt, "time": case.dt * (steps + 0.5)},
+             "output": {"output start time": 1e9, "output interval time": 1e9, "time avg start time": 1e9},
+             "inflow": {"value": list(case.uin)}, "cfd": {"Re": case.Re, "Cs": case.Cs}, "eq": eq}
+    if case.turbine is not None and case.sites:
+        setup["windturbine array"] = [s.to_dict() for s in case.sites]
+        setup["alm"] = {"blade points": case.ap_per_blade, "projection width": case.width}
+        with open(os.path.join(path, "windturbine.json"), "w") as fh:
+            fh.write(case.turbine.to_json())
+    with open(os.path.join(path, "setup.json"), "w") as fh:
+        fh.write(json.dumps(setup, indent=1))
+    with open(os.path.join(path, "mesh.json"), "w") as fh:
+        fh.write(case.mesh.to_json())

@@ -103,9 +103,10 @@ struct M1 {
     const double *x, *y, *z;
 };
 
-// AP forces (src/alm.h:287-371)
-__global__ void k_ap_force(Dims d, M1 m, const double* __restrict__ u0, const double* __restrict__ u1,
-                           const double* __restrict__ u2, const double* __restrict__ wt,
+// AP forces (src/alm.h:287-371); TU is the storage type of the cell velocity
+template <typename TU>
+__global__ void k_ap_force(Dims d, M1 m, const TU* __restrict__ u0, const TU* __restrict__ u1,
+                           const TU* __restrict__ u2, const double* __restrict__ wt,
                            double* __restrict__ ap, int n, int blade_per_wt, int ap_per_blade,
                            const double* __restrict__ cdt, const double* __restrict__ clt,
                            const double* __restrict__ atk, int natk) {
@@ -122,12 +123,12 @@ __global__ void k_ap_force(Dims d, M1 m, const double* __restrict__ u0, const do
         double fx = (P[0] - m.x[i]) / (m.x[i + 1] - m.x[i]);
         double fy = (P[1] - m.y[j]) / (m.y[j + 1] - m.y[j]);
         double fz = (P[2] - m.z[k]) / (m.z[k + 1] - m.z[k]);
-        const double* U[3] = {u0, u1, u2};
+        const TU* U[3] = {u0, u1, u2};
         double Uap[3];
         for (int q = 0; q < 3; ++q) {
             double v[8];
             for (int o = 0; o < 8; ++o)
-                v[o] = U[q][d.at(i + (o & 1), j + ((o >> 1) & 1), k + ((o >> 2) & 1))];
+                v[o] = (double)U[q][d.at(i + (o & 1), j + ((o >> 1) & 1), k + ((o >> 2) & 1))];
             double v01 = v[0] * (1 - fx) + v[1] * fx, v23 = v[2] * (1 - fx) + v[3] * fx;
             double v45 = v[4] * (1 - fx) + v[5] * fx, v67 = v[6] * (1 - fx) + v[7] * fx;
             double lo = v01 * (1 - fy) + v23 * fy, hi = v45 * (1 - fy) + v67 * fy;
@@ -188,13 +189,14 @@ __global__ void k_wt_sum(const double* __restrict__ ap, int wt_count, int per_wt
 }
 
 // Gaussian projection (src/alm.h:416-448), truncated per turbine box and at
-// r > 8 eps.
+// r > 8 eps; TF is the storage type of the force field.
+template <typename TF>
 __global__ __launch_bounds__(BLOCK) void k_ap_project(Dims d, M1 m, const double* __restrict__ ap,
                                                       const int* __restrict__ box, int wt_count,
                                                       int per_wt, double norm, double w2,
-                                                      double cut2, double* __restrict__ f0,
-                                                      double* __restrict__ f1,
-                                                      double* __restrict__ f2) {
+                                                      double cut2, TF* __restrict__ f0,
+                                                      TF* __restrict__ f1,
+                                                      TF* __restrict__ f2) {
     TileRange tr = block_tiles(d);
     for (int t = tr.begin; t < tr.end; ++t) {
         Cell cc = tile_cell(d, t);
@@ -217,9 +219,9 @@ __global__ __launch_bounds__(BLOCK) void k_ap_project(Dims d, M1 m, const double
                 sz += wgt * P[10];
             }
         }
-        f0[c] = sx;
-        f1[c] = sy;
-        f2[c] = sz;
+        f0[c] = (TF)sx;
+        f1[c] = (TF)sy;
+        f2[c] = (TF)sz;
     }
 }
 
@@ -291,9 +293,14 @@ int alm_run(dalm_ctx* ctx, double t, bool sync_scalars) {
                        A.blade_per_wt, A.ap_per_blade, t);
     LAUNCH_CHECK();
     M1 m{ctx->x, ctx->y, ctx->z};
-    hipLaunchKernelGGL(k_ap_force, dim3(nb), dim3(64), 0, ctx->stream, d, m, ctx->U[0], ctx->U[1],
-                       ctx->U[2], A.wt, A.ap, A.ap_count, A.blade_per_wt, A.ap_per_blade, A.cd, A.cl,
-                       A.atk, A.atk_count);
+    if (ctx->precision == DALM_MIXED)
+        hipLaunchKernelGGL(k_ap_force<float>, dim3(nb), dim3(64), 0, ctx->stream, d, m, ctx->U32[0],
+                           ctx->U32[1], ctx->U32[2], A.wt, A.ap, A.ap_count, A.blade_per_wt, A.ap_per_blade,
+                           A.cd, A.cl, A.atk, A.atk_count);
+    else
+        hipLaunchKernelGGL(k_ap_force<double>, dim3(nb), dim3(64), 0, ctx->stream, d, m, ctx->U[0], ctx->U[1],
+                           ctx->U[2], A.wt, A.ap, A.ap_count, A.blade_per_wt, A.ap_per_blade, A.cd, A.cl,
+                           A.atk, A.atk_count);
     LAUNCH_CHECK();
     int per_wt = A.blade_per_wt * A.ap_per_blade;
     hipLaunchKernelGGL(k_wt_sum, dim3(1), dim3(64), 0, ctx->stream, A.ap, A.wt_count, per_wt, ctx->scal);
@@ -302,8 +309,12 @@ int alm_run(dalm_ctx* ctx, double t, bool sync_scalars) {
     double norm = 1 / (sp * sp * sp);
     double w2 = A.width * A.width;
     double cut2 = reach * reach;
-    hipLaunchKernelGGL(k_ap_project, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, m, A.ap,
-                       A.box_dev, A.wt_count, per_wt, norm, w2, cut2, ctx->f[0], ctx->f[1], ctx->f[2]);
+    if (ctx->precision == DALM_MIXED)
+        hipLaunchKernelGGL(k_ap_project<float>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, m, A.ap,
+                           A.box_dev, A.wt_count, per_wt, norm, w2, cut2, ctx->f32[0], ctx->f32[1], ctx->f32[2]);
+    else
+        hipLaunchKernelGGL(k_ap_project<double>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, m, A.ap,
+                           A.box_dev, A.wt_count, per_wt, norm, w2, cut2, ctx->f[0], ctx->f[1], ctx->f[2]);
     LAUNCH_CHECK();
     if (sync_scalars) {
         HIP_TRY(hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(StepScalars), hipMemcpyDeviceToHost,

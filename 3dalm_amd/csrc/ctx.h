@@ -66,6 +66,22 @@ struct PCoef {
     double max_diag;
 };
 
+// FP32 metric reciprocals of the precision form (1-D device arrays): 1/width,
+// 1/(c[i+1]-c[i]), 1/(c[i]-c[i-1]), 1/(c[i+1]-c[i-1]) per direction, and widths.
+struct MetricF {
+    const float *rw[3];
+    const float *rp[3];
+    const float *rm[3];
+    const float *r2[3];
+    const float *w[3];
+};
+
+// FP32 pieces of the normalised operator for the inner Poisson sweeps.
+struct PCoefF {
+    const float *ae, *aw, *an, *as, *at, *ab;  // normalised off-diagonals
+    const float *acx, *anu, *asu, *atu, *abu;  // normalised diagonal pieces
+};
+
 struct AlmTables {
     int wt_count = 0, ap_count = 0, blade_per_wt = 0, ap_per_blade = 0, atk_count = 0;
     double width = 0;
@@ -114,6 +130,19 @@ struct dalm_ctx {
 
     PoissonCoef pc{};
     bool have_A = false;
+
+    // precision form (DALM_MIXED): FP32 cell fields and inner Krylov vectors;
+    // JU, p, b, div stay FP64
+    float* U32[3] = {};
+    float* Uold32[3] = {};
+    float* f32[3] = {};
+    float* nut32 = nullptr;
+    float *fr = nullptr, *fr0 = nullptr, *fp = nullptr, *fpp = nullptr, *fq = nullptr, *fs = nullptr,
+          *fss = nullptr, *ft = nullptr, *ftmp = nullptr, *ftmp2 = nullptr, *fe = nullptr;
+    MetricF mf{};
+    PCoefF pcf{};
+    std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
+    long long outer_it = 0;       // FP64 refinement passes of the last mixed solve
 
     // reduction scratch
     double* partials = nullptr;  // [8][grid]
@@ -175,3 +204,15 @@ int zero_device(dalm_ctx* ctx, double* a, long long n);
 void probe_begin(dalm_ctx* ctx, int kernel);
 void probe_end(dalm_ctx* ctx, int kernel);
 int probe_collect(dalm_ctx* ctx);
+
+// precision form (cfd_mixed.hip, poisson_mixed.hip, alm.hip)
+int setup_mixed(dalm_ctx* ctx);
+int launch_ghost_copy_m(dalm_ctx* ctx);
+int launch_predictor_m(dalm_ctx* ctx, double Re, double dt);
+int launch_flux_rhs_m(dalm_ctx* ctx, const double Uin[3], double dt, double scale, bool rhs);
+int launch_project_m(dalm_ctx* ctx, double dt, const double Uin[3]);
+int launch_ubc_m(dalm_ctx* ctx, const double Uin[3], double dt, int retracted, double floor_x);
+int launch_solid_m(dalm_ctx* ctx);
+int launch_post_m(dalm_ctx* ctx, double Cs, double dt, bool store_div);
+int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
+                      long long* it, double* err);

@@ -38,13 +38,13 @@ void probe_begin(dalm_ctx* ctx, int kernel) {
             P.ev.push_back(e);
         }
     }
-    hipEventRecord(P.ev[P.used], ctx->stream);
+    (void)hipEventRecord(P.ev[P.used], ctx->stream);
 }
 
 void probe_end(dalm_ctx* ctx, int kernel) {
     KernelProbe& P = ctx->probe;
     if (P.kernel == 0 || P.kernel != kernel || P.used + 2 > P.ev.size()) return;
-    hipEventRecord(P.ev[P.used + 1], ctx->stream);
+    (void)hipEventRecord(P.ev[P.used + 1], ctx->stream);
     P.used += 2;
 }
 
@@ -133,16 +133,27 @@ int build_operator(dalm_ctx* ctx) {
     return DALM_OK;
 }
 
-// field id -> device component pointers (FP64 context)
-int field_ptrs(dalm_ctx* ctx, int64_t field, double** c, int* ncomp) {
+// field id -> device component pointers; *fp32 tells whether they hold floats
+int field_ptrs(dalm_ctx* ctx, int64_t field, void** c, int* ncomp, bool* fp32) {
     *ncomp = 1;
+    const bool m = ctx->precision == DALM_MIXED;
+    *fp32 = false;
     switch (field) {
-    case DALM_F_U: c[0] = ctx->U[0]; c[1] = ctx->U[1]; c[2] = ctx->U[2]; *ncomp = 3; break;
-    case DALM_F_UOLD: c[0] = ctx->Uold[0]; c[1] = ctx->Uold[1]; c[2] = ctx->Uold[2]; *ncomp = 3; break;
+    case DALM_F_U:
+    case DALM_F_UOLD:
+    case DALM_F_F: {
+        *ncomp = 3;
+        *fp32 = m;
+        for (int q = 0; q < 3; ++q) {
+            if (field == DALM_F_U) c[q] = m ? (void*)ctx->U32[q] : (void*)ctx->U[q];
+            else if (field == DALM_F_UOLD) c[q] = m ? (void*)ctx->Uold32[q] : (void*)ctx->Uold[q];
+            else c[q] = m ? (void*)ctx->f32[q] : (void*)ctx->f[q];
+        }
+        break;
+    }
     case DALM_F_JU: c[0] = ctx->JU[0]; c[1] = ctx->JU[1]; c[2] = ctx->JU[2]; *ncomp = 3; break;
-    case DALM_F_F: c[0] = ctx->f[0]; c[1] = ctx->f[1]; c[2] = ctx->f[2]; *ncomp = 3; break;
     case DALM_F_P: c[0] = ctx->p; break;
-    case DALM_F_NUT: c[0] = ctx->nut; break;
+    case DALM_F_NUT: c[0] = m ? (void*)ctx->nut32 : (void*)ctx->nut; *fp32 = m; break;
     case DALM_F_DIV: c[0] = ctx->div; break;
     case DALM_F_SOLID: c[0] = ctx->solid; break;
     case DALM_F_B: c[0] = ctx->b; break;
@@ -181,6 +192,12 @@ int read_scalars(dalm_ctx* ctx) {
 #define CHECK_CTX()                             \
     do {                                        \
         if (!ctx) return DALM_BAD_ARG;          \
+    } while (0)
+#define CHECK_FP64()                                                                        \
+    do {                                                                                    \
+        if (!ctx) return DALM_BAD_ARG;                                                      \
+        if (ctx->precision != DALM_FP64)                                                    \
+            return dalm_fail_msg(ctx, DALM_BAD_ARG, "per-kernel entry points need a DALM_FP64 context"); \
     } while (0)
 #define TRY(x)              \
     do {                    \
@@ -264,33 +281,52 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     for (int q = 0; q < 11; ++q)
         if ((rc = dev_alloc(ctx, pc1[q], pcn[q]))) return fail(rc);
 
+    const bool mixed = precision == DALM_MIXED;
     for (int q = 0; q < 3; ++q) {
-        if ((rc = dev_alloc(ctx, &ctx->U[q], L)) || (rc = dev_alloc(ctx, &ctx->Uold[q], L)) ||
-            (rc = dev_alloc(ctx, &ctx->JU[q], L)) || (rc = dev_alloc(ctx, &ctx->f[q], L)))
-            return fail(rc);
+        if ((rc = dev_alloc(ctx, &ctx->JU[q], L))) return fail(rc);
+        if (mixed) {
+            if ((rc = dev_alloc(ctx, &ctx->U32[q], L)) || (rc = dev_alloc(ctx, &ctx->Uold32[q], L)) ||
+                (rc = dev_alloc(ctx, &ctx->f32[q], L)))
+                return fail(rc);
+        } else {
+            if ((rc = dev_alloc(ctx, &ctx->U[q], L)) || (rc = dev_alloc(ctx, &ctx->Uold[q], L)) ||
+                (rc = dev_alloc(ctx, &ctx->f[q], L)))
+                return fail(rc);
+        }
     }
-    double** sc[15] = {&ctx->p, &ctx->nut, &ctx->div, &ctx->b, &ctx->solid, &ctx->r, &ctx->r0,
-                       &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t, &ctx->tmp, &ctx->tmp2};
-    for (int q = 0; q < 15; ++q)
-        if ((rc = dev_alloc(ctx, sc[q], L))) return fail(rc);
+    double** sc64[5] = {&ctx->p, &ctx->div, &ctx->b, &ctx->solid, &ctx->r};
+    for (int q = 0; q < 5; ++q)
+        if ((rc = dev_alloc(ctx, sc64[q], L))) return fail(rc);
+    if (mixed) {
+        float** scf[12] = {&ctx->nut32, &ctx->fr, &ctx->fr0, &ctx->fp, &ctx->fpp, &ctx->fq,
+                           &ctx->fs, &ctx->fss, &ctx->ft, &ctx->ftmp, &ctx->ftmp2, &ctx->fe};
+        for (int q = 0; q < 12; ++q)
+            if ((rc = dev_alloc(ctx, scf[q], L))) return fail(rc);
+    } else {
+        double** sc[10] = {&ctx->nut, &ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t,
+                           &ctx->tmp, &ctx->tmp2};
+        for (int q = 0; q < 10; ++q)
+            if ((rc = dev_alloc(ctx, sc[q], L))) return fail(rc);
+    }
     if ((rc = dev_alloc(ctx, &ctx->partials, 8 * (size_t)ctx->grid)) ||
-        (rc = dev_alloc(ctx, &ctx->counter, 16)) || (rc = dev_alloc(ctx, &ctx->solve, 1)) ||
+        (rc = dev_alloc(ctx, &ctx->counter, 16)) || (rc = dev_alloc(ctx, &ctx->solve, 2)) ||
         (rc = dev_alloc(ctx, &ctx->scal, 1)))
         return fail(rc);
-    if (hipHostMalloc((void**)&ctx->h_solve, sizeof(SolveState)) != hipSuccess ||
+    if (hipHostMalloc((void**)&ctx->h_solve, 2 * sizeof(SolveState)) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_scal, sizeof(StepScalars)) != hipSuccess)
         return fail(DALM_HIP_ERROR);
-    std::memset(ctx->h_solve, 0, sizeof(SolveState));
+    std::memset(ctx->h_solve, 0, 2 * sizeof(SolveState));
     std::memset(ctx->h_scal, 0, sizeof(StepScalars));
     if ((rc = build_operator(ctx))) return fail(rc);
+    if (mixed && (rc = setup_mixed(ctx))) return fail(rc);
     *out = ctx;
     return DALM_OK;
 }
 
 int dalm_destroy(dalm_ctx* ctx) {
     if (!ctx) return DALM_OK;
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (hipEvent_t e : ctx->probe.ev) hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->probe.ev) (void)hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
                       ctx->t, ctx->tmp, ctx->tmp2, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
@@ -298,20 +334,30 @@ int dalm_destroy(dalm_ctx* ctx) {
                       ctx->pc.an_u, ctx->pc.as_u, ctx->pc.at_u, ctx->pc.ab_u, ctx->alm.wt, ctx->alm.ap,
                       ctx->alm.cd, ctx->alm.cl, ctx->alm.atk};
     for (double* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (int q = 0; q < 3; ++q) {
-        if (ctx->U[q]) hipFree(ctx->U[q]);
-        if (ctx->Uold[q]) hipFree(ctx->Uold[q]);
-        if (ctx->JU[q]) hipFree(ctx->JU[q]);
-        if (ctx->f[q]) hipFree(ctx->f[q]);
+        if (ctx->U[q]) (void)hipFree(ctx->U[q]);
+        if (ctx->Uold[q]) (void)hipFree(ctx->Uold[q]);
+        if (ctx->JU[q]) (void)hipFree(ctx->JU[q]);
+        if (ctx->f[q]) (void)hipFree(ctx->f[q]);
     }
-    if (ctx->alm.box_dev) hipFree(ctx->alm.box_dev);
-    if (ctx->counter) hipFree(ctx->counter);
-    if (ctx->solve) hipFree(ctx->solve);
-    if (ctx->scal) hipFree(ctx->scal);
-    if (ctx->h_solve) hipHostFree(ctx->h_solve);
-    if (ctx->h_scal) hipHostFree(ctx->h_scal);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    for (int q = 0; q < 3; ++q) {
+        if (ctx->U32[q]) (void)hipFree(ctx->U32[q]);
+        if (ctx->Uold32[q]) (void)hipFree(ctx->Uold32[q]);
+        if (ctx->f32[q]) (void)hipFree(ctx->f32[q]);
+    }
+    float* fptrs[] = {ctx->nut32, ctx->fr, ctx->fr0, ctx->fp, ctx->fpp, ctx->fq, ctx->fs, ctx->fss,
+                      ctx->ft, ctx->ftmp, ctx->ftmp2, ctx->fe};
+    for (float* p : fptrs)
+        if (p) (void)hipFree(p);
+    for (void* p : ctx->owned_f) (void)hipFree(p);
+    if (ctx->alm.box_dev) (void)hipFree(ctx->alm.box_dev);
+    if (ctx->counter) (void)hipFree(ctx->counter);
+    if (ctx->solve) (void)hipFree(ctx->solve);
+    if (ctx->scal) (void)hipFree(ctx->scal);
+    if (ctx->h_solve) (void)hipHostFree(ctx->h_solve);
+    if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DALM_OK;
 }
@@ -319,15 +365,20 @@ int dalm_destroy(dalm_ctx* ctx) {
 int dalm_upload(dalm_ctx* ctx, int64_t field, const double* host) {
     CHECK_CTX();
     if (!host) return DALM_BAD_ARG;
-    double* c[3];
+    void* c[3];
     int nc;
-    TRY(field_ptrs(ctx, field, c, &nc));
+    bool f32;
+    TRY(field_ptrs(ctx, field, c, &nc, &f32));
     std::vector<double> tmp(ctx->L);
+    std::vector<float> tmpf(f32 ? ctx->L : 0);
     for (int q = 0; q < nc; ++q) {
-        for (long long e = 0; e < ctx->L; ++e) tmp[e] = host[e * nc + q];
-        HIP_TRY(hipMemcpyAsync(c[q], tmp.data(), ctx->L * sizeof(double), hipMemcpyHostToDevice,
-                               ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (f32) {
+            for (long long e = 0; e < ctx->L; ++e) tmpf[e] = (float)host[e * nc + q];
+            HIP_TRY(hipMemcpy(c[q], tmpf.data(), ctx->L * sizeof(float), hipMemcpyHostToDevice));
+        } else {
+            for (long long e = 0; e < ctx->L; ++e) tmp[e] = host[e * nc + q];
+            HIP_TRY(hipMemcpy(c[q], tmp.data(), ctx->L * sizeof(double), hipMemcpyHostToDevice));
+        }
     }
     if (field == DALM_F_SOLID) {
         ctx->has_solid = false;
@@ -343,15 +394,21 @@ int dalm_upload(dalm_ctx* ctx, int64_t field, const double* host) {
 int dalm_download(dalm_ctx* ctx, int64_t field, double* host) {
     CHECK_CTX();
     if (!host) return DALM_BAD_ARG;
-    double* c[3];
+    void* c[3];
     int nc;
-    TRY(field_ptrs(ctx, field, c, &nc));
-    std::vector<double> tmp(ctx->L);
+    bool f32;
+    TRY(field_ptrs(ctx, field, c, &nc, &f32));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<double> tmp(f32 ? 0 : ctx->L);
+    std::vector<float> tmpf(f32 ? ctx->L : 0);
     for (int q = 0; q < nc; ++q) {
-        HIP_TRY(hipMemcpyAsync(tmp.data(), c[q], ctx->L * sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        for (long long e = 0; e < ctx->L; ++e) host[e * nc + q] = tmp[e];
+        if (f32) {
+            HIP_TRY(hipMemcpy(tmpf.data(), c[q], ctx->L * sizeof(float), hipMemcpyDeviceToHost));
+            for (long long e = 0; e < ctx->L; ++e) host[e * nc + q] = tmpf[e];
+        } else {
+            HIP_TRY(hipMemcpy(tmp.data(), c[q], ctx->L * sizeof(double), hipMemcpyDeviceToHost));
+            for (long long e = 0; e < ctx->L; ++e) host[e * nc + q] = tmp[e];
+        }
     }
     return DALM_OK;
 }
@@ -369,28 +426,28 @@ int dalm_set_params(dalm_ctx* ctx, const dalm_params* prm) {
 
 // ---- per-kernel entry points --------------------------------------------------
 int dalm_copy_u_to_uold(dalm_ctx* ctx) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_copy_uold(ctx));
     return sync_stream(ctx);
 }
 int dalm_predictor(dalm_ctx* ctx, double Re, double dt) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_predictor(ctx, Re, dt));
     return sync_stream(ctx);
 }
 int dalm_interpolate_JU(dalm_ctx* ctx) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_flux(ctx, nullptr, false));
     return sync_stream(ctx);
 }
 int dalm_apply_JUbc(dalm_ctx* ctx, const double Uin[3]) {
-    CHECK_CTX();
+    CHECK_FP64();
     if (!Uin) return DALM_BAD_ARG;
     TRY(launch_jubc(ctx, Uin));
     return sync_stream(ctx);
 }
 int dalm_poisson_rhs(dalm_ctx* ctx, double dt, double scale) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_rhs(ctx, dt, scale));
     return sync_stream(ctx);
 }
@@ -401,7 +458,7 @@ int dalm_build_A(dalm_ctx* ctx, double* max_diag) {
     return DALM_OK;
 }
 int dalm_run_sor(dalm_ctx* ctx, double omega, int64_t max_it, double tol, int64_t* it, double* err) {
-    CHECK_CTX();
+    CHECK_FP64();
     if (max_it < 1) return DALM_BAD_ARG;
     long long i = 0;
     double e = 0;
@@ -412,7 +469,7 @@ int dalm_run_sor(dalm_ctx* ctx, double omega, int64_t max_it, double tol, int64_
 }
 int dalm_run_bicgstab(dalm_ctx* ctx, int64_t pc_kind, double pc_omega, int64_t pc_it, int64_t max_it,
                       double tol, int64_t* it, double* err) {
-    CHECK_CTX();
+    CHECK_FP64();
     if (max_it < 1 || pc_it < 0 || (pc_kind != DALM_PC_JACOBI && pc_kind != DALM_PC_SOR))
         return DALM_BAD_ARG;
     long long i = 0;
@@ -423,7 +480,7 @@ int dalm_run_bicgstab(dalm_ctx* ctx, int64_t pc_kind, double pc_omega, int64_t p
     return rc;
 }
 int dalm_residual(dalm_ctx* ctx, double* out) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_residual(ctx, ctx->p, ctx->b, ctx->r));
     TRY(sync_stream(ctx));
     if (out) {
@@ -434,47 +491,48 @@ int dalm_residual(dalm_ctx* ctx, double* out) {
     return DALM_OK;
 }
 int dalm_apply_pbc(dalm_ctx* ctx) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_pbc(ctx));
     return sync_stream(ctx);
 }
 int dalm_project_p(dalm_ctx* ctx, double dt) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_project(ctx, dt, nullptr));
     return sync_stream(ctx);
 }
 int dalm_set_solid_U(dalm_ctx* ctx) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_solid(ctx));
     return sync_stream(ctx);
 }
 int dalm_apply_Ubc(dalm_ctx* ctx, const double Uin[3], double dt, int64_t retracted_floor,
                    double floor_start_x) {
-    CHECK_CTX();
+    CHECK_FP64();
     if (!Uin) return DALM_BAD_ARG;
     TRY(launch_ubc(ctx, Uin, dt, (int)retracted_floor, floor_start_x));
     return sync_stream(ctx);
 }
 int dalm_calc_nut(dalm_ctx* ctx, double Cs) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_nut(ctx, Cs));
     return sync_stream(ctx);
 }
 int dalm_calc_divergence(dalm_ctx* ctx) {
-    CHECK_CTX();
+    CHECK_FP64();
     TRY(launch_divergence(ctx));
     return sync_stream(ctx);
 }
 int dalm_l2_norm(dalm_ctx* ctx, int64_t field, double* out) {
-    CHECK_CTX();
-    double* c[3];
+    CHECK_FP64();
+    void* c[3];
     int nc;
-    TRY(field_ptrs(ctx, field, c, &nc));
-    if (nc != 1 || !out) return DALM_BAD_ARG;
-    return launch_l2(ctx, c[0], out);
+    bool f32;
+    TRY(field_ptrs(ctx, field, c, &nc, &f32));
+    if (nc != 1 || f32 || !out) return DALM_BAD_ARG;
+    return launch_l2(ctx, (const double*)c[0], out);
 }
 int dalm_max_cfl(dalm_ctx* ctx, double dt, double* out) {
-    CHECK_CTX();
+    CHECK_FP64();
     if (!out) return DALM_BAD_ARG;
     return launch_max_cfl(ctx, dt, out);
 }
@@ -508,7 +566,7 @@ int dalm_alm_setup(dalm_ctx* ctx, const double* wt, int64_t wt_count, const doub
     if (!A.wt && ((rc = dev_alloc(ctx, &A.wt, DALM_WT_STRIDE * DALM_MAX_TURBINES)) ||
                   (rc = dev_alloc(ctx, &A.box_dev, 6 * DALM_MAX_TURBINES))))
         return rc;
-    if (A.ap) { hipFree(A.ap); hipFree(A.cd); hipFree(A.cl); hipFree(A.atk); }
+    if (A.ap) { (void)hipFree(A.ap); (void)hipFree(A.cd); (void)hipFree(A.cl); (void)hipFree(A.atk); }
     if ((rc = dev_alloc(ctx, &A.ap, DALM_AP_STRIDE * ap_count)) ||
         (rc = dev_alloc(ctx, &A.cd, ap_count * atk_count)) ||
         (rc = dev_alloc(ctx, &A.cl, ap_count * atk_count)) || (rc = dev_alloc(ctx, &A.atk, atk_count)))
@@ -554,10 +612,17 @@ int dalm_initialize(dalm_ctx* ctx, dalm_step_stats* stats) {
     const dalm_params& P = ctx->prm;
     ctx->step = 0;
     if (!ctx->have_A) TRY(build_operator(ctx));
-    if (ctx->has_solid) TRY(launch_solid(ctx));
-    TRY(launch_ubc(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
-    TRY(launch_flux(ctx, P.Uin, true));
-    TRY(launch_post(ctx, P.Cs, P.dt, true));
+    if (ctx->precision == DALM_MIXED) {
+        if (ctx->has_solid) TRY(launch_solid_m(ctx));
+        TRY(launch_ubc_m(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
+        TRY(launch_flux_rhs_m(ctx, P.Uin, P.dt, ctx->pc.max_diag, false));
+        TRY(launch_post_m(ctx, P.Cs, P.dt, true));
+    } else {
+        if (ctx->has_solid) TRY(launch_solid(ctx));
+        TRY(launch_ubc(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
+        TRY(launch_flux(ctx, P.Uin, true));
+        TRY(launch_post(ctx, P.Cs, P.dt, true));
+    }
     TRY(alm_run(ctx, 0.0, false));
     TRY(read_scalars(ctx));
     dalm_step_stats st;
@@ -566,12 +631,38 @@ int dalm_initialize(dalm_ctx* ctx, dalm_step_stats* stats) {
     return DALM_OK;
 }
 
+// One time step of the precision form (same order as the FP64 step below).
+static int step_mixed(dalm_ctx* ctx, double t, dalm_step_stats* stats) {
+    const dalm_params& P = ctx->prm;
+    for (int q = 0; q < 3; ++q) std::swap(ctx->U32[q], ctx->Uold32[q]);
+    TRY(launch_ghost_copy_m(ctx));
+    TRY(launch_predictor_m(ctx, P.Re, P.dt));
+    TRY(launch_flux_rhs_m(ctx, P.Uin, P.dt, ctx->pc.max_diag, true));
+    long long it = 0;
+    double err = 0;
+    int status = run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+    if (status == DALM_HIP_ERROR) return status;
+    TRY(launch_pbc(ctx));
+    TRY(launch_project_m(ctx, P.dt, P.Uin));
+    if (ctx->has_solid) TRY(launch_solid_m(ctx));
+    TRY(launch_ubc_m(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
+    TRY(launch_post_m(ctx, P.Cs, P.dt, true));
+    TRY(alm_run(ctx, t, false));
+    TRY(read_scalars(ctx));
+    TRY(probe_collect(ctx));
+    dalm_step_stats st;
+    fill_stats(ctx, &st, it, err, status);
+    if (stats) *stats = st;
+    return status == DALM_NONCONVERGED ? DALM_OK : status;
+}
+
 int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
     CHECK_CTX();
     if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
     const dalm_params& P = ctx->prm;
     ctx->step += 1;
     const double t = P.dt * (double)ctx->step;
+    if (ctx->precision == DALM_MIXED) return step_mixed(ctx, t, stats);
     // Uold <- U: swap the buffers, then restore U's ghost shell from Uold
     for (int q = 0; q < 3; ++q) std::swap(ctx->U[q], ctx->Uold[q]);
     TRY(launch_ghost_copy(ctx));

@@ -1,0 +1,378 @@
+// poisson_mixed.hip -- Poisson solve of the precision form: FP64 iterative
+// refinement around an FP32 preconditioned BiCGStab.
+//
+//   repeat:  r = b - A p                      (FP64 operator, FP64 norm)
+//            stop when |r| / sqrt(N) <= tol    (the reference's test, src/eq.h:399-401)
+//            solve A e = r in FP32 BiCGStab (Jacobi / SOR preconditioned, same
+//            recurrences as src/eq.h:333-473) to max(tol/2, 1e-4 |r|/sqrt(N)),
+//            with every dot product and norm accumulated in FP64
+//            p += e                            (FP64)
+//
+// Inner vectors are FP32, so each inner iteration moves half the bytes of the FP64
+// solver; the pressure, the right-hand side and the stopping test stay FP64, so the
+// final residual -- and with it div(JU) = dt max_diag (b - A p) -- meets the same
+// tolerance as the drop-in.  Iteration count reported = total inner iterations.
+#include <float.h>
+
+#include "ctx.h"
+
+namespace {
+
+__device__ __forceinline__ float diag_f(const PCoefF& A, int i, int j, int k) {
+    return (((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k];
+}
+__device__ __forceinline__ float apply7f(const Dims& d, const PCoefF& A, const float* x, int c, int i, int j,
+                                         int k, float ac) {
+    return ac * x[c] + A.ae[i] * x[c + d.si] + A.aw[i] * x[c - d.si] + A.an[j] * x[c + d.sj] +
+           A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
+}
+__device__ __forceinline__ double diag64(const PCoef& A, int i, int j, int k) {
+    return ((((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k]) / A.max_diag;
+}
+__device__ __forceinline__ double apply7d(const Dims& d, const PCoef& A, const double* x, int c, int i, int j,
+                                          int k, double ac) {
+    return ac * x[c] + A.ae[i] * x[c + d.si] + A.aw[i] * x[c - d.si] + A.an[j] * x[c + d.sj] +
+           A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
+}
+
+// FP64 residual r = b - A p, its norm (outer test) and the start of an inner solve:
+// r_in = r0_in = (float) r, e = 0, rho = <r0_in, r_in>.
+__global__ __launch_bounds__(BLOCK) void km_res64(Dims d, PCoef A, const double* __restrict__ p,
+                                                  const double* __restrict__ b, float* __restrict__ r,
+                                                  float* __restrict__ r0, float* __restrict__ e,
+                                                  double* partials, unsigned* counter, SolveState* inner,
+                                                  SolveState* outer, double eta, long long inner_max) {
+    TileRange tr = block_tiles(d);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double rv = b[c] - apply7d(d, A, p, c, cc.i, cc.j, cc.k, diag64(A, cc.i, cc.j, cc.k));
+        float rf = (float)rv;
+        r[c] = rf;
+        r0[c] = rf;
+        e[c] = 0.f;
+        v[0] += rv * rv;
+        v[1] += (double)rf * (double)rf;
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    if (publish_partials<2>(pv, partials, counter)) {
+        double s[2];
+        reduce_partials<2>(partials, gridDim.x, s);
+        if (threadIdx.x == 0) {
+            double err = sqrt(s[0]) / outer->root_n;
+            outer->err = err;
+            outer->it += 1;
+            inner->it = 0;
+            inner->rho = s[1];
+            inner->alpha = inner->omega = 1.0;
+            inner->beta = 0;
+            inner->err = err;
+            inner->root_n = outer->root_n;
+            inner->tol = fmax(0.5 * outer->tol, eta * err);
+            inner->max_it = inner_max;
+            inner->status = DALM_OK;
+            inner->done = 0;
+            if (!(err == err) || isinf(err)) {
+                outer->status = DALM_NONFINITE;
+                outer->done = 1;
+                inner->done = 1;
+            } else if (err <= outer->tol || inner_max <= 0) {
+                outer->done = 1;
+                outer->status = err <= outer->tol ? DALM_OK : DALM_NONCONVERGED;
+                inner->done = 1;
+            } else if (fabs(s[1]) < (double)FLT_MIN) {
+                inner->done = 1;
+                inner->status = DALM_BREAKDOWN;
+            }
+        }
+    }
+}
+
+__global__ void km_outer_reset(SolveState* outer, double tol, double root_n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        outer->err = 0;
+        outer->tol = tol;
+        outer->root_n = root_n;
+        outer->it = 0;
+        outer->done = 0;
+        outer->status = DALM_OK;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_p(Dims d, PCoefF A, const float* __restrict__ r,
+                                              float* __restrict__ p, const float* __restrict__ q,
+                                              float* __restrict__ z, int pc_first, const SolveState* st) {
+    if (st->done) return;
+    const bool first = st->it == 0;
+    const float beta = (float)st->beta, om = (float)st->omega;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        float pv = first ? r[c] : r[c] + beta * (p[c] - om * q[c]);
+        p[c] = pv;
+        z[c] = pc_first ? pv * __builtin_amdgcn_rcpf(diag_f(A, cc.i, cc.j, cc.k)) : 0.f;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float* __restrict__ zin,
+                                                   float* __restrict__ zout, const float* __restrict__ v,
+                                                   const SolveState* st) {
+    if (st->done) return;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        float ac = diag_f(A, cc.i, cc.j, cc.k);
+        zout[c] = zin[c] + (v[c] - apply7f(d, A, zin, c, cc.i, cc.j, cc.k, ac)) * __builtin_amdgcn_rcpf(ac);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restrict__ x,
+                                                const float* __restrict__ b, float omega, int color,
+                                                const SolveState* st) {
+    if (st->done) return;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside || ((cc.i + cc.j + cc.k) & 1) != color) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        float ac = diag_f(A, cc.i, cc.j, cc.k);
+        x[c] = x[c] + omega * (b[c] - apply7f(d, A, x, c, cc.i, cc.j, cc.k, ac)) * __builtin_amdgcn_rcpf(ac);
+    }
+}
+
+template <int WHICH>
+__global__ __launch_bounds__(BLOCK) void kf_spmv(Dims d, PCoefF A, const float* __restrict__ z,
+                                                 float* __restrict__ y, const float* __restrict__ a1,
+                                                 double* partials, unsigned* counter, SolveState* st) {
+    if (st->done) return;
+    TileRange tr = block_tiles(d);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        float yv = apply7f(d, A, z, c, cc.i, cc.j, cc.k, diag_f(A, cc.i, cc.j, cc.k));
+        y[c] = yv;
+        double yd = yv;
+        if (WHICH == 0) {
+            v[0] += (double)a1[c] * yd;
+        } else {
+            v[0] += yd * (double)a1[c];
+            v[1] += yd * yd;
+        }
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    if (publish_partials<2>(pv, partials, counter)) {
+        double s[2];
+        reduce_partials<2>(partials, gridDim.x, s);
+        if (threadIdx.x == 0) {
+            if (WHICH == 0) st->alpha = st->rho / s[0];
+            else st->omega = s[0] / s[1];
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_s(Dims d, PCoefF A, const float* __restrict__ r,
+                                              const float* __restrict__ q, float* __restrict__ s,
+                                              float* __restrict__ z, int pc_first, const SolveState* st) {
+    if (st->done) return;
+    const float alpha = (float)st->alpha;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        float sv = r[c] - alpha * q[c];
+        s[c] = sv;
+        z[c] = pc_first ? sv * __builtin_amdgcn_rcpf(diag_f(A, cc.i, cc.j, cc.k)) : 0.f;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_xr(Dims d, float* __restrict__ x, const float* __restrict__ pp,
+                                               const float* __restrict__ ss, const float* __restrict__ s,
+                                               const float* __restrict__ t_, float* __restrict__ r,
+                                               const float* __restrict__ r0, double* partials,
+                                               unsigned* counter, SolveState* st) {
+    if (st->done) return;
+    const float alpha = (float)st->alpha, om = (float)st->omega;
+    TileRange tr = block_tiles(d);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        x[c] = x[c] + alpha * pp[c] + om * ss[c];
+        float rv = s[c] - om * t_[c];
+        r[c] = rv;
+        double rd = rv;
+        v[0] += rd * rd;
+        v[1] += (double)r0[c] * rd;
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    if (publish_partials<2>(pv, partials, counter)) {
+        double sm[2];
+        reduce_partials<2>(partials, gridDim.x, sm);
+        if (threadIdx.x == 0) {
+            double err = sqrt(sm[0]) / st->root_n;
+            long long it = st->it + 1;
+            st->it = it;
+            st->err = err;
+            if (!(err == err) || isinf(err)) {
+                st->done = 1;
+                st->status = DALM_NONFINITE;
+            } else if (it < st->max_it && err > st->tol) {
+                double rho_new = sm[1];
+                if (fabs(rho_new) < (double)FLT_MIN) {
+                    st->done = 1;
+                    st->status = DALM_BREAKDOWN;
+                } else {
+                    st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
+                    st->rho_prev = st->rho;
+                    st->rho = rho_new;
+                }
+            } else {
+                st->done = 1;
+                st->status = err > st->tol ? DALM_NONCONVERGED : DALM_OK;
+            }
+        }
+    }
+}
+
+// p += e over the interior (FP64 accumulate of the FP32 correction)
+__global__ __launch_bounds__(BLOCK) void km_add(Dims d, double* __restrict__ p, const float* __restrict__ e) {
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        p[c] += (double)e[c];
+    }
+}
+
+int read_states(dalm_ctx* ctx) {
+    HIP_TRY(hipMemcpyAsync(ctx->h_solve, ctx->solve, 2 * sizeof(SolveState), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DALM_OK;
+}
+
+long long chunk_for(long long done_it, long long predicted, long long max_it) {
+    long long left = predicted - done_it;
+    long long c = left > 8 ? left : 8;
+    if (c > 256) c = 256;
+    if (done_it + c > max_it) c = max_it - done_it;
+    return c < 1 ? 1 : c;
+}
+
+}  // namespace
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
+                      long long* it, double* err) {
+    const Dims& d = ctx->d;
+    const PCoefF A = ctx->pcf;
+    const PCoef A64 = ctx->coef();
+    SolveState* inner = ctx->solve;
+    SolveState* outer = ctx->solve + 1;
+    const double eta = 1e-4;
+    const int jac = pc_kind == DALM_PC_JACOBI;
+    const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
+    hipLaunchKernelGGL(km_outer_reset, dim3(1), dim3(64), 0, ctx->stream, outer, tol, sqrt((double)ctx->N));
+    LAUNCH_CHECK();
+    long long total = 0;
+    int status = DALM_OK;
+    auto precondition = [&](float* z0, float* spare, const float* v) -> float* {
+        float* cur = z0;
+        if (jac) {
+            float* other = spare;
+            for (int m = 1; m < pc_it; ++m) {
+                probe_begin(ctx, DALM_K_JACOBI);
+                hipLaunchKernelGGL(kf_jacobi, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, other, v,
+                                   (const SolveState*)inner);
+                probe_end(ctx, DALM_K_JACOBI);
+                float* tsw = cur;
+                cur = other;
+                other = tsw;
+            }
+        } else {
+            for (int m = 0; m < pc_it; ++m)
+                for (int color = 0; color < 2; ++color)
+                    hipLaunchKernelGGL(kf_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
+                                       (float)pc_omega, color, (const SolveState*)inner);
+        }
+        return cur;
+    };
+    for (int pass = 0; pass < 64; ++pass) {
+        hipLaunchKernelGGL(km_res64, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A64, ctx->p, ctx->b, ctx->fr,
+                           ctx->fr0, ctx->fe, ctx->partials, ctx->counter, inner, outer, eta, max_it - total);
+        LAUNCH_CHECK();
+        int rc = read_states(ctx);
+        if (rc) return rc;
+        if (ctx->h_solve[1].done) {
+            status = ctx->h_solve[1].status;
+            break;
+        }
+        if (ctx->h_solve[0].done) {  // breakdown at the start of the inner solve
+            status = ctx->h_solve[0].status;
+            break;
+        }
+        long long launched = 0, budget = max_it - total;
+        while (!ctx->h_solve[0].done && launched < budget) {
+            long long chunk = chunk_for(launched, ctx->prev_it[1] - total, budget);
+            for (long long c = 0; c < chunk; ++c) {
+                probe_begin(ctx, DALM_K_PUPDATE);
+                hipLaunchKernelGGL(kf_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->fr, ctx->fp,
+                                   ctx->fq, ctx->fpp, pc_first, (const SolveState*)inner);
+                probe_end(ctx, DALM_K_PUPDATE);
+                float* ppz = precondition(ctx->fpp, ctx->ftmp, ctx->fp);
+                probe_begin(ctx, DALM_K_SPMV);
+                hipLaunchKernelGGL(kf_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz, ctx->fq,
+                                   ctx->fr0, ctx->partials, ctx->counter, inner);
+                probe_end(ctx, DALM_K_SPMV);
+                probe_begin(ctx, DALM_K_SUPDATE);
+                hipLaunchKernelGGL(kf_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->fr, ctx->fq,
+                                   ctx->fs, ctx->fss, pc_first, (const SolveState*)inner);
+                probe_end(ctx, DALM_K_SUPDATE);
+                float* ssz = precondition(ctx->fss, ctx->ftmp2, ctx->fs);
+                probe_begin(ctx, DALM_K_SPMV);
+                hipLaunchKernelGGL(kf_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz, ctx->ft,
+                                   ctx->fs, ctx->partials, ctx->counter, inner);
+                probe_end(ctx, DALM_K_SPMV);
+                probe_begin(ctx, DALM_K_UPDATE);
+                hipLaunchKernelGGL(kf_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->fe, ppz, ssz,
+                                   ctx->fs, ctx->ft, ctx->fr, ctx->fr0, ctx->partials, ctx->counter, inner);
+                probe_end(ctx, DALM_K_UPDATE);
+            }
+            LAUNCH_CHECK();
+            launched += chunk;
+            rc = read_states(ctx);
+            if (rc) return rc;
+        }
+        total += ctx->h_solve[0].it;
+        hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe);
+        LAUNCH_CHECK();
+        if (ctx->h_solve[0].status == DALM_NONFINITE) {
+            status = DALM_NONFINITE;
+            break;
+        }
+        if (ctx->h_solve[0].it == 0) {  // no progress possible
+            status = DALM_BREAKDOWN;
+            break;
+        }
+    }
+    *it = total;
+    *err = ctx->h_solve[1].err;
+    ctx->outer_it = ctx->h_solve[1].it;
+    ctx->prev_it[1] = total;
+    return status;
+}

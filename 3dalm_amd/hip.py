@@ -21,6 +21,7 @@ PI64 = C.POINTER(C.c_int64)
 OK, NONCONVERGED, BREAKDOWN, NONFINITE, HIP_ERROR, BAD_ARG = range(6)
 STATUS_NAMES = {0: "OK", 1: "NONCONVERGED", 2: "BREAKDOWN", 3: "NONFINITE", 4: "HIP_ERROR", 5: "BAD_ARG"}
 FP64, MIXED = 0, 1
+MIXED_BUILT = True  # the precision form is part of this library
 F_U, F_UOLD, F_JU, F_P, F_NUT, F_DIV, F_F, F_SOLID, F_B = range(9)
 FIELDS = {"U": F_U, "Uold": F_UOLD, "JU": F_JU, "p": F_P, "nut": F_NUT, "div": F_DIV, "f": F_F,
           "solid": F_SOLID, "b": F_B}
@@ -102,6 +103,8 @@ def lib():
         "dalm_initialize": [V, C.POINTER(StepStats)],
         "dalm_step": [V, C.POINTER(StepStats)],
         "dalm_run": [V, i64, C.POINTER(StepStats)],
+        "dalm_probe_kernel": [V, i64],
+        "dalm_probe_read": [V, PI64, P],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -127,6 +127,8 @@ class _StdoutToStderr:
 
     def __exit__(self, *exc):
         sys.stdout.flush()
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
         os.dup2(self.saved, 1)
         os.close(self.saved)
 

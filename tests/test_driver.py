@@ -1,0 +1,63 @@
+"""The C++ case driver (3dalm_amd/lib/dalm_solver): reads the reference's case files
+and prints the reference's per-step line.  The GPU test runs a small case and
+compares every printed line with the CPU oracle; the CPU tests cover argument and
+case-file errors that end before any device call."""
+import importlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+cases = importlib.import_module("3dalm_amd.cases")
+build = importlib.import_module("3dalm_amd.build")
+po = importlib.import_module("pyoracle")
+
+
+def run(args, timeout=300):
+    return subprocess.run([build.DRIVER, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_refuses_turbulence_grid(tmp_path):
+    case = cases.case_c1(8, 6, 6, steps=1)
+    cases.write_case_dir(case, str(tmp_path))
+    setup = json.load(open(tmp_path / "setup.json"))
+    setup["turbulence grid"] = {"bar": 0.1, "mesh": 0.5, "thick": 0.05, "placement": [0, 0, 0]}
+    json.dump(setup, open(tmp_path / "setup.json", "w"))
+    r = run([str(tmp_path)])
+    assert r.returncode == 1 and "turbulence-generating grid" in r.stderr
+
+
+def test_missing_case_file(tmp_path):
+    r = run([str(tmp_path / "nowhere")])
+    assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_usage():
+    r = run([])
+    assert r.returncode == 1 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("builder", [lambda: cases.case_c1(12, 10, 10, steps=4),
+                                     lambda: cases.case_c2(24, 12, 12, steps=3, ap_per_blade=6)])
+def test_driver_matches_oracle(tmp_path, builder):
+    case = builder()
+    print(f"[driver case {case.name} {case.mesh.interior}]", flush=True)
+    cases.write_case_dir(case, str(tmp_path))
+    r = run([str(tmp_path), "--perturb", str(case.noise * case.uin[0]), "--seed", str(case.seed), "--power"])
+    assert r.returncode == 0, r.stderr
+    lines = [l.split() for l in r.stdout.splitlines() if l and not l.startswith("#")]
+    power = [float(l.split()[-1]) for l in r.stdout.splitlines() if l.startswith("# wt")]
+    o = po.OracleSolver(case)
+    assert len(lines) == case.steps
+    for n, l in enumerate(lines):
+        s = o.step()
+        assert int(l[0]) == s["step"] and int(l[2]) == s["it"]
+        # printed with %e (7 significant digits)
+        for q, key in ((1, None), (3, "err"), (4, "avg_div"), (5, "max_cfl")):
+            ref = s["step"] * case.dt if key is None else s[key]
+            assert abs(float(l[q]) - ref) <= 1e-6 * abs(ref) + 1e-300, (n, key, l[q], ref)
+        if power:
+            assert abs(power[n] - case.sites[0].rot_speed * s["torque"][0]) <= 1e-10 * abs(power[n])
