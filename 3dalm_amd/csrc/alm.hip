@@ -174,54 +174,89 @@ __global__ void k_ap_force(Dims d, M1 m, const TU* __restrict__ u0, const TU* __
     }
 }
 
-// per-turbine torque / thrust, summed serially in AP order (src/alm.h:384-393)
-__global__ void k_wt_sum(const double* __restrict__ ap, int wt_count, int per_wt,
-                         StepScalars* out) {
-    int w = threadIdx.x;
-    if (w >= wt_count) return;
-    double tq = 0, th = 0;
-    for (int a = w * per_wt; a < (w + 1) * per_wt; ++a) {
-        tq += ap[DALM_AP_STRIDE * a + 11];
-        th += ap[DALM_AP_STRIDE * a + 12];
+// per-turbine torque / thrust, summed serially in AP order (src/alm.h:384-393);
+// the AP values are first staged in LDS by the whole block
+constexpr int MAX_PER_WT = 1024;
+__global__ __launch_bounds__(256) void k_wt_sum(const double* __restrict__ ap, int wt_count, int per_wt,
+                                                StepScalars* out) {
+    __shared__ double tq_s[MAX_PER_WT], th_s[MAX_PER_WT];
+    const int w = blockIdx.x;
+    for (int a = threadIdx.x; a < per_wt; a += blockDim.x) {
+        const double* P = ap + DALM_AP_STRIDE * (long long)(w * per_wt + a);
+        tq_s[a] = P[11];
+        th_s[a] = P[12];
     }
-    out->torque[w] = tq;
-    out->thrust[w] = th;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tq = 0, th = 0;
+        for (int a = 0; a < per_wt; ++a) {
+            tq += tq_s[a];
+            th += th_s[a];
+        }
+        out->torque[w] = tq;
+        out->thrust[w] = th;
+    }
 }
 
-// Gaussian projection (src/alm.h:416-448), truncated per turbine box and at
-// r > 8 eps; TF is the storage type of the force field.
+// Zero the force in a box of cells (half-open index ranges).
 template <typename TF>
-__global__ __launch_bounds__(BLOCK) void k_ap_project(Dims d, M1 m, const double* __restrict__ ap,
-                                                      const int* __restrict__ box, int wt_count,
-                                                      int per_wt, double norm, double w2,
-                                                      double cut2, TF* __restrict__ f0,
-                                                      TF* __restrict__ f1,
-                                                      TF* __restrict__ f2) {
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        const int i = cc.i, j = cc.j, k = cc.k;
-        const int c = d.at(i, j, k);
-        double sx = 0, sy = 0, sz = 0;
+__global__ void k_box_zero(Dims d, int i0, int i1, int j0, int j1, int k0, int k1, TF* __restrict__ f0,
+                           TF* __restrict__ f1, TF* __restrict__ f2) {
+    long long nk = k1 - k0, nj = j1 - j0, n = (long long)(i1 - i0) * nj * nk;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        int k = k0 + (int)(t % nk);
+        long long r = t / nk;
+        int j = j0 + (int)(r % nj);
+        int i = i0 + (int)(r / nj);
+        int c = d.at(i, j, k);
+        f0[c] = 0;
+        f1[c] = 0;
+        f2[c] = 0;
+    }
+}
+
+// Gaussian projection of one turbine's AP forces (src/alm.h:416-448) onto the
+// cells of its box, added to f.  The turbine's AP positions and forces are staged
+// in LDS; pairs beyond the cutoff are skipped.
+template <typename TF>
+__global__ __launch_bounds__(256) void k_ap_project(Dims d, M1 m, const double* __restrict__ ap, int a0,
+                                                    int napw, int i0, int i1, int j0, int j1, int k0, int k1,
+                                                    double norm, double w2, double cut2, TF* __restrict__ f0,
+                                                    TF* __restrict__ f1, TF* __restrict__ f2) {
+    __shared__ double P_s[MAX_PER_WT][6];
+    for (int a = threadIdx.x; a < napw; a += blockDim.x) {
+        const double* P = ap + DALM_AP_STRIDE * (long long)(a0 + a);
+        P_s[a][0] = P[0];
+        P_s[a][1] = P[1];
+        P_s[a][2] = P[2];
+        P_s[a][3] = P[8];
+        P_s[a][4] = P[9];
+        P_s[a][5] = P[10];
+    }
+    __syncthreads();
+    long long nk = k1 - k0, nj = j1 - j0, n = (long long)(i1 - i0) * nj * nk;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        int k = k0 + (int)(t % nk);
+        long long r = t / nk;
+        int j = j0 + (int)(r % nj);
+        int i = i0 + (int)(r / nj);
+        int c = d.at(i, j, k);
         const double xc = m.x[i], yc = m.y[j], zc = m.z[k];
-        for (int w = 0; w < wt_count; ++w) {
-            const int* B = box + 6 * w;
-            if (i < B[0] || i >= B[1] || j < B[2] || j >= B[3] || k < B[4] || k >= B[5]) continue;
-            for (int a = w * per_wt; a < (w + 1) * per_wt; ++a) {
-                const double* P = ap + DALM_AP_STRIDE * a;
-                double ex = xc - P[0], ey = yc - P[1], ez = zc - P[2];
-                double r2 = ex * ex + ey * ey + ez * ez;
-                if (r2 > cut2) continue;
-                double wgt = norm * exp(-r2 / w2);
-                sx += wgt * P[8];
-                sy += wgt * P[9];
-                sz += wgt * P[10];
-            }
+        double sx = 0, sy = 0, sz = 0;
+        for (int a = 0; a < napw; ++a) {
+            double ex = xc - P_s[a][0], ey = yc - P_s[a][1], ez = zc - P_s[a][2];
+            double r2 = ex * ex + ey * ey + ez * ez;
+            if (r2 > cut2) continue;
+            double wgt = norm * exp(-r2 / w2);
+            sx += wgt * P_s[a][3];
+            sy += wgt * P_s[a][4];
+            sz += wgt * P_s[a][5];
         }
-        f0[c] = (TF)sx;
-        f1[c] = (TF)sy;
-        f2[c] = (TF)sz;
+        f0[c] = (TF)((double)f0[c] + sx);
+        f1[c] = (TF)((double)f1[c] + sy);
+        f2[c] = (TF)((double)f2[c] + sz);
     }
 }
 
@@ -284,8 +319,6 @@ int alm_run(dalm_ctx* ctx, double t, bool sync_scalars) {
             box[6 * w + 2 * q + 1] = upper_idx(*cs[q], h[q] + R, d.gc, n[q] - d.gc);
         }
     }
-    HIP_TRY(hipMemcpyAsync(A.box_dev, box.data(), sizeof(int) * box.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
     HIP_TRY(hipMemcpyAsync(A.wt, A.h_wt.data(), sizeof(double) * A.h_wt.size(), hipMemcpyHostToDevice,
                            ctx->stream));
     int nb = (A.ap_count + 63) / 64;
@@ -303,19 +336,53 @@ int alm_run(dalm_ctx* ctx, double t, bool sync_scalars) {
                            A.atk, A.atk_count);
     LAUNCH_CHECK();
     int per_wt = A.blade_per_wt * A.ap_per_blade;
-    hipLaunchKernelGGL(k_wt_sum, dim3(1), dim3(64), 0, ctx->stream, A.ap, A.wt_count, per_wt, ctx->scal);
+    hipLaunchKernelGGL(k_wt_sum, dim3(A.wt_count), dim3(256), 0, ctx->stream, A.ap, A.wt_count, per_wt, ctx->scal);
     LAUNCH_CHECK();
     double sp = A.width * sqrt(PI_);
     double norm = 1 / (sp * sp * sp);
     double w2 = A.width * A.width;
     double cut2 = reach * reach;
-    if (ctx->precision == DALM_MIXED)
-        hipLaunchKernelGGL(k_ap_project<float>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, m, A.ap,
-                           A.box_dev, A.wt_count, per_wt, norm, w2, cut2, ctx->f32[0], ctx->f32[1], ctx->f32[2]);
-    else
-        hipLaunchKernelGGL(k_ap_project<double>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, m, A.ap,
-                           A.box_dev, A.wt_count, per_wt, norm, w2, cut2, ctx->f[0], ctx->f[1], ctx->f[2]);
-    LAUNCH_CHECK();
+    const bool mixed = ctx->precision == DALM_MIXED;
+    auto zero_box = [&](const int* B) -> int {
+        long long n = (long long)(B[1] - B[0]) * (B[3] - B[2]) * (B[5] - B[4]);
+        if (n <= 0) return DALM_OK;
+        int nb = (int)std::min<long long>((n + 255) / 256, 2048);
+        if (mixed)
+            hipLaunchKernelGGL(k_box_zero<float>, dim3(nb), dim3(256), 0, ctx->stream, d, B[0], B[1], B[2], B[3],
+                               B[4], B[5], ctx->f32[0], ctx->f32[1], ctx->f32[2]);
+        else
+            hipLaunchKernelGGL(k_box_zero<double>, dim3(nb), dim3(256), 0, ctx->stream, d, B[0], B[1], B[2], B[3],
+                               B[4], B[5], ctx->f[0], ctx->f[1], ctx->f[2]);
+        HIP_TRY(hipGetLastError());
+        return DALM_OK;
+    };
+    // clear the cells of the previous and the current boxes (the whole interior on
+    // the first call), then add every turbine's contribution in AP order
+    if (!A.f_cleared) {
+        int whole[6] = {d.gc, d.n0 - d.gc, d.gc, d.n1 - d.gc, d.gc, d.n2 - d.gc};
+        if (int rc = zero_box(whole)) return rc;
+        A.f_cleared = true;
+    }
+    for (size_t q = 0; q < A.prev_box.size(); q += 6)
+        if (int rc = zero_box(&A.prev_box[q])) return rc;
+    for (int w = 0; w < A.wt_count; ++w)
+        if (int rc = zero_box(&box[6 * w])) return rc;
+    for (int w = 0; w < A.wt_count; ++w) {
+        const int* B = &box[6 * w];
+        long long n = (long long)(B[1] - B[0]) * (B[3] - B[2]) * (B[5] - B[4]);
+        if (n <= 0) continue;
+        int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+        if (mixed)
+            hipLaunchKernelGGL(k_ap_project<float>, dim3(nb), dim3(256), 0, ctx->stream, d, m, A.ap, w * per_wt,
+                               per_wt, B[0], B[1], B[2], B[3], B[4], B[5], norm, w2, cut2, ctx->f32[0], ctx->f32[1],
+                               ctx->f32[2]);
+        else
+            hipLaunchKernelGGL(k_ap_project<double>, dim3(nb), dim3(256), 0, ctx->stream, d, m, A.ap, w * per_wt,
+                               per_wt, B[0], B[1], B[2], B[3], B[4], B[5], norm, w2, cut2, ctx->f[0], ctx->f[1],
+                               ctx->f[2]);
+        LAUNCH_CHECK();
+    }
+    A.prev_box = box;
     if (sync_scalars) {
         HIP_TRY(hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(StepScalars), hipMemcpyDeviceToHost,
                                ctx->stream));

@@ -471,15 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dims d, M1 m, const double* __re
     block_sum<1>(v);
     double bm = block_max(cfl);
     double pv[2] = {v[0], bm};
-    if (publish_partials<2>(pv, partials, counter)) {
-        double s[1];
-        reduce_partials<1>(partials, gridDim.x, s);
-        double mx = reduce_partials_max(partials + gridDim.x, gridDim.x);
-        if (threadIdx.x == 0) {
-            out->avg_div_sq = s[0];
-            out->max_cfl = mx;
-        }
-    }
+    store_partials<2>(pv, partials);
 }
 
 __global__ void k_scalar_sum(Dims d, const double* __restrict__ a, const double* __restrict__ b,
@@ -494,11 +486,7 @@ __global__ void k_scalar_sum(Dims d, const double* __restrict__ a, const double*
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
-    if (publish_partials<1>(pv, partials, counter)) {
-        double s[1];
-        reduce_partials<1>(partials, gridDim.x, s);
-        if (threadIdx.x == 0) *result = s[0];
-    }
+    store_partials<1>(pv, partials);
 }
 
 int grid_for(long long n) {
@@ -602,7 +590,7 @@ int launch_post(dalm_ctx* ctx, double Cs, double dt, bool store_div) {
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
                        store_div ? ctx->div : nullptr, Cs, dt, 1, ctx->partials, ctx->counter, ctx->scal);
     LAUNCH_CHECK();
-    return DALM_OK;
+    return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr, nullptr);
 }
 
 int launch_nut(dalm_ctx* ctx, double Cs) {
@@ -615,7 +603,7 @@ int launch_divergence(dalm_ctx* ctx) {
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
                        ctx->div, 0.0, 0.0, 0, ctx->partials, ctx->counter, ctx->scal);
     LAUNCH_CHECK();
-    return DALM_OK;
+    return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr, nullptr);
 }
 
 int launch_l2(dalm_ctx* ctx, const double* v, double* out_host) {
@@ -623,6 +611,7 @@ int launch_l2(dalm_ctx* ctx, const double* v, double* out_host) {
     hipLaunchKernelGGL(k_scalar_sum, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, v, v,
                        ctx->partials, ctx->counter, res);
     LAUNCH_CHECK();
+    if (int rc = launch_finalize(ctx, FIN_SUM, ctx->grid, nullptr, nullptr, res)) return rc;
     double h = 0;
     HIP_TRY(hipMemcpyAsync(&h, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -635,6 +624,7 @@ int launch_max_cfl(dalm_ctx* ctx, double dt, double* out_host) {
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
                        nullptr, 0.0, dt, 0, ctx->partials, ctx->counter, ctx->scal);
     LAUNCH_CHECK();
+    if (int rc = launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr, nullptr)) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(StepScalars), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

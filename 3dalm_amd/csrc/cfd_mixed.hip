@@ -359,15 +359,7 @@ __global__ __launch_bounds__(BLOCK) void km_post(Dims d, M1 m, MetricF mf, const
     block_sum<1>(v);
     double bm = block_max(cfl);
     double pv[2] = {v[0], bm};
-    if (publish_partials<2>(pv, partials, counter)) {
-        double s[1];
-        reduce_partials<1>(partials, gridDim.x, s);
-        double mx = reduce_partials_max(partials + gridDim.x, gridDim.x);
-        if (threadIdx.x == 0) {
-            out->avg_div_sq = s[0];
-            out->max_cfl = mx;
-        }
-    }
+    store_partials<2>(pv, partials);
 }
 
 int grid_for(long long n) {
@@ -494,5 +486,5 @@ int launch_post_m(dalm_ctx* ctx, double Cs, double dt, bool store_div) {
                        ctx->U32[0], ctx->U32[1], ctx->U32[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut32,
                        store_div ? ctx->div : nullptr, (float)Cs, dt, ctx->partials, ctx->counter, ctx->scal);
     LAUNCH_CHECK();
-    return DALM_OK;
+    return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr, nullptr);
 }

@@ -130,46 +130,28 @@ __device__ __forceinline__ double block_max(double v) {
     return v;
 }
 
-// Device-coherent load/store for cross-block hand-off of partial results.
-template <typename T>
-__device__ __forceinline__ T coherent_load(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void coherent_store(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Last-block-done protocol: every block stores its NV partials, then the block
-// that increments the counter last reduces all partials in block order and
-// returns true (in every thread) so the caller can finish on the device.
+// Per-block partial results: thread 0 stores the block's NV values with plain
+// stores; a separate one-block finalize kernel (reduce.hip) sums them in a fixed
+// order after the producer kernel has ended.  No device-scope fence or atomic is
+// needed inside the producer: with one L2 per XCD such a fence writes the L2 back,
+// which made every reducing kernel several times slower than a plain sweep.
 template <int NV>
-__device__ __forceinline__ bool publish_partials(const double (&v)[NV], double* partials,
-                                                 unsigned* counter) {
-    __shared__ bool last;
+__device__ __forceinline__ void store_partials(const double (&v)[NV], double* partials) {
     if (threadIdx.x == 0) {
-        for (int q = 0; q < NV; ++q) coherent_store(&partials[q * gridDim.x + blockIdx.x], v[q]);
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = v[q];
     }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        if (threadIdx.x == 0) coherent_store(counter, 0u);
-    }
-    return last;
 }
 
-// In the last block: reduce partials[q*nb + b] over b (fixed order).  Result in
-// all threads' out[] after the call.
+// Reduce partials[q*nb + b] over b in a fixed order (one block).  Result in all
+// threads' out[] after the call.
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* partials, int nb, double (&out)[NV]) {
     double v[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double s = 0;
-        for (int b = (int)threadIdx.x; b < nb; b += BLOCK) s += coherent_load(&partials[q * nb + b]);
+        for (int b = (int)threadIdx.x; b < nb; b += BLOCK) s += partials[q * nb + b];
         v[q] = s;
     }
     block_sum<NV>(v);
@@ -185,7 +167,7 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int nb, 
 __device__ __forceinline__ double reduce_partials_max(const double* partials, int nb) {
     double v = 0;
     for (int b = (int)threadIdx.x; b < nb; b += BLOCK) {
-        double w = coherent_load(&partials[b]);
+        double w = partials[b];
         v = w > v ? w : v;
     }
     v = block_max(v);

@@ -92,7 +92,9 @@ struct AlmTables {
     double* atk = nullptr;    // [atk]
     std::vector<double> h_wt; // host copy of turbine records (angles are set on the host)
     std::vector<double> h_rmax; // largest AP radius per turbine
-    int* box_dev = nullptr;   // [wt][6] i0,i1,j0,j1,k0,k1 (half-open) cells any AP can reach
+    int* box_dev = nullptr;   // unused (kept for layout stability)
+    std::vector<int> prev_box; // [wt][6] i0,i1,j0,j1,k0,k1 (half-open) boxes written last step
+    bool f_cleared = false;    // interior force zeroed once before the first projection
 };
 
 // Event timing of every launch of one selected kernel on the context's stream
@@ -138,7 +140,8 @@ struct dalm_ctx {
     float* f32[3] = {};
     float* nut32 = nullptr;
     float *fr = nullptr, *fr0 = nullptr, *fp = nullptr, *fpp = nullptr, *fq = nullptr, *fs = nullptr,
-          *fss = nullptr, *ft = nullptr, *ftmp = nullptr, *ftmp2 = nullptr, *fe = nullptr;
+          *fss = nullptr, *ft = nullptr, *ftmp = nullptr, *ftmp2 = nullptr, *fe = nullptr,
+          *fe_prev = nullptr;
     MetricF mf{};
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
@@ -172,6 +175,11 @@ struct dalm_ctx {
         return c;
     }
 };
+
+// finalize modes of reduce.hip
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_RES64 };
+int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, SolveState* st2, double* out, double eta = 0,
+                    long long inner_max = 0);
 
 int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
 int dalm_fail_msg(dalm_ctx* ctx, int status, const std::string& msg);

@@ -298,9 +298,9 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     for (int q = 0; q < 5; ++q)
         if ((rc = dev_alloc(ctx, sc64[q], L))) return fail(rc);
     if (mixed) {
-        float** scf[12] = {&ctx->nut32, &ctx->fr, &ctx->fr0, &ctx->fp, &ctx->fpp, &ctx->fq,
-                           &ctx->fs, &ctx->fss, &ctx->ft, &ctx->ftmp, &ctx->ftmp2, &ctx->fe};
-        for (int q = 0; q < 12; ++q)
+        float** scf[13] = {&ctx->nut32, &ctx->fr, &ctx->fr0, &ctx->fp, &ctx->fpp, &ctx->fq,
+                           &ctx->fs, &ctx->fss, &ctx->ft, &ctx->ftmp, &ctx->ftmp2, &ctx->fe, &ctx->fe_prev};
+        for (int q = 0; q < 13; ++q)
             if ((rc = dev_alloc(ctx, scf[q], L))) return fail(rc);
     } else {
         double** sc[10] = {&ctx->nut, &ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t,
@@ -347,7 +347,7 @@ int dalm_destroy(dalm_ctx* ctx) {
         if (ctx->f32[q]) (void)hipFree(ctx->f32[q]);
     }
     float* fptrs[] = {ctx->nut32, ctx->fr, ctx->fr0, ctx->fp, ctx->fpp, ctx->fq, ctx->fs, ctx->fss,
-                      ctx->ft, ctx->ftmp, ctx->ftmp2, ctx->fe};
+                      ctx->ft, ctx->ftmp, ctx->ftmp2, ctx->fe, ctx->fe_prev};
     for (float* p : fptrs)
         if (p) (void)hipFree(p);
     for (void* p : ctx->owned_f) (void)hipFree(p);
@@ -380,6 +380,7 @@ int dalm_upload(dalm_ctx* ctx, int64_t field, const double* host) {
             HIP_TRY(hipMemcpy(c[q], tmp.data(), ctx->L * sizeof(double), hipMemcpyHostToDevice));
         }
     }
+    if (field == DALM_F_F) ctx->alm.f_cleared = false;
     if (field == DALM_F_SOLID) {
         ctx->has_solid = false;
         for (long long e = 0; e < ctx->L; ++e)
@@ -544,6 +545,7 @@ int dalm_alm_setup(dalm_ctx* ctx, const double* wt, int64_t wt_count, const doub
     CHECK_CTX();
     if (wt_count < 0 || wt_count > DALM_MAX_TURBINES || ap_count < 0 ||
         ap_count != wt_count * blade_per_wt * ap_per_blade || (ap_count > 0 && atk_count < 1) ||
+        blade_per_wt * ap_per_blade > 1024 ||
         (ap_count > 0 && !(width > 0)))
         return dalm_fail_msg(ctx, DALM_BAD_ARG, "invalid ALM set-up");
     AlmTables& A = ctx->alm;

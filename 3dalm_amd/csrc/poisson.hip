@@ -75,26 +75,7 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Dims d, PCoef A, const double
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
-    if (publish_partials<1>(pv, partials, counter)) {
-        double s[1];
-        reduce_partials<1>(partials, gridDim.x, s);
-        if (threadIdx.x == 0) {
-            double err = sqrt(s[0]) / st->root_n;
-            st->err = err;
-            st->aux0 = s[0];
-            if (count_it) {
-                long long it = st->it + 1;
-                st->it = it;
-                if (!(err == err) || isinf(err)) {
-                    st->done = 1;
-                    st->status = DALM_NONFINITE;
-                } else if (!(it < st->max_it && err > st->tol)) {
-                    st->done = 1;
-                    st->status = err > st->tol ? DALM_NONCONVERGED : DALM_OK;
-                }
-            }
-        }
-    }
+    store_partials<1>(pv, partials);
 }
 
 __global__ void k_state_reset(SolveState* st, double tol, long long max_it, double root_n) {
@@ -132,19 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_init(Dims d, PCoef A, const double
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
-    if (publish_partials<1>(pv, partials, counter)) {
-        double s[1];
-        reduce_partials<1>(partials, gridDim.x, s);
-        if (threadIdx.x == 0) {
-            double rho = s[0];
-            st->rho = rho;
-            if (fabs(rho) < (double)FLT_MIN) {
-                st->err = rho;
-                st->done = 1;
-                st->status = DALM_BREAKDOWN;
-            }
-        }
-    }
+    store_partials<1>(pv, partials);
 }
 
 // p = r (first iteration) or r + beta (p - omega q); then the first
@@ -212,14 +181,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv(Dims d, PCoef A, const double
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    if (publish_partials<2>(pv, partials, counter)) {
-        double s[2];
-        reduce_partials<2>(partials, gridDim.x, s);
-        if (threadIdx.x == 0) {
-            if (WHICH == 0) st->alpha = st->rho / s[0];
-            else st->omega = s[0] / s[1];
-        }
-    }
+    store_partials<2>(pv, partials);
 }
 
 // s = r - alpha q, then the first preconditioner sweep for ss (src/eq.h:373-380)
@@ -267,34 +229,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr(Dims d, double* __restrict__ x,
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    if (publish_partials<2>(pv, partials, counter)) {
-        double sm[2];
-        reduce_partials<2>(partials, gridDim.x, sm);
-        if (threadIdx.x == 0) {
-            double err = sqrt(sm[0]) / st->root_n;
-            long long it = st->it + 1;
-            st->it = it;
-            st->err = err;
-            if (!(err == err) || isinf(err)) {
-                st->done = 1;
-                st->status = DALM_NONFINITE;
-            } else if (it < st->max_it && err > st->tol) {
-                double rho_new = sm[1];
-                if (fabs(rho_new) < (double)FLT_MIN) {
-                    st->err = rho_new;
-                    st->done = 1;
-                    st->status = DALM_BREAKDOWN;
-                } else {
-                    st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
-                    st->rho_prev = st->rho;
-                    st->rho = rho_new;
-                }
-            } else {
-                st->done = 1;
-                st->status = err > st->tol ? DALM_NONCONVERGED : DALM_OK;
-            }
-        }
-    }
+    store_partials<2>(pv, partials);
 }
 
 __global__ void k_zero(long long n, double* __restrict__ a) {
@@ -334,7 +269,7 @@ int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r) 
     hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x,
                        b, r, ctx->partials, ctx->counter, ctx->solve, 0);
     LAUNCH_CHECK();
-    return DALM_OK;
+    return launch_finalize(ctx, FIN_RESID, ctx->grid, ctx->solve, nullptr, nullptr);
 }
 
 __global__ void k_Ax_plain(Dims d, PCoef A, const double* __restrict__ z, double* __restrict__ y) {
@@ -373,6 +308,7 @@ int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long
             hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A,
                                ctx->p, ctx->b, ctx->r, ctx->partials, ctx->counter, ctx->solve, 1);
             probe_end(ctx, DALM_K_RESNORM);
+            if (int rc = launch_finalize(ctx, FIN_SOR, ctx->grid, ctx->solve, nullptr, nullptr)) return rc;
         }
         LAUNCH_CHECK();
         launched += chunk;
@@ -395,6 +331,7 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
     hipLaunchKernelGGL(k_bi_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b,
                        ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
     LAUNCH_CHECK();
+    if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->grid, ctx->solve, nullptr, nullptr)) return rc0;
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     // preconditioner application for one vector; returns the buffer holding z
@@ -436,6 +373,7 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
             hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
                                ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
+            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->grid, ctx->solve, nullptr, nullptr)) return rc1;
             probe_begin(ctx, DALM_K_SUPDATE);
             hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
                                ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
@@ -445,10 +383,12 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
             hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
                                ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
+            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->grid, ctx->solve, nullptr, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
             hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ppz, ssz,
                                ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_UPDATE);
+            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->grid, ctx->solve, nullptr, nullptr)) return rc3;
         }
         LAUNCH_CHECK();
         launched += chunk;

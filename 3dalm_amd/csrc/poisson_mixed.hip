@@ -58,37 +58,7 @@ __global__ __launch_bounds__(BLOCK) void km_res64(Dims d, PCoef A, const double*
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    if (publish_partials<2>(pv, partials, counter)) {
-        double s[2];
-        reduce_partials<2>(partials, gridDim.x, s);
-        if (threadIdx.x == 0) {
-            double err = sqrt(s[0]) / outer->root_n;
-            outer->err = err;
-            outer->it += 1;
-            inner->it = 0;
-            inner->rho = s[1];
-            inner->alpha = inner->omega = 1.0;
-            inner->beta = 0;
-            inner->err = err;
-            inner->root_n = outer->root_n;
-            inner->tol = fmax(0.5 * outer->tol, eta * err);
-            inner->max_it = inner_max;
-            inner->status = DALM_OK;
-            inner->done = 0;
-            if (!(err == err) || isinf(err)) {
-                outer->status = DALM_NONFINITE;
-                outer->done = 1;
-                inner->done = 1;
-            } else if (err <= outer->tol || inner_max <= 0) {
-                outer->done = 1;
-                outer->status = err <= outer->tol ? DALM_OK : DALM_NONCONVERGED;
-                inner->done = 1;
-            } else if (fabs(s[1]) < (double)FLT_MIN) {
-                inner->done = 1;
-                inner->status = DALM_BREAKDOWN;
-            }
-        }
-    }
+    store_partials<2>(pv, partials);
 }
 
 __global__ void km_outer_reset(SolveState* outer, double tol, double root_n) {
@@ -170,14 +140,7 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv(Dims d, PCoefF A, const float* 
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    if (publish_partials<2>(pv, partials, counter)) {
-        double s[2];
-        reduce_partials<2>(partials, gridDim.x, s);
-        if (threadIdx.x == 0) {
-            if (WHICH == 0) st->alpha = st->rho / s[0];
-            else st->omega = s[0] / s[1];
-        }
-    }
+    store_partials<2>(pv, partials);
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_s(Dims d, PCoefF A, const float* __restrict__ r,
@@ -218,43 +181,18 @@ __global__ __launch_bounds__(BLOCK) void kf_xr(Dims d, float* __restrict__ x, co
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    if (publish_partials<2>(pv, partials, counter)) {
-        double sm[2];
-        reduce_partials<2>(partials, gridDim.x, sm);
-        if (threadIdx.x == 0) {
-            double err = sqrt(sm[0]) / st->root_n;
-            long long it = st->it + 1;
-            st->it = it;
-            st->err = err;
-            if (!(err == err) || isinf(err)) {
-                st->done = 1;
-                st->status = DALM_NONFINITE;
-            } else if (it < st->max_it && err > st->tol) {
-                double rho_new = sm[1];
-                if (fabs(rho_new) < (double)FLT_MIN) {
-                    st->done = 1;
-                    st->status = DALM_BREAKDOWN;
-                } else {
-                    st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
-                    st->rho_prev = st->rho;
-                    st->rho = rho_new;
-                }
-            } else {
-                st->done = 1;
-                st->status = err > st->tol ? DALM_NONCONVERGED : DALM_OK;
-            }
-        }
-    }
+    store_partials<2>(pv, partials);
 }
 
-// p += e over the interior (FP64 accumulate of the FP32 correction)
-__global__ __launch_bounds__(BLOCK) void km_add(Dims d, double* __restrict__ p, const float* __restrict__ e) {
+// p += sign * e over the interior (FP64 accumulate of the FP32 correction)
+__global__ __launch_bounds__(BLOCK) void km_add(Dims d, double* __restrict__ p, const float* __restrict__ e,
+                                                double sign) {
     TileRange tr = block_tiles(d);
     for (int t = tr.begin; t < tr.end; ++t) {
         Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         int c = d.at(cc.i, cc.j, cc.k);
-        p[c] += (double)e[c];
+        p[c] += sign * (double)e[c];
     }
 }
 
@@ -277,6 +215,10 @@ long long chunk_for(long long done_it, long long predicted, long long max_it) {
 
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
+// Refinement passes.  Each inner FP32 solve is capped (the attainable FP32
+// residual reduction is limited, and a short inner solve restarted from the true
+// FP64 residual is more robust than a long one); a pass whose correction raises
+// the FP64 residual is undone and the cap halved.
 int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
                       long long* it, double* err) {
     const Dims& d = ctx->d;
@@ -284,13 +226,15 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     const PCoef A64 = ctx->coef();
     SolveState* inner = ctx->solve;
     SolveState* outer = ctx->solve + 1;
-    const double eta = 1e-4;
+    const double eta = 1e-3;
+    long long cap = 120;
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     hipLaunchKernelGGL(km_outer_reset, dim3(1), dim3(64), 0, ctx->stream, outer, tol, sqrt((double)ctx->N));
     LAUNCH_CHECK();
     long long total = 0;
     int status = DALM_OK;
+    double prev_err = -1;
     auto precondition = [&](float* z0, float* spare, const float* v) -> float* {
         float* cur = z0;
         if (jac) {
@@ -312,12 +256,28 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         }
         return cur;
     };
-    for (int pass = 0; pass < 64; ++pass) {
+    for (int pass = 0; pass < 200; ++pass) {
+        long long budget = max_it - total;
+        long long inner_max = budget < cap ? budget : cap;
         hipLaunchKernelGGL(km_res64, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A64, ctx->p, ctx->b, ctx->fr,
-                           ctx->fr0, ctx->fe, ctx->partials, ctx->counter, inner, outer, eta, max_it - total);
+                           ctx->fr0, ctx->fe, ctx->partials, ctx->counter, inner, outer, eta, inner_max);
         LAUNCH_CHECK();
+        if (int rc = launch_finalize(ctx, FIN_RES64, ctx->grid, inner, outer, nullptr, eta, inner_max)) return rc;
         int rc = read_states(ctx);
         if (rc) return rc;
+        const double e64 = ctx->h_solve[1].err;
+        if (pass > 0 && prev_err >= 0 && e64 > prev_err && !ctx->h_solve[1].done) {
+            // the last correction made things worse: undo it, shorten the inner solve
+            hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe_prev, -1.0);
+            LAUNCH_CHECK();
+            cap /= 2;
+            if (cap < 8) {
+                status = DALM_NONCONVERGED;
+                break;
+            }
+            continue;
+        }
+        prev_err = e64;
         if (ctx->h_solve[1].done) {
             status = ctx->h_solve[1].status;
             break;
@@ -326,9 +286,9 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             status = ctx->h_solve[0].status;
             break;
         }
-        long long launched = 0, budget = max_it - total;
-        while (!ctx->h_solve[0].done && launched < budget) {
-            long long chunk = chunk_for(launched, ctx->prev_it[1] - total, budget);
+        long long launched = 0;
+        while (!ctx->h_solve[0].done && launched < inner_max) {
+            long long chunk = chunk_for(launched, ctx->prev_it[1] - total, inner_max);
             for (long long c = 0; c < chunk; ++c) {
                 probe_begin(ctx, DALM_K_PUPDATE);
                 hipLaunchKernelGGL(kf_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->fr, ctx->fp,
@@ -339,6 +299,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 hipLaunchKernelGGL(kf_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz, ctx->fq,
                                    ctx->fr0, ctx->partials, ctx->counter, inner);
                 probe_end(ctx, DALM_K_SPMV);
+                if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->grid, inner, nullptr, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_SUPDATE);
                 hipLaunchKernelGGL(kf_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->fr, ctx->fq,
                                    ctx->fs, ctx->fss, pc_first, (const SolveState*)inner);
@@ -348,10 +309,12 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 hipLaunchKernelGGL(kf_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz, ctx->ft,
                                    ctx->fs, ctx->partials, ctx->counter, inner);
                 probe_end(ctx, DALM_K_SPMV);
+                if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->grid, inner, nullptr, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_UPDATE);
                 hipLaunchKernelGGL(kf_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->fe, ppz, ssz,
                                    ctx->fs, ctx->ft, ctx->fr, ctx->fr0, ctx->partials, ctx->counter, inner);
                 probe_end(ctx, DALM_K_UPDATE);
+                if ((rc = launch_finalize(ctx, FIN_BI_XR, ctx->grid, inner, nullptr, nullptr))) return rc;
             }
             LAUNCH_CHECK();
             launched += chunk;
@@ -359,8 +322,6 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             if (rc) return rc;
         }
         total += ctx->h_solve[0].it;
-        hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe);
-        LAUNCH_CHECK();
         if (ctx->h_solve[0].status == DALM_NONFINITE) {
             status = DALM_NONFINITE;
             break;
@@ -369,6 +330,9 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             status = DALM_BREAKDOWN;
             break;
         }
+        hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe, 1.0);
+        LAUNCH_CHECK();
+        std::swap(ctx->fe, ctx->fe_prev);  // keep the applied correction for a possible undo
     }
     *it = total;
     *err = ctx->h_solve[1].err;

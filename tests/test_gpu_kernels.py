@@ -112,15 +112,17 @@ def test_bicgstab(mesh, ok, pc):
     x = p0.copy()
     work = np.zeros(9 * mesh.L)
     it, err = po.C.c_int64(), po.C.c_double()
+    # converged to 1e-13 so that the solution, not the Krylov path, is compared:
+    # the dot products differ from the reference's serial loop in summation order,
+    # which BiCGStab amplifies into a different iteration count
     po.lib("oracle")[0].or_run_bicgstab(po.C.byref(g), po.ptr(A), po.ptr(x), po.ptr(b), po.ptr(work), pc,
-                                        1.1, 3, po.C.byref(it), 3000, po.C.byref(err), 1e-10)
-    git, gerr, st = c.run_bicgstab(pc, 1.1, 3, 3000, 1e-10)
-    assert st == hip.OK and err.value <= 1e-10 and gerr <= 1e-10
-    # dot products differ from the oracle only in summation order: the iteration
-    # count may move by one when the last residual sits next to the tolerance
-    assert abs(git - it.value) <= 1, (git, it.value)
+                                        1.1, 3, po.C.byref(it), 5000, po.C.byref(err), 1e-13)
+    git, gerr, st = c.run_bicgstab(pc, 1.1, 3, 5000, 1e-13)
+    print(f"bicgstab pc={pc}: it {git} vs {it.value}, err {gerr:.3e} vs {err.value:.3e}", flush=True)
+    assert st == hip.OK and err.value <= 1e-13 and gerr <= 1e-13
+    assert abs(git - it.value) <= max(4, 0.2 * it.value), (git, it.value)
     m = interior_mask(mesh)
-    assert rel_l2(c.download("p")[m], x[m]) <= 1e-8
+    assert rel_l2(c.download("p")[m], x[m]) <= 1e-10
 
 
 def test_pbc_project_ubc(mesh, ok):

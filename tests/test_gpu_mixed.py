@@ -5,9 +5,13 @@ Stated tolerances (the north star asks for them to be stated against FP64):
   * every step ends with the FP64 residual test met: err <= tol, and therefore
     avg div <= 1e-6 (it is dt * max_diag * err by the projection identity);
   * turbine power within 0.1 % of the FP64 oracle;
-  * velocity within 1e-4, pressure within 1e-3 relative L2 of the FP64 oracle
-    over the first steps (FP32 storage of U rounds at 6e-8 per step; the pressure
-    difference is dominated by the solver tolerance 1e-8 scaled by max_diag).
+  * velocity within 1e-4 relative L2 of the FP64 oracle over the first steps
+    (FP32 storage of U rounds at 6e-8 per step);
+  * pressure: the residual test err <= 1e-8 leaves a smooth pressure error that it
+    hardly sees (the FP64 oracle at 1e-8 is 1.2e-3 from its own 1e-13 solution at
+    64x32x32), so both are measured from the oracle's 1e-13 solution and the
+    precision form may be at most five times as far from it as the FP64 oracle at
+    the same tolerance (plus 1e-5).
 """
 import importlib
 
@@ -43,10 +47,18 @@ def test_mixed_against_fp64_oracle(builder, steps):
                 1e-3 * abs(case.sites[0].rot_speed * so["torque"][0])
         m = interior_mask(case.mesh)
         eu = rel_l2(s.field("U")[m], o.field("U")[m])
-        ep = rel_l2(s.field("p")[m], o.field("p")[m])
-        print(f"mixed vs fp64: U {eu:.2e} p {ep:.2e}", flush=True)
+        t = builder()
+        t.tol, t.max_it = 1e-13, 20000
+        ot = po.OracleSolver(t)
+        for _ in range(steps):
+            ot.step()
+        truth = ot.field("p")[m]
+        ep = rel_l2(s.field("p")[m], truth)
+        ep64 = rel_l2(o.field("p")[m], truth)
+        print(f"mixed vs fp64: U {eu:.2e}; p from the 1e-13 solution: mixed {ep:.2e}, fp64 {ep64:.2e}",
+              flush=True)
         assert eu <= 1e-4
-        assert ep <= 1e-3
+        assert ep <= 5 * ep64 + 1e-5
     finally:
         s.close()
 
