@@ -198,6 +198,7 @@ class TurbineSite:
     rot_speed: float
     angle: tuple = (0.0, 0.0, 0.0)  # roll, pitch, yaw: number (deg) or dict {amp,T,phase,offset}
     pitch: float = 0.0              # collective blade pitch, degrees (not in the reference)
+    angle_type: Optional[int] = None  # force the angle type (0 roll, 1 pitch, 2 yaw) even at angle 0
 
     def to_dict(self):
         return {"base": list(self.base), "angle": list(self.angle), "rotation speed": self.rot_speed}
@@ -251,19 +252,47 @@ def case_c2(nx=256, ny=128, nz=128, steps=1000, ap_per_blade=64) -> Case:
                 ap_per_blade=ap_per_blade, width=2 * dx)
 
 
+def case_c4(nx=192, ny=96, nz=96, steps=200, ap_per_blade=64, yaw_deg=(5.0, 0.0, -5.0),
+            pitch_deg=(0.0, 0.0, 0.0)) -> Case:
+    """Differentiation form: three rotors (D=1) in tandem at x = 3, 8, 13 on a 24x8x8
+    box, yaw set explicitly on each (angle type yaw even at 0), collective pitch,
+    TSR 6, eps = 2 dx, BiCGStab(Jacobi 3) to 1e-8."""
+    m = make_mesh(nx, ny, nz, ((0.0, 24.0), (-4.0, 4.0), (0.0, 8.0)), 1.5, 1.5)
+    dx = 24.0 / nx
+    t = default_turbine(R=0.5, tower=4.0)
+    sites = [TurbineSite((x, 0.0, 0.0), 12.0, angle=(0.0, 0.0, yw), pitch=pt, angle_type=2)
+             for x, yw, pt in zip((3.0, 8.0, 13.0), yaw_deg, pitch_deg)]
+    return Case("c4", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000, steps=steps,
+                turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
+
+
+def case_adjoint_small(nx=20, ny=12, nz=12, steps=4, ap_per_blade=6) -> Case:
+    """A few-thousand-cell rotor case for finite-difference checks of the adjoint:
+    one rotor (R = 0.5) at x = 1 on a 4x3x3 box, yaw 10 deg, pitch 2 deg,
+    BiCGStab(SOR 2) converged to 1e-13."""
+    m = make_mesh(nx, ny, nz, ((0.0, 4.0), (-1.5, 1.5), (0.0, 3.0)), 1.3, 1.3)
+    t = default_turbine(R=0.5, tower=1.5)
+    site = TurbineSite((1.0, 0.0, 0.0), 12.0, angle=(0.0, 0.0, 10.0), pitch=2.0, angle_type=2)
+    return Case("adj", m, method="BiCG", pc="SOR", pc_it=2, pc_omega=1.2, tol=1e-13, max_it=5000,
+                steps=steps, turbine=t, sites=[site], ap_per_blade=ap_per_blade, width=4.0 / nx * 2)
+
+
 def turbine_records(turbine: TurbineModel, sites: List[TurbineSite]) -> np.ndarray:
     """Turbine rows (include/dalm_hip.h layout), src/alm.h:144-206: rotation centre
     (-overhang, 0, tower); the angle type is the first non-zero or object entry of
     (roll, pitch, yaw), with formula (0, 0, 0, rad(a)) for a number and
     (rad(amp), 2 pi / T, rad(phase), rad(offset)) for an object.  All-zero angles
-    give type "none" (the reference leaves that case unset)."""
+    give type "none" (the reference leaves that case unset) unless the site names
+    its type explicitly."""
     wt = np.zeros((len(sites), 16))
     for w, s in enumerate(sites):
         wt[w, 0:3] = s.base
         wt[w, 3:6] = (-turbine.overhang, 0.0, turbine.tower)
         wt[w, 6] = 3
         for j, a in enumerate(s.angle):
-            if isinstance(a, dict) or a != 0:
+            if s.angle_type is not None and j != s.angle_type:
+                continue
+            if isinstance(a, dict) or a != 0 or s.angle_type is not None:
                 wt[w, 6] = j
                 if isinstance(a, dict):
                     wt[w, 9:13] = (a["amp"] * math.pi / 180, 2 * math.pi / a["T"],

@@ -98,6 +98,29 @@ void or_calc_ap_force(const or_grid *g, const double *U, double *wt, i64 wt_coun
 void or_distribute_ap_force(const or_grid *g, double *f, const double *ap, i64 ap_count,
                             double width);
 
+/* ---- discrete adjoint (adjoint.c) ---------------------------------------- *
+ * Reverse mode of the routines above; "_b" arrays are bars (sensitivities) and
+ * accumulate (+=) unless stated.  Not in the reference: pinned by central
+ * differences of the forward routines (tests/test_oracle_adjoint.py). */
+void or_predictor_adj(const or_grid *g, const double *Ub, const double *Uold, const double *JU,
+                      const double *nut, double Re, double dt, double *Uold_b, double *JU_b,
+                      double *nut_b, double *f_b);
+void or_interpolate_JU_adj(const or_grid *g, const double *JU_b, double *U_b);
+void or_apply_JUbc_adj(const or_grid *g, double *JU_b);
+void or_poisson_rhs_adj(const or_grid *g, const double *b_b, double *JU_b, double dt, double scale);
+void or_apply_pbc_adj(const or_grid *g, double *p_b);
+void or_project_adj(const or_grid *g, const double *U_b, const double *JU_b, double *p_b, double dt);
+void or_set_solid_adj(const or_grid *g, double *U_b, const double *solid);
+void or_apply_Ubc_adj(const or_grid *g, double *U_b, const double *U_pre, const double *Uold,
+                      double *Uold_b, double dt, i64 retracted_floor, double floor_start_x);
+void or_nut_adj(const or_grid *g, const double *U, const double *nut_b, double *U_b, double Cs);
+void or_alm_adj(const or_grid *g, const double *U, double *wt, i64 wt_count, double *ap,
+                i64 ap_count, i64 blade_per_wt, i64 ap_per_blade, const double *cd_tbl,
+                const double *cl_tbl, const double *atk, i64 atk_count, double width, double t,
+                const double *f_b, const double *tq_w, double *U_b, double *ap_grad);
+void or_poisson_adj(const or_grid *g, const double *A, const double *p_b, double *y, double *b_b,
+                    double *work, double tol_rel, i64 max_it, i64 *it, double *err);
+
 /* ---- whole step ---------------------------------------------------------- */
 typedef struct {
     /* fields (caller-owned, length L or L*3) */
@@ -126,6 +149,11 @@ typedef struct {
 
 void or_state_initialize(const or_grid *g, or_state *s);
 void or_state_step(const or_grid *g, or_state *s);
+/* forward `steps` steps keeping every state, J = sum_n sum_w tq_w[w] torque_w(n),
+ * then the reverse sweep; ap_grad is [ap_count][4], state_b (optional) 10*L bars of
+ * the initial U, JU, nut, f (adjoint.c) */
+void or_adjoint_run(const or_grid *g, or_state *s, i64 steps, const double *tq_w, double adj_tol,
+                    double *J, double *ap_grad, double *state_b);
 
 #ifdef __cplusplus
 }

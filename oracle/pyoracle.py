@@ -98,6 +98,24 @@ def lib(kind="oracle"):
             L.or_build_ap_props.argtypes = [P, P, P, P, P, i64, i64, dbl, dbl, i64, i64, i64, P, P, P]
             L.or_state_initialize.argtypes = [G, C.c_void_p]
             L.or_state_step.argtypes = [G, C.c_void_p]
+            adj = {
+                "predictor_adj": [G, P, P, P, P, dbl, dbl, P, P, P, P],
+                "interpolate_JU_adj": [G, P, P],
+                "apply_JUbc_adj": [G, P],
+                "poisson_rhs_adj": [G, P, P, dbl, dbl],
+                "apply_pbc_adj": [G, P],
+                "project_adj": [G, P, P, P, dbl],
+                "set_solid_adj": [G, P, P],
+                "apply_Ubc_adj": [G, P, P, P, P, dbl, i64, dbl],
+                "nut_adj": [G, P, P, P, dbl],
+                "alm_adj": [G, P, P, i64, P, i64, i64, i64, P, P, P, i64, dbl, dbl, P, P, P, P],
+                "poisson_adj": [G, P, P, P, P, P, dbl, i64, C.POINTER(i64), P],
+            }
+            for name, args in adj.items():
+                f = getattr(L, "or_" + name)
+                f.restype = None
+                f.argtypes = args
+            L.or_adjoint_run.argtypes = [G, C.c_void_p, i64, P, dbl, P, P, P]
         else:
             L.ref_build_mesh.argtypes = [C.c_char_p, P, P, P, P, P, P, C.POINTER(i64), C.POINTER(i64)]
             L.ref_build_ap_props.restype = i64
@@ -185,7 +203,7 @@ class OrState(C.Structure):
 class OracleSolver:
     """Whole-step CPU oracle on a Case (cases.Case)."""
 
-    def __init__(self, case):
+    def __init__(self, case, ap_hook=None):
         self.case = case
         m = case.mesh
         self.g = make_grid(m)
@@ -220,6 +238,8 @@ class OracleSolver:
             s.wt_count, s.ap_count = len(self.wt), len(self.ap)
             s.blade_per_wt, s.ap_per_blade = case.turbine.blades, case.ap_per_blade
             s.atk_count, s.width = len(self.atk), case.width
+            if ap_hook is not None:
+                ap_hook(self.ap, self.wt)
         self.s = s
         self.L, _ = lib("oracle")
         self.L.or_state_initialize(C.byref(self.g), C.byref(self.s))
@@ -227,6 +247,21 @@ class OracleSolver:
     def step(self):
         self.L.or_state_step(C.byref(self.g), C.byref(self.s))
         return self.stats()
+
+    def adjoint(self, steps, tq_w, adj_tol=1e-13):
+        """Run `steps` steps and the reverse sweep (adjoint.c): returns J, the
+        per-AP sensitivities [ap][4] (turbine angle, pitch, chord, twist) and the
+        bars of the initial state {U, JU, nut, f}."""
+        tq = np.ascontiguousarray(tq_w, dtype=np.float64)
+        J = dbl()
+        g = np.zeros((max(len(getattr(self, "ap", [])), 1), 4))
+        L = self.case.mesh.L
+        sb = np.zeros(10 * L)
+        self.L.or_adjoint_run(C.byref(self.g), C.byref(self.s), steps, ptr(tq), adj_tol, C.byref(J), ptr(g),
+                              ptr(sb))
+        bars = {"U": sb[:3 * L].reshape(L, 3), "JU": sb[3 * L:6 * L].reshape(L, 3),
+                "nut": sb[6 * L:7 * L], "f": sb[7 * L:].reshape(L, 3)}
+        return J.value, g, bars
 
     def stats(self):
         s = self.s
