@@ -128,7 +128,9 @@ struct dalm_ctx {
     // Krylov work vectors
     double *r = nullptr, *r0 = nullptr, *kp = nullptr, *pp = nullptr, *q = nullptr, *s = nullptr,
            *ss = nullptr, *t = nullptr, *tmp = nullptr, *tmp2 = nullptr;
-    long long prev_it[2] = {0, 0};  // iterations of the previous SOR / BiCGStab solve
+    // iterations of the previous solve per use: 0 SOR, 1 BiCGStab, 2 the FP64 finish of
+    // the precision form, 3 the adjoint solve (sizes the chunks enqueued per host check)
+    long long prev_it[4] = {0, 0, 0, 0};
 
     PoissonCoef pc{};
     bool have_A = false;
@@ -146,6 +148,7 @@ struct dalm_ctx {
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
     long long outer_it = 0;       // FP64 refinement passes of the last mixed solve
+    long long fp64_finish_it = 0; // FP64 BiCGStab iterations that finished it (0: FP32 passes sufficed)
 
     // reduction scratch
     double* partials = nullptr;  // [8][grid]
@@ -203,6 +206,12 @@ int launch_max_cfl(dalm_ctx* ctx, double dt, double* out_host);
 int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long* it, double* err);
 int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
                  long long* it, double* err);
+// the same solvers on any (x, b) pair; `slot` indexes prev_it
+int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega, long long max_it, double tol,
+               long long* it, double* err);
+int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_kind, double pc_omega, int pc_it,
+                    long long max_it, double tol, long long* it, double* err);
+int ensure_fp64_krylov(dalm_ctx* ctx);
 int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r);
 int launch_Ax(dalm_ctx* ctx, const double* x, double* y);
 int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color);

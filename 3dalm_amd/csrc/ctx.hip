@@ -189,6 +189,16 @@ int read_scalars(dalm_ctx* ctx) {
 
 }  // namespace
 
+// FP64 Krylov vectors, allocated on first use in a precision-form context (its FP64
+// finish) -- an FP64 context has them from creation.
+int ensure_fp64_krylov(dalm_ctx* ctx) {
+    double** sc[9] = {&ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t, &ctx->tmp, &ctx->tmp2};
+    for (int q = 0; q < 9; ++q)
+        if (!*sc[q])
+            if (int rc = dev_alloc(ctx, sc[q], ctx->L)) return rc;
+    return DALM_OK;
+}
+
 #define CHECK_CTX()                             \
     do {                                        \
         if (!ctx) return DALM_BAD_ARG;          \

@@ -289,24 +289,25 @@ int launch_Ax(dalm_ctx* ctx, const double* x, double* y) {
     return DALM_OK;
 }
 
-int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long* it, double* err) {
+int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega, long long max_it, double tol,
+               long long* it, double* err) {
     const double root_n = sqrt((double)ctx->N);
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it, root_n);
     LAUNCH_CHECK();
     PCoef A = ctx->coef();
     long long launched = 0;
     for (;;) {
-        long long chunk = next_chunk(launched, ctx->prev_it[0], max_it);
+        long long chunk = next_chunk(launched, ctx->prev_it[slot], max_it);
         for (long long c = 0; c < chunk; ++c) {
             for (int color = 0; color < 2; ++color) {
                 probe_begin(ctx, DALM_K_SOR);
-                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, ctx->p,
-                                   ctx->b, omega, color, ctx->solve);
+                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b, omega,
+                                   color, ctx->solve);
                 probe_end(ctx, DALM_K_SOR);
             }
             probe_begin(ctx, DALM_K_RESNORM);
-            hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A,
-                               ctx->p, ctx->b, ctx->r, ctx->partials, ctx->counter, ctx->solve, 1);
+            hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b,
+                               ctx->r, ctx->partials, ctx->counter, ctx->solve, 1);
             probe_end(ctx, DALM_K_RESNORM);
             if (int rc = launch_finalize(ctx, FIN_SOR, ctx->grid, ctx->solve, nullptr, nullptr)) return rc;
         }
@@ -318,18 +319,22 @@ int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long
     }
     *it = ctx->h_solve->it;
     *err = ctx->h_solve->err;
-    ctx->prev_it[0] = *it;
+    ctx->prev_it[slot] = *it;
     return ctx->h_solve->status;
 }
 
-int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
-                 long long* it, double* err) {
+int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long* it, double* err) {
+    return run_sor_on(ctx, ctx->p, ctx->b, 0, omega, max_it, tol, it, err);
+}
+
+int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_kind, double pc_omega, int pc_it,
+                    long long max_it, double tol, long long* it, double* err) {
     const double root_n = sqrt((double)ctx->N);
     PCoef A = ctx->coef();
     const Dims& d = ctx->d;
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it, root_n);
-    hipLaunchKernelGGL(k_bi_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b,
-                       ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+    hipLaunchKernelGGL(k_bi_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, x, b, ctx->r, ctx->r0,
+                       ctx->partials, ctx->counter, ctx->solve);
     LAUNCH_CHECK();
     if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->grid, ctx->solve, nullptr, nullptr)) return rc0;
     const int jac = pc_kind == DALM_PC_JACOBI;
@@ -362,7 +367,7 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
     int rc = read_state(ctx);
     if (rc) return rc;
     while (!ctx->h_solve->done && launched < max_it) {
-        long long chunk = next_chunk(launched, ctx->prev_it[1], max_it);
+        long long chunk = next_chunk(launched, ctx->prev_it[slot], max_it);
         for (long long c = 0; c < chunk; ++c) {
             probe_begin(ctx, DALM_K_PUPDATE);
             hipLaunchKernelGGL(k_bi_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
@@ -385,7 +390,7 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
             probe_end(ctx, DALM_K_SPMV);
             if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->grid, ctx->solve, nullptr, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
-            hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ppz, ssz,
+            hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
                                ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_UPDATE);
             if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->grid, ctx->solve, nullptr, nullptr)) return rc3;
@@ -397,8 +402,13 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
     }
     *it = ctx->h_solve->it;
     *err = ctx->h_solve->err;
-    ctx->prev_it[1] = *it;
+    ctx->prev_it[slot] = *it;
     return ctx->h_solve->status;
+}
+
+int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
+                 long long* it, double* err) {
+    return run_bicgstab_on(ctx, ctx->p, ctx->b, 1, pc_kind, pc_omega, pc_it, max_it, tol, it, err);
 }
 
 int zero_device(dalm_ctx* ctx, double* a, long long n) {

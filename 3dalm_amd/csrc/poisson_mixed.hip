@@ -215,10 +215,13 @@ long long chunk_for(long long done_it, long long predicted, long long max_it) {
 
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
-// Refinement passes.  Each inner FP32 solve is capped (the attainable FP32
-// residual reduction is limited, and a short inner solve restarted from the true
-// FP64 residual is more robust than a long one); a pass whose correction raises
-// the FP64 residual is undone and the cap halved.
+// Refinement passes: each pass computes the FP64 residual of the current p, solves
+// A e = r in FP32 (BiCGStab with FP64 dot products) down to eta times that residual,
+// and adds e to p in FP64.  The FP32 inner solve can only reach a relative residual
+// of about u_32 * cond(A); when a pass no longer halves the FP64 residual (the
+// accuracy asked for is beyond FP32), the solve is finished by the FP64 BiCGStab
+// from the current p, so the stopping test err <= tol is always met as in the FP64
+// form.  A pass that raises the residual is undone first.
 int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
                       long long* it, double* err) {
     const Dims& d = ctx->d;
@@ -226,8 +229,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     const PCoef A64 = ctx->coef();
     SolveState* inner = ctx->solve;
     SolveState* outer = ctx->solve + 1;
-    const double eta = 1e-3;
-    long long cap = 120;
+    const double eta = 2e-2;
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     hipLaunchKernelGGL(km_outer_reset, dim3(1), dim3(64), 0, ctx->stream, outer, tol, sqrt((double)ctx->N));
@@ -235,6 +237,8 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     long long total = 0;
     int status = DALM_OK;
     double prev_err = -1;
+    bool finish64 = false;
+    ctx->fp64_finish_it = 0;
     auto precondition = [&](float* z0, float* spare, const float* v) -> float* {
         float* cur = z0;
         if (jac) {
@@ -257,8 +261,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         return cur;
     };
     for (int pass = 0; pass < 200; ++pass) {
-        long long budget = max_it - total;
-        long long inner_max = budget < cap ? budget : cap;
+        long long inner_max = max_it - total;
         hipLaunchKernelGGL(km_res64, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A64, ctx->p, ctx->b, ctx->fr,
                            ctx->fr0, ctx->fe, ctx->partials, ctx->counter, inner, outer, eta, inner_max);
         LAUNCH_CHECK();
@@ -266,24 +269,23 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         int rc = read_states(ctx);
         if (rc) return rc;
         const double e64 = ctx->h_solve[1].err;
-        if (pass > 0 && prev_err >= 0 && e64 > prev_err && !ctx->h_solve[1].done) {
-            // the last correction made things worse: undo it, shorten the inner solve
-            hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe_prev, -1.0);
-            LAUNCH_CHECK();
-            cap /= 2;
-            if (cap < 8) {
-                status = DALM_NONCONVERGED;
-                break;
-            }
-            continue;
-        }
-        prev_err = e64;
         if (ctx->h_solve[1].done) {
             status = ctx->h_solve[1].status;
             break;
         }
+        if (pass > 0 && e64 > 0.5 * prev_err) {
+            // FP32 no longer makes progress at this accuracy
+            if (e64 > prev_err) {
+                hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe_prev,
+                                   -1.0);
+                LAUNCH_CHECK();
+            }
+            finish64 = true;
+            break;
+        }
+        prev_err = e64;
         if (ctx->h_solve[0].done) {  // breakdown at the start of the inner solve
-            status = ctx->h_solve[0].status;
+            finish64 = true;
             break;
         }
         long long launched = 0;
@@ -326,17 +328,32 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             status = DALM_NONFINITE;
             break;
         }
-        if (ctx->h_solve[0].it == 0) {  // no progress possible
-            status = DALM_BREAKDOWN;
+        if (ctx->h_solve[0].it == 0) {  // no progress possible in FP32
+            finish64 = true;
             break;
         }
         hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe, 1.0);
         LAUNCH_CHECK();
         std::swap(ctx->fe, ctx->fe_prev);  // keep the applied correction for a possible undo
+        if (total >= max_it) {
+            status = DALM_NONCONVERGED;
+            break;
+        }
     }
-    *it = total;
-    *err = ctx->h_solve[1].err;
     ctx->outer_it = ctx->h_solve[1].it;
     ctx->prev_it[1] = total;
+    double e_out = ctx->h_solve[1].err;
+    if (finish64 && total < max_it) {
+        if (int rc = ensure_fp64_krylov(ctx)) return rc;
+        long long it64 = 0;
+        status = run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
+        if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
+        ctx->fp64_finish_it = it64;
+        total += it64;
+    } else if (finish64) {
+        status = DALM_NONCONVERGED;
+    }
+    *it = total;
+    *err = e_out;
     return status;
 }

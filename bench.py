@@ -239,11 +239,14 @@ def main():
     cpu = None
     if not a.no_cpu:
         try:
-            cpu = cpu_baseline(make_case(grid, 1), R["it_mean"])
+            # the reference solves in FP64: extrapolate with the FP64 form's iterations
+            it_ref = results["fp64"]["it_mean"] if "fp64" in results else R["it_mean"]
+            cpu = cpu_baseline(make_case(grid, 1), it_ref)
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
     forms_out = {f: {"mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
                      "it_per_step": round(r["it_mean"], 2), "avg_div_max": r["avg_div_max"],
+                     "div_within_1e-6": bool(r["avg_div_max"] <= 1e-6),
                      "power_last": r["power_last"]} for f, r in results.items()}
     line = {
         "metric": METRIC,
