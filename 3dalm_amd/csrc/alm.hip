@@ -299,12 +299,52 @@ void hub(const double* T, double out[3]) {
 
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
-int alm_run(dalm_ctx* ctx, double t, bool sync_scalars) {
+// AP positions and forces at time t (src/alm.h:233-414) from an FP64 velocity
+int alm_forces(dalm_ctx* ctx, double t, double* const U[3]) {
     AlmTables& A = ctx->alm;
     if (A.ap_count <= 0) return DALM_OK;
     turbine_angles(A, t);
-    // bounding boxes: sphere of radius max r around each hub, plus the cutoff
+    HIP_TRY(hipMemcpyAsync(A.wt, A.h_wt.data(), sizeof(double) * A.h_wt.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    int nb = (A.ap_count + 63) / 64;
+    hipLaunchKernelGGL(k_ap_position, dim3(nb), dim3(64), 0, ctx->stream, A.wt, A.ap, A.ap_count,
+                       A.blade_per_wt, A.ap_per_blade, t);
+    LAUNCH_CHECK();
+    M1 m{ctx->x, ctx->y, ctx->z};
+    hipLaunchKernelGGL(k_ap_force<double>, dim3(nb), dim3(64), 0, ctx->stream, ctx->d, m, U[0], U[1], U[2], A.wt,
+                       A.ap, A.ap_count, A.blade_per_wt, A.ap_per_blade, A.cd, A.cl, A.atk, A.atk_count);
+    LAUNCH_CHECK();
+    int per_wt = A.blade_per_wt * A.ap_per_blade;
+    hipLaunchKernelGGL(k_wt_sum, dim3(A.wt_count), dim3(256), 0, ctx->stream, A.ap, A.wt_count, per_wt, ctx->scal);
+    LAUNCH_CHECK();
+    return DALM_OK;
+}
+
+int alm_run(dalm_ctx* ctx, double t, bool sync_scalars) {
+    AlmTables& A = ctx->alm;
+    if (A.ap_count <= 0) return DALM_OK;
     const Dims& d = ctx->d;
+    M1 m{ctx->x, ctx->y, ctx->z};
+    if (ctx->precision == DALM_MIXED) {
+        turbine_angles(A, t);
+        HIP_TRY(hipMemcpyAsync(A.wt, A.h_wt.data(), sizeof(double) * A.h_wt.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        int nb = (A.ap_count + 63) / 64;
+        hipLaunchKernelGGL(k_ap_position, dim3(nb), dim3(64), 0, ctx->stream, A.wt, A.ap, A.ap_count,
+                           A.blade_per_wt, A.ap_per_blade, t);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_ap_force<float>, dim3(nb), dim3(64), 0, ctx->stream, d, m, ctx->U32[0],
+                           ctx->U32[1], ctx->U32[2], A.wt, A.ap, A.ap_count, A.blade_per_wt, A.ap_per_blade,
+                           A.cd, A.cl, A.atk, A.atk_count);
+        LAUNCH_CHECK();
+        int per_wt = A.blade_per_wt * A.ap_per_blade;
+        hipLaunchKernelGGL(k_wt_sum, dim3(A.wt_count), dim3(256), 0, ctx->stream, A.ap, A.wt_count, per_wt,
+                           ctx->scal);
+        LAUNCH_CHECK();
+    } else {
+        if (int rc = alm_forces(ctx, t, ctx->U)) return rc;
+    }
+    // bounding boxes: sphere of radius max r around each hub, plus the cutoff
     const double reach = A.width * CUT;
     std::vector<int> box(6 * A.wt_count);
     for (int w = 0; w < A.wt_count; ++w) {
@@ -319,25 +359,7 @@ int alm_run(dalm_ctx* ctx, double t, bool sync_scalars) {
             box[6 * w + 2 * q + 1] = upper_idx(*cs[q], h[q] + R, d.gc, n[q] - d.gc);
         }
     }
-    HIP_TRY(hipMemcpyAsync(A.wt, A.h_wt.data(), sizeof(double) * A.h_wt.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
-    int nb = (A.ap_count + 63) / 64;
-    hipLaunchKernelGGL(k_ap_position, dim3(nb), dim3(64), 0, ctx->stream, A.wt, A.ap, A.ap_count,
-                       A.blade_per_wt, A.ap_per_blade, t);
-    LAUNCH_CHECK();
-    M1 m{ctx->x, ctx->y, ctx->z};
-    if (ctx->precision == DALM_MIXED)
-        hipLaunchKernelGGL(k_ap_force<float>, dim3(nb), dim3(64), 0, ctx->stream, d, m, ctx->U32[0],
-                           ctx->U32[1], ctx->U32[2], A.wt, A.ap, A.ap_count, A.blade_per_wt, A.ap_per_blade,
-                           A.cd, A.cl, A.atk, A.atk_count);
-    else
-        hipLaunchKernelGGL(k_ap_force<double>, dim3(nb), dim3(64), 0, ctx->stream, d, m, ctx->U[0], ctx->U[1],
-                           ctx->U[2], A.wt, A.ap, A.ap_count, A.blade_per_wt, A.ap_per_blade, A.cd, A.cl,
-                           A.atk, A.atk_count);
-    LAUNCH_CHECK();
     int per_wt = A.blade_per_wt * A.ap_per_blade;
-    hipLaunchKernelGGL(k_wt_sum, dim3(A.wt_count), dim3(256), 0, ctx->stream, A.ap, A.wt_count, per_wt, ctx->scal);
-    LAUNCH_CHECK();
     double sp = A.width * sqrt(PI_);
     double norm = 1 / (sp * sp * sp);
     double w2 = A.width * A.width;

@@ -590,7 +590,7 @@ int launch_post(dalm_ctx* ctx, double Cs, double dt, bool store_div) {
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
                        store_div ? ctx->div : nullptr, Cs, dt, 1, ctx->partials, ctx->counter, ctx->scal);
     LAUNCH_CHECK();
-    return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr, nullptr);
+    return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr);
 }
 
 int launch_nut(dalm_ctx* ctx, double Cs) {
@@ -603,7 +603,7 @@ int launch_divergence(dalm_ctx* ctx) {
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
                        ctx->div, 0.0, 0.0, 0, ctx->partials, ctx->counter, ctx->scal);
     LAUNCH_CHECK();
-    return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr, nullptr);
+    return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr);
 }
 
 int launch_l2(dalm_ctx* ctx, const double* v, double* out_host) {
@@ -611,7 +611,7 @@ int launch_l2(dalm_ctx* ctx, const double* v, double* out_host) {
     hipLaunchKernelGGL(k_scalar_sum, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, v, v,
                        ctx->partials, ctx->counter, res);
     LAUNCH_CHECK();
-    if (int rc = launch_finalize(ctx, FIN_SUM, ctx->grid, nullptr, nullptr, res)) return rc;
+    if (int rc = launch_finalize(ctx, FIN_SUM, ctx->grid, nullptr, res)) return rc;
     double h = 0;
     HIP_TRY(hipMemcpyAsync(&h, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -624,7 +624,7 @@ int launch_max_cfl(dalm_ctx* ctx, double dt, double* out_host) {
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
                        nullptr, 0.0, dt, 0, ctx->partials, ctx->counter, ctx->scal);
     LAUNCH_CHECK();
-    if (int rc = launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr, nullptr)) return rc;
+    if (int rc = launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr)) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(StepScalars), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -80,6 +80,7 @@ struct MetricF {
 struct PCoefF {
     const float *ae, *aw, *an, *as, *at, *ab;  // normalised off-diagonals
     const float *acx, *anu, *asu, *atu, *abu;  // normalised diagonal pieces
+    const float* adir;                         // [n0] row sum: -2/dx^2/max_diag at the x+ row, else 0
 };
 
 struct AlmTables {
@@ -107,6 +108,36 @@ struct KernelProbe {
     long long launches = 0;
 };
 
+// Discrete adjoint of the FP64 step (adjoint.hip): bars of the state, the
+// checkpoint / pressure tape of a recorded forward run, and its objective.
+struct AdjointState {
+    bool have_buffers = false;
+    // bars of the step's state; "n" arrays receive the bars of the previous step
+    double *Ub[3] = {}, *Ubn[3] = {}, *JUb[3] = {}, *JUbn[3] = {}, *fb[3] = {}, *fbn[3] = {};
+    double *nutb = nullptr, *nutbn = nullptr;
+    double *pb = nullptr;   // bar of p (interior and first ghost layer)
+    double *ab = nullptr;   // adjoint Poisson right-hand side V^-1 p_b, then bar of b
+    double *ya = nullptr;   // adjoint Poisson solution (warm start for the next step)
+    double *xp = nullptr;   // [4][3][ny*nz] x+ outflow bars of the previous velocity
+    double *apred = nullptr;  // [ap][6] per-AP projection sums
+    double *apg = nullptr;    // [ap][4] dJ/d(turbine angle, pitch, chord, twist) per AP
+    int ap_cap = 0;           // APs the two arrays above hold
+    double *tqw = nullptr;    // [wt] objective weights on the torques
+    // recorded forward run
+    bool taping = false;
+    long long steps = 0, every = 0, recorded = 0, start_step = 0;
+    bool keep_p = false;
+    double* ck = nullptr;     // checkpoints [steps/every + 1][11][L]: U, JU, p, nut, f
+    double* ptape = nullptr;  // pressure after each step [steps][L] (keep_p)
+    double* seg = nullptr;    // states of one segment [every + 1][7][L]: U, JU, nut
+    double* fin = nullptr;    // final state [11][L], restored after the reverse sweep
+    std::vector<double> h_tqw;
+    double J = 0;
+    // statistics of the last reverse sweep
+    long long adj_it = 0;
+    double replay_ms = 0, reverse_ms = 0;
+};
+
 struct dalm_ctx {
     Dims d;
     long long L = 0, N = 0;
@@ -128,27 +159,26 @@ struct dalm_ctx {
     // Krylov work vectors
     double *r = nullptr, *r0 = nullptr, *kp = nullptr, *pp = nullptr, *q = nullptr, *s = nullptr,
            *ss = nullptr, *t = nullptr, *tmp = nullptr, *tmp2 = nullptr;
-    // iterations of the previous solve per use: 0 SOR, 1 BiCGStab, 2 the FP64 finish of
-    // the precision form, 3 the adjoint solve (sizes the chunks enqueued per host check)
+    // iterations of the previous solve per use: 0 SOR, 1 BiCGStab (either form),
+    // 3 the adjoint solve (sizes the chunks enqueued per host check)
     long long prev_it[4] = {0, 0, 0, 0};
 
     PoissonCoef pc{};
     bool have_A = false;
 
-    // precision form (DALM_MIXED): FP32 cell fields and inner Krylov vectors;
-    // JU, p, b, div stay FP64
+    // precision form (DALM_MIXED): FP32 cell fields and preconditioner vectors
+    // (shadow residual, copies of the preconditioned vectors, Jacobi iterates);
+    // JU, p, b, div and the BiCGStab recurrence vectors stay FP64
     float* U32[3] = {};
     float* Uold32[3] = {};
     float* f32[3] = {};
     float* nut32 = nullptr;
-    float *fr = nullptr, *fr0 = nullptr, *fp = nullptr, *fpp = nullptr, *fq = nullptr, *fs = nullptr,
-          *fss = nullptr, *ft = nullptr, *ftmp = nullptr, *ftmp2 = nullptr, *fe = nullptr,
-          *fe_prev = nullptr;
+    float *fr0 = nullptr, *fp = nullptr, *fpp = nullptr, *fs = nullptr, *fss = nullptr, *ftmp = nullptr,
+          *ftmp2 = nullptr;
     MetricF mf{};
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
-    long long outer_it = 0;       // FP64 refinement passes of the last mixed solve
-    long long fp64_finish_it = 0; // FP64 BiCGStab iterations that finished it (0: FP32 passes sufficed)
+    long long outer_it = 0;       // FP64 true-residual passes of the last mixed solve
 
     // reduction scratch
     double* partials = nullptr;  // [8][grid]
@@ -166,6 +196,7 @@ struct dalm_ctx {
 
     AlmTables alm;
     KernelProbe probe;
+    AdjointState adj;
 
     std::string err;
     int hip_err = 0;
@@ -180,9 +211,8 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_RES64 };
-int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, SolveState* st2, double* out, double eta = 0,
-                    long long inner_max = 0);
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_HX_INIT };
+int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
 
 int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
 int dalm_fail_msg(dalm_ctx* ctx, int status, const std::string& msg);
@@ -211,11 +241,18 @@ int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega
                long long* it, double* err);
 int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_kind, double pc_omega, int pc_it,
                     long long max_it, double tol, long long* it, double* err);
-int ensure_fp64_krylov(dalm_ctx* ctx);
 int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r);
 int launch_Ax(dalm_ctx* ctx, const double* x, double* y);
 int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color);
 int alm_run(dalm_ctx* ctx, double t, bool sync_scalars);
+// AP positions and forces at time t from the FP64 velocity U (no projection)
+int alm_forces(dalm_ctx* ctx, double t, double* const U[3]);
+// one FP64 step; with p_given the Poisson solve is replaced by copying p_given
+int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given);
+// adjoint.hip: record a forward step on the tape, free the adjoint memory
+int adjoint_record(dalm_ctx* ctx);
+void adjoint_free(dalm_ctx* ctx);
+int ensure_adjoint_buffers(dalm_ctx* ctx);
 int sync_stream(dalm_ctx* ctx);
 int zero_device(dalm_ctx* ctx, double* a, long long n);
 void probe_begin(dalm_ctx* ctx, int kernel);

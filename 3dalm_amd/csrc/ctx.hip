@@ -157,7 +157,32 @@ int field_ptrs(dalm_ctx* ctx, int64_t field, void** c, int* ncomp, bool* fp32) {
     case DALM_F_DIV: c[0] = ctx->div; break;
     case DALM_F_SOLID: c[0] = ctx->solid; break;
     case DALM_F_B: c[0] = ctx->b; break;
-    default: return DALM_BAD_ARG;
+    default: {
+        if (field < DALM_F_U_B || field > DALM_F_B_B) return DALM_BAD_ARG;
+        if (int rc = ensure_adjoint_buffers(ctx)) return rc;
+        AdjointState& S = ctx->adj;
+        double* const* v3 = nullptr;
+        double* one = nullptr;
+        switch (field) {
+        case DALM_F_U_B: v3 = S.Ub; break;
+        case DALM_F_UOLD_B: v3 = S.Ubn; break;
+        case DALM_F_JU_B: v3 = S.JUb; break;
+        case DALM_F_JUOLD_B: v3 = S.JUbn; break;
+        case DALM_F_F_B: v3 = S.fb; break;
+        case DALM_F_FOLD_B: v3 = S.fbn; break;
+        case DALM_F_NUT_B: one = S.nutb; break;
+        case DALM_F_NUTOLD_B: one = S.nutbn; break;
+        case DALM_F_P_B: one = S.pb; break;
+        default: one = S.ab; break;
+        }
+        if (v3) {
+            *ncomp = 3;
+            for (int q = 0; q < 3; ++q) c[q] = v3[q];
+        } else {
+            c[0] = one;
+        }
+        break;
+    }
     }
     return DALM_OK;
 }
@@ -189,16 +214,6 @@ int read_scalars(dalm_ctx* ctx) {
 
 }  // namespace
 
-// FP64 Krylov vectors, allocated on first use in a precision-form context (its FP64
-// finish) -- an FP64 context has them from creation.
-int ensure_fp64_krylov(dalm_ctx* ctx) {
-    double** sc[9] = {&ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t, &ctx->tmp, &ctx->tmp2};
-    for (int q = 0; q < 9; ++q)
-        if (!*sc[q])
-            if (int rc = dev_alloc(ctx, sc[q], ctx->L)) return rc;
-    return DALM_OK;
-}
-
 #define CHECK_CTX()                             \
     do {                                        \
         if (!ctx) return DALM_BAD_ARG;          \
@@ -214,6 +229,45 @@ int ensure_fp64_krylov(dalm_ctx* ctx) {
         int rc_ = (x);      \
         if (rc_) return rc_; \
     } while (0)
+
+// One FP64 step in Solver::main_loop_once order (src/solver.h:1042-1186).  With
+// p_given the Poisson solve is replaced by a copy of p_given (replay of a recorded
+// step during the reverse sweep).
+int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given) {
+    const dalm_params& P = ctx->prm;
+    ctx->step += 1;
+    const double t = P.dt * (double)ctx->step;
+    // Uold <- U: swap the buffers, then restore U's ghost shell from Uold
+    for (int q = 0; q < 3; ++q) std::swap(ctx->U[q], ctx->Uold[q]);
+    TRY(launch_ghost_copy(ctx));
+    TRY(launch_predictor(ctx, P.Re, P.dt));
+    TRY(launch_flux(ctx, P.Uin, true));
+    long long it = 0;
+    double err = 0;
+    int status = DALM_OK;
+    if (p_given) {
+        HIP_TRY(hipMemcpyAsync(ctx->p, p_given, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        TRY(launch_rhs(ctx, P.dt, ctx->pc.max_diag));
+        if (P.method == DALM_BICGSTAB)
+            status = run_bicgstab(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+        else
+            status = run_sor(ctx, P.omega, P.max_it, P.tol, &it, &err);
+        if (status == DALM_HIP_ERROR) return status;
+    }
+    TRY(launch_pbc(ctx));
+    TRY(launch_project(ctx, P.dt, P.Uin));
+    if (ctx->has_solid) TRY(launch_solid(ctx));
+    TRY(launch_ubc(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
+    TRY(launch_post(ctx, P.Cs, P.dt, true));
+    TRY(alm_run(ctx, t, false));
+    TRY(read_scalars(ctx));
+    TRY(probe_collect(ctx));
+    dalm_step_stats st;
+    fill_stats(ctx, &st, it, err, status);
+    if (stats) *stats = st;
+    return status == DALM_NONCONVERGED ? DALM_OK : status;
+}
 
 extern "C" {
 
@@ -308,10 +362,13 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     for (int q = 0; q < 5; ++q)
         if ((rc = dev_alloc(ctx, sc64[q], L))) return fail(rc);
     if (mixed) {
-        float** scf[13] = {&ctx->nut32, &ctx->fr, &ctx->fr0, &ctx->fp, &ctx->fpp, &ctx->fq,
-                           &ctx->fs, &ctx->fss, &ctx->ft, &ctx->ftmp, &ctx->ftmp2, &ctx->fe, &ctx->fe_prev};
-        for (int q = 0; q < 13; ++q)
+        // FP32 preconditioner vectors and the FP64 BiCGStab recurrence vectors
+        float** scf[8] = {&ctx->nut32, &ctx->fr0, &ctx->fp, &ctx->fpp, &ctx->fs, &ctx->fss, &ctx->ftmp, &ctx->ftmp2};
+        for (int q = 0; q < 8; ++q)
             if ((rc = dev_alloc(ctx, scf[q], L))) return fail(rc);
+        double** sd[4] = {&ctx->kp, &ctx->q, &ctx->s, &ctx->t};
+        for (int q = 0; q < 4; ++q)
+            if ((rc = dev_alloc(ctx, sd[q], L))) return fail(rc);
     } else {
         double** sc[10] = {&ctx->nut, &ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t,
                            &ctx->tmp, &ctx->tmp2};
@@ -336,6 +393,7 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
 int dalm_destroy(dalm_ctx* ctx) {
     if (!ctx) return DALM_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    adjoint_free(ctx);
     for (hipEvent_t e : ctx->probe.ev) (void)hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
@@ -356,8 +414,7 @@ int dalm_destroy(dalm_ctx* ctx) {
         if (ctx->Uold32[q]) (void)hipFree(ctx->Uold32[q]);
         if (ctx->f32[q]) (void)hipFree(ctx->f32[q]);
     }
-    float* fptrs[] = {ctx->nut32, ctx->fr, ctx->fr0, ctx->fp, ctx->fpp, ctx->fq, ctx->fs, ctx->fss,
-                      ctx->ft, ctx->ftmp, ctx->ftmp2, ctx->fe, ctx->fe_prev};
+    float* fptrs[] = {ctx->nut32, ctx->fr0, ctx->fp, ctx->fpp, ctx->fs, ctx->fss, ctx->ftmp, ctx->ftmp2};
     for (float* p : fptrs)
         if (p) (void)hipFree(p);
     for (void* p : ctx->owned_f) (void)hipFree(p);
@@ -671,36 +728,14 @@ static int step_mixed(dalm_ctx* ctx, double t, dalm_step_stats* stats) {
 int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
     CHECK_CTX();
     if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
-    const dalm_params& P = ctx->prm;
-    ctx->step += 1;
-    const double t = P.dt * (double)ctx->step;
-    if (ctx->precision == DALM_MIXED) return step_mixed(ctx, t, stats);
-    // Uold <- U: swap the buffers, then restore U's ghost shell from Uold
-    for (int q = 0; q < 3; ++q) std::swap(ctx->U[q], ctx->Uold[q]);
-    TRY(launch_ghost_copy(ctx));
-    TRY(launch_predictor(ctx, P.Re, P.dt));
-    TRY(launch_flux(ctx, P.Uin, true));
-    TRY(launch_rhs(ctx, P.dt, ctx->pc.max_diag));
-    long long it = 0;
-    double err = 0;
-    int status;
-    if (P.method == DALM_BICGSTAB)
-        status = run_bicgstab(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
-    else
-        status = run_sor(ctx, P.omega, P.max_it, P.tol, &it, &err);
-    if (status == DALM_HIP_ERROR) return status;
-    TRY(launch_pbc(ctx));
-    TRY(launch_project(ctx, P.dt, P.Uin));
-    if (ctx->has_solid) TRY(launch_solid(ctx));
-    TRY(launch_ubc(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
-    TRY(launch_post(ctx, P.Cs, P.dt, true));
-    TRY(alm_run(ctx, t, false));
-    TRY(read_scalars(ctx));
-    TRY(probe_collect(ctx));
-    dalm_step_stats st;
-    fill_stats(ctx, &st, it, err, status);
-    if (stats) *stats = st;
-    return status == DALM_NONCONVERGED ? DALM_OK : status;
+    if (ctx->precision == DALM_MIXED) {
+        ctx->step += 1;
+        return step_mixed(ctx, ctx->prm.dt * (double)ctx->step, stats);
+    }
+    int rc = step_fp64(ctx, stats, nullptr);
+    if (rc) return rc;
+    if (ctx->adj.taping) return adjoint_record(ctx);
+    return DALM_OK;
 }
 
 int dalm_probe_kernel(dalm_ctx* ctx, int64_t kernel) {

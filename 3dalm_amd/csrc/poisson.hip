@@ -269,7 +269,7 @@ int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r) 
     hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x,
                        b, r, ctx->partials, ctx->counter, ctx->solve, 0);
     LAUNCH_CHECK();
-    return launch_finalize(ctx, FIN_RESID, ctx->grid, ctx->solve, nullptr, nullptr);
+    return launch_finalize(ctx, FIN_RESID, ctx->grid, ctx->solve, nullptr);
 }
 
 __global__ void k_Ax_plain(Dims d, PCoef A, const double* __restrict__ z, double* __restrict__ y) {
@@ -309,7 +309,7 @@ int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega
             hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b,
                                ctx->r, ctx->partials, ctx->counter, ctx->solve, 1);
             probe_end(ctx, DALM_K_RESNORM);
-            if (int rc = launch_finalize(ctx, FIN_SOR, ctx->grid, ctx->solve, nullptr, nullptr)) return rc;
+            if (int rc = launch_finalize(ctx, FIN_SOR, ctx->grid, ctx->solve, nullptr)) return rc;
         }
         LAUNCH_CHECK();
         launched += chunk;
@@ -336,7 +336,7 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     hipLaunchKernelGGL(k_bi_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, x, b, ctx->r, ctx->r0,
                        ctx->partials, ctx->counter, ctx->solve);
     LAUNCH_CHECK();
-    if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->grid, ctx->solve, nullptr, nullptr)) return rc0;
+    if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->grid, ctx->solve, nullptr)) return rc0;
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     // preconditioner application for one vector; returns the buffer holding z
@@ -378,7 +378,7 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
                                ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
-            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->grid, ctx->solve, nullptr, nullptr)) return rc1;
+            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->grid, ctx->solve, nullptr)) return rc1;
             probe_begin(ctx, DALM_K_SUPDATE);
             hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
                                ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
@@ -388,12 +388,12 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
                                ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
-            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->grid, ctx->solve, nullptr, nullptr)) return rc2;
+            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->grid, ctx->solve, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
             hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
                                ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_UPDATE);
-            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->grid, ctx->solve, nullptr, nullptr)) return rc3;
+            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->grid, ctx->solve, nullptr)) return rc3;
         }
         LAUNCH_CHECK();
         launched += chunk;

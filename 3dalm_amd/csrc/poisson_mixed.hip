@@ -1,18 +1,29 @@
-// poisson_mixed.hip -- Poisson solve of the precision form: FP64 iterative
-// refinement around an FP32 preconditioned BiCGStab.
+// poisson_mixed.hip -- Poisson solve of the precision form.
 //
-//   repeat:  r = b - A p                      (FP64 operator, FP64 norm)
-//            stop when |r| / sqrt(N) <= tol    (the reference's test, src/eq.h:399-401)
-//            solve A e = r in FP32 BiCGStab (Jacobi / SOR preconditioned, same
-//            recurrences as src/eq.h:333-473) to max(tol/2, 1e-4 |r|/sqrt(N)),
-//            with every dot product and norm accumulated in FP64
-//            p += e                            (FP64)
+// Preconditioned BiCGStab (the recurrences of src/eq.h:333-473) whose iterate,
+// residual and direction vectors (x = p, r, p, s, q = A z, t = A z') and every dot
+// product stay FP64, while the preconditioner -- m Jacobi sweeps or SOR double
+// sweeps per application, more than half of the solver's memory traffic -- runs on
+// FP32 vectors: FP32 copies of the vector being preconditioned and FP32 iterates z.
+// The operator products A z are evaluated in FP64 from the FP32 z, so the residual
+// recurrence stays consistent with the FP64 iterate; the shadow vector r0 (only
+// ever used in dot products) is FP32.  A solve ends with an FP64 true-residual
+// check |b - A p| / sqrt(N) <= tol (the reference's test, src/eq.h:399-401); if the
+// recursive residual drifted above it, the solve restarts from the true residual
+// (outer iterative refinement in FP64).
 //
-// Inner vectors are FP32, so each inner iteration moves half the bytes of the FP64
-// solver; the pressure, the right-hand side and the stopping test stay FP64, so the
-// final residual -- and with it div(JU) = dt max_diag (b - A p) -- meets the same
-// tolerance as the drop-in.  Iteration count reported = total inner iterations.
+// Why not FP32 recurrences: the normalised operator of a stretched grid is badly
+// conditioned (about 2.5e6 at 256x128x128 with tanh beta = 1.5), so an FP32
+// Krylov recurrence loses the smooth pressure modes (u32 * cond ~ 0.15) and
+// stagnates well above the 1e-8 tolerance.  The preconditioner is an approximate
+// inverse anyway; FP32 rounding only perturbs it slightly.
+//
+// Algorithmic bytes per iteration (m = 3): 208 per cell against 280 for the FP64
+// solver; iterations reported = BiCGStab iterations over all passes.
 #include <float.h>
+
+#include <cstdio>
+#include <cstdlib>
 
 #include "ctx.h"
 
@@ -21,10 +32,19 @@ namespace {
 __device__ __forceinline__ float diag_f(const PCoefF& A, int i, int j, int k) {
     return (((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k];
 }
+// A x in difference form: sum_nb a_nb (x_nb - x_c) + a_dir x_c.  The operator's
+// rows sum to zero except the x+ Dirichlet row (a_dir = -2/dx^2 / max_diag there),
+// so this equals the 7-point product; but for smooth x the 7 terms of the plain
+// product cancel to ~1e-5 of their size (the normalised operator's smallest
+// eigenvalues), which in FP32 leaves a few per cent of error on exactly the modes
+// the Krylov solve needs last.  Differences of nearby FP32 values are exact
+// (Sterbenz), so here each term keeps FP32 relative accuracy.
 __device__ __forceinline__ float apply7f(const Dims& d, const PCoefF& A, const float* x, int c, int i, int j,
-                                         int k, float ac) {
-    return ac * x[c] + A.ae[i] * x[c + d.si] + A.aw[i] * x[c - d.si] + A.an[j] * x[c + d.sj] +
-           A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
+                                         int k) {
+    const float xc = x[c];
+    return A.ae[i] * (x[c + d.si] - xc) + A.aw[i] * (x[c - d.si] - xc) + A.an[j] * (x[c + d.sj] - xc) +
+           A.as[j] * (x[c - d.sj] - xc) + A.at[k] * (x[c + 1] - xc) + A.ab[k] * (x[c - 1] - xc) +
+           A.adir[i] * xc;
 }
 __device__ __forceinline__ double diag64(const PCoef& A, int i, int j, int k) {
     return ((((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k]) / A.max_diag;
@@ -33,60 +53,6 @@ __device__ __forceinline__ double apply7d(const Dims& d, const PCoef& A, const d
                                           int k, double ac) {
     return ac * x[c] + A.ae[i] * x[c + d.si] + A.aw[i] * x[c - d.si] + A.an[j] * x[c + d.sj] +
            A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
-}
-
-// FP64 residual r = b - A p, its norm (outer test) and the start of an inner solve:
-// r_in = r0_in = (float) r, e = 0, rho = <r0_in, r_in>.
-__global__ __launch_bounds__(BLOCK) void km_res64(Dims d, PCoef A, const double* __restrict__ p,
-                                                  const double* __restrict__ b, float* __restrict__ r,
-                                                  float* __restrict__ r0, float* __restrict__ e,
-                                                  double* partials, unsigned* counter, SolveState* inner,
-                                                  SolveState* outer, double eta, long long inner_max) {
-    TileRange tr = block_tiles(d);
-    double v[2] = {0.0, 0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double rv = b[c] - apply7d(d, A, p, c, cc.i, cc.j, cc.k, diag64(A, cc.i, cc.j, cc.k));
-        float rf = (float)rv;
-        r[c] = rf;
-        r0[c] = rf;
-        e[c] = 0.f;
-        v[0] += rv * rv;
-        v[1] += (double)rf * (double)rf;
-    }
-    block_sum<2>(v);
-    double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
-}
-
-__global__ void km_outer_reset(SolveState* outer, double tol, double root_n) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        outer->err = 0;
-        outer->tol = tol;
-        outer->root_n = root_n;
-        outer->it = 0;
-        outer->done = 0;
-        outer->status = DALM_OK;
-    }
-}
-
-__global__ __launch_bounds__(BLOCK) void kf_p(Dims d, PCoefF A, const float* __restrict__ r,
-                                              float* __restrict__ p, const float* __restrict__ q,
-                                              float* __restrict__ z, int pc_first, const SolveState* st) {
-    if (st->done) return;
-    const bool first = st->it == 0;
-    const float beta = (float)st->beta, om = (float)st->omega;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        float pv = first ? r[c] : r[c] + beta * (p[c] - om * q[c]);
-        p[c] = pv;
-        z[c] = pc_first ? pv * __builtin_amdgcn_rcpf(diag_f(A, cc.i, cc.j, cc.k)) : 0.f;
-    }
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float* __restrict__ zin,
@@ -99,7 +65,7 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float
         if (!cc.inside) continue;
         int c = d.at(cc.i, cc.j, cc.k);
         float ac = diag_f(A, cc.i, cc.j, cc.k);
-        zout[c] = zin[c] + (v[c] - apply7f(d, A, zin, c, cc.i, cc.j, cc.k, ac)) * __builtin_amdgcn_rcpf(ac);
+        zout[c] = zin[c] + (v[c] - apply7f(d, A, zin, c, cc.i, cc.j, cc.k)) * __builtin_amdgcn_rcpf(ac);
     }
 }
 
@@ -113,14 +79,103 @@ __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restr
         if (!cc.inside || ((cc.i + cc.j + cc.k) & 1) != color) continue;
         int c = d.at(cc.i, cc.j, cc.k);
         float ac = diag_f(A, cc.i, cc.j, cc.k);
-        x[c] = x[c] + omega * (b[c] - apply7f(d, A, x, c, cc.i, cc.j, cc.k, ac)) * __builtin_amdgcn_rcpf(ac);
+        x[c] = x[c] + omega * (b[c] - apply7f(d, A, x, c, cc.i, cc.j, cc.k)) * __builtin_amdgcn_rcpf(ac);
     }
 }
 
-template <int WHICH>
-__global__ __launch_bounds__(BLOCK) void kf_spmv(Dims d, PCoefF A, const float* __restrict__ z,
-                                                 float* __restrict__ y, const float* __restrict__ a1,
-                                                 double* partials, unsigned* counter, SolveState* st) {
+
+// A z in FP64 arithmetic from an FP32 vector, in the FP64 form's summation order
+__device__ __forceinline__ double apply7h(const Dims& d, const PCoef& A, const float* z, int c, int i, int j, int k,
+                                          double ac) {
+    return ac * (double)z[c] + A.ae[i] * (double)z[c + d.si] + A.aw[i] * (double)z[c - d.si] +
+           A.an[j] * (double)z[c + d.sj] + A.as[j] * (double)z[c - d.sj] + A.at[k] * (double)z[c + 1] +
+           A.ab[k] * (double)z[c - 1];
+}
+
+__global__ void kh_state_reset(SolveState* st, double tol, long long max_it, double root_n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->rho = st->rho_prev = st->alpha = st->omega = 1.0;
+        st->beta = 0;
+        st->err = 0;
+        st->tol = tol;
+        st->root_n = root_n;
+        st->it = 0;
+        st->max_it = max_it;
+        st->done = 0;
+        st->status = DALM_OK;
+    }
+}
+
+// start of a pass: r = b - A x (FP64), r0 = (float) r; partials <r0, r>, |r|^2
+__global__ __launch_bounds__(BLOCK) void kh_init(Dims d, PCoef A, const double* __restrict__ x,
+                                                 const double* __restrict__ b, double* __restrict__ r,
+                                                 float* __restrict__ r0, double* partials) {
+    TileRange tr = block_tiles(d);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double rv = b[c] - apply7d(d, A, x, c, cc.i, cc.j, cc.k, diag64(A, cc.i, cc.j, cc.k));
+        float rf = (float)rv;
+        r[c] = rv;
+        r0[c] = rf;
+        v[0] += (double)rf * rv;
+        v[1] += rv * rv;
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    store_partials<2>(pv, partials);
+}
+
+// p = r (first iteration) or r + beta (p - omega q) in FP64; its FP32 copy, and
+// the first preconditioner sweep from zero (Jacobi: z = p / a_c; SOR: z = 0)
+__global__ __launch_bounds__(BLOCK) void kh_p(Dims d, PCoefF Af, const double* __restrict__ r,
+                                              double* __restrict__ p, const double* __restrict__ q,
+                                              float* __restrict__ p32, float* __restrict__ z, int pc_first,
+                                              const SolveState* st) {
+    if (st->done) return;
+    const bool first = st->it == 0;
+    const double beta = st->beta, om = st->omega;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double pv = first ? r[c] : r[c] + beta * (p[c] - om * q[c]);
+        p[c] = pv;
+        float pf = (float)pv;
+        p32[c] = pf;
+        z[c] = pc_first ? pf * __builtin_amdgcn_rcpf(diag_f(Af, cc.i, cc.j, cc.k)) : 0.f;
+    }
+}
+
+// s = r - alpha q in FP64; its FP32 copy and the first preconditioner sweep
+__global__ __launch_bounds__(BLOCK) void kh_s(Dims d, PCoefF Af, const double* __restrict__ r,
+                                              const double* __restrict__ q, double* __restrict__ s,
+                                              float* __restrict__ s32, float* __restrict__ z, int pc_first,
+                                              const SolveState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        int c = d.at(cc.i, cc.j, cc.k);
+        double sv = r[c] - alpha * q[c];
+        s[c] = sv;
+        float sf = (float)sv;
+        s32[c] = sf;
+        z[c] = pc_first ? sf * __builtin_amdgcn_rcpf(diag_f(Af, cc.i, cc.j, cc.k)) : 0.f;
+    }
+}
+
+// y = A z (FP64 from FP32 z) with <a1, y> (WHICH 0: a1 = r0) or <y, a1>, <y, y>
+// (WHICH 1: a1 = s)
+template <int WHICH, typename TA>
+__global__ __launch_bounds__(BLOCK) void kh_spmv(Dims d, PCoef A, const float* __restrict__ z,
+                                                 double* __restrict__ y, const TA* __restrict__ a1,
+                                                 double* partials, const SolveState* st) {
     if (st->done) return;
     TileRange tr = block_tiles(d);
     double v[2] = {0.0, 0.0};
@@ -128,14 +183,13 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv(Dims d, PCoefF A, const float* 
         Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         int c = d.at(cc.i, cc.j, cc.k);
-        float yv = apply7f(d, A, z, c, cc.i, cc.j, cc.k, diag_f(A, cc.i, cc.j, cc.k));
+        double yv = apply7h(d, A, z, c, cc.i, cc.j, cc.k, diag64(A, cc.i, cc.j, cc.k));
         y[c] = yv;
-        double yd = yv;
         if (WHICH == 0) {
-            v[0] += (double)a1[c] * yd;
+            v[0] += (double)a1[c] * yv;
         } else {
-            v[0] += yd * (double)a1[c];
-            v[1] += yd * yd;
+            v[0] += yv * (double)a1[c];
+            v[1] += yv * yv;
         }
     }
     block_sum<2>(v);
@@ -143,57 +197,29 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv(Dims d, PCoefF A, const float* 
     store_partials<2>(pv, partials);
 }
 
-__global__ __launch_bounds__(BLOCK) void kf_s(Dims d, PCoefF A, const float* __restrict__ r,
-                                              const float* __restrict__ q, float* __restrict__ s,
-                                              float* __restrict__ z, int pc_first, const SolveState* st) {
-    if (st->done) return;
-    const float alpha = (float)st->alpha;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        float sv = r[c] - alpha * q[c];
-        s[c] = sv;
-        z[c] = pc_first ? sv * __builtin_amdgcn_rcpf(diag_f(A, cc.i, cc.j, cc.k)) : 0.f;
-    }
-}
-
-__global__ __launch_bounds__(BLOCK) void kf_xr(Dims d, float* __restrict__ x, const float* __restrict__ pp,
-                                               const float* __restrict__ ss, const float* __restrict__ s,
-                                               const float* __restrict__ t_, float* __restrict__ r,
+// x += alpha z_p + omega z_s; r = s - omega t; partials |r|^2, <r0, r>
+__global__ __launch_bounds__(BLOCK) void kh_xr(Dims d, double* __restrict__ x, const float* __restrict__ pp,
+                                               const float* __restrict__ ss, const double* __restrict__ s,
+                                               const double* __restrict__ t_, double* __restrict__ r,
                                                const float* __restrict__ r0, double* partials,
-                                               unsigned* counter, SolveState* st) {
+                                               const SolveState* st) {
     if (st->done) return;
-    const float alpha = (float)st->alpha, om = (float)st->omega;
+    const double alpha = st->alpha, om = st->omega;
     TileRange tr = block_tiles(d);
     double v[2] = {0.0, 0.0};
     for (int t = tr.begin; t < tr.end; ++t) {
         Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         int c = d.at(cc.i, cc.j, cc.k);
-        x[c] = x[c] + alpha * pp[c] + om * ss[c];
-        float rv = s[c] - om * t_[c];
+        x[c] = x[c] + alpha * (double)pp[c] + om * (double)ss[c];
+        double rv = s[c] - om * t_[c];
         r[c] = rv;
-        double rd = rv;
-        v[0] += rd * rd;
-        v[1] += (double)r0[c] * rd;
+        v[0] += rv * rv;
+        v[1] += (double)r0[c] * rv;
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
     store_partials<2>(pv, partials);
-}
-
-// p += sign * e over the interior (FP64 accumulate of the FP32 correction)
-__global__ __launch_bounds__(BLOCK) void km_add(Dims d, double* __restrict__ p, const float* __restrict__ e,
-                                                double sign) {
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        p[c] += sign * (double)e[c];
-    }
 }
 
 int read_states(dalm_ctx* ctx) {
@@ -215,38 +241,25 @@ long long chunk_for(long long done_it, long long predicted, long long max_it) {
 
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
-// Refinement passes: each pass computes the FP64 residual of the current p, solves
-// A e = r in FP32 (BiCGStab with FP64 dot products) down to eta times that residual,
-// and adds e to p in FP64.  The FP32 inner solve can only reach a relative residual
-// of about u_32 * cond(A); when a pass no longer halves the FP64 residual (the
-// accuracy asked for is beyond FP32), the solve is finished by the FP64 BiCGStab
-// from the current p, so the stopping test err <= tol is always met as in the FP64
-// form.  A pass that raises the residual is undone first.
 int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
                       long long* it, double* err) {
     const Dims& d = ctx->d;
-    const PCoefF A = ctx->pcf;
-    const PCoef A64 = ctx->coef();
-    SolveState* inner = ctx->solve;
-    SolveState* outer = ctx->solve + 1;
-    const double eta = 2e-2;
+    const PCoefF Af = ctx->pcf;
+    const PCoef A = ctx->coef();
+    SolveState* st = ctx->solve;
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
-    hipLaunchKernelGGL(km_outer_reset, dim3(1), dim3(64), 0, ctx->stream, outer, tol, sqrt((double)ctx->N));
-    LAUNCH_CHECK();
-    long long total = 0;
-    int status = DALM_OK;
-    double prev_err = -1;
-    bool finish64 = false;
-    ctx->fp64_finish_it = 0;
-    auto precondition = [&](float* z0, float* spare, const float* v) -> float* {
+    static const bool trace = getenv("DALM_TRACE") != nullptr;
+    const double root_n = sqrt((double)ctx->N);
+    // FP32 preconditioner application on v32 into z0 (spare: the other Jacobi buffer)
+    auto precondition = [&](float* z0, float* spare, const float* v32) -> float* {
         float* cur = z0;
         if (jac) {
             float* other = spare;
             for (int m = 1; m < pc_it; ++m) {
                 probe_begin(ctx, DALM_K_JACOBI);
-                hipLaunchKernelGGL(kf_jacobi, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, other, v,
-                                   (const SolveState*)inner);
+                hipLaunchKernelGGL(kf_jacobi, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, cur, other, v32,
+                                   (const SolveState*)st);
                 probe_end(ctx, DALM_K_JACOBI);
                 float* tsw = cur;
                 cur = other;
@@ -255,68 +268,60 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         } else {
             for (int m = 0; m < pc_it; ++m)
                 for (int color = 0; color < 2; ++color)
-                    hipLaunchKernelGGL(kf_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
-                                       (float)pc_omega, color, (const SolveState*)inner);
+                    hipLaunchKernelGGL(kf_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, cur, v32,
+                                       (float)pc_omega, color, (const SolveState*)st);
         }
         return cur;
     };
-    for (int pass = 0; pass < 200; ++pass) {
-        long long inner_max = max_it - total;
-        hipLaunchKernelGGL(km_res64, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A64, ctx->p, ctx->b, ctx->fr,
-                           ctx->fr0, ctx->fe, ctx->partials, ctx->counter, inner, outer, eta, inner_max);
+    long long total = 0;
+    int status = DALM_NONCONVERGED;
+    double e_out = 0;
+    ctx->outer_it = 0;
+    for (int pass = 0; pass < 8; ++pass) {
+        hipLaunchKernelGGL(kh_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it - total, root_n);
+        hipLaunchKernelGGL(kh_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
+                           ctx->fr0, ctx->partials);
         LAUNCH_CHECK();
-        if (int rc = launch_finalize(ctx, FIN_RES64, ctx->grid, inner, outer, nullptr, eta, inner_max)) return rc;
+        if (int rc = launch_finalize(ctx, FIN_HX_INIT, ctx->grid, st, nullptr)) return rc;
         int rc = read_states(ctx);
         if (rc) return rc;
-        const double e64 = ctx->h_solve[1].err;
-        if (ctx->h_solve[1].done) {
-            status = ctx->h_solve[1].status;
+        ctx->outer_it = pass + 1;
+        e_out = ctx->h_solve[0].err;
+        if (trace) fprintf(stderr, "[mixed] pass %d: fp64 residual %.3e after %lld iterations\n", pass, e_out, total);
+        if (ctx->h_solve[0].done) {
+            status = ctx->h_solve[0].status;
             break;
         }
-        if (pass > 0 && e64 > 0.5 * prev_err) {
-            // FP32 no longer makes progress at this accuracy
-            if (e64 > prev_err) {
-                hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe_prev,
-                                   -1.0);
-                LAUNCH_CHECK();
-            }
-            finish64 = true;
-            break;
-        }
-        prev_err = e64;
-        if (ctx->h_solve[0].done) {  // breakdown at the start of the inner solve
-            finish64 = true;
-            break;
-        }
+        if (total >= max_it) break;
         long long launched = 0;
-        while (!ctx->h_solve[0].done && launched < inner_max) {
-            long long chunk = chunk_for(launched, ctx->prev_it[1] - total, inner_max);
+        while (!ctx->h_solve[0].done && launched < max_it - total) {
+            long long chunk = chunk_for(launched, ctx->prev_it[1] - total, max_it - total);
             for (long long c = 0; c < chunk; ++c) {
                 probe_begin(ctx, DALM_K_PUPDATE);
-                hipLaunchKernelGGL(kf_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->fr, ctx->fp,
-                                   ctx->fq, ctx->fpp, pc_first, (const SolveState*)inner);
+                hipLaunchKernelGGL(kh_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->r, ctx->kp,
+                                   ctx->q, ctx->fp, ctx->fpp, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_PUPDATE);
                 float* ppz = precondition(ctx->fpp, ctx->ftmp, ctx->fp);
                 probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(kf_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz, ctx->fq,
-                                   ctx->fr0, ctx->partials, ctx->counter, inner);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<0, float>), dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d,
+                                   A, ppz, ctx->q, ctx->fr0, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SPMV);
-                if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->grid, inner, nullptr, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->grid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_SUPDATE);
-                hipLaunchKernelGGL(kf_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->fr, ctx->fq,
-                                   ctx->fs, ctx->fss, pc_first, (const SolveState*)inner);
+                hipLaunchKernelGGL(kh_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->r, ctx->q, ctx->s,
+                                   ctx->fs, ctx->fss, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SUPDATE);
                 float* ssz = precondition(ctx->fss, ctx->ftmp2, ctx->fs);
                 probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(kf_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz, ctx->ft,
-                                   ctx->fs, ctx->partials, ctx->counter, inner);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<1, double>), dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream,
+                                   d, A, ssz, ctx->t, ctx->s, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SPMV);
-                if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->grid, inner, nullptr, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->grid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_UPDATE);
-                hipLaunchKernelGGL(kf_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->fe, ppz, ssz,
-                                   ctx->fs, ctx->ft, ctx->fr, ctx->fr0, ctx->partials, ctx->counter, inner);
+                hipLaunchKernelGGL(kh_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ppz, ssz, ctx->s,
+                                   ctx->t, ctx->r, ctx->fr0, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_UPDATE);
-                if ((rc = launch_finalize(ctx, FIN_BI_XR, ctx->grid, inner, nullptr, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_BI_XR, ctx->grid, st, nullptr))) return rc;
             }
             LAUNCH_CHECK();
             launched += chunk;
@@ -324,35 +329,12 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             if (rc) return rc;
         }
         total += ctx->h_solve[0].it;
-        if (ctx->h_solve[0].status == DALM_NONFINITE) {
-            status = DALM_NONFINITE;
-            break;
-        }
-        if (ctx->h_solve[0].it == 0) {  // no progress possible in FP32
-            finish64 = true;
-            break;
-        }
-        hipLaunchKernelGGL(km_add, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fe, 1.0);
-        LAUNCH_CHECK();
-        std::swap(ctx->fe, ctx->fe_prev);  // keep the applied correction for a possible undo
-        if (total >= max_it) {
-            status = DALM_NONCONVERGED;
-            break;
-        }
+        status = ctx->h_solve[0].status;
+        e_out = ctx->h_solve[0].err;
+        if (status != DALM_OK) break;  // max_it, breakdown or non-finite: as the FP64 solver
+        // converged by the recursive residual: the next pass checks the true one
     }
-    ctx->outer_it = ctx->h_solve[1].it;
     ctx->prev_it[1] = total;
-    double e_out = ctx->h_solve[1].err;
-    if (finish64 && total < max_it) {
-        if (int rc = ensure_fp64_krylov(ctx)) return rc;
-        long long it64 = 0;
-        status = run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
-        if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
-        ctx->fp64_finish_it = it64;
-        total += it64;
-    } else if (finish64) {
-        status = DALM_NONCONVERGED;
-    }
     *it = total;
     *err = e_out;
     return status;

@@ -17,12 +17,9 @@ struct FinArgs {
     int mode;
     int nb;                 // number of partials per column
     const double* partials;
-    SolveState* st;         // primary state (FP64 solve, or the inner solve)
-    SolveState* st2;        // outer state (precision form)
+    SolveState* st;         // state of the running solve
     StepScalars* sc;
     double* out;
-    double eta;
-    long long inner_max;
 };
 
 __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
@@ -99,32 +96,21 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         }
         break;
     }
-    case FIN_RES64: {
-        // start of one refinement pass of the precision form
-        SolveState* outer = a.st2;
-        double err = sqrt(s[0]) / outer->root_n;
-        outer->aux1 = outer->err;  // previous pass's residual
-        outer->err = err;
-        outer->it += 1;
-        st->it = 0;
-        st->rho = s[1];
-        st->alpha = st->omega = 1.0;
-        st->beta = 0;
+    case FIN_HX_INIT: {
+        // start of a precision-form pass: s[0] = <r0, r>, s[1] = |r|^2 of the
+        // true FP64 residual; a pass whose residual already meets the test ends
+        // here (the restart after a converged pass)
+        double err = sqrt(s[1]) / st->root_n;
+        st->rho = s[0];
         st->err = err;
-        st->root_n = outer->root_n;
-        st->tol = fmax(0.5 * outer->tol, a.eta * err);
-        st->max_it = a.inner_max;
-        st->status = DALM_OK;
-        st->done = 0;
         if (!(err == err) || isinf(err)) {
-            outer->status = DALM_NONFINITE;
-            outer->done = 1;
             st->done = 1;
-        } else if (err <= outer->tol || a.inner_max <= 0) {
-            outer->done = 1;
-            outer->status = err <= outer->tol ? DALM_OK : DALM_NONCONVERGED;
+            st->status = DALM_NONFINITE;
+        } else if (err <= st->tol) {
             st->done = 1;
-        } else if (fabs(s[1]) < (double)FLT_MIN) {
+            st->status = DALM_OK;
+        } else if (fabs(s[0]) < (double)FLT_MIN) {
+            st->err = s[0];
             st->done = 1;
             st->status = DALM_BREAKDOWN;
         }
@@ -137,18 +123,14 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
 
 }  // namespace
 
-int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, SolveState* st2, double* out, double eta,
-                    long long inner_max) {
+int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out) {
     FinArgs a;
     a.mode = mode;
     a.nb = nb;
     a.partials = ctx->partials;
     a.st = st;
-    a.st2 = st2;
     a.sc = ctx->scal;
     a.out = out;
-    a.eta = eta;
-    a.inner_max = inner_max;
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, ctx->stream, a);
     HIP_TRY(hipGetLastError());
     return DALM_OK;

@@ -23,9 +23,14 @@ STATUS_NAMES = {0: "OK", 1: "NONCONVERGED", 2: "BREAKDOWN", 3: "NONFINITE", 4: "
 FP64, MIXED = 0, 1
 MIXED_BUILT = True  # the precision form is part of this library
 F_U, F_UOLD, F_JU, F_P, F_NUT, F_DIV, F_F, F_SOLID, F_B = range(9)
+# sensitivities of the discrete adjoint (FP64 contexts)
+(F_U_B, F_UOLD_B, F_JU_B, F_JUOLD_B, F_NUT_B, F_NUTOLD_B, F_F_B, F_FOLD_B, F_P_B, F_B_B) = range(9, 19)
 FIELDS = {"U": F_U, "Uold": F_UOLD, "JU": F_JU, "p": F_P, "nut": F_NUT, "div": F_DIV, "f": F_F,
-          "solid": F_SOLID, "b": F_B}
-VECTOR_FIELDS = {F_U, F_UOLD, F_JU, F_F}
+          "solid": F_SOLID, "b": F_B,
+          "U_b": F_U_B, "Uold_b": F_UOLD_B, "JU_b": F_JU_B, "JUold_b": F_JUOLD_B, "nut_b": F_NUT_B,
+          "nutold_b": F_NUTOLD_B, "f_b": F_F_B, "fold_b": F_FOLD_B, "p_b": F_P_B, "b_b": F_B_B}
+VECTOR_FIELDS = {F_U, F_UOLD, F_JU, F_F, F_U_B, F_UOLD_B, F_JU_B, F_JUOLD_B, F_F_B, F_FOLD_B}
+ADJ_KEEP_PRESSURE = 1
 SOR, BICGSTAB = 0, 1
 PC_JACOBI, PC_SOR = 0, 1
 WT_STRIDE = 16
@@ -54,6 +59,13 @@ class StepStats(C.Structure):
             d["thrust"] = np.array(self.thrust[:n])
             d["power"] = np.array(self.power[:n])
         return d
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("steps", i64), ("every", i64), ("adj_it", i64), ("replay_ms", dbl), ("reverse_ms", dbl)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class HipError(RuntimeError):
@@ -105,6 +117,20 @@ def lib():
         "dalm_run": [V, i64, C.POINTER(StepStats)],
         "dalm_probe_kernel": [V, i64],
         "dalm_probe_read": [V, PI64, P],
+        "dalm_predictor_adj": [V, dbl, dbl],
+        "dalm_interpolate_JU_adj": [V],
+        "dalm_apply_JUbc_adj": [V],
+        "dalm_poisson_rhs_adj": [V, dbl, dbl],
+        "dalm_apply_pbc_adj": [V],
+        "dalm_project_p_adj": [V, dbl],
+        "dalm_set_solid_U_adj": [V],
+        "dalm_apply_Ubc_adj": [V, dbl, i64, dbl],
+        "dalm_calc_nut_adj": [V, dbl],
+        "dalm_poisson_adj": [V, dbl, i64, PI64, P],
+        "dalm_run_alm_adj": [V, dbl, P, P],
+        "dalm_adjoint_begin": [V, i64, i64, P, i64],
+        "dalm_adjoint_run": [V, dbl, P, P, P, C.POINTER(AdjointStats)],
+        "dalm_adjoint_end": [V],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -145,6 +171,7 @@ class Context:
         if rc != OK:
             raise HipError(f"dalm_create failed with status {STATUS_NAMES.get(rc, rc)}")
         self.precision = precision
+        self.n_wt, self.n_ap = 0, 0
 
     def check(self, rc, what, allow=(OK,)):
         if rc not in allow:
@@ -289,6 +316,64 @@ class Context:
         st = StepStats()
         self.check(self.L.dalm_run(self.h, n, C.byref(st)), "run")
         return st.as_dict()
+
+    # ---- discrete adjoint (bars are the *_b fields)
+    def predictor_adj(self, Re, dt):
+        self.check(self.L.dalm_predictor_adj(self.h, Re, dt), "predictor_adj")
+
+    def interpolate_JU_adj(self):
+        self.check(self.L.dalm_interpolate_JU_adj(self.h), "interpolate_JU_adj")
+
+    def apply_JUbc_adj(self):
+        self.check(self.L.dalm_apply_JUbc_adj(self.h), "apply_JUbc_adj")
+
+    def poisson_rhs_adj(self, dt, scale):
+        self.check(self.L.dalm_poisson_rhs_adj(self.h, dt, scale), "poisson_rhs_adj")
+
+    def apply_pbc_adj(self):
+        self.check(self.L.dalm_apply_pbc_adj(self.h), "apply_pbc_adj")
+
+    def project_p_adj(self, dt):
+        self.check(self.L.dalm_project_p_adj(self.h, dt), "project_p_adj")
+
+    def set_solid_U_adj(self):
+        self.check(self.L.dalm_set_solid_U_adj(self.h), "set_solid_U_adj")
+
+    def apply_Ubc_adj(self, dt, retracted=0, floor_x=0.0):
+        self.check(self.L.dalm_apply_Ubc_adj(self.h, dt, retracted, floor_x), "apply_Ubc_adj")
+
+    def calc_nut_adj(self, Cs):
+        self.check(self.L.dalm_calc_nut_adj(self.h, Cs), "calc_nut_adj")
+
+    def poisson_adj(self, tol_rel, max_it):
+        it, err = i64(), dbl()
+        self.check(self.L.dalm_poisson_adj(self.h, tol_rel, max_it, C.byref(it), C.byref(err)), "poisson_adj")
+        return it.value, err.value
+
+    def run_alm_adj(self, t, tq_w):
+        tq = np.zeros(MAX_TURBINES)
+        tq[:len(tq_w)] = tq_w
+        g = np.zeros((self.n_ap, 4))
+        self.check(self.L.dalm_run_alm_adj(self.h, t, ptr(tq), ptr(g)), "run_alm_adj")
+        return g
+
+    def adjoint_begin(self, steps, every, tq_w, keep_pressure=True):
+        tq = np.zeros(MAX_TURBINES)
+        tq[:len(tq_w)] = tq_w
+        self.check(self.L.dalm_adjoint_begin(self.h, steps, every, ptr(tq),
+                                             ADJ_KEEP_PRESSURE if keep_pressure else 0), "adjoint_begin")
+
+    def adjoint_run(self, tol_rel=1e-10):
+        J = dbl()
+        ap_g = np.zeros((max(self.n_ap, 1), 4))
+        wt_g = np.zeros((max(self.n_wt, 1), 2))
+        st = AdjointStats()
+        self.check(self.L.dalm_adjoint_run(self.h, tol_rel, C.byref(J), ptr(ap_g), ptr(wt_g), C.byref(st)),
+                   "adjoint_run")
+        return J.value, ap_g[:self.n_ap], wt_g[:self.n_wt], st.as_dict()
+
+    def adjoint_end(self):
+        self.check(self.L.dalm_adjoint_end(self.h), "adjoint_end")
 
 
 def build_ap_props(turbine, wt_count, ap_per_blade):

@@ -27,7 +27,10 @@ def params_for(case: Case) -> hip.Params:
 
 
 class Solver:
-    def __init__(self, case: Case, precision: int = hip.FP64, U0: np.ndarray | None = None):
+    """``ap_hook(ap, wt)``, when given, may change the actuator-point and turbine
+    records (chord, twist, pitch, angle formula) before the start-up pass."""
+
+    def __init__(self, case: Case, precision: int = hip.FP64, U0: np.ndarray | None = None, ap_hook=None):
         self.case = case
         self.ctx = hip.Context(case.mesh, precision)
         L = case.mesh.L
@@ -45,6 +48,8 @@ class Solver:
             for a in range(len(ap)):
                 ap[a, 13] = case.sites[a // per_wt].pitch
             wt = turbine_records(case.turbine, case.sites)
+            if ap_hook is not None:
+                ap_hook(ap, wt)
             self.ctx.alm_setup(wt, ap, case.turbine.blades, case.ap_per_blade, cd, cl, atk, case.width)
             self.ap = ap
         self.stats = self.ctx.initialize()

@@ -51,7 +51,19 @@ enum dalm_field {
     DALM_F_DIV = 5,   /* [L]    divergence             (Cfd::div)   */
     DALM_F_F = 6,     /* [L][3] ALM body force         (Cfd::f)     */
     DALM_F_SOLID = 7, /* [L]    solid fraction         (Cfd::solid) */
-    DALM_F_B = 8      /* [L]    Poisson right-hand side (Eq::b)     */
+    DALM_F_B = 8,     /* [L]    Poisson right-hand side (Eq::b)     */
+    /* sensitivities (bars) of the discrete adjoint, FP64 contexts only; the
+       "OLD" bars are what the predictor's adjoint produces for the previous step */
+    DALM_F_U_B = 9,       /* [L][3] bar of U                        */
+    DALM_F_UOLD_B = 10,   /* [L][3] bar of Uold                     */
+    DALM_F_JU_B = 11,     /* [L][3] bar of JU                       */
+    DALM_F_JUOLD_B = 12,  /* [L][3] bar of the previous step's JU   */
+    DALM_F_NUT_B = 13,    /* [L]    bar of nu_t                     */
+    DALM_F_NUTOLD_B = 14, /* [L]    bar of the previous step's nu_t */
+    DALM_F_F_B = 15,      /* [L][3] bar of f                        */
+    DALM_F_FOLD_B = 16,   /* [L][3] bar of the previous step's f    */
+    DALM_F_P_B = 17,      /* [L]    bar of p                        */
+    DALM_F_B_B = 18       /* [L]    bar of b (Poisson right-hand side) */
 };
 
 enum dalm_method { DALM_SOR = 0, DALM_BICGSTAB = 1 };
@@ -222,6 +234,52 @@ int dalm_initialize(dalm_ctx *ctx, dalm_step_stats *stats);
 int dalm_step(dalm_ctx *ctx, dalm_step_stats *stats);
 /* n steps back to back; stats of the last one. */
 int dalm_run(dalm_ctx *ctx, int64_t n, dalm_step_stats *stats);
+
+/* ---- differentiation form: discrete adjoint of the FP64 step --------------------
+ * No counterpart in the reference (it has no adjoint).  Each *_adj entry point is
+ * the transpose of the Jacobian of the forward entry point of the same name,
+ * acting on the bar fields above: outputs named "(overwrites)" are written, the
+ * others accumulate.  The Poisson solve is differentiated as the exact solve. */
+int dalm_predictor_adj(dalm_ctx *ctx, double Re, double dt);
+    /* U_B (bar of U*) with values UOLD, JU, NUT -> UOLD_B, JUOLD_B, NUTOLD_B,
+       FOLD_B (overwrites) */
+int dalm_interpolate_JU_adj(dalm_ctx *ctx);                 /* U_B += I^T JU_B */
+int dalm_apply_JUbc_adj(dalm_ctx *ctx);                     /* JU_B = 0 on the boundary faces */
+int dalm_poisson_rhs_adj(dalm_ctx *ctx, double dt, double scale);  /* JU_B += R^T B_B */
+int dalm_apply_pbc_adj(dalm_ctx *ctx);        /* P_B ghosts folded onto the interior */
+int dalm_project_p_adj(dalm_ctx *ctx, double dt);
+    /* P_B = -dt (Gc^T U_B + Gf^T JU_B) on the interior and one ghost layer
+       (overwrites) */
+int dalm_set_solid_U_adj(dalm_ctx *ctx);                    /* U_B *= 1 - solid */
+int dalm_apply_Ubc_adj(dalm_ctx *ctx, double dt, int64_t retracted_floor, double floor_start_x);
+    /* in place on U_B with values U (before the pass) and UOLD; UOLD_B += */
+int dalm_calc_nut_adj(dalm_ctx *ctx, double Cs);            /* U_B += (dnut/dU)^T NUT_B */
+int dalm_poisson_adj(dalm_ctx *ctx, double tol_rel, int64_t max_it, int64_t *it, double *err);
+    /* B_B = A^-T P_B (overwrites) with the configured solver (dalm_set_params),
+       to tol_rel times |V^-1 P_B| */
+int dalm_run_alm_adj(dalm_ctx *ctx, double t, const double *tq_w, double *ap_grad);
+    /* ALM at time t from U: U_B += bars of F_B and of sum_w tq_w[w] torque_w;
+       ap_grad[ap][4] += dJ/d(turbine angle, collective pitch, chord, twist) */
+
+/* Recorded run and reverse sweep.  dalm_adjoint_begin records the current state
+ * as checkpoint 0 and sets the objective J = sum over the next `steps` steps of
+ * sum_w tq_w[w] torque_w; the next `steps` dalm_step calls record checkpoints
+ * every `every` steps (and, with DALM_ADJ_KEEP_PRESSURE, the pressure of every
+ * step, so the replay needs no Poisson solve).  dalm_adjoint_run then replays
+ * each segment from its checkpoint and runs the reverse steps, and restores the
+ * final state.  ap_grad [ap][4] as above; wt_grad [wt][2]: dJ/d(turbine angle
+ * offset D), dJ/d(collective pitch). */
+#define DALM_ADJ_KEEP_PRESSURE 1
+typedef struct {
+    int64_t steps, every;  /* recorded steps and checkpoint interval */
+    int64_t adj_it;        /* adjoint Poisson iterations over the sweep */
+    double replay_ms;      /* wall time of the segment replays */
+    double reverse_ms;     /* wall time of the reverse steps */
+} dalm_adjoint_stats;
+int dalm_adjoint_begin(dalm_ctx *ctx, int64_t steps, int64_t every, const double *tq_w, int64_t flags);
+int dalm_adjoint_run(dalm_ctx *ctx, double tol_rel, double *J, double *ap_grad, double *wt_grad,
+                     dalm_adjoint_stats *stats);
+int dalm_adjoint_end(dalm_ctx *ctx);
 
 #ifdef __cplusplus
 }

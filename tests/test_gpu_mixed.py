@@ -1,5 +1,6 @@
-"""The precision form (FP32 stencils and inner Poisson sweeps, FP64 reductions,
-FP64 face fluxes / pressure / refinement) against the FP64 oracle.
+"""The precision form (FP32 cell stencils and FP32 preconditioner sweeps; FP64 face
+fluxes, pressure, BiCGStab recurrences, dot products and the true-residual check)
+against the FP64 oracle.
 
 Stated tolerances (the north star asks for them to be stated against FP64):
   * every step ends with the FP64 residual test met: err <= tol, and therefore
