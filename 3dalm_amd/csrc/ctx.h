@@ -160,7 +160,8 @@ struct dalm_ctx {
     double *r = nullptr, *r0 = nullptr, *kp = nullptr, *pp = nullptr, *q = nullptr, *s = nullptr,
            *ss = nullptr, *t = nullptr, *tmp = nullptr, *tmp2 = nullptr;
     // iterations of the previous solve per use: 0 SOR, 1 BiCGStab (either form),
-    // 3 the adjoint solve (sizes the chunks enqueued per host check)
+    // 2 the FP64 finish of the precision form, 3 the adjoint solve (sizes the
+    // chunks enqueued per host check)
     long long prev_it[4] = {0, 0, 0, 0};
 
     PoissonCoef pc{};
@@ -179,6 +180,7 @@ struct dalm_ctx {
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
     long long outer_it = 0;       // FP64 true-residual passes of the last mixed solve
+    long long fp64_finish_it = 0; // iterations of its FP64 finish (0: FP32 preconditioning sufficed)
 
     // reduction scratch
     double* partials = nullptr;  // [8][grid]

@@ -366,8 +366,10 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
         float** scf[8] = {&ctx->nut32, &ctx->fr0, &ctx->fp, &ctx->fpp, &ctx->fs, &ctx->fss, &ctx->ftmp, &ctx->ftmp2};
         for (int q = 0; q < 8; ++q)
             if ((rc = dev_alloc(ctx, scf[q], L))) return fail(rc);
-        double** sd[4] = {&ctx->kp, &ctx->q, &ctx->s, &ctx->t};
-        for (int q = 0; q < 4; ++q)
+        // FP64 vectors: the recurrences of the FP32-preconditioned BiCGStab (kp, q, s,
+        // t) and, for its FP64 finish, the full FP64 solver's set
+        double** sd[9] = {&ctx->kp, &ctx->q, &ctx->s, &ctx->t, &ctx->r0, &ctx->pp, &ctx->ss, &ctx->tmp, &ctx->tmp2};
+        for (int q = 0; q < 9; ++q)
             if ((rc = dev_alloc(ctx, sd[q], L))) return fail(rc);
     } else {
         double** sc[10] = {&ctx->nut, &ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t,

@@ -12,11 +12,15 @@
 // recursive residual drifted above it, the solve restarts from the true residual
 // (outer iterative refinement in FP64).
 //
-// Why not FP32 recurrences: the normalised operator of a stretched grid is badly
-// conditioned (about 2.5e6 at 256x128x128 with tanh beta = 1.5), so an FP32
-// Krylov recurrence loses the smooth pressure modes (u32 * cond ~ 0.15) and
-// stagnates well above the 1e-8 tolerance.  The preconditioner is an approximate
-// inverse anyway; FP32 rounding only perturbs it slightly.
+// The limit of FP32 here: the normalised operator of a stretched grid is badly
+// conditioned (about 2.5e6 at 256x128x128 with tanh beta = 1.5).  Rounding a
+// preconditioned direction z to FP32 adds high-frequency noise of relative size
+// u32, which A amplifies by up to cond(A) against the smooth part of A z -- so
+// once the residual is dominated by smooth modes (about 4e-7 at that size) the
+// FP32-preconditioned iteration stagnates or diverges.  A watchdog on the
+// recursive residual detects that; the pass is then abandoned (the pressure
+// restored to its start if the pass made it worse) and the solve is finished by
+// the FP64 BiCGStab, so every step meets the same tolerance as the FP64 form.
 //
 // Algorithmic bytes per iteration (m = 3): 208 per cell against 280 for the FP64
 // solver; iterations reported = BiCGStab iterations over all passes.
@@ -276,7 +280,9 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     long long total = 0;
     int status = DALM_NONCONVERGED;
     double e_out = 0;
+    bool finish64 = false;
     ctx->outer_it = 0;
+    ctx->fp64_finish_it = 0;
     for (int pass = 0; pass < 8; ++pass) {
         hipLaunchKernelGGL(kh_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it - total, root_n);
         hipLaunchKernelGGL(kh_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
@@ -293,9 +299,16 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             break;
         }
         if (total >= max_it) break;
+        const double start_err = e_out;
+        // the pressure at the start of the pass, restored if the pass makes it worse
+        HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
+        double best = start_err, anchor = start_err;  // lowest residual; last 30% milestone
+        long long anchor_it = 0;
+        bool stalled = false;
         long long launched = 0;
         while (!ctx->h_solve[0].done && launched < max_it - total) {
             long long chunk = chunk_for(launched, ctx->prev_it[1] - total, max_it - total);
+            if (chunk > 32) chunk = 32;
             for (long long c = 0; c < chunk; ++c) {
                 probe_begin(ctx, DALM_K_PUPDATE);
                 hipLaunchKernelGGL(kh_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->r, ctx->kp,
@@ -327,14 +340,58 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             launched += chunk;
             rc = read_states(ctx);
             if (rc) return rc;
+            if (trace)
+                fprintf(stderr, "[mixed]   it %lld: recursive residual %.3e alpha %.3e omega %.3e\n",
+                        ctx->h_solve[0].it, ctx->h_solve[0].err, ctx->h_solve[0].alpha, ctx->h_solve[0].omega);
+            const double e = ctx->h_solve[0].err;
+            const long long itn = ctx->h_solve[0].it;
+            if (ctx->h_solve[0].done) break;
+            if (e < best) best = e;
+            if (e < 0.7 * anchor) {
+                anchor = e;
+                anchor_it = itn;
+            }
+            // divergence, or no 30% gain in 96 iterations
+            if (!(e == e) || e > 100 * best || itn - anchor_it >= 96) {
+                stalled = true;
+                break;
+            }
         }
         total += ctx->h_solve[0].it;
         status = ctx->h_solve[0].status;
         e_out = ctx->h_solve[0].err;
-        if (status != DALM_OK) break;  // max_it, breakdown or non-finite: as the FP64 solver
+        if (stalled || status == DALM_BREAKDOWN || status == DALM_NONFINITE) {
+            // true residual of the pass's end; keep the better of start and end
+            hipLaunchKernelGGL(kh_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, 1, root_n);
+            hipLaunchKernelGGL(kh_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
+                               ctx->fr0, ctx->partials);
+            LAUNCH_CHECK();
+            if ((rc = launch_finalize(ctx, FIN_HX_INIT, ctx->grid, st, nullptr))) return rc;
+            if ((rc = read_states(ctx))) return rc;
+            const double end_err = ctx->h_solve[0].err;
+            if (!(end_err <= start_err))
+                HIP_TRY(hipMemcpyAsync(ctx->p, ctx->tmp2, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+            if (trace)
+                fprintf(stderr, "[mixed] FP32 preconditioning stalled at %.3e (pass start %.3e): FP64 finish\n",
+                        end_err, start_err);
+            finish64 = true;
+            break;
+        }
+        if (status != DALM_OK) break;  // max_it: as the FP64 solver
         // converged by the recursive residual: the next pass checks the true one
     }
     ctx->prev_it[1] = total;
+    if (finish64 && total < max_it) {
+        long long it64 = 0;
+        status = run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
+        if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
+        ctx->fp64_finish_it = it64;
+        total += it64;
+        if (trace) fprintf(stderr, "[mixed] FP64 finish: %lld iterations, residual %.3e\n", it64, e_out);
+    } else if (finish64) {
+        status = DALM_NONCONVERGED;
+    }
     *it = total;
     *err = e_out;
     return status;

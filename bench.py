@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed]
                     [--grid 256,128,128] [--no-cpu] [--probe jacobi]
+                    [--adjoint-steps 20] [--adjoint-every 10] [--no-adjoint]
 
 Workload (one "step" = one Solver::main_loop_once): the single-turbine ALM case on
 256x128x128 cells (box 16x8x8, y/z tanh beta=1.5, rotor D=1 at x=3, 3 blades x 64
@@ -16,6 +17,11 @@ launch over its mean launch time, timed with HIP events on the library's stream.
 cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) when
 that library is present and loads, else the C restatement in oracle/ (one core),
 on a bounded sample of this workload (see "sample").
+adjoint_forward_ratio: on the differentiation-form workload (192x96x96, three
+rotors, FP64, BiCGStab(Jacobi 3) to 1e-8), the wall time per step of the reverse
+sweep (segment replays from checkpoints plus reverse steps, adjoint solves to 1e-8
+relative) over the wall time per step of the recorded forward run of the same
+steps (which includes keeping the pressure of every step).
 """
 from __future__ import annotations
 
@@ -191,6 +197,46 @@ def _cpu_baseline(case, gpu_it_mean):
             "seconds_per_step": t_step}
 
 
+def run_adjoint(steps, every, warmup=2):
+    """Forward vs reverse-sweep cost on the differentiation-form workload."""
+    cases = importlib.import_module("3dalm_amd.cases")
+    solver = importlib.import_module("3dalm_amd.solver")
+    case = cases.case_c4(steps=steps)
+    omega = np.array([st.rot_speed for st in case.sites])
+    log(f"[bench] adjoint: grid={case.mesh.interior} steps={steps} every={every}")
+    s = solver.Solver(case)
+    try:
+        for _ in range(warmup):
+            s.step()
+        its = []
+        s.ctx.adjoint_begin(steps, every, omega / steps, True)
+        sync()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            its.append(s.step()["it"])
+        sync()
+        t_rec = (time.perf_counter() - t1) / steps
+        t2 = time.perf_counter()
+        J, ap_g, wt_g, st = s.ctx.adjoint_run(1e-8)
+        sync()
+        t_rev = (time.perf_counter() - t2) / steps
+        s.ctx.adjoint_end()
+    finally:
+        s.close()
+    log(f"[bench] adjoint: recorded forward {t_rec * 1e3:.2f} ms/step, reverse "
+        f"{t_rev * 1e3:.2f} ms/step (replay {st['replay_ms'] / steps:.2f} + reverse steps "
+        f"{st['reverse_ms'] / steps:.2f}); adjoint solve {st['adj_it'] / steps:.1f} it/step, forward "
+        f"{np.mean(its):.1f} it/step")
+    return {"ratio": t_rev / t_rec, "forward_ms": t_rec * 1e3,
+            "reverse_ms": t_rev * 1e3, "replay_ms": st["replay_ms"] / steps,
+            "reverse_steps_ms": st["reverse_ms"] / steps, "adjoint_it_per_step": st["adj_it"] / steps,
+            "forward_it_per_step": float(np.mean(its)), "steps": steps, "checkpoint_every": every,
+            "workload": (f"{case.mesh.interior[0]}x{case.mesh.interior[1]}x{case.mesh.interior[2]}, "
+                         f"{len(case.sites)} rotors x 3 blades x {case.ap_per_blade} points, FP64, "
+                         "pressure kept per step"),
+            "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +246,9 @@ def main():
     ap.add_argument("--grid", default="256,128,128")
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--adjoint-every", type=int, default=10)
+    ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the whole domain lives on one MI355X and this build has "
@@ -236,6 +285,12 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": pr["kernel"],
                 "launches": pr["launches"], "mean_us": round(mean_s * 1e6, 2),
                 "bytes_per_launch": bytes_per_launch}
+    adj = None
+    if not a.no_adjoint:
+        try:
+            adj = run_adjoint(a.adjoint_steps, a.adjoint_every)
+        except Exception as e:
+            log(f"[bench] adjoint leg failed: {e}")
     cpu = None
     if not a.no_cpu:
         try:
@@ -268,7 +323,8 @@ def main():
         "forms": forms_out,
         "mixed_vs_fp64": (round(results["mixed"]["mcell_steps_s"] / results["fp64"]["mcell_steps_s"], 3)
                           if "mixed" in results and "fp64" in results else None),
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": round(adj["ratio"], 3) if adj else None,
+        "adjoint": adj,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
