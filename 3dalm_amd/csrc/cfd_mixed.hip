@@ -423,12 +423,6 @@ int setup_mixed(dalm_ctx* ctx) {
         (rc = down(P.as_u, d.n1, &ctx->pcf.asu, true)) || (rc = down(P.at_u, d.n2, &ctx->pcf.atu, true)) ||
         (rc = down(P.ab_u, d.n2, &ctx->pcf.abu, true)))
         return rc;
-    // row sums of the normalised operator (src/eq.h:742-747): only the x+ row keeps
-    // its Dirichlet term
-    std::vector<float> adir(d.n0, 0.f);
-    const int il = d.n0 - d.gc - 1;
-    adir[il] = (float)(-2 / (ctx->hdx[il] * ctx->hdx[il]) / P.max_diag);
-    if ((rc = dev_upload(ctx, adir, &ctx->pcf.adir))) return rc;
     return DALM_OK;
 }
 

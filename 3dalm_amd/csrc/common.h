@@ -27,6 +27,9 @@ struct Dims {
     int si, sj;         // strides of i and j (k stride is 1)
     int tk, tj;         // tiles along k and j: ceil(nz/TK), ceil(ny/TJ)
     int ntiles;         // nx * tj * tk  (one tile = 1 x TJ x TK interior cells)
+    int ni;             // planes per column unit (column-streaming stencil kernels)
+    int nch;            // column units along i: ceil(nx / ni)
+    int nunits;         // tj * tk * nch
     DALM_HD int at(int i, int j, int k) const { return i * si + j * sj + k; }
 };
 
@@ -56,6 +59,30 @@ __device__ __forceinline__ Cell tile_cell(const Dims& d, int t) {
     c.i = d.gc + it;
     c.inside = (c.k < d.gc + d.nz) && (c.j < d.gc + d.ny);
     return c;
+}
+
+// Column unit of a stencil kernel that streams along i: one block owns a TK x TJ
+// (k, j) tile and ni consecutive interior planes, so each thread walks its own
+// (j, k) line and keeps x[i-1], x[i], x[i+1] in registers -- every value of the
+// streamed vector is read once per unit (plus one halo plane on each end) instead
+// of three times from the caches.  Units are numbered column-major over i so that
+// the consecutive units a single XCD receives (virtual_block) share halo planes.
+struct Col {
+    int i0, i1, j, k;
+    bool ok;  // (j, k) inside the interior (ragged tiles have idle lanes)
+};
+__device__ __forceinline__ Col col_unit(const Dims& d) {
+    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
+    const int ic = u % d.nch, col = u / d.nch;
+    const int kt = col % d.tk, jt = col / d.tk;
+    Col r;
+    r.k = d.gc + kt * TK + (int)(threadIdx.x % TK);
+    r.j = d.gc + jt * TJ + (int)(threadIdx.x / TK);
+    r.i0 = d.gc + ic * d.ni;
+    r.i1 = r.i0 + d.ni;
+    if (r.i1 > d.gc + d.nx) r.i1 = d.gc + d.nx;
+    r.ok = r.k < d.gc + d.nz && r.j < d.gc + d.ny;
+    return r;
 }
 
 // Contiguous chunk of tiles for this block.

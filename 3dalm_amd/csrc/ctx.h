@@ -80,7 +80,6 @@ struct MetricF {
 struct PCoefF {
     const float *ae, *aw, *an, *as, *at, *ab;  // normalised off-diagonals
     const float *acx, *anu, *asu, *atu, *abu;  // normalised diagonal pieces
-    const float* adir;                         // [n0] row sum: -2/dx^2/max_diag at the x+ row, else 0
 };
 
 struct AlmTables {
@@ -143,7 +142,8 @@ struct dalm_ctx {
     long long L = 0, N = 0;
     int precision = DALM_FP64;
     hipStream_t stream = nullptr;
-    int grid = 0;  // blocks for the cell kernels
+    int grid = 0;   // blocks for the cell kernels (tile loop)
+    int sgrid = 0;  // blocks of the column-streaming stencil kernels (= d.nunits)
 
     // mesh (device)
     double *x = nullptr, *y = nullptr, *z = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr;

@@ -315,6 +315,17 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     ctx->precision = (int)precision;
     int g = d.ntiles < 2048 ? d.ntiles : 2048;
     ctx->grid = g >= 64 ? (g / 8) * 8 : g;
+    // column units: about 2048 of them, at least 4 planes each
+    {
+        long long per_plane = (long long)d.tj * d.tk;
+        long long ni = (d.nx * per_plane + 2047) / 2048;
+        if (ni < 4) ni = 4;
+        if (ni > d.nx) ni = d.nx;
+        d.ni = (int)ni;
+        d.nch = (d.nx + d.ni - 1) / d.ni;
+        d.nunits = (int)(per_plane * d.nch);
+        ctx->sgrid = d.nunits;
+    }
 
     ctx->hx.assign(x, x + d.n0);
     ctx->hy.assign(y, y + d.n1);
@@ -377,7 +388,7 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
         for (int q = 0; q < 10; ++q)
             if ((rc = dev_alloc(ctx, sc[q], L))) return fail(rc);
     }
-    if ((rc = dev_alloc(ctx, &ctx->partials, 8 * (size_t)ctx->grid)) ||
+    if ((rc = dev_alloc(ctx, &ctx->partials, 8 * (size_t)std::max(ctx->grid, ctx->sgrid))) ||
         (rc = dev_alloc(ctx, &ctx->counter, 16)) || (rc = dev_alloc(ctx, &ctx->solve, 2)) ||
         (rc = dev_alloc(ctx, &ctx->scal, 1)))
         return fail(rc);
@@ -742,7 +753,7 @@ int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
 
 int dalm_probe_kernel(dalm_ctx* ctx, int64_t kernel) {
     CHECK_CTX();
-    if (kernel < 0 || kernel > DALM_K_SUPDATE) return DALM_BAD_ARG;
+    if (kernel < 0 || kernel > DALM_K_SUPDATE_MIXED) return DALM_BAD_ARG;
     TRY(probe_collect(ctx));
     ctx->probe.kernel = (int)kernel;
     ctx->probe.total_ms = 0;

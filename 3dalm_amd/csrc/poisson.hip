@@ -34,6 +34,14 @@ __device__ __forceinline__ double apply7(const Dims& d, const PCoef& A, const do
            A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
 }
 
+// the same product with the i-neighbours supplied from registers (column
+// streaming) -- identical operands in identical order
+__device__ __forceinline__ double apply7s(const Dims& d, const PCoef& A, const double* x, int c, int i, int j,
+                                          int k, double ac, double x0, double xp, double xm) {
+    return ac * x0 + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * x[c + d.sj] + A.as[j] * x[c - d.sj] +
+           A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
+}
+
 __device__ __forceinline__ bool finished(const SolveState* st) {
     return st != nullptr && st->done;
 }
@@ -43,14 +51,22 @@ __global__ __launch_bounds__(BLOCK) void k_sor(Dims d, PCoef A, double* __restri
                                                const double* __restrict__ b, double omega,
                                                int color, const SolveState* st) {
     if (finished(st)) return;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside || ((cc.i + cc.j + cc.k) & 1) != color) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double ac = diag_at(A, cc.i, cc.j, cc.k);
-        double rel = (b[c] - apply7(d, A, x, c, cc.i, cc.j, cc.k, ac)) / ac;
-        x[c] = x[c] + omega * rel;
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const int j = cl.j, k = cl.k;
+    int c = d.at(cl.i0, j, k);
+    // the i-neighbours of a cell of this colour have the other colour, which this
+    // sweep does not change
+    double xm = x[c - d.si], x0 = x[c];
+    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+        const double xp = x[c + d.si];
+        if (((i + j + k) & 1) == color) {
+            double ac = diag_at(A, i, j, k);
+            double rel = (b[c] - apply7s(d, A, x, c, i, j, k, ac, x0, xp, xm)) / ac;
+            x[c] = x0 + omega * rel;
+        }
+        xm = x0;
+        x0 = xp;
     }
 }
 
@@ -62,16 +78,21 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Dims d, PCoef A, const double
                                                    double* __restrict__ r, double* partials,
                                                    unsigned* counter, SolveState* st, int count_it) {
     if (count_it && finished(st)) return;
-    TileRange tr = block_tiles(d);
+    const Col cl = col_unit(d);
     double v[1] = {0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double ac = diag_at(A, cc.i, cc.j, cc.k);
-        double rv = b[c] - apply7(d, A, x, c, cc.i, cc.j, cc.k, ac);
-        if (r) r[c] = rv;
-        v[0] += rv * rv;
+    if (cl.ok) {
+        const int j = cl.j, k = cl.k;
+        int c = d.at(cl.i0, j, k);
+        double xm = x[c - d.si], x0 = x[c];
+        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+            const double xp = x[c + d.si];
+            double ac = diag_at(A, i, j, k);
+            double rv = b[c] - apply7s(d, A, x, c, i, j, k, ac, x0, xp, xm);
+            if (r) r[c] = rv;
+            v[0] += rv * rv;
+            xm = x0;
+            x0 = xp;
+        }
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
@@ -99,17 +120,22 @@ __global__ __launch_bounds__(BLOCK) void k_bi_init(Dims d, PCoef A, const double
                                                    double* __restrict__ r, double* __restrict__ r0,
                                                    double* partials, unsigned* counter,
                                                    SolveState* st) {
-    TileRange tr = block_tiles(d);
+    const Col cl = col_unit(d);
     double v[1] = {0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double ac = diag_at(A, cc.i, cc.j, cc.k);
-        double rv = b[c] - apply7(d, A, x, c, cc.i, cc.j, cc.k, ac);
-        r[c] = rv;
-        r0[c] = rv;
-        v[0] += rv * rv;
+    if (cl.ok) {
+        const int j = cl.j, k = cl.k;
+        int c = d.at(cl.i0, j, k);
+        double xm = x[c - d.si], x0 = x[c];
+        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+            const double xp = x[c + d.si];
+            double ac = diag_at(A, i, j, k);
+            double rv = b[c] - apply7s(d, A, x, c, i, j, k, ac, x0, xp, xm);
+            r[c] = rv;
+            r0[c] = rv;
+            v[0] += rv * rv;
+            xm = x0;
+            x0 = xp;
+        }
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
@@ -143,14 +169,18 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dims d, PCoef A, const double*
                                                   const double* __restrict__ v,
                                                   const SolveState* st) {
     if (finished(st)) return;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double ac = diag_at(A, cc.i, cc.j, cc.k);
-        double rel = (v[c] - apply7(d, A, zin, c, cc.i, cc.j, cc.k, ac)) / ac;
-        zout[c] = zin[c] + rel;
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const int j = cl.j, k = cl.k;
+    int c = d.at(cl.i0, j, k);
+    double xm = zin[c - d.si], x0 = zin[c];
+    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+        const double xp = zin[c + d.si];
+        double ac = diag_at(A, i, j, k);
+        double rel = (v[c] - apply7s(d, A, zin, c, i, j, k, ac, x0, xp, xm)) / ac;
+        zout[c] = x0 + rel;
+        xm = x0;
+        x0 = xp;
     }
 }
 
@@ -163,20 +193,25 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv(Dims d, PCoef A, const double
                                                    const double* __restrict__ a1, double* partials,
                                                    unsigned* counter, SolveState* st) {
     if (st->done) return;
-    TileRange tr = block_tiles(d);
+    const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double ac = diag_at(A, cc.i, cc.j, cc.k);
-        double yv = apply7(d, A, z, c, cc.i, cc.j, cc.k, ac);
-        y[c] = yv;
-        if (WHICH == 0) {
-            v[0] += a1[c] * yv;   // <r0, q>
-        } else {
-            v[0] += yv * a1[c];   // <t, s>
-            v[1] += yv * yv;      // <t, t>
+    if (cl.ok) {
+        const int j = cl.j, k = cl.k;
+        int c = d.at(cl.i0, j, k);
+        double xm = z[c - d.si], x0 = z[c];
+        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+            const double xp = z[c + d.si];
+            double ac = diag_at(A, i, j, k);
+            double yv = apply7s(d, A, z, c, i, j, k, ac, x0, xp, xm);
+            y[c] = yv;
+            if (WHICH == 0) {
+                v[0] += a1[c] * yv;   // <r0, q>
+            } else {
+                v[0] += yv * a1[c];   // <t, s>
+                v[1] += yv * yv;      // <t, t>
+            }
+            xm = x0;
+            x0 = xp;
         }
     }
     block_sum<2>(v);
@@ -259,31 +294,35 @@ long long next_chunk(long long done_it, long long predicted, long long max_it) {
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
 int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color) {
-    hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x, b,
+    hipLaunchKernelGGL(k_sor, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x, b,
                        omega, color, (const SolveState*)nullptr);
     LAUNCH_CHECK();
     return DALM_OK;
 }
 
 int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r) {
-    hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x,
+    hipLaunchKernelGGL(k_resnorm, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(), x,
                        b, r, ctx->partials, ctx->counter, ctx->solve, 0);
     LAUNCH_CHECK();
-    return launch_finalize(ctx, FIN_RESID, ctx->grid, ctx->solve, nullptr);
+    return launch_finalize(ctx, FIN_RESID, ctx->sgrid, ctx->solve, nullptr);
 }
 
 __global__ void k_Ax_plain(Dims d, PCoef A, const double* __restrict__ z, double* __restrict__ y) {
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        y[c] = apply7(d, A, z, c, cc.i, cc.j, cc.k, diag_at(A, cc.i, cc.j, cc.k));
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const int j = cl.j, k = cl.k;
+    int c = d.at(cl.i0, j, k);
+    double xm = z[c - d.si], x0 = z[c];
+    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+        const double xp = z[c + d.si];
+        y[c] = apply7s(d, A, z, c, i, j, k, diag_at(A, i, j, k), x0, xp, xm);
+        xm = x0;
+        x0 = xp;
     }
 }
 
 int launch_Ax(dalm_ctx* ctx, const double* x, double* y) {
-    hipLaunchKernelGGL(k_Ax_plain, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(),
+    hipLaunchKernelGGL(k_Ax_plain, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->coef(),
                        x, y);
     LAUNCH_CHECK();
     return DALM_OK;
@@ -301,15 +340,15 @@ int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega
         for (long long c = 0; c < chunk; ++c) {
             for (int color = 0; color < 2; ++color) {
                 probe_begin(ctx, DALM_K_SOR);
-                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b, omega,
+                hipLaunchKernelGGL(k_sor, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b, omega,
                                    color, ctx->solve);
                 probe_end(ctx, DALM_K_SOR);
             }
             probe_begin(ctx, DALM_K_RESNORM);
-            hipLaunchKernelGGL(k_resnorm, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b,
+            hipLaunchKernelGGL(k_resnorm, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b,
                                ctx->r, ctx->partials, ctx->counter, ctx->solve, 1);
             probe_end(ctx, DALM_K_RESNORM);
-            if (int rc = launch_finalize(ctx, FIN_SOR, ctx->grid, ctx->solve, nullptr)) return rc;
+            if (int rc = launch_finalize(ctx, FIN_SOR, ctx->sgrid, ctx->solve, nullptr)) return rc;
         }
         LAUNCH_CHECK();
         launched += chunk;
@@ -333,10 +372,10 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     PCoef A = ctx->coef();
     const Dims& d = ctx->d;
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it, root_n);
-    hipLaunchKernelGGL(k_bi_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, x, b, ctx->r, ctx->r0,
+    hipLaunchKernelGGL(k_bi_init, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, x, b, ctx->r, ctx->r0,
                        ctx->partials, ctx->counter, ctx->solve);
     LAUNCH_CHECK();
-    if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->grid, ctx->solve, nullptr)) return rc0;
+    if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->sgrid, ctx->solve, nullptr)) return rc0;
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     // preconditioner application for one vector; returns the buffer holding z
@@ -346,7 +385,7 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             double* other = spare;
             for (int m = 1; m < pc_it; ++m) {
                 probe_begin(ctx, DALM_K_JACOBI);
-                hipLaunchKernelGGL(k_jacobi, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
+                hipLaunchKernelGGL(k_jacobi, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
                                    other, v, (const SolveState*)ctx->solve);
                 probe_end(ctx, DALM_K_JACOBI);
                 double* tsw = cur;
@@ -355,9 +394,9 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             }
         } else {
             for (int m = 0; m < pc_it; ++m) {
-                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
+                hipLaunchKernelGGL(k_sor, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
                                    pc_omega, 0, (const SolveState*)ctx->solve);
-                hipLaunchKernelGGL(k_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
+                hipLaunchKernelGGL(k_sor, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
                                    pc_omega, 1, (const SolveState*)ctx->solve);
             }
         }
@@ -375,20 +414,20 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             probe_end(ctx, DALM_K_PUPDATE);
             double* ppz = precondition(ctx->pp, ctx->tmp, ctx->kp);
             probe_begin(ctx, DALM_K_SPMV);
-            hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
+            hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
                                ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
-            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->grid, ctx->solve, nullptr)) return rc1;
+            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, ctx->solve, nullptr)) return rc1;
             probe_begin(ctx, DALM_K_SUPDATE);
             hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
                                ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
             probe_end(ctx, DALM_K_SUPDATE);
             double* ssz = precondition(ctx->ss, ctx->tmp2, ctx->s);
             probe_begin(ctx, DALM_K_SPMV);
-            hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
+            hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
                                ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
-            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->grid, ctx->solve, nullptr)) return rc2;
+            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, ctx->solve, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
             hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
                                ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);

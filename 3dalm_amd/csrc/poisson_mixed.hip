@@ -36,40 +36,32 @@ namespace {
 __device__ __forceinline__ float diag_f(const PCoefF& A, int i, int j, int k) {
     return (((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k];
 }
-// A x in difference form: sum_nb a_nb (x_nb - x_c) + a_dir x_c.  The operator's
-// rows sum to zero except the x+ Dirichlet row (a_dir = -2/dx^2 / max_diag there),
-// so this equals the 7-point product; but for smooth x the 7 terms of the plain
-// product cancel to ~1e-5 of their size (the normalised operator's smallest
-// eigenvalues), which in FP32 leaves a few per cent of error on exactly the modes
-// the Krylov solve needs last.  Differences of nearby FP32 values are exact
-// (Sterbenz), so here each term keeps FP32 relative accuracy.
-__device__ __forceinline__ float apply7f(const Dims& d, const PCoefF& A, const float* x, int c, int i, int j,
-                                         int k) {
-    const float xc = x[c];
-    return A.ae[i] * (x[c + d.si] - xc) + A.aw[i] * (x[c - d.si] - xc) + A.an[j] * (x[c + d.sj] - xc) +
-           A.as[j] * (x[c - d.sj] - xc) + A.at[k] * (x[c + 1] - xc) + A.ab[k] * (x[c - 1] - xc) +
-           A.adir[i] * xc;
+// 7-point product in FP32 in the reference's order, the i-neighbours from
+// registers (column streaming)
+__device__ __forceinline__ float apply7fs(const Dims& d, const PCoefF& A, const float* x, int c, int i, int j,
+                                          int k, float ac, float xc, float xp, float xm) {
+    return ac * xc + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * x[c + d.sj] + A.as[j] * x[c - d.sj] +
+           A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
 }
 __device__ __forceinline__ double diag64(const PCoef& A, int i, int j, int k) {
     return ((((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k]) / A.max_diag;
-}
-__device__ __forceinline__ double apply7d(const Dims& d, const PCoef& A, const double* x, int c, int i, int j,
-                                          int k, double ac) {
-    return ac * x[c] + A.ae[i] * x[c + d.si] + A.aw[i] * x[c - d.si] + A.an[j] * x[c + d.sj] +
-           A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float* __restrict__ zin,
                                                    float* __restrict__ zout, const float* __restrict__ v,
                                                    const SolveState* st) {
     if (st->done) return;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        float ac = diag_f(A, cc.i, cc.j, cc.k);
-        zout[c] = zin[c] + (v[c] - apply7f(d, A, zin, c, cc.i, cc.j, cc.k)) * __builtin_amdgcn_rcpf(ac);
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const int j = cl.j, k = cl.k;
+    int c = d.at(cl.i0, j, k);
+    float xm = zin[c - d.si], x0 = zin[c];
+    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+        const float xp = zin[c + d.si];
+        float ac = diag_f(A, i, j, k);
+        zout[c] = x0 + (v[c] - apply7fs(d, A, zin, c, i, j, k, ac, x0, xp, xm)) * __builtin_amdgcn_rcpf(ac);
+        xm = x0;
+        x0 = xp;
     }
 }
 
@@ -77,13 +69,19 @@ __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restr
                                                 const float* __restrict__ b, float omega, int color,
                                                 const SolveState* st) {
     if (st->done) return;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside || ((cc.i + cc.j + cc.k) & 1) != color) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        float ac = diag_f(A, cc.i, cc.j, cc.k);
-        x[c] = x[c] + omega * (b[c] - apply7f(d, A, x, c, cc.i, cc.j, cc.k)) * __builtin_amdgcn_rcpf(ac);
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const int j = cl.j, k = cl.k;
+    int c = d.at(cl.i0, j, k);
+    float xm = x[c - d.si], x0 = x[c];
+    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+        const float xp = x[c + d.si];
+        if (((i + j + k) & 1) == color) {
+            float ac = diag_f(A, i, j, k);
+            x[c] = x0 + omega * (b[c] - apply7fs(d, A, x, c, i, j, k, ac, x0, xp, xm)) * __builtin_amdgcn_rcpf(ac);
+        }
+        xm = x0;
+        x0 = xp;
     }
 }
 
@@ -114,18 +112,26 @@ __global__ void kh_state_reset(SolveState* st, double tol, long long max_it, dou
 __global__ __launch_bounds__(BLOCK) void kh_init(Dims d, PCoef A, const double* __restrict__ x,
                                                  const double* __restrict__ b, double* __restrict__ r,
                                                  float* __restrict__ r0, double* partials) {
-    TileRange tr = block_tiles(d);
+    const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double rv = b[c] - apply7d(d, A, x, c, cc.i, cc.j, cc.k, diag64(A, cc.i, cc.j, cc.k));
-        float rf = (float)rv;
-        r[c] = rv;
-        r0[c] = rf;
-        v[0] += (double)rf * rv;
-        v[1] += rv * rv;
+    if (cl.ok) {
+        const int j = cl.j, k = cl.k;
+        int c = d.at(cl.i0, j, k);
+        double xm = x[c - d.si], x0 = x[c];
+        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+            const double xp = x[c + d.si];
+            double ac = diag64(A, i, j, k);
+            double s = ac * x0 + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * x[c + d.sj] + A.as[j] * x[c - d.sj] +
+                       A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
+            double rv = b[c] - s;
+            float rf = (float)rv;
+            r[c] = rv;
+            r0[c] = rf;
+            v[0] += (double)rf * rv;
+            v[1] += rv * rv;
+            xm = x0;
+            x0 = xp;
+        }
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
@@ -181,19 +187,26 @@ __global__ __launch_bounds__(BLOCK) void kh_spmv(Dims d, PCoef A, const float* _
                                                  double* __restrict__ y, const TA* __restrict__ a1,
                                                  double* partials, const SolveState* st) {
     if (st->done) return;
-    TileRange tr = block_tiles(d);
+    const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double yv = apply7h(d, A, z, c, cc.i, cc.j, cc.k, diag64(A, cc.i, cc.j, cc.k));
-        y[c] = yv;
-        if (WHICH == 0) {
-            v[0] += (double)a1[c] * yv;
-        } else {
-            v[0] += yv * (double)a1[c];
-            v[1] += yv * yv;
+    if (cl.ok) {
+        const int j = cl.j, k = cl.k;
+        int c = d.at(cl.i0, j, k);
+        double xm = z[c - d.si], x0 = z[c];
+        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+            const double xp = z[c + d.si];
+            double ac = diag64(A, i, j, k);
+            double yv = ac * x0 + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * (double)z[c + d.sj] +
+                        A.as[j] * (double)z[c - d.sj] + A.at[k] * (double)z[c + 1] + A.ab[k] * (double)z[c - 1];
+            y[c] = yv;
+            if (WHICH == 0) {
+                v[0] += (double)a1[c] * yv;
+            } else {
+                v[0] += yv * (double)a1[c];
+                v[1] += yv * yv;
+            }
+            xm = x0;
+            x0 = xp;
         }
     }
     block_sum<2>(v);
@@ -261,10 +274,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         if (jac) {
             float* other = spare;
             for (int m = 1; m < pc_it; ++m) {
-                probe_begin(ctx, DALM_K_JACOBI);
-                hipLaunchKernelGGL(kf_jacobi, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, cur, other, v32,
+                probe_begin(ctx, DALM_K_JACOBI32);
+                hipLaunchKernelGGL(kf_jacobi, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, cur, other, v32,
                                    (const SolveState*)st);
-                probe_end(ctx, DALM_K_JACOBI);
+                probe_end(ctx, DALM_K_JACOBI32);
                 float* tsw = cur;
                 cur = other;
                 other = tsw;
@@ -272,7 +285,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         } else {
             for (int m = 0; m < pc_it; ++m)
                 for (int color = 0; color < 2; ++color)
-                    hipLaunchKernelGGL(kf_sor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, cur, v32,
+                    hipLaunchKernelGGL(kf_sor, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, cur, v32,
                                        (float)pc_omega, color, (const SolveState*)st);
         }
         return cur;
@@ -285,10 +298,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     ctx->fp64_finish_it = 0;
     for (int pass = 0; pass < 8; ++pass) {
         hipLaunchKernelGGL(kh_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it - total, root_n);
-        hipLaunchKernelGGL(kh_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
+        hipLaunchKernelGGL(kh_init, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
                            ctx->fr0, ctx->partials);
         LAUNCH_CHECK();
-        if (int rc = launch_finalize(ctx, FIN_HX_INIT, ctx->grid, st, nullptr)) return rc;
+        if (int rc = launch_finalize(ctx, FIN_HX_INIT, ctx->sgrid, st, nullptr)) return rc;
         int rc = read_states(ctx);
         if (rc) return rc;
         ctx->outer_it = pass + 1;
@@ -310,30 +323,30 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             long long chunk = chunk_for(launched, ctx->prev_it[1] - total, max_it - total);
             if (chunk > 32) chunk = 32;
             for (long long c = 0; c < chunk; ++c) {
-                probe_begin(ctx, DALM_K_PUPDATE);
+                probe_begin(ctx, DALM_K_PUPDATE_MIXED);
                 hipLaunchKernelGGL(kh_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->r, ctx->kp,
                                    ctx->q, ctx->fp, ctx->fpp, pc_first, (const SolveState*)st);
-                probe_end(ctx, DALM_K_PUPDATE);
+                probe_end(ctx, DALM_K_PUPDATE_MIXED);
                 float* ppz = precondition(ctx->fpp, ctx->ftmp, ctx->fp);
-                probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<0, float>), dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d,
+                probe_begin(ctx, DALM_K_SPMV_MIXED);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<0, float>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d,
                                    A, ppz, ctx->q, ctx->fr0, ctx->partials, (const SolveState*)st);
-                probe_end(ctx, DALM_K_SPMV);
-                if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->grid, st, nullptr))) return rc;
-                probe_begin(ctx, DALM_K_SUPDATE);
+                probe_end(ctx, DALM_K_SPMV_MIXED);
+                if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, st, nullptr))) return rc;
+                probe_begin(ctx, DALM_K_SUPDATE_MIXED);
                 hipLaunchKernelGGL(kh_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->r, ctx->q, ctx->s,
                                    ctx->fs, ctx->fss, pc_first, (const SolveState*)st);
-                probe_end(ctx, DALM_K_SUPDATE);
+                probe_end(ctx, DALM_K_SUPDATE_MIXED);
                 float* ssz = precondition(ctx->fss, ctx->ftmp2, ctx->fs);
-                probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<1, double>), dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream,
+                probe_begin(ctx, DALM_K_SPMV_MIXED);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<1, double>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream,
                                    d, A, ssz, ctx->t, ctx->s, ctx->partials, (const SolveState*)st);
-                probe_end(ctx, DALM_K_SPMV);
-                if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->grid, st, nullptr))) return rc;
-                probe_begin(ctx, DALM_K_UPDATE);
+                probe_end(ctx, DALM_K_SPMV_MIXED);
+                if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, st, nullptr))) return rc;
+                probe_begin(ctx, DALM_K_UPDATE_MIXED);
                 hipLaunchKernelGGL(kh_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ppz, ssz, ctx->s,
                                    ctx->t, ctx->r, ctx->fr0, ctx->partials, (const SolveState*)st);
-                probe_end(ctx, DALM_K_UPDATE);
+                probe_end(ctx, DALM_K_UPDATE_MIXED);
                 if ((rc = launch_finalize(ctx, FIN_BI_XR, ctx->grid, st, nullptr))) return rc;
             }
             LAUNCH_CHECK();
@@ -363,10 +376,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         if (stalled || status == DALM_BREAKDOWN || status == DALM_NONFINITE) {
             // true residual of the pass's end; keep the better of start and end
             hipLaunchKernelGGL(kh_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, 1, root_n);
-            hipLaunchKernelGGL(kh_init, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
+            hipLaunchKernelGGL(kh_init, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
                                ctx->fr0, ctx->partials);
             LAUNCH_CHECK();
-            if ((rc = launch_finalize(ctx, FIN_HX_INIT, ctx->grid, st, nullptr))) return rc;
+            if ((rc = launch_finalize(ctx, FIN_HX_INIT, ctx->sgrid, st, nullptr))) return rc;
             if ((rc = read_states(ctx))) return rc;
             const double end_err = ctx->h_solve[0].err;
             if (!(end_err <= start_err))

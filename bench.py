@@ -51,9 +51,16 @@ PROBE_BYTES = {
     "predictor": 104,  # read Uold(3), JU(3), nut, f(3); write U(3)
     "sor": 16,        # half-sweep: read x, b over the colour + neighbours ~ 2w, write half
     "resnorm": 16,    # read x, b
+    # precision form (FP32 preconditioner vectors, FP64 recurrences)
+    "jacobi32": 12,   # read z_in, v (FP32); write z_out (FP32)
+    "spmv_mixed": 18,  # read z (FP32), r0 (FP32) or s (FP64); write y (FP64): 16 / 20
+    "update_mixed": 52,  # read x, s, t (FP64), pp, ss, r0 (FP32); write x, r
+    "pupdate_mixed": 40,  # read r, p, q; write p, p32, z
+    "supdate_mixed": 32,  # read r, q; write s, s32, z
 }
 PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
-             "supdate": 8}
+             "supdate": 8, "jacobi32": 9, "spmv_mixed": 10, "update_mixed": 11, "pupdate_mixed": 12,
+             "supdate_mixed": 13}
 
 
 def log(msg):
@@ -266,7 +273,10 @@ def main():
     for form in forms:
         case = make_case(grid, a.steps)
         N = case.mesh.N
-        probe = "jacobi" if a.probe == "auto" else (a.probe if a.probe != "none" else None)
+        if a.probe == "auto":
+            probe = "jacobi" if form == "fp64" else "jacobi32"
+        else:
+            probe = a.probe if a.probe != "none" else None
         prec = hip.FP64 if form == "fp64" else hip.MIXED
         res, s = run_form(case, prec, a.steps, a.warmup, probe)
         s.close()
@@ -275,16 +285,19 @@ def main():
         results[form] = res
     main_form = "mixed" if "mixed" in results else "fp64"
     R = results[main_form]
-    roof = None
-    if "probe" in R and R["probe"]["launches"]:
-        pr = R["probe"]
+    def roofline(res):
+        if "probe" not in res or not res["probe"]["launches"]:
+            return None
+        pr = res["probe"]
         mean_s = pr["total_ms"] / pr["launches"] / 1e3
-        bytes_per_launch = PROBE_BYTES[pr["kernel"]] * N * (0.5 if main_form == "mixed" else 1.0)
+        bytes_per_launch = PROBE_BYTES[pr["kernel"]] * N
         ach = bytes_per_launch / mean_s / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": pr["kernel"],
                 "launches": pr["launches"], "mean_us": round(mean_s * 1e6, 2),
                 "bytes_per_launch": bytes_per_launch}
+    roof = roofline(R)
+    roof_fp64 = roofline(results["fp64"]) if "fp64" in results and main_form != "fp64" else None
     adj = None
     if not a.no_adjoint:
         try:
@@ -326,6 +339,7 @@ def main():
         "adjoint_forward_ratio": round(adj["ratio"], 3) if adj else None,
         "adjoint": adj,
         "roofline": roof,
+        "roofline_fp64": roof_fp64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

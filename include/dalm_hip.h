@@ -215,7 +215,13 @@ enum dalm_kernel {
     DALM_K_SOR = 5,       /* one red-black SOR colour sweep           */
     DALM_K_RESNORM = 6,   /* SOR residual + norm                      */
     DALM_K_PUPDATE = 7,   /* BiCGStab p update + first PC sweep       */
-    DALM_K_SUPDATE = 8    /* BiCGStab s update + first PC sweep       */
+    DALM_K_SUPDATE = 8,   /* BiCGStab s update + first PC sweep       */
+    /* the precision form's kernels of its FP32-preconditioned BiCGStab */
+    DALM_K_JACOBI32 = 9,  /* FP32 Jacobi preconditioner sweep         */
+    DALM_K_SPMV_MIXED = 10,    /* A z (FP64 from FP32 z) + dot products */
+    DALM_K_UPDATE_MIXED = 11,  /* x/r update + norms                    */
+    DALM_K_PUPDATE_MIXED = 12, /* p update, FP32 copy, first PC sweep   */
+    DALM_K_SUPDATE_MIXED = 13  /* s update, FP32 copy, first PC sweep   */
 };
 /* Time every launch of `kernel` with HIP events on the context's stream from now
  * on (DALM_K_NONE stops); resets the totals. */
