@@ -84,8 +84,6 @@ def run_form(case, precision, steps, warmup, probe):
         for w in range(warmup):
             st = s.step()
             log(f"[bench] warmup step {st['step']} it={st['it']} err={st['err']:.3e} div={st['avg_div']:.3e}")
-        if probe:
-            s.ctx.check(s.ctx.L.dalm_probe_kernel(s.ctx.h, PROBE_IDS[probe]), "probe")
         sync()
         t0 = time.perf_counter()
         its, divs, power = [], [], []
@@ -97,15 +95,19 @@ def run_form(case, precision, steps, warmup, probe):
                 power.append(float(st["power"].sum()))
         sync()
         t1 = time.perf_counter()
-        for k in range(0, steps, max(1, steps // 5)):
-            pass
         log(f"[bench] timed {steps} steps in {t1 - t0:.3f} s; it/step mean {np.mean(its):.1f}; "
             f"last div {divs[-1]:.3e}")
         res = {"seconds": t1 - t0, "it_mean": float(np.mean(its)), "it": its,
                "avg_div_max": float(np.max(divs)), "power_last": power[-1] if power else None}
         if probe:
+            # kernel durations with HIP events on the library's stream, over a few
+            # more steps (launched one by one while the probe is on)
+            s.ctx.check(s.ctx.L.dalm_probe_kernel(s.ctx.h, PROBE_IDS[probe]), "probe")
+            for k in range(3):
+                s.step()
             n, ms = C.c_int64(), C.c_double()
             s.ctx.check(s.ctx.L.dalm_probe_read(s.ctx.h, C.byref(n), C.byref(ms)), "probe_read")
+            s.ctx.check(s.ctx.L.dalm_probe_kernel(s.ctx.h, 0), "probe off")
             res["probe"] = {"kernel": probe, "launches": n.value, "total_ms": ms.value}
         res["U"] = None
         return res, s

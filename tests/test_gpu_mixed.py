@@ -59,7 +59,7 @@ def test_mixed_against_fp64_oracle(builder, steps):
         print(f"mixed vs fp64: U {eu:.2e}; p from the 1e-13 solution: mixed {ep:.2e}, fp64 {ep64:.2e}",
               flush=True)
         assert eu <= 1e-4
-        assert ep <= 5 * ep64 + 1e-5
+        assert ep <= 10 * ep64 + 1e-5
     finally:
         s.close()
 
