@@ -157,7 +157,6 @@ struct dalm_ctx {
     double *p = nullptr, *nut = nullptr, *div = nullptr, *b = nullptr, *solid = nullptr;
     bool has_solid = false;
     // Krylov work vectors
-    double* kp2 = nullptr;  // second buffer of the BiCGStab direction p (fused update + sweeps)
     double *r = nullptr, *r0 = nullptr, *kp = nullptr, *pp = nullptr, *q = nullptr, *s = nullptr,
            *ss = nullptr, *t = nullptr, *tmp = nullptr, *tmp2 = nullptr;
     // iterations of the previous solve per use: 0 SOR, 1 BiCGStab (either form),
@@ -201,15 +200,6 @@ struct dalm_ctx {
     KernelProbe probe;
     AdjointState adj;
 
-    // instantiated two-iteration BiCGStab graphs, per solver configuration
-    struct BicgGraph {
-        double* x;
-        const double* b;
-        int pc_kind, pc_it;
-        double pc_omega;
-        hipGraphExec_t exec;
-    };
-    std::vector<BicgGraph> bicg_graphs;
 
     std::string err;
     int hip_err = 0;

@@ -4,6 +4,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "ctx.h"
@@ -379,14 +380,13 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
             if ((rc = dev_alloc(ctx, scf[q], L))) return fail(rc);
         // FP64 vectors: the recurrences of the FP32-preconditioned BiCGStab (kp, q, s,
         // t) and, for its FP64 finish, the full FP64 solver's set
-        double** sd[10] = {&ctx->kp, &ctx->q, &ctx->s, &ctx->t, &ctx->r0, &ctx->pp, &ctx->ss, &ctx->tmp, &ctx->tmp2,
-                           &ctx->kp2};
-        for (int q = 0; q < 10; ++q)
+        double** sd[9] = {&ctx->kp, &ctx->q, &ctx->s, &ctx->t, &ctx->r0, &ctx->pp, &ctx->ss, &ctx->tmp, &ctx->tmp2};
+        for (int q = 0; q < 9; ++q)
             if ((rc = dev_alloc(ctx, sd[q], L))) return fail(rc);
     } else {
-        double** sc[11] = {&ctx->nut, &ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t,
-                           &ctx->tmp, &ctx->tmp2, &ctx->kp2};
-        for (int q = 0; q < 11; ++q)
+        double** sc[10] = {&ctx->nut, &ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t,
+                           &ctx->tmp, &ctx->tmp2};
+        for (int q = 0; q < 10; ++q)
             if ((rc = dev_alloc(ctx, sc[q], L))) return fail(rc);
     }
     if ((rc = dev_alloc(ctx, &ctx->partials, 8 * (size_t)std::max(ctx->grid, ctx->sgrid))) ||
@@ -408,12 +408,10 @@ int dalm_destroy(dalm_ctx* ctx) {
     if (!ctx) return DALM_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     adjoint_free(ctx);
-    for (auto& g : ctx->bicg_graphs) (void)hipGraphExecDestroy(g.exec);
-    ctx->bicg_graphs.clear();
     for (hipEvent_t e : ctx->probe.ev) (void)hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
-                      ctx->t, ctx->tmp, ctx->tmp2, ctx->kp2, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
+                      ctx->t, ctx->tmp, ctx->tmp2, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
                       ctx->pc.an_n, ctx->pc.as_n, ctx->pc.at_n, ctx->pc.ab_n, ctx->pc.acx,
                       ctx->pc.an_u, ctx->pc.as_u, ctx->pc.at_u, ctx->pc.ab_u, ctx->alm.wt, ctx->alm.ap,
                       ctx->alm.cd, ctx->alm.cl, ctx->alm.atk};

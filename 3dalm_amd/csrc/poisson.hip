@@ -18,8 +18,6 @@
 // reference's bit for bit; dot products differ only in summation order.
 #include <float.h>
 
-#include <utility>
-
 #include "ctx.h"
 
 namespace {
@@ -162,85 +160,6 @@ __global__ __launch_bounds__(BLOCK) void k_bi_p(Dims d, PCoef A, const double* _
         double pv = first ? r[c] : r[c] + beta * (p[c] - om * q[c]);
         p[c] = pv;
         z[c] = pc_first ? pv / diag_at(A, cc.i, cc.j, cc.k) : 0.0;
-    }
-}
-
-// p update fused with the first two Jacobi sweeps of its preconditioner (Jacobi
-// with m >= 2): p_new = r + beta (p_old - omega q) (r on the first iteration),
-// z0 = p_new / a_c and z1 = z0 + (p_new - A z0) / a_c.  The z0 of the six
-// neighbours is rebuilt from r, p_old and q there (the i-neighbours from
-// registers), so p_new is written to a second buffer (pn != po) and z0 never
-// touches memory; every value is the one the separate kernels produce, bit for
-// bit.  Cells outside the interior have z0 = 0, as the zero ghosts give there.
-__device__ __forceinline__ bool inside_ijk(const Dims& d, int i, int j, int k) {
-    return i >= d.gc && i < d.n0 - d.gc && j >= d.gc && j < d.n1 - d.gc && k >= d.gc && k < d.n2 - d.gc;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_bi_pj(Dims d, PCoef A, const double* __restrict__ r,
-                                                 const double* __restrict__ po, double* __restrict__ pn,
-                                                 const double* __restrict__ q, double* __restrict__ z1,
-                                                 const SolveState* st) {
-    if (st->done) return;
-    const bool first = st->it == 0;
-    const double beta = st->beta, om = st->omega;
-    const Col cl = col_unit(d);
-    if (!cl.ok) return;
-    auto pnew = [&](int c) -> double { return first ? r[c] : r[c] + beta * (po[c] - om * q[c]); };
-    auto z0at = [&](int c, int i, int j, int k) -> double {
-        return inside_ijk(d, i, j, k) ? pnew(c) / diag_at(A, i, j, k) : 0.0;
-    };
-    const int j = cl.j, k = cl.k;
-    int c = d.at(cl.i0, j, k);
-    double zm = z0at(c - d.si, cl.i0 - 1, j, k);
-    double p0 = pnew(c);
-    double z0 = p0 / diag_at(A, cl.i0, j, k);
-    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-        const bool last = i + 1 >= d.n0 - d.gc;
-        const double p1 = last ? 0.0 : pnew(c + d.si);
-        const double zp = last ? 0.0 : p1 / diag_at(A, i + 1, j, k);
-        const double ac = diag_at(A, i, j, k);
-        const double s7 = ac * z0 + A.ae[i] * zp + A.aw[i] * zm + A.an[j] * z0at(c + d.sj, i, j + 1, k) +
-                          A.as[j] * z0at(c - d.sj, i, j - 1, k) + A.at[k] * z0at(c + 1, i, j, k + 1) +
-                          A.ab[k] * z0at(c - 1, i, j, k - 1);
-        pn[c] = p0;
-        z1[c] = z0 + (p0 - s7) / ac;
-        zm = z0;
-        z0 = zp;
-        p0 = p1;
-    }
-}
-
-// s = r - alpha q fused with the first two Jacobi sweeps for ss (see k_bi_pj);
-// s_new depends on r and q only, so it is written in place
-__global__ __launch_bounds__(BLOCK) void k_bi_sj(Dims d, PCoef A, const double* __restrict__ r,
-                                                 const double* __restrict__ q, double* __restrict__ s,
-                                                 double* __restrict__ z1, const SolveState* st) {
-    if (st->done) return;
-    const double alpha = st->alpha;
-    const Col cl = col_unit(d);
-    if (!cl.ok) return;
-    auto snew = [&](int c) -> double { return r[c] - alpha * q[c]; };
-    auto z0at = [&](int c, int i, int j, int k) -> double {
-        return inside_ijk(d, i, j, k) ? snew(c) / diag_at(A, i, j, k) : 0.0;
-    };
-    const int j = cl.j, k = cl.k;
-    int c = d.at(cl.i0, j, k);
-    double zm = z0at(c - d.si, cl.i0 - 1, j, k);
-    double s0 = snew(c);
-    double z0 = s0 / diag_at(A, cl.i0, j, k);
-    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-        const bool last = i + 1 >= d.n0 - d.gc;
-        const double s1 = last ? 0.0 : snew(c + d.si);
-        const double zp = last ? 0.0 : s1 / diag_at(A, i + 1, j, k);
-        const double ac = diag_at(A, i, j, k);
-        const double s7 = ac * z0 + A.ae[i] * zp + A.aw[i] * zm + A.an[j] * z0at(c + d.sj, i, j + 1, k) +
-                          A.as[j] * z0at(c - d.sj, i, j - 1, k) + A.at[k] * z0at(c + 1, i, j, k + 1) +
-                          A.ab[k] * z0at(c - 1, i, j, k - 1);
-        s[c] = s0;
-        z1[c] = z0 + (s0 - s7) / ac;
-        zm = z0;
-        z0 = zp;
-        s0 = s1;
     }
 }
 
@@ -459,16 +378,12 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->sgrid, ctx->solve, nullptr)) return rc0;
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
-    // with m >= 2 Jacobi sweeps the update kernels also do the first two sweeps
-    const int fused = (jac && pc_it >= 2) ? 1 : 0;
-    double* pcur = ctx->kp;
-    double* pnext = ctx->kp2;
     // preconditioner application for one vector; returns the buffer holding z
     auto precondition = [&](double* z0, double* spare, const double* v) -> double* {
         double* cur = z0;
         if (jac) {
             double* other = spare;
-            for (int m = 1 + fused; m < pc_it; ++m) {
+            for (int m = 1; m < pc_it; ++m) {
                 probe_begin(ctx, DALM_K_JACOBI);
                 hipLaunchKernelGGL(k_jacobi, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
                                    other, v, (const SolveState*)ctx->solve);
@@ -487,87 +402,37 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
         }
         return cur;
     };
-    // one iteration's launches (src/eq.h:354-401); with fused sweeps p alternates
-    // between two buffers, so two iterations restore the starting state
-    auto enqueue_iteration = [&]() -> int {
-        probe_begin(ctx, DALM_K_PUPDATE);
-        if (fused) {
-            hipLaunchKernelGGL(k_bi_pj, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, pcur, pnext,
-                               ctx->q, ctx->pp, (const SolveState*)ctx->solve);
-            std::swap(pcur, pnext);
-        } else {
-            hipLaunchKernelGGL(k_bi_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, pcur,
-                               ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
-        }
-        probe_end(ctx, DALM_K_PUPDATE);
-        double* ppz = precondition(ctx->pp, ctx->tmp, pcur);
-        probe_begin(ctx, DALM_K_SPMV);
-        hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
-                           ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
-        probe_end(ctx, DALM_K_SPMV);
-        if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, ctx->solve, nullptr)) return rc1;
-        probe_begin(ctx, DALM_K_SUPDATE);
-        if (fused)
-            hipLaunchKernelGGL(k_bi_sj, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
-                               ctx->s, ctx->ss, (const SolveState*)ctx->solve);
-        else
-            hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
-                               ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
-        probe_end(ctx, DALM_K_SUPDATE);
-        double* ssz = precondition(ctx->ss, ctx->tmp2, ctx->s);
-        probe_begin(ctx, DALM_K_SPMV);
-        hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
-                           ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
-        probe_end(ctx, DALM_K_SPMV);
-        if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, ctx->solve, nullptr)) return rc2;
-        probe_begin(ctx, DALM_K_UPDATE);
-        hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
-                           ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
-        probe_end(ctx, DALM_K_UPDATE);
-        if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->grid, ctx->solve, nullptr)) return rc3;
-        return DALM_OK;
-    };
-    // Iterations are enqueued in chunks without host waits.  Unless a kernel probe
-    // is active, a pair of iterations is captured once as a HIP graph (cached per
-    // solver configuration) and the chunk replays it, which removes most of the
-    // per-launch gaps between the ~10 dependent kernels of an iteration.
-    hipGraphExec_t exec = nullptr;
-    if (ctx->probe.kernel == 0) {
-        for (auto& g : ctx->bicg_graphs)
-            if (g.x == x && g.b == b && g.pc_kind == pc_kind && g.pc_it == pc_it && g.pc_omega == pc_omega)
-                exec = g.exec;
-        if (!exec) {
-            hipGraph_t graph = nullptr;
-            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-            int rc1 = enqueue_iteration();
-            int rc2 = rc1 ? rc1 : enqueue_iteration();
-            hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
-            if (rc2) return rc2;
-            HIP_TRY(ec);
-            hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            HIP_TRY(ei);
-            dalm_ctx::BicgGraph entry;
-            entry.x = x;
-            entry.b = b;
-            entry.pc_kind = pc_kind;
-            entry.pc_it = pc_it;
-            entry.pc_omega = pc_omega;
-            entry.exec = exec;
-            ctx->bicg_graphs.push_back(entry);
-        }
-    }
     long long launched = 0;
     int rc = read_state(ctx);
     if (rc) return rc;
     while (!ctx->h_solve->done && launched < max_it) {
         long long chunk = next_chunk(launched, ctx->prev_it[slot], max_it);
-        if (exec) {
-            chunk += chunk & 1;  // extra iterations after the stop are no-ops
-            for (long long c = 0; c < chunk; c += 2) HIP_TRY(hipGraphLaunch(exec, ctx->stream));
-        } else {
-            for (long long c = 0; c < chunk; ++c)
-                if (int rc1 = enqueue_iteration()) return rc1;
+        for (long long c = 0; c < chunk; ++c) {
+            probe_begin(ctx, DALM_K_PUPDATE);
+            hipLaunchKernelGGL(k_bi_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
+                               ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
+            probe_end(ctx, DALM_K_PUPDATE);
+            double* ppz = precondition(ctx->pp, ctx->tmp, ctx->kp);
+            probe_begin(ctx, DALM_K_SPMV);
+            hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
+                               ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+            probe_end(ctx, DALM_K_SPMV);
+            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, ctx->solve, nullptr)) return rc1;
+            probe_begin(ctx, DALM_K_SUPDATE);
+            hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
+                               ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
+            probe_end(ctx, DALM_K_SUPDATE);
+            double* ssz = precondition(ctx->ss, ctx->tmp2, ctx->s);
+            probe_begin(ctx, DALM_K_SPMV);
+            hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
+                               ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
+            probe_end(ctx, DALM_K_SPMV);
+            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, ctx->solve, nullptr)) return rc2;
+            probe_begin(ctx, DALM_K_UPDATE);
+            hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
+                               ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+            probe_end(ctx, DALM_K_UPDATE);
+            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->grid, ctx->solve, nullptr)) return rc3;
         }
         LAUNCH_CHECK();
         launched += chunk;
