@@ -258,12 +258,25 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a kernel of known traffic for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the whole domain lives on one MI355X and this build has "
               "no multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     hip = importlib.import_module("3dalm_amd.hip")
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {a.copy_calibration} MiB done")
+        del src, dst
     grid = tuple(int(v) for v in a.grid.split(","))
     forms = [f for f in a.forms.split(",") if f]
     have_mixed = getattr(hip, "MIXED_BUILT", False)
