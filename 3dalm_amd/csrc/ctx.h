@@ -30,8 +30,16 @@ struct SolveState {
     long long it, max_it;
     int done;       // 1 once the stopping test (or a breakdown) has ended the solve
     int status;     // DALM_OK, DALM_NONCONVERGED, DALM_BREAKDOWN, DALM_NONFINITE
-    int pad[2];
+    int ru;         // precision form: a residual replacement is due
+    int reliable;   // CG: 1 with reduced-precision vectors (stop on the true residual)
+    double maxr;    // precision form: largest |r| since the last replacement
+    long long n_ru; // precision form: replacements so far
 };
+
+// Precision form: the recursive FP32 residual is replaced by the true FP64 one
+// once its norm falls below this fraction of the largest norm since the last
+// replacement.
+constexpr double kMixedReliableDelta = 0.1;
 
 // Per-step quantities returned to the host in one copy.
 struct StepScalars {
@@ -102,7 +110,14 @@ struct AlmTables {
 struct KernelProbe {
     int kernel = 0;
     std::vector<hipEvent_t> ev;
-    size_t used = 0;
+    size_t used = 0;               // events recorded (two per launch)
+    // Solver iterations are enqueued in chunks ahead of the device's stopping
+    // test, so launches past it return at once; they are tagged with their
+    // iteration and left out of the totals once the solve's count is known.
+    std::vector<long long> tag;    // per recorded launch: solver iteration, -1 outside a solve
+    std::vector<char> idle;        // per recorded launch: enqueued past the stopping point
+    long long cur_tag = -1;        // iteration being enqueued by the running solve
+    size_t solve_first = 0;        // first launch recorded by the running solve
     double total_ms = 0;
     long long launches = 0;
 };
@@ -167,15 +182,16 @@ struct dalm_ctx {
     PoissonCoef pc{};
     bool have_A = false;
 
-    // precision form (DALM_MIXED): FP32 cell fields and preconditioner vectors
-    // (shadow residual, copies of the preconditioned vectors, Jacobi iterates);
-    // JU, p, b, div and the BiCGStab recurrence vectors stay FP64
+    // precision form (DALM_MIXED): FP32 cell fields and the FP32 BiCGStab vectors
+    // (r, r0, p, q, s, t, preconditioner iterates z, z' and their Jacobi partners);
+    // JU, p, b and div stay FP64, and the FP64 Krylov set above serves the FP64
+    // finish of a stalled solve
     float* U32[3] = {};
     float* Uold32[3] = {};
     float* f32[3] = {};
     float* nut32 = nullptr;
-    float *fr0 = nullptr, *fp = nullptr, *fpp = nullptr, *fs = nullptr, *fss = nullptr, *ftmp = nullptr,
-          *ftmp2 = nullptr;
+    float *fr = nullptr, *fr0 = nullptr, *fp = nullptr, *fq = nullptr, *fs = nullptr, *ft = nullptr,
+          *fz = nullptr, *fz2 = nullptr, *ftmp = nullptr, *ftmp2 = nullptr;
     MetricF mf{};
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
@@ -214,7 +230,8 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_HX_INIT };
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU,
+               FIN_CG_INIT, FIN_CG_RHO, FIN_CG_ALPHA, FIN_CG_XR, FIN_CG_RU };
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
 
 int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
@@ -244,6 +261,11 @@ int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega
                long long* it, double* err);
 int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_kind, double pc_omega, int pc_it,
                     long long max_it, double tol, long long* it, double* err);
+// poisson_cg.hip: Jacobi-preconditioned CG in the cell-volume inner product (FP64
+// vectors), and the precision form's CG with FP32 vectors on p / b
+int run_cg_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_it, long long max_it, double tol,
+              long long* it, double* err);
+int run_cg_mixed(dalm_ctx* ctx, int pc_it, long long max_it, double tol, long long* it, double* err);
 int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r);
 int launch_Ax(dalm_ctx* ctx, const double* x, double* y);
 int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color);
@@ -260,6 +282,9 @@ int sync_stream(dalm_ctx* ctx);
 int zero_device(dalm_ctx* ctx, double* a, long long n);
 void probe_begin(dalm_ctx* ctx, int kernel);
 void probe_end(dalm_ctx* ctx, int kernel);
+void probe_solve_begin(dalm_ctx* ctx);
+void probe_iteration(dalm_ctx* ctx, long long i);
+void probe_solve_end(dalm_ctx* ctx, long long iterations);
 int probe_collect(dalm_ctx* ctx);
 
 // precision form (cfd_mixed.hip, poisson_mixed.hip, alm.hip)

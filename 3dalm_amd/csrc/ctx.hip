@@ -47,6 +47,23 @@ void probe_end(dalm_ctx* ctx, int kernel) {
     if (P.kernel == 0 || P.kernel != kernel || P.used + 2 > P.ev.size()) return;
     (void)hipEventRecord(P.ev[P.used + 1], ctx->stream);
     P.used += 2;
+    P.tag.push_back(P.cur_tag);
+    P.idle.push_back(0);
+}
+
+void probe_solve_begin(dalm_ctx* ctx) {
+    ctx->probe.solve_first = ctx->probe.used / 2;
+    ctx->probe.cur_tag = -1;
+}
+
+void probe_iteration(dalm_ctx* ctx, long long i) { ctx->probe.cur_tag = i; }
+
+void probe_solve_end(dalm_ctx* ctx, long long iterations) {
+    KernelProbe& P = ctx->probe;
+    for (size_t q = P.solve_first; q < P.used / 2; ++q)
+        if (P.tag[q] >= iterations) P.idle[q] = 1;
+    P.cur_tag = -1;
+    P.solve_first = P.used / 2;
 }
 
 int probe_collect(dalm_ctx* ctx) {
@@ -54,12 +71,16 @@ int probe_collect(dalm_ctx* ctx) {
     if (P.used == 0) return DALM_OK;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (size_t q = 0; q + 1 < P.used; q += 2) {
+        if (P.idle[q / 2]) continue;
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, P.ev[q], P.ev[q + 1]));
         P.total_ms += ms;
         P.launches += 1;
     }
     P.used = 0;
+    P.solve_first = 0;
+    P.tag.clear();
+    P.idle.clear();
     return DALM_OK;
 }
 
@@ -252,6 +273,8 @@ int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given) {
         TRY(launch_rhs(ctx, P.dt, ctx->pc.max_diag));
         if (P.method == DALM_BICGSTAB)
             status = run_bicgstab(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+        else if (P.method == DALM_CG)
+            status = run_cg_on(ctx, ctx->p, ctx->b, 1, (int)P.pc_it, P.max_it, P.tol, &it, &err);
         else
             status = run_sor(ctx, P.omega, P.max_it, P.tol, &it, &err);
         if (status == DALM_HIP_ERROR) return status;
@@ -374,12 +397,13 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     for (int q = 0; q < 5; ++q)
         if ((rc = dev_alloc(ctx, sc64[q], L))) return fail(rc);
     if (mixed) {
-        // FP32 preconditioner vectors and the FP64 BiCGStab recurrence vectors
-        float** scf[8] = {&ctx->nut32, &ctx->fr0, &ctx->fp, &ctx->fpp, &ctx->fs, &ctx->fss, &ctx->ftmp, &ctx->ftmp2};
-        for (int q = 0; q < 8; ++q)
+        // FP32 BiCGStab vectors (ghost shells stay zero: kernels write interiors only)
+        float** scf[11] = {&ctx->nut32, &ctx->fr, &ctx->fr0, &ctx->fp, &ctx->fq,   &ctx->fs,
+                           &ctx->ft,    &ctx->fz, &ctx->fz2, &ctx->ftmp, &ctx->ftmp2};
+        for (int q = 0; q < 11; ++q)
             if ((rc = dev_alloc(ctx, scf[q], L))) return fail(rc);
-        // FP64 vectors: the recurrences of the FP32-preconditioned BiCGStab (kp, q, s,
-        // t) and, for its FP64 finish, the full FP64 solver's set
+        // FP64 vectors for the FP64 finish of a stalled solve (the FP64 solver's set;
+        // tmp2 also keeps the pressure at the start of the solve)
         double** sd[9] = {&ctx->kp, &ctx->q, &ctx->s, &ctx->t, &ctx->r0, &ctx->pp, &ctx->ss, &ctx->tmp, &ctx->tmp2};
         for (int q = 0; q < 9; ++q)
             if ((rc = dev_alloc(ctx, sd[q], L))) return fail(rc);
@@ -428,7 +452,8 @@ int dalm_destroy(dalm_ctx* ctx) {
         if (ctx->Uold32[q]) (void)hipFree(ctx->Uold32[q]);
         if (ctx->f32[q]) (void)hipFree(ctx->f32[q]);
     }
-    float* fptrs[] = {ctx->nut32, ctx->fr0, ctx->fp, ctx->fpp, ctx->fs, ctx->fss, ctx->ftmp, ctx->ftmp2};
+    float* fptrs[] = {ctx->nut32, ctx->fr, ctx->fr0, ctx->fp,   ctx->fq,   ctx->fs,
+                      ctx->ft,    ctx->fz, ctx->fz2, ctx->ftmp, ctx->ftmp2};
     for (float* p : fptrs)
         if (p) (void)hipFree(p);
     for (void* p : ctx->owned_f) (void)hipFree(p);
@@ -498,9 +523,11 @@ int dalm_download(dalm_ctx* ctx, int64_t field, double* host) {
 int dalm_set_params(dalm_ctx* ctx, const dalm_params* prm) {
     CHECK_CTX();
     if (!prm || !(prm->dt > 0) || !(prm->Re > 0) || prm->max_it < 1 ||
-        (prm->method != DALM_SOR && prm->method != DALM_BICGSTAB) ||
+        (prm->method != DALM_SOR && prm->method != DALM_BICGSTAB && prm->method != DALM_CG) ||
         (prm->pc_kind != DALM_PC_JACOBI && prm->pc_kind != DALM_PC_SOR) || prm->pc_it < 0)
         return dalm_fail_msg(ctx, DALM_BAD_ARG, "invalid dalm_params");
+    if (prm->method == DALM_CG && (prm->pc_kind != DALM_PC_JACOBI || prm->pc_it < 1))
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "CG needs the Jacobi preconditioner with at least one sweep");
     ctx->prm = *prm;
     ctx->have_prm = true;
     return DALM_OK;
@@ -557,6 +584,16 @@ int dalm_run_bicgstab(dalm_ctx* ctx, int64_t pc_kind, double pc_omega, int64_t p
     long long i = 0;
     double e = 0;
     int rc = run_bicgstab(ctx, (int)pc_kind, pc_omega, (int)pc_it, max_it, tol, &i, &e);
+    if (it) *it = i;
+    if (err) *err = e;
+    return rc;
+}
+int dalm_run_cg(dalm_ctx* ctx, int64_t pc_it, int64_t max_it, double tol, int64_t* it, double* err) {
+    CHECK_FP64();
+    if (max_it < 1 || pc_it < 1) return DALM_BAD_ARG;
+    long long i = 0;
+    double e = 0;
+    int rc = run_cg_on(ctx, ctx->p, ctx->b, 1, (int)pc_it, max_it, tol, &i, &e);
     if (it) *it = i;
     if (err) *err = e;
     return rc;
@@ -723,7 +760,9 @@ static int step_mixed(dalm_ctx* ctx, double t, dalm_step_stats* stats) {
     TRY(launch_flux_rhs_m(ctx, P.Uin, P.dt, ctx->pc.max_diag, true));
     long long it = 0;
     double err = 0;
-    int status = run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+    int status = P.method == DALM_CG
+                     ? run_cg_mixed(ctx, (int)P.pc_it, P.max_it, P.tol, &it, &err)
+                     : run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
     if (status == DALM_HIP_ERROR) return status;
     TRY(launch_pbc(ctx));
     TRY(launch_project_m(ctx, P.dt, P.Uin));
@@ -754,7 +793,7 @@ int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
 
 int dalm_probe_kernel(dalm_ctx* ctx, int64_t kernel) {
     CHECK_CTX();
-    if (kernel < 0 || kernel > DALM_K_SUPDATE_MIXED) return DALM_BAD_ARG;
+    if (kernel < 0 || kernel > DALM_K_CG_UPDATE) return DALM_BAD_ARG;
     TRY(probe_collect(ctx));
     ctx->probe.kernel = (int)kernel;
     ctx->probe.total_ms = 0;

@@ -335,9 +335,11 @@ int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega
     LAUNCH_CHECK();
     PCoef A = ctx->coef();
     long long launched = 0;
+    probe_solve_begin(ctx);
     for (;;) {
         long long chunk = next_chunk(launched, ctx->prev_it[slot], max_it);
         for (long long c = 0; c < chunk; ++c) {
+            probe_iteration(ctx, launched + c);
             for (int color = 0; color < 2; ++color) {
                 probe_begin(ctx, DALM_K_SOR);
                 hipLaunchKernelGGL(k_sor, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, ctx->d, A, x, b, omega,
@@ -356,6 +358,7 @@ int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega
         if (rc) return rc;
         if (ctx->h_solve->done || launched >= max_it) break;
     }
+    probe_solve_end(ctx, ctx->h_solve->it);
     *it = ctx->h_solve->it;
     *err = ctx->h_solve->err;
     ctx->prev_it[slot] = *it;
@@ -405,9 +408,11 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     long long launched = 0;
     int rc = read_state(ctx);
     if (rc) return rc;
+    probe_solve_begin(ctx);
     while (!ctx->h_solve->done && launched < max_it) {
         long long chunk = next_chunk(launched, ctx->prev_it[slot], max_it);
         for (long long c = 0; c < chunk; ++c) {
+            probe_iteration(ctx, launched + c);
             probe_begin(ctx, DALM_K_PUPDATE);
             hipLaunchKernelGGL(k_bi_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
                                ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
@@ -439,6 +444,7 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
         rc = read_state(ctx);
         if (rc) return rc;
     }
+    probe_solve_end(ctx, ctx->h_solve->it);
     *it = ctx->h_solve->it;
     *err = ctx->h_solve->err;
     ctx->prev_it[slot] = *it;

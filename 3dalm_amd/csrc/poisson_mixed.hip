@@ -1,29 +1,36 @@
 // poisson_mixed.hip -- Poisson solve of the precision form.
 //
-// Preconditioned BiCGStab (the recurrences of src/eq.h:333-473) whose iterate,
-// residual and direction vectors (x = p, r, p, s, q = A z, t = A z') and every dot
-// product stay FP64, while the preconditioner -- m Jacobi sweeps or SOR double
-// sweeps per application, more than half of the solver's memory traffic -- runs on
-// FP32 vectors: FP32 copies of the vector being preconditioned and FP32 iterates z.
-// The operator products A z are evaluated in FP64 from the FP32 z, so the residual
-// recurrence stays consistent with the FP64 iterate; the shadow vector r0 (only
-// ever used in dot products) is FP32.  A solve ends with an FP64 true-residual
-// check |b - A p| / sqrt(N) <= tol (the reference's test, src/eq.h:399-401); if the
-// recursive residual drifted above it, the solve restarts from the true residual
-// (outer iterative refinement in FP64).
+// Preconditioned BiCGStab with the recurrences of src/eq.h:333-473, run on FP32
+// vectors with FP64 dot products, FP64 norms and an FP64 iterate:
 //
-// The limit of FP32 here: the normalised operator of a stretched grid is badly
-// conditioned (about 2.5e6 at 256x128x128 with tanh beta = 1.5).  Rounding a
-// preconditioned direction z to FP32 adds high-frequency noise of relative size
-// u32, which A amplifies by up to cond(A) against the smooth part of A z -- so
-// once the residual is dominated by smooth modes (about 4e-7 at that size) the
-// FP32-preconditioned iteration stagnates or diverges.  A watchdog on the
-// recursive residual detects that; the pass is then abandoned (the pressure
-// restored to its start if the pass made it worse) and the solve is finished by
-// the FP64 BiCGStab, so every step meets the same tolerance as the FP64 form.
+//   FP32 storage  r, r0, p, q = A z, s, t = A z', the preconditioner iterates z, z'
+//                 (Jacobi sweeps or SOR double sweeps in FP32 arithmetic)
+//   FP64          the pressure x (updated every iteration: x += alpha z + omega z',
+//                 exactly the FP32 z that produced q and t), every dot product and
+//                 norm (FP32 products are exact in FP64), the scalars alpha, omega,
+//                 beta, rho, and the products A z (rounded to FP32 once)
 //
-// Algorithmic bytes per iteration (m = 3): 208 per cell against 280 for the FP64
-// solver; iterations reported = BiCGStab iterations over all passes.
+// Reliable updates (residual replacement, Sleijpen & van der Vorst 1996; van der
+// Vorst & Ye 2000): the FP32 recursive residual drifts from b - A x by the FP32
+// rounding it accumulates, so whenever its norm has fallen below delta = 0.1 of
+// the largest norm since the last replacement -- and whenever it meets the
+// stopping test -- the true residual b - A x is computed in FP64 from the FP64
+// pressure and replaces it (rounded to FP32); the search direction p is kept, so
+// the Krylov process continues instead of restarting.  The solve ends only when
+// the TRUE residual meets the reference's test |b - A x| / sqrt(N) <= tol
+// (src/eq.h:399-401), so the pressure meets the same tolerance as the FP64 form.
+// The decisions are taken on the device (SolveState.ru); the replacement kernel
+// and its finalize are enqueued every iteration and return at once when no
+// replacement is due.
+//
+// Safety net: a watchdog on the residual (no 30% gain in 192 iterations, or a
+// rise above 1e3 x the best, or a breakdown) abandons the FP32 iteration and
+// finishes the solve with the FP64 BiCGStab from the better of the pressures
+// before and after, so every step meets the tolerance whatever the conditioning.
+//
+// Algorithmic bytes per iteration, Jacobi m = 3, per cell: p update 20, Jacobi
+// sweeps 4 x 12, q = A z 12, s update 16, t = A z' 12, x/r update 40 = 148,
+// against 280 for the FP64 solver; a replacement costs 24.
 #include <float.h>
 
 #include <cstdio>
@@ -85,16 +92,7 @@ __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restr
     }
 }
 
-
-// A z in FP64 arithmetic from an FP32 vector, in the FP64 form's summation order
-__device__ __forceinline__ double apply7h(const Dims& d, const PCoef& A, const float* z, int c, int i, int j, int k,
-                                          double ac) {
-    return ac * (double)z[c] + A.ae[i] * (double)z[c + d.si] + A.aw[i] * (double)z[c - d.si] +
-           A.an[j] * (double)z[c + d.sj] + A.as[j] * (double)z[c - d.sj] + A.at[k] * (double)z[c + 1] +
-           A.ab[k] * (double)z[c - 1];
-}
-
-__global__ void kh_state_reset(SolveState* st, double tol, long long max_it, double root_n) {
+__global__ void kg_state_reset(SolveState* st, double tol, long long max_it, double root_n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->rho = st->rho_prev = st->alpha = st->omega = 1.0;
         st->beta = 0;
@@ -105,13 +103,20 @@ __global__ void kh_state_reset(SolveState* st, double tol, long long max_it, dou
         st->max_it = max_it;
         st->done = 0;
         st->status = DALM_OK;
+        st->ru = 0;
+        st->n_ru = 0;
+        st->maxr = 0;
     }
 }
 
-// start of a pass: r = b - A x (FP64), r0 = (float) r; partials <r0, r>, |r|^2
-__global__ __launch_bounds__(BLOCK) void kh_init(Dims d, PCoef A, const double* __restrict__ x,
-                                                 const double* __restrict__ b, double* __restrict__ r,
-                                                 float* __restrict__ r0, double* partials) {
+// True residual in FP64: r64 = b - A x, stored rounded as r32; INIT also sets the
+// shadow r0 = r32.  Partials <r0, r32>, |r64|^2.  Without INIT this is the
+// reliable update, which returns at once unless the device state asks for one.
+template <bool INIT>
+__global__ __launch_bounds__(BLOCK) void kg_resid(Dims d, PCoef A, const double* __restrict__ x,
+                                                  const double* __restrict__ b, float* __restrict__ r,
+                                                  float* __restrict__ r0, double* partials, const SolveState* st) {
+    if (!INIT && (st->done || !st->ru)) return;
     const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
     if (cl.ok) {
@@ -125,9 +130,15 @@ __global__ __launch_bounds__(BLOCK) void kh_init(Dims d, PCoef A, const double* 
                        A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
             double rv = b[c] - s;
             float rf = (float)rv;
-            r[c] = rv;
-            r0[c] = rf;
-            v[0] += (double)rf * rv;
+            r[c] = rf;
+            float r0v;
+            if (INIT) {
+                r0[c] = rf;
+                r0v = rf;
+            } else {
+                r0v = r0[c];
+            }
+            v[0] += (double)r0v * (double)rf;
             v[1] += rv * rv;
             xm = x0;
             x0 = xp;
@@ -138,12 +149,11 @@ __global__ __launch_bounds__(BLOCK) void kh_init(Dims d, PCoef A, const double* 
     store_partials<2>(pv, partials);
 }
 
-// p = r (first iteration) or r + beta (p - omega q) in FP64; its FP32 copy, and
-// the first preconditioner sweep from zero (Jacobi: z = p / a_c; SOR: z = 0)
-__global__ __launch_bounds__(BLOCK) void kh_p(Dims d, PCoefF Af, const double* __restrict__ r,
-                                              double* __restrict__ p, const double* __restrict__ q,
-                                              float* __restrict__ p32, float* __restrict__ z, int pc_first,
-                                              const SolveState* st) {
+// p = r (first iteration) or r + beta (p - omega q), and the first preconditioner
+// sweep from zero (Jacobi: z = p / a_c; SOR: z = 0)
+__global__ __launch_bounds__(BLOCK) void kg_p(Dims d, PCoefF Af, const float* __restrict__ r,
+                                              float* __restrict__ p, const float* __restrict__ q,
+                                              float* __restrict__ z, int pc_first, const SolveState* st) {
     if (st->done) return;
     const bool first = st->it == 0;
     const double beta = st->beta, om = st->omega;
@@ -152,19 +162,16 @@ __global__ __launch_bounds__(BLOCK) void kh_p(Dims d, PCoefF Af, const double* _
         Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         int c = d.at(cc.i, cc.j, cc.k);
-        double pv = first ? r[c] : r[c] + beta * (p[c] - om * q[c]);
-        p[c] = pv;
-        float pf = (float)pv;
-        p32[c] = pf;
+        float pf = first ? r[c] : (float)((double)r[c] + beta * ((double)p[c] - om * (double)q[c]));
+        p[c] = pf;
         z[c] = pc_first ? pf * __builtin_amdgcn_rcpf(diag_f(Af, cc.i, cc.j, cc.k)) : 0.f;
     }
 }
 
-// s = r - alpha q in FP64; its FP32 copy and the first preconditioner sweep
-__global__ __launch_bounds__(BLOCK) void kh_s(Dims d, PCoefF Af, const double* __restrict__ r,
-                                              const double* __restrict__ q, double* __restrict__ s,
-                                              float* __restrict__ s32, float* __restrict__ z, int pc_first,
-                                              const SolveState* st) {
+// s = r - alpha q and the first preconditioner sweep
+__global__ __launch_bounds__(BLOCK) void kg_s(Dims d, PCoefF Af, const float* __restrict__ r,
+                                              const float* __restrict__ q, float* __restrict__ s,
+                                              float* __restrict__ z, int pc_first, const SolveState* st) {
     if (st->done) return;
     const double alpha = st->alpha;
     TileRange tr = block_tiles(d);
@@ -172,19 +179,17 @@ __global__ __launch_bounds__(BLOCK) void kh_s(Dims d, PCoefF Af, const double* _
         Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         int c = d.at(cc.i, cc.j, cc.k);
-        double sv = r[c] - alpha * q[c];
-        s[c] = sv;
-        float sf = (float)sv;
-        s32[c] = sf;
+        float sf = (float)((double)r[c] - alpha * (double)q[c]);
+        s[c] = sf;
         z[c] = pc_first ? sf * __builtin_amdgcn_rcpf(diag_f(Af, cc.i, cc.j, cc.k)) : 0.f;
     }
 }
 
-// y = A z (FP64 from FP32 z) with <a1, y> (WHICH 0: a1 = r0) or <y, a1>, <y, y>
-// (WHICH 1: a1 = s)
-template <int WHICH, typename TA>
-__global__ __launch_bounds__(BLOCK) void kh_spmv(Dims d, PCoef A, const float* __restrict__ z,
-                                                 double* __restrict__ y, const TA* __restrict__ a1,
+// y = A z in FP64 arithmetic, stored rounded to FP32, with <a1, y> (WHICH 0:
+// a1 = r0) or <y, a1>, <y, y> (WHICH 1: a1 = s) of the stored values
+template <int WHICH>
+__global__ __launch_bounds__(BLOCK) void kg_spmv(Dims d, PCoef A, const float* __restrict__ z,
+                                                 float* __restrict__ y, const float* __restrict__ a1,
                                                  double* partials, const SolveState* st) {
     if (st->done) return;
     const Col cl = col_unit(d);
@@ -198,12 +203,14 @@ __global__ __launch_bounds__(BLOCK) void kh_spmv(Dims d, PCoef A, const float* _
             double ac = diag64(A, i, j, k);
             double yv = ac * x0 + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * (double)z[c + d.sj] +
                         A.as[j] * (double)z[c - d.sj] + A.at[k] * (double)z[c + 1] + A.ab[k] * (double)z[c - 1];
-            y[c] = yv;
+            const float yf = (float)yv;
+            y[c] = yf;
+            const double ys = (double)yf;
             if (WHICH == 0) {
-                v[0] += (double)a1[c] * yv;
+                v[0] += (double)a1[c] * ys;
             } else {
-                v[0] += yv * (double)a1[c];
-                v[1] += yv * yv;
+                v[0] += ys * (double)a1[c];
+                v[1] += ys * ys;
             }
             xm = x0;
             x0 = xp;
@@ -214,10 +221,10 @@ __global__ __launch_bounds__(BLOCK) void kh_spmv(Dims d, PCoef A, const float* _
     store_partials<2>(pv, partials);
 }
 
-// x += alpha z_p + omega z_s; r = s - omega t; partials |r|^2, <r0, r>
-__global__ __launch_bounds__(BLOCK) void kh_xr(Dims d, double* __restrict__ x, const float* __restrict__ pp,
-                                               const float* __restrict__ ss, const double* __restrict__ s,
-                                               const double* __restrict__ t_, double* __restrict__ r,
+// x += alpha z + omega z' (FP64); r = s - omega t; partials |r|^2, <r0, r>
+__global__ __launch_bounds__(BLOCK) void kg_xr(Dims d, double* __restrict__ x, const float* __restrict__ zp,
+                                               const float* __restrict__ zs, const float* __restrict__ s,
+                                               const float* __restrict__ t_, float* __restrict__ r,
                                                const float* __restrict__ r0, double* partials,
                                                const SolveState* st) {
     if (st->done) return;
@@ -228,9 +235,10 @@ __global__ __launch_bounds__(BLOCK) void kh_xr(Dims d, double* __restrict__ x, c
         Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         int c = d.at(cc.i, cc.j, cc.k);
-        x[c] = x[c] + alpha * (double)pp[c] + om * (double)ss[c];
-        double rv = s[c] - om * t_[c];
-        r[c] = rv;
+        x[c] = x[c] + alpha * (double)zp[c] + om * (double)zs[c];
+        const float rf = (float)((double)s[c] - om * (double)t_[c]);
+        r[c] = rf;
+        const double rv = (double)rf;
         v[0] += rv * rv;
         v[1] += (double)r0[c] * rv;
     }
@@ -249,7 +257,7 @@ int read_states(dalm_ctx* ctx) {
 long long chunk_for(long long done_it, long long predicted, long long max_it) {
     long long left = predicted - done_it;
     long long c = left > 8 ? left : 8;
-    if (c > 256) c = 256;
+    if (c > 64) c = 64;
     if (done_it + c > max_it) c = max_it - done_it;
     return c < 1 ? 1 : c;
 }
@@ -290,120 +298,109 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         }
         return cur;
     };
-    long long total = 0;
-    int status = DALM_NONCONVERGED;
-    double e_out = 0;
-    bool finish64 = false;
     ctx->outer_it = 0;
     ctx->fp64_finish_it = 0;
-    for (int pass = 0; pass < 8; ++pass) {
-        hipLaunchKernelGGL(kh_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it - total, root_n);
-        hipLaunchKernelGGL(kh_init, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
-                           ctx->fr0, ctx->partials);
-        LAUNCH_CHECK();
-        if (int rc = launch_finalize(ctx, FIN_HX_INIT, ctx->sgrid, st, nullptr)) return rc;
-        int rc = read_states(ctx);
-        if (rc) return rc;
-        ctx->outer_it = pass + 1;
-        e_out = ctx->h_solve[0].err;
-        if (trace) fprintf(stderr, "[mixed] pass %d: fp64 residual %.3e after %lld iterations\n", pass, e_out, total);
-        if (ctx->h_solve[0].done) {
-            status = ctx->h_solve[0].status;
-            break;
-        }
-        if (total >= max_it) break;
-        const double start_err = e_out;
-        // the pressure at the start of the pass, restored if the pass makes it worse
+    hipLaunchKernelGGL(kg_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it, root_n);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<true>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p,
+                       ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+    LAUNCH_CHECK();
+    if (int rc = launch_finalize(ctx, FIN_MX_INIT, ctx->sgrid, st, nullptr)) return rc;
+    int rc = read_states(ctx);
+    if (rc) return rc;
+    const double start_err = ctx->h_solve[0].err;
+    if (trace) fprintf(stderr, "[mixed] fp64 residual %.3e at the start\n", start_err);
+    bool stalled = false;
+    if (!ctx->h_solve[0].done) {
+        // the pressure at the start, restored if the FP32 iteration makes it worse
         HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
         double best = start_err, anchor = start_err;  // lowest residual; last 30% milestone
-        long long anchor_it = 0;
-        bool stalled = false;
-        long long launched = 0;
-        while (!ctx->h_solve[0].done && launched < max_it - total) {
-            long long chunk = chunk_for(launched, ctx->prev_it[1] - total, max_it - total);
-            if (chunk > 32) chunk = 32;
+        long long anchor_it = 0, launched = 0;
+        probe_solve_begin(ctx);
+        while (!ctx->h_solve[0].done && launched < max_it) {
+            const long long chunk = chunk_for(launched, ctx->prev_it[1], max_it);
             for (long long c = 0; c < chunk; ++c) {
+                probe_iteration(ctx, launched + c);
                 probe_begin(ctx, DALM_K_PUPDATE_MIXED);
-                hipLaunchKernelGGL(kh_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->r, ctx->kp,
-                                   ctx->q, ctx->fp, ctx->fpp, pc_first, (const SolveState*)st);
+                hipLaunchKernelGGL(kg_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fp,
+                                   ctx->fq, ctx->fz, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_PUPDATE_MIXED);
-                float* ppz = precondition(ctx->fpp, ctx->ftmp, ctx->fp);
+                float* zp = precondition(ctx->fz, ctx->ftmp, ctx->fp);
                 probe_begin(ctx, DALM_K_SPMV_MIXED);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<0, float>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d,
-                                   A, ppz, ctx->q, ctx->fr0, ctx->partials, (const SolveState*)st);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv<0>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
+                                   zp, ctx->fq, ctx->fr0, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SPMV_MIXED);
                 if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_SUPDATE_MIXED);
-                hipLaunchKernelGGL(kh_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->r, ctx->q, ctx->s,
-                                   ctx->fs, ctx->fss, pc_first, (const SolveState*)st);
+                hipLaunchKernelGGL(kg_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fq,
+                                   ctx->fs, ctx->fz2, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SUPDATE_MIXED);
-                float* ssz = precondition(ctx->fss, ctx->ftmp2, ctx->fs);
+                float* zs = precondition(ctx->fz2, ctx->ftmp2, ctx->fs);
                 probe_begin(ctx, DALM_K_SPMV_MIXED);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(kh_spmv<1, double>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream,
-                                   d, A, ssz, ctx->t, ctx->s, ctx->partials, (const SolveState*)st);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv<1>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
+                                   zs, ctx->ft, ctx->fs, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SPMV_MIXED);
                 if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_UPDATE_MIXED);
-                hipLaunchKernelGGL(kh_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ppz, ssz, ctx->s,
-                                   ctx->t, ctx->r, ctx->fr0, ctx->partials, (const SolveState*)st);
+                hipLaunchKernelGGL(kg_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, zp, zs, ctx->fs,
+                                   ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_UPDATE_MIXED);
-                if ((rc = launch_finalize(ctx, FIN_BI_XR, ctx->grid, st, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_MX_XR, ctx->grid, st, nullptr))) return rc;
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<false>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d,
+                                   A, ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+                if ((rc = launch_finalize(ctx, FIN_MX_RU, ctx->sgrid, st, nullptr))) return rc;
             }
             LAUNCH_CHECK();
             launched += chunk;
-            rc = read_states(ctx);
-            if (rc) return rc;
+            if ((rc = read_states(ctx))) return rc;
+            const SolveState& h = ctx->h_solve[0];
             if (trace)
-                fprintf(stderr, "[mixed]   it %lld: recursive residual %.3e alpha %.3e omega %.3e\n",
-                        ctx->h_solve[0].it, ctx->h_solve[0].err, ctx->h_solve[0].alpha, ctx->h_solve[0].omega);
-            const double e = ctx->h_solve[0].err;
-            const long long itn = ctx->h_solve[0].it;
-            if (ctx->h_solve[0].done) break;
+                fprintf(stderr, "[mixed]   it %lld: residual %.3e (%lld replacements) alpha %.3e omega %.3e\n", h.it,
+                        h.err, h.n_ru, h.alpha, h.omega);
+            if (h.done) break;
+            const double e = h.err;
             if (e < best) best = e;
             if (e < 0.7 * anchor) {
                 anchor = e;
-                anchor_it = itn;
+                anchor_it = h.it;
             }
-            // divergence, or no 30% gain in 96 iterations
-            if (!(e == e) || e > 100 * best || itn - anchor_it >= 96) {
+            if (!(e == e) || e > 1e3 * best || h.it - anchor_it >= 192) {
                 stalled = true;
                 break;
             }
         }
-        total += ctx->h_solve[0].it;
-        status = ctx->h_solve[0].status;
-        e_out = ctx->h_solve[0].err;
-        if (stalled || status == DALM_BREAKDOWN || status == DALM_NONFINITE) {
-            // true residual of the pass's end; keep the better of start and end
-            hipLaunchKernelGGL(kh_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, 1, root_n);
-            hipLaunchKernelGGL(kh_init, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p, ctx->b, ctx->r,
-                               ctx->fr0, ctx->partials);
-            LAUNCH_CHECK();
-            if ((rc = launch_finalize(ctx, FIN_HX_INIT, ctx->sgrid, st, nullptr))) return rc;
-            if ((rc = read_states(ctx))) return rc;
-            const double end_err = ctx->h_solve[0].err;
-            if (!(end_err <= start_err))
-                HIP_TRY(hipMemcpyAsync(ctx->p, ctx->tmp2, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
-                                       ctx->stream));
-            if (trace)
-                fprintf(stderr, "[mixed] FP32 preconditioning stalled at %.3e (pass start %.3e): FP64 finish\n",
-                        end_err, start_err);
-            finish64 = true;
-            break;
-        }
-        if (status != DALM_OK) break;  // max_it: as the FP64 solver
-        // converged by the recursive residual: the next pass checks the true one
     }
+    const SolveState& h = ctx->h_solve[0];
+    probe_solve_end(ctx, h.it);
+    long long total = h.it;
+    int status = h.status;
+    double e_out = h.err;
+    ctx->outer_it = h.n_ru;
+    if (!h.done && !stalled) status = DALM_NONCONVERGED;
     ctx->prev_it[1] = total;
-    if (finish64 && total < max_it) {
-        long long it64 = 0;
-        status = run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
-        if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
-        ctx->fp64_finish_it = it64;
-        total += it64;
-        if (trace) fprintf(stderr, "[mixed] FP64 finish: %lld iterations, residual %.3e\n", it64, e_out);
-    } else if (finish64) {
-        status = DALM_NONCONVERGED;
+    if (stalled || status == DALM_BREAKDOWN || status == DALM_NONFINITE) {
+        // true residual at the end; keep the better of start and end, then finish in FP64
+        hipLaunchKernelGGL(kg_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, 1, root_n);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<true>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
+                           ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+        LAUNCH_CHECK();
+        if ((rc = launch_finalize(ctx, FIN_MX_INIT, ctx->sgrid, st, nullptr))) return rc;
+        if ((rc = read_states(ctx))) return rc;
+        const double end_err = ctx->h_solve[0].err;
+        if (!(end_err <= start_err))
+            HIP_TRY(hipMemcpyAsync(ctx->p, ctx->tmp2, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
+        if (trace)
+            fprintf(stderr, "[mixed] FP32 iteration stalled at %.3e (start %.3e): FP64 finish\n", end_err, start_err);
+        if (total < max_it) {
+            long long it64 = 0;
+            status =
+                run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
+            if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
+            ctx->fp64_finish_it = it64;
+            total += it64;
+            if (trace) fprintf(stderr, "[mixed] FP64 finish: %lld iterations, residual %.3e\n", it64, e_out);
+        } else {
+            status = DALM_NONCONVERGED;
+        }
     }
     *it = total;
     *err = e_out;

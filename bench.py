@@ -1,6 +1,6 @@
 """Benchmark of the 3dalm per-time-step hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed,fp64_cg,mixed_bicg]
                     [--grid 256,128,128] [--no-cpu] [--probe jacobi]
                     [--adjoint-steps 20] [--adjoint-every 10] [--no-adjoint]
 
@@ -10,8 +10,14 @@ actuator points, TSR 6, eps = 2 dx, 5% seeded noise, dt = 0.005), Poisson by
 BiCGStab with 3 Jacobi sweeps as preconditioner to 1e-8.  Synthetic inputs,
 resident in HBM before the timed region.
 
+Forms: "fp64" is the FP64 drop-in with the case's BiCGStab; "mixed" is the
+precision form (FP32 stencils, FP32 CG vectors with FP64 dot products, FP64
+pressure and residual replacements, to the same residual test); "fp64_cg" (CG in
+FP64) and "mixed_bicg" (the precision form with the case's BiCGStab) separate what
+the Krylov method and what the precision contribute.
+
 Prints ONE JSON line (rank 0).  value = Mcell-steps/s = N_interior * K / t / 1e6
-of the precision form when it is built, else of the FP64 drop-in; both forms are
+of the precision form when it is built, else of the FP64 drop-in; every form is
 reported under "forms".  roofline: the dominant kernel's algorithmic bytes per
 launch over its mean launch time, timed with HIP events on the library's stream.
 cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) when
@@ -51,34 +57,48 @@ PROBE_BYTES = {
     "predictor": 104,  # read Uold(3), JU(3), nut, f(3); write U(3)
     "sor": 16,        # half-sweep: read x, b over the colour + neighbours ~ 2w, write half
     "resnorm": 16,    # read x, b
-    # precision form (FP32 preconditioner vectors, FP64 recurrences)
-    "jacobi32": 12,   # read z_in, v (FP32); write z_out (FP32)
-    "spmv_mixed": 18,  # read z (FP32), r0 (FP32) or s (FP64); write y (FP64): 16 / 20
-    "update_mixed": 52,  # read x, s, t (FP64), pp, ss, r0 (FP32); write x, r
-    "pupdate_mixed": 40,  # read r, p, q; write p, p32, z
-    "supdate_mixed": 32,  # read r, q; write s, s32, z
+    # CG (FP32 vectors in the precision form; FP64 values in PROBE_BYTES_FP64)
+    "cg_spmv": 16,    # read z, p_old; write p_new, q
+    "cg_update": 36,  # read x (FP64), p, q, r; write x (FP64), r, z
+    # precision form's BiCGStab (FP32 Krylov vectors, FP64 pressure)
+    "jacobi32": 12,   # read z_in, v; write z_out (FP32)
+    "spmv_mixed": 12,  # read z, r0 or s; write y (FP32)
+    "update_mixed": 40,  # read x (FP64), z, z', s, t, r0 (FP32); write x (FP64), r (FP32)
+    "pupdate_mixed": 20,  # read r, p, q; write p, z (FP32)
+    "supdate_mixed": 16,  # read r, q; write s, z (FP32)
 }
+PROBE_BYTES_FP64 = {"cg_spmv": 32, "cg_update": 56}
 PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
              "supdate": 8, "jacobi32": 9, "spmv_mixed": 10, "update_mixed": 11, "pupdate_mixed": 12,
-             "supdate_mixed": 13}
+             "supdate_mixed": 13, "cg_spmv": 14, "cg_update": 15}
+# form -> (precision, Poisson method)
+FORMS = {"fp64": ("fp64", "BiCG"), "mixed": ("mixed", "CG"), "fp64_cg": ("fp64", "CG"),
+         "mixed_bicg": ("mixed", "BiCG")}
+POISSON = {"BiCG": "BiCGStab+Jacobi3", "CG": "PCG+Jacobi3 (cell-volume inner product)"}
+
+
+def probe_bytes(probe, form):
+    if FORMS[form][0] == "fp64" and probe in PROBE_BYTES_FP64:
+        return PROBE_BYTES_FP64[probe]
+    return PROBE_BYTES[probe]
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_case(grid, steps):
+def make_case(grid, steps, method="BiCG"):
     cases = importlib.import_module("3dalm_amd.cases")
     nx, ny, nz = grid
-    return cases.case_c2(nx, ny, nz, steps=steps)
+    case = cases.case_c2(nx, ny, nz, steps=steps)
+    case.method = method
+    return case
 
 
-def run_form(case, precision, steps, warmup, probe):
-    hip = importlib.import_module("3dalm_amd.hip")
+def run_form(case, precision, steps, warmup, probe, form):
     solver = importlib.import_module("3dalm_amd.solver")
     import ctypes as C
-    log(f"[bench] form={'fp64' if precision == hip.FP64 else 'mixed'} grid={case.mesh.interior} "
-        f"warmup={warmup} steps={steps}")
+    log(f"[bench] form={form} ({case.method}) grid={case.mesh.interior} warmup={warmup} steps={steps}")
     s = solver.Solver(case, precision)
     try:
         for w in range(warmup):
@@ -251,7 +271,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--forms", default="fp64,mixed,fp64_cg,mixed_bicg")
     ap.add_argument("--grid", default="256,128,128")
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
@@ -279,40 +299,46 @@ def main():
         del src, dst
     grid = tuple(int(v) for v in a.grid.split(","))
     forms = [f for f in a.forms.split(",") if f]
+    unknown = [f for f in forms if f not in FORMS]
+    if unknown:
+        print(f"bench.py: unknown form(s) {unknown}; choose from {list(FORMS)}", file=sys.stderr)
+        sys.exit(2)
     have_mixed = getattr(hip, "MIXED_BUILT", False)
-    if "mixed" in forms and not have_mixed:
-        log("[bench] precision form not built in this library; measuring the FP64 drop-in only")
-        forms = [f for f in forms if f != "mixed"]
+    if not have_mixed and any(FORMS[f][0] == "mixed" for f in forms):
+        log("[bench] precision form not built in this library; measuring the FP64 forms only")
+        forms = [f for f in forms if FORMS[f][0] != "mixed"]
     results = {}
     N = None
     for form in forms:
-        case = make_case(grid, a.steps)
+        prec_name, method = FORMS[form]
+        case = make_case(grid, a.steps, method)
         N = case.mesh.N
         if a.probe == "auto":
-            probe = "jacobi" if form == "fp64" else "jacobi32"
+            probe = "jacobi" if prec_name == "fp64" else "jacobi32"
         else:
             probe = a.probe if a.probe != "none" else None
-        prec = hip.FP64 if form == "fp64" else hip.MIXED
-        res, s = run_form(case, prec, a.steps, a.warmup, probe)
+        prec = hip.FP64 if prec_name == "fp64" else hip.MIXED
+        res, s = run_form(case, prec, a.steps, a.warmup, probe, form)
         s.close()
         res["mcell_steps_s"] = N * a.steps / res["seconds"] / 1e6
         res["ms_per_step"] = 1e3 * res["seconds"] / a.steps
         results[form] = res
-    main_form = "mixed" if "mixed" in results else "fp64"
+    main_form = "mixed" if "mixed" in results else next(iter(results))
     R = results[main_form]
-    def roofline(res):
+
+    def roofline(res, form):
         if "probe" not in res or not res["probe"]["launches"]:
             return None
         pr = res["probe"]
         mean_s = pr["total_ms"] / pr["launches"] / 1e3
-        bytes_per_launch = PROBE_BYTES[pr["kernel"]] * N
+        bytes_per_launch = probe_bytes(pr["kernel"], form) * N
         ach = bytes_per_launch / mean_s / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": pr["kernel"],
                 "launches": pr["launches"], "mean_us": round(mean_s * 1e6, 2),
                 "bytes_per_launch": bytes_per_launch}
-    roof = roofline(R)
-    roof_fp64 = roofline(results["fp64"]) if "fp64" in results and main_form != "fp64" else None
+    roof = roofline(R, main_form)
+    roof_fp64 = roofline(results["fp64"], "fp64") if "fp64" in results and main_form != "fp64" else None
     adj = None
     if not a.no_adjoint:
         try:
@@ -327,10 +353,16 @@ def main():
             cpu = cpu_baseline(make_case(grid, 1), it_ref)
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
-    forms_out = {f: {"mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
+    forms_out = {f: {"precision": FORMS[f][0], "poisson": POISSON[FORMS[f][1]],
+                     "mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
                      "it_per_step": round(r["it_mean"], 2), "avg_div_max": r["avg_div_max"],
                      "div_within_1e-6": bool(r["avg_div_max"] <= 1e-6),
                      "power_last": r["power_last"]} for f, r in results.items()}
+
+    def ratio(x, y):
+        if x in results and y in results:
+            return round(results[x]["mcell_steps_s"] / results[y]["mcell_steps_s"], 3)
+        return None
     line = {
         "metric": METRIC,
         "value": round(R["mcell_steps_s"], 3),
@@ -345,12 +377,14 @@ def main():
         "dtype": "f64" if main_form == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": (f"single 3-blade turbine ALM, {grid[0]}x{grid[1]}x{grid[2]} cells, "
-                                f"BiCGStab(Jacobi x3) to 1e-8, form={main_form}"),
-                   "cells": N, "turbines": 1, "actuator_points": 192, "poisson": "BiCGStab+Jacobi3",
-                   "tol": 1e-8},
+                                f"Poisson to 1e-8 (FP64 drop-in: BiCGStab(Jacobi x3)), form={main_form}"),
+                   "cells": N, "turbines": 1, "actuator_points": 192,
+                   "poisson": POISSON[FORMS[main_form][1]], "tol": 1e-8},
         "forms": forms_out,
-        "mixed_vs_fp64": (round(results["mixed"]["mcell_steps_s"] / results["fp64"]["mcell_steps_s"], 3)
-                          if "mixed" in results and "fp64" in results else None),
+        "mixed_vs_fp64": ratio("mixed", "fp64"),
+        "same_method_ratios": {"mixed_bicg_vs_fp64": ratio("mixed_bicg", "fp64"),
+                               "mixed_vs_fp64_cg": ratio("mixed", "fp64_cg"),
+                               "fp64_cg_vs_fp64": ratio("fp64_cg", "fp64")},
         "adjoint_forward_ratio": round(adj["ratio"], 3) if adj else None,
         "adjoint": adj,
         "roofline": roof,

@@ -1,6 +1,7 @@
-"""The precision form (FP32 cell stencils and FP32 preconditioner sweeps; FP64 face
-fluxes, pressure, BiCGStab recurrences, dot products and the true-residual check)
-against the FP64 oracle.
+"""The precision form (FP32 cell stencils, FP32 BiCGStab vectors and preconditioner
+sweeps; FP64 face fluxes, pressure, dot products, norms and the residual
+replacements that end every solve on the true FP64 residual) against the FP64
+oracle.
 
 Stated tolerances (the north star asks for them to be stated against FP64):
   * every step ends with the FP64 residual test met: err <= tol, and therefore
@@ -11,7 +12,7 @@ Stated tolerances (the north star asks for them to be stated against FP64):
   * pressure: the residual test err <= 1e-8 leaves a smooth pressure error that it
     hardly sees (the FP64 oracle at 1e-8 is 1.2e-3 from its own 1e-13 solution at
     64x32x32), so both are measured from the oracle's 1e-13 solution and the
-    precision form may be at most five times as far from it as the FP64 oracle at
+    precision form may be at most ten times as far from it as the FP64 oracle at
     the same tolerance (plus 1e-5).
 """
 import importlib
@@ -29,15 +30,20 @@ po = importlib.import_module("pyoracle")
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("method", ["BiCG", "CG"])
 @pytest.mark.parametrize("builder,steps", [
     (lambda: cases.case_c2(32, 16, 16, steps=6, ap_per_blade=8), 6),
     (lambda: cases.case_c2(64, 32, 32, steps=4, ap_per_blade=16), 4),
 ])
-def test_mixed_against_fp64_oracle(builder, steps):
+def test_mixed_against_fp64_oracle(builder, steps, method):
+    """The FP64 oracle solves with the case's BiCGStab; the precision form with
+    BiCGStab or with its CG (FP32 vectors), to the same residual test."""
     case = builder()
-    print(f"[mixed {case.mesh.interior} steps={steps}]", flush=True)
+    print(f"[mixed {method} {case.mesh.interior} steps={steps}]", flush=True)
     o = po.OracleSolver(case)
-    s = solver.Solver(case, hip.MIXED)
+    mcase = builder()
+    mcase.method = method
+    s = solver.Solver(mcase, hip.MIXED)
     try:
         for n in range(steps):
             so, sm = o.step(), s.step()
@@ -75,5 +81,29 @@ def test_mixed_sor_pc_and_c1():
             assert sm["err"] <= c.tol and sm["avg_div"] <= 1e-6
         m = interior_mask(c.mesh)
         assert rel_l2(s.field("U")[m], o.field("U")[m]) <= 1e-4
+    finally:
+        s.close()
+
+
+def test_fp64_cg_steps_match_oracle():
+    """Whole FP64 steps with the pressure solved by CG instead of BiCGStab: with
+    both solves driven to a 1e-13 residual the steps agree with the oracle's to
+    the solution error that tolerance leaves."""
+    case = cases.case_c2(32, 16, 16, steps=3, ap_per_blade=8)
+    case.tol, case.max_it = 1e-13, 20000
+    print(f"[fp64 cg {case.mesh.interior}]", flush=True)
+    o = po.OracleSolver(case)
+    ccase = cases.case_c2(32, 16, 16, steps=3, ap_per_blade=8)
+    ccase.tol, ccase.max_it, ccase.method = 1e-13, 20000, "CG"
+    s = solver.Solver(ccase, hip.FP64)
+    try:
+        for _ in range(3):
+            so, sg = o.step(), s.step()
+            assert sg["status"] == hip.OK and sg["err"] <= 1e-13
+        m = interior_mask(case.mesh)
+        eu = rel_l2(s.field("U")[m], o.field("U")[m])
+        ep = rel_l2(s.field("p")[m], o.field("p")[m])
+        print(f"fp64 cg vs oracle bicgstab: U {eu:.2e} p {ep:.2e}", flush=True)
+        assert eu <= 1e-10 and ep <= 1e-8
     finally:
         s.close()
