@@ -217,7 +217,7 @@ class Case:
     Re: float = 1e4
     Cs: float = 0.1
     dt: float = 0.005
-    method: str = "SOR"          # "SOR" | "BiCG" | "CG" (CG has no counterpart in the reference)
+    method: str = "SOR"          # "SOR" | "BiCG"
     pc: str = "Jacobi"           # "Jacobi" | "SOR"
     omega: float = 1.2
     pc_omega: float = 1.0
@@ -311,8 +311,8 @@ def write_case_dir(case: Case, path: str, time_steps: Optional[int] = None) -> N
     os.makedirs(path, exist_ok=True)
     steps = case.steps if time_steps is None else time_steps
     eq = {"tolerance": case.tol, "max iteration": case.max_it,
-          "method": case.method if case.method in ("BiCG", "CG") else "SOR"}
-    if case.method in ("BiCG", "CG"):
+          "method": "BiCG" if case.method == "BiCG" else "SOR"}
+    if case.method == "BiCG":
         eq["preconditioner"] = {"method": case.pc, "max iteration": case.pc_it,
                                 "relaxation rate": case.pc_omega}
     else:

@@ -1048,8 +1048,6 @@ int adjoint_solve(dalm_ctx* ctx, int fold_pbc, double tol_rel, long long max_it,
     if (P.method == DALM_BICGSTAB)
         st = run_bicgstab_on(ctx, S.ya, S.ab, 3, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, max_it, tol, it_out,
                              err_out);
-    else if (P.method == DALM_CG)
-        st = run_cg_on(ctx, S.ya, S.ab, 3, (int)P.pc_it, max_it, tol, it_out, err_out);
     else
         st = run_sor_on(ctx, S.ya, S.ab, 3, P.omega, max_it, tol, it_out, err_out);
     if (st == DALM_HIP_ERROR || st == DALM_BAD_ARG) return st;

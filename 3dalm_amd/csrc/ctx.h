@@ -31,7 +31,7 @@ struct SolveState {
     int done;       // 1 once the stopping test (or a breakdown) has ended the solve
     int status;     // DALM_OK, DALM_NONCONVERGED, DALM_BREAKDOWN, DALM_NONFINITE
     int ru;         // precision form: a residual replacement is due
-    int reliable;   // CG: 1 with reduced-precision vectors (stop on the true residual)
+    int pad;
     double maxr;    // precision form: largest |r| since the last replacement
     long long n_ru; // precision form: replacements so far
 };
@@ -230,8 +230,7 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU,
-               FIN_CG_INIT, FIN_CG_RHO, FIN_CG_ALPHA, FIN_CG_XR, FIN_CG_RU };
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU };
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
 
 int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
@@ -261,11 +260,6 @@ int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega
                long long* it, double* err);
 int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_kind, double pc_omega, int pc_it,
                     long long max_it, double tol, long long* it, double* err);
-// poisson_cg.hip: Jacobi-preconditioned CG in the cell-volume inner product (FP64
-// vectors), and the precision form's CG with FP32 vectors on p / b
-int run_cg_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_it, long long max_it, double tol,
-              long long* it, double* err);
-int run_cg_mixed(dalm_ctx* ctx, int pc_it, long long max_it, double tol, long long* it, double* err);
 int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r);
 int launch_Ax(dalm_ctx* ctx, const double* x, double* y);
 int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color);

@@ -273,8 +273,6 @@ int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given) {
         TRY(launch_rhs(ctx, P.dt, ctx->pc.max_diag));
         if (P.method == DALM_BICGSTAB)
             status = run_bicgstab(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
-        else if (P.method == DALM_CG)
-            status = run_cg_on(ctx, ctx->p, ctx->b, 1, (int)P.pc_it, P.max_it, P.tol, &it, &err);
         else
             status = run_sor(ctx, P.omega, P.max_it, P.tol, &it, &err);
         if (status == DALM_HIP_ERROR) return status;
@@ -523,11 +521,9 @@ int dalm_download(dalm_ctx* ctx, int64_t field, double* host) {
 int dalm_set_params(dalm_ctx* ctx, const dalm_params* prm) {
     CHECK_CTX();
     if (!prm || !(prm->dt > 0) || !(prm->Re > 0) || prm->max_it < 1 ||
-        (prm->method != DALM_SOR && prm->method != DALM_BICGSTAB && prm->method != DALM_CG) ||
+        (prm->method != DALM_SOR && prm->method != DALM_BICGSTAB) ||
         (prm->pc_kind != DALM_PC_JACOBI && prm->pc_kind != DALM_PC_SOR) || prm->pc_it < 0)
         return dalm_fail_msg(ctx, DALM_BAD_ARG, "invalid dalm_params");
-    if (prm->method == DALM_CG && (prm->pc_kind != DALM_PC_JACOBI || prm->pc_it < 1))
-        return dalm_fail_msg(ctx, DALM_BAD_ARG, "CG needs the Jacobi preconditioner with at least one sweep");
     ctx->prm = *prm;
     ctx->have_prm = true;
     return DALM_OK;
@@ -584,16 +580,6 @@ int dalm_run_bicgstab(dalm_ctx* ctx, int64_t pc_kind, double pc_omega, int64_t p
     long long i = 0;
     double e = 0;
     int rc = run_bicgstab(ctx, (int)pc_kind, pc_omega, (int)pc_it, max_it, tol, &i, &e);
-    if (it) *it = i;
-    if (err) *err = e;
-    return rc;
-}
-int dalm_run_cg(dalm_ctx* ctx, int64_t pc_it, int64_t max_it, double tol, int64_t* it, double* err) {
-    CHECK_FP64();
-    if (max_it < 1 || pc_it < 1) return DALM_BAD_ARG;
-    long long i = 0;
-    double e = 0;
-    int rc = run_cg_on(ctx, ctx->p, ctx->b, 1, (int)pc_it, max_it, tol, &i, &e);
     if (it) *it = i;
     if (err) *err = e;
     return rc;
@@ -760,9 +746,7 @@ static int step_mixed(dalm_ctx* ctx, double t, dalm_step_stats* stats) {
     TRY(launch_flux_rhs_m(ctx, P.Uin, P.dt, ctx->pc.max_diag, true));
     long long it = 0;
     double err = 0;
-    int status = P.method == DALM_CG
-                     ? run_cg_mixed(ctx, (int)P.pc_it, P.max_it, P.tol, &it, &err)
-                     : run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+    int status = run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
     if (status == DALM_HIP_ERROR) return status;
     TRY(launch_pbc(ctx));
     TRY(launch_project_m(ctx, P.dt, P.Uin));
@@ -793,7 +777,7 @@ int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
 
 int dalm_probe_kernel(dalm_ctx* ctx, int64_t kernel) {
     CHECK_CTX();
-    if (kernel < 0 || kernel > DALM_K_CG_UPDATE) return DALM_BAD_ARG;
+    if (kernel < 0 || kernel > DALM_K_SUPDATE_MIXED) return DALM_BAD_ARG;
     TRY(probe_collect(ctx));
     ctx->probe.kernel = (int)kernel;
     ctx->probe.total_ms = 0;

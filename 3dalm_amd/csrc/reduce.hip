@@ -182,107 +182,6 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         }
         break;
     }
-    case FIN_CG_INIT: {
-        // CG, start of the solve (or a true-residual check): s[0] = <r, z>_V,
-        // s[1] = |b - A x|^2; a residual that already meets the test ends the solve
-        double rn = sqrt(s[1]);
-        double err = rn / st->root_n;
-        st->err = err;
-        st->maxr = rn;
-        st->ru = 0;
-        if (!(err == err) || isinf(err)) {
-            st->done = 1;
-            st->status = DALM_NONFINITE;
-        } else if (err <= st->tol) {
-            st->done = 1;
-            st->status = DALM_OK;
-        }
-        break;
-    }
-    case FIN_CG_RHO: {
-        // CG: rho = <r, z>_V of the preconditioned residual, beta = rho / rho_prev
-        if (st->done) break;
-        double rho_new = s[0];
-        if (!(rho_new == rho_new) || isinf(rho_new)) {
-            st->done = 1;
-            st->status = DALM_NONFINITE;
-        } else if (rho_new == 0.0) {
-            st->err = rho_new;
-            st->done = 1;
-            st->status = DALM_BREAKDOWN;
-        } else {
-            st->beta = st->it == 0 ? 0.0 : rho_new / st->rho;
-            st->rho_prev = st->rho;
-            st->rho = rho_new;
-        }
-        break;
-    }
-    case FIN_CG_ALPHA: {
-        // CG: alpha = rho / <p, A p>_V
-        if (st->done) break;
-        double den = s[0];
-        double alpha = st->rho / den;
-        if (!(den != 0.0) || !(alpha == alpha) || isinf(alpha)) {
-            st->err = den;
-            st->done = 1;
-            st->status = DALM_BREAKDOWN;
-        } else {
-            st->alpha = alpha;
-        }
-        break;
-    }
-    case FIN_CG_XR: {
-        // CG, end of an iteration: s[1] = |r|^2 of the recursive residual.  With
-        // FP64 vectors it carries the stopping test (as the reference's BiCGStab);
-        // with reduced-precision vectors meeting it, reaching the iteration limit or
-        // falling below delta x the largest norm since the last replacement asks for
-        // a residual replacement (FIN_CG_RU decides then).
-        if (st->done) break;
-        double rn = sqrt(s[1]);
-        double err = rn / st->root_n;
-        long long it = st->it + 1;
-        st->it = it;
-        st->err = err;
-        if (!(err == err) || isinf(err)) {
-            st->done = 1;
-            st->status = DALM_NONFINITE;
-        } else if (!st->reliable) {
-            if (err <= st->tol) {
-                st->done = 1;
-                st->status = DALM_OK;
-            } else if (it >= st->max_it) {
-                st->done = 1;
-                st->status = DALM_NONCONVERGED;
-            }
-        } else if (err <= st->tol || it >= st->max_it || rn < kMixedReliableDelta * st->maxr) {
-            st->ru = 1;
-        } else if (rn > st->maxr) {
-            st->maxr = rn;
-        }
-        break;
-    }
-    case FIN_CG_RU: {
-        // CG residual replacement: s[1] = |b - A x|^2 in FP64; the solve ends only
-        // on the true residual
-        if (st->done || !st->ru) break;
-        st->ru = 0;
-        st->n_ru += 1;
-        double rn = sqrt(s[1]);
-        double err = rn / st->root_n;
-        st->err = err;
-        st->maxr = rn;
-        if (!(err == err) || isinf(err)) {
-            st->done = 1;
-            st->status = DALM_NONFINITE;
-        } else if (err <= st->tol) {
-            st->done = 1;
-            st->status = DALM_OK;
-        } else if (st->it >= st->max_it) {
-            st->done = 1;
-            st->status = DALM_NONCONVERGED;
-        }
-        break;
-    }
     default:
         break;
     }

@@ -31,7 +31,7 @@ FIELDS = {"U": F_U, "Uold": F_UOLD, "JU": F_JU, "p": F_P, "nut": F_NUT, "div": F
           "nutold_b": F_NUTOLD_B, "f_b": F_F_B, "fold_b": F_FOLD_B, "p_b": F_P_B, "b_b": F_B_B}
 VECTOR_FIELDS = {F_U, F_UOLD, F_JU, F_F, F_U_B, F_UOLD_B, F_JU_B, F_JUOLD_B, F_F_B, F_FOLD_B}
 ADJ_KEEP_PRESSURE = 1
-SOR, BICGSTAB, CG = 0, 1, 2
+SOR, BICGSTAB = 0, 1
 PC_JACOBI, PC_SOR = 0, 1
 WT_STRIDE = 16
 AP_STRIDE = 16
@@ -99,7 +99,6 @@ def lib():
         "dalm_build_A": [V, P],
         "dalm_run_sor": [V, dbl, i64, dbl, PI64, P],
         "dalm_run_bicgstab": [V, i64, dbl, i64, i64, dbl, PI64, P],
-        "dalm_run_cg": [V, i64, i64, dbl, PI64, P],
         "dalm_residual": [V, P],
         "dalm_apply_pbc": [V],
         "dalm_project_p": [V, dbl],
@@ -239,12 +238,6 @@ class Context:
         rc = self.check(self.L.dalm_run_bicgstab(self.h, pc_kind, pc_omega, pc_it, max_it, tol, C.byref(it),
                                                  C.byref(err)),
                         "run_bicgstab", allow=(OK, NONCONVERGED, BREAKDOWN, NONFINITE))
-        return it.value, err.value, rc
-
-    def run_cg(self, pc_it, max_it, tol):
-        it, err = i64(), dbl()
-        rc = self.check(self.L.dalm_run_cg(self.h, pc_it, max_it, tol, C.byref(it), C.byref(err)),
-                        "run_cg", allow=(OK, NONCONVERGED, BREAKDOWN, NONFINITE))
         return it.value, err.value, rc
 
     def residual(self):

@@ -9,10 +9,8 @@
 //
 // Beyond the reference (needed by the measured workloads): --steps N replaces
 // time/dt, --perturb A --seed S adds the seeded uniform perturbation of the
-// interior velocity, --precision mixed selects the FP32 + FP64-refined form,
-// --poisson cg solves the pressure with Jacobi-preconditioned CG instead of the
-// case's method (eq.method "CG" in setup.json does the same; neither exists in the
-// reference), and --power prints per-turbine power after each step line.  Snapshot output, probes,
+// interior velocity, --precision mixed selects the FP32 + FP64-refined form, and
+// --power prints per-turbine power after each step line.  Snapshot output, probes,
 // time averaging and the turbulence-generating grid are diagnostics / features
 // outside this build; a case that asks for the grid is refused.
 #include <cmath>
@@ -67,7 +65,6 @@ struct Args {
     double perturb = 0;
     uint64_t seed = 1;
     bool power = false;
-    bool cg = false;
 };
 
 Args parse_args(int argc, char** argv) {
@@ -84,17 +81,11 @@ Args parse_args(int argc, char** argv) {
         else if (s == "--perturb") a.perturb = std::stod(next());
         else if (s == "--seed") a.seed = std::stoull(next());
         else if (s == "--power") a.power = true;
-        else if (s == "--poisson") {
-            std::string m = next();
-            if (m != "cg" && m != "case") throw std::runtime_error("--poisson takes cg or case");
-            a.cg = m == "cg";
-        }
         else if (!s.empty() && s[0] == '-') throw std::runtime_error("unknown option " + s);
         else a.dir = s;
     }
     if (a.dir.empty()) throw std::runtime_error("usage: dalm_solver <case_dir> [--steps N] "
-                                                "[--precision fp64|mixed] [--poisson case|cg] [--perturb A --seed S] "
-                                                "[--power]");
+                                                "[--precision fp64|mixed] [--perturb A --seed S] [--power]");
     return a;
 }
 
@@ -129,8 +120,8 @@ int main(int argc, char** argv) {
         P.tol = eq["tolerance"].d();
         P.max_it = eq["max iteration"].i();
         std::string method = eq["method"].s();
-        if (method == "BiCG" || method == "CG") {
-            P.method = method == "CG" ? DALM_CG : DALM_BICGSTAB;
+        if (method == "BiCG") {
+            P.method = DALM_BICGSTAB;
             const jl::Value& pc = eq["preconditioner"];
             P.pc_kind = pc["method"].s() == "SOR" ? DALM_PC_SOR : DALM_PC_JACOBI;
             P.pc_it = pc["max iteration"].i();
@@ -140,11 +131,6 @@ int main(int argc, char** argv) {
             P.omega = eq["relaxation rate"].d();
         } else {
             throw std::runtime_error("unknown eq.method " + method);
-        }
-        if (a.cg) {
-            if (P.method != DALM_CG && !(P.method == DALM_BICGSTAB && P.pc_kind == DALM_PC_JACOBI)) P.pc_it = 3;
-            P.method = DALM_CG;
-            P.pc_kind = DALM_PC_JACOBI;
         }
         if (setup.has("floor start")) {
             P.retracted_floor = 1;

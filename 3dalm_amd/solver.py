@@ -18,7 +18,7 @@ def params_for(case: Case) -> hip.Params:
     p = hip.Params()
     p.Uin[:] = case.uin
     p.Re, p.Cs, p.dt = case.Re, case.Cs, case.dt
-    p.method = {"BiCG": hip.BICGSTAB, "CG": hip.CG}.get(case.method, hip.SOR)
+    p.method = hip.BICGSTAB if case.method == "BiCG" else hip.SOR
     p.pc_kind = hip.PC_SOR if case.pc == "SOR" else hip.PC_JACOBI
     p.omega, p.pc_omega, p.pc_it = case.omega, case.pc_omega, case.pc_it
     p.max_it, p.tol = case.max_it, case.tol

@@ -1,6 +1,6 @@
 """Benchmark of the 3dalm per-time-step hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed,fp64_cg,mixed_bicg]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed]
                     [--grid 256,128,128] [--no-cpu] [--probe jacobi]
                     [--adjoint-steps 20] [--adjoint-every 10] [--no-adjoint]
 
@@ -10,11 +10,9 @@ actuator points, TSR 6, eps = 2 dx, 5% seeded noise, dt = 0.005), Poisson by
 BiCGStab with 3 Jacobi sweeps as preconditioner to 1e-8.  Synthetic inputs,
 resident in HBM before the timed region.
 
-Forms: "fp64" is the FP64 drop-in with the case's BiCGStab; "mixed" is the
-precision form (FP32 stencils, FP32 CG vectors with FP64 dot products, FP64
-pressure and residual replacements, to the same residual test); "fp64_cg" (CG in
-FP64) and "mixed_bicg" (the precision form with the case's BiCGStab) separate what
-the Krylov method and what the precision contribute.
+Forms: "fp64" is the FP64 drop-in; "mixed" is the precision form (FP32 stencils,
+FP32 BiCGStab vectors with FP64 dot products, norms, pressure and residual
+replacements, to the same true-residual test).
 
 Prints ONE JSON line (rank 0).  value = Mcell-steps/s = N_interior * K / t / 1e6
 of the precision form when it is built, else of the FP64 drop-in; every form is
@@ -57,9 +55,6 @@ PROBE_BYTES = {
     "predictor": 104,  # read Uold(3), JU(3), nut, f(3); write U(3)
     "sor": 16,        # half-sweep: read x, b over the colour + neighbours ~ 2w, write half
     "resnorm": 16,    # read x, b
-    # CG (FP32 vectors in the precision form; FP64 values in PROBE_BYTES_FP64)
-    "cg_spmv": 16,    # read z, p_old; write p_new, q
-    "cg_update": 36,  # read x (FP64), p, q, r; write x (FP64), r, z
     # precision form's BiCGStab (FP32 Krylov vectors, FP64 pressure)
     "jacobi32": 12,   # read z_in, v; write z_out (FP32)
     "spmv_mixed": 12,  # read z, r0 or s; write y (FP32)
@@ -67,38 +62,26 @@ PROBE_BYTES = {
     "pupdate_mixed": 20,  # read r, p, q; write p, z (FP32)
     "supdate_mixed": 16,  # read r, q; write s, z (FP32)
 }
-PROBE_BYTES_FP64 = {"cg_spmv": 32, "cg_update": 56}
 PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
              "supdate": 8, "jacobi32": 9, "spmv_mixed": 10, "update_mixed": 11, "pupdate_mixed": 12,
-             "supdate_mixed": 13, "cg_spmv": 14, "cg_update": 15}
-# form -> (precision, Poisson method)
-FORMS = {"fp64": ("fp64", "BiCG"), "mixed": ("mixed", "CG"), "fp64_cg": ("fp64", "CG"),
-         "mixed_bicg": ("mixed", "BiCG")}
-POISSON = {"BiCG": "BiCGStab+Jacobi3", "CG": "PCG+Jacobi3 (cell-volume inner product)"}
-
-
-def probe_bytes(probe, form):
-    if FORMS[form][0] == "fp64" and probe in PROBE_BYTES_FP64:
-        return PROBE_BYTES_FP64[probe]
-    return PROBE_BYTES[probe]
+             "supdate_mixed": 13}
+FORMS = ("fp64", "mixed")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_case(grid, steps, method="BiCG"):
+def make_case(grid, steps):
     cases = importlib.import_module("3dalm_amd.cases")
     nx, ny, nz = grid
-    case = cases.case_c2(nx, ny, nz, steps=steps)
-    case.method = method
-    return case
+    return cases.case_c2(nx, ny, nz, steps=steps)
 
 
 def run_form(case, precision, steps, warmup, probe, form):
     solver = importlib.import_module("3dalm_amd.solver")
     import ctypes as C
-    log(f"[bench] form={form} ({case.method}) grid={case.mesh.interior} warmup={warmup} steps={steps}")
+    log(f"[bench] form={form} grid={case.mesh.interior} warmup={warmup} steps={steps}")
     s = solver.Solver(case, precision)
     try:
         for w in range(warmup):
@@ -271,7 +254,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--forms", default="fp64,mixed,fp64_cg,mixed_bicg")
+    ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--grid", default="256,128,128")
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
@@ -304,20 +287,19 @@ def main():
         print(f"bench.py: unknown form(s) {unknown}; choose from {list(FORMS)}", file=sys.stderr)
         sys.exit(2)
     have_mixed = getattr(hip, "MIXED_BUILT", False)
-    if not have_mixed and any(FORMS[f][0] == "mixed" for f in forms):
-        log("[bench] precision form not built in this library; measuring the FP64 forms only")
-        forms = [f for f in forms if FORMS[f][0] != "mixed"]
+    if not have_mixed and "mixed" in forms:
+        log("[bench] precision form not built in this library; measuring the FP64 drop-in only")
+        forms = [f for f in forms if f != "mixed"]
     results = {}
     N = None
     for form in forms:
-        prec_name, method = FORMS[form]
-        case = make_case(grid, a.steps, method)
+        case = make_case(grid, a.steps)
         N = case.mesh.N
         if a.probe == "auto":
-            probe = "jacobi" if prec_name == "fp64" else "jacobi32"
+            probe = "jacobi" if form == "fp64" else "jacobi32"
         else:
             probe = a.probe if a.probe != "none" else None
-        prec = hip.FP64 if prec_name == "fp64" else hip.MIXED
+        prec = hip.FP64 if form == "fp64" else hip.MIXED
         res, s = run_form(case, prec, a.steps, a.warmup, probe, form)
         s.close()
         res["mcell_steps_s"] = N * a.steps / res["seconds"] / 1e6
@@ -326,19 +308,19 @@ def main():
     main_form = "mixed" if "mixed" in results else next(iter(results))
     R = results[main_form]
 
-    def roofline(res, form):
+    def roofline(res):
         if "probe" not in res or not res["probe"]["launches"]:
             return None
         pr = res["probe"]
         mean_s = pr["total_ms"] / pr["launches"] / 1e3
-        bytes_per_launch = probe_bytes(pr["kernel"], form) * N
+        bytes_per_launch = PROBE_BYTES[pr["kernel"]] * N
         ach = bytes_per_launch / mean_s / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": pr["kernel"],
                 "launches": pr["launches"], "mean_us": round(mean_s * 1e6, 2),
                 "bytes_per_launch": bytes_per_launch}
-    roof = roofline(R, main_form)
-    roof_fp64 = roofline(results["fp64"], "fp64") if "fp64" in results and main_form != "fp64" else None
+    roof = roofline(R)
+    roof_fp64 = roofline(results["fp64"]) if "fp64" in results and main_form != "fp64" else None
     adj = None
     if not a.no_adjoint:
         try:
@@ -353,16 +335,11 @@ def main():
             cpu = cpu_baseline(make_case(grid, 1), it_ref)
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
-    forms_out = {f: {"precision": FORMS[f][0], "poisson": POISSON[FORMS[f][1]],
-                     "mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
+    forms_out = {f: {"mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
                      "it_per_step": round(r["it_mean"], 2), "avg_div_max": r["avg_div_max"],
                      "div_within_1e-6": bool(r["avg_div_max"] <= 1e-6),
                      "power_last": r["power_last"]} for f, r in results.items()}
 
-    def ratio(x, y):
-        if x in results and y in results:
-            return round(results[x]["mcell_steps_s"] / results[y]["mcell_steps_s"], 3)
-        return None
     line = {
         "metric": METRIC,
         "value": round(R["mcell_steps_s"], 3),
@@ -377,14 +354,12 @@ def main():
         "dtype": "f64" if main_form == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": (f"single 3-blade turbine ALM, {grid[0]}x{grid[1]}x{grid[2]} cells, "
-                                f"Poisson to 1e-8 (FP64 drop-in: BiCGStab(Jacobi x3)), form={main_form}"),
-                   "cells": N, "turbines": 1, "actuator_points": 192,
-                   "poisson": POISSON[FORMS[main_form][1]], "tol": 1e-8},
+                                f"BiCGStab(Jacobi x3) to 1e-8, form={main_form}"),
+                   "cells": N, "turbines": 1, "actuator_points": 192, "poisson": "BiCGStab+Jacobi3",
+                   "tol": 1e-8},
         "forms": forms_out,
-        "mixed_vs_fp64": ratio("mixed", "fp64"),
-        "same_method_ratios": {"mixed_bicg_vs_fp64": ratio("mixed_bicg", "fp64"),
-                               "mixed_vs_fp64_cg": ratio("mixed", "fp64_cg"),
-                               "fp64_cg_vs_fp64": ratio("fp64_cg", "fp64")},
+        "mixed_vs_fp64": (round(results["mixed"]["mcell_steps_s"] / results["fp64"]["mcell_steps_s"], 3)
+                          if "mixed" in results and "fp64" in results else None),
         "adjoint_forward_ratio": round(adj["ratio"], 3) if adj else None,
         "adjoint": adj,
         "roofline": roof,

@@ -66,10 +66,7 @@ enum dalm_field {
     DALM_F_B_B = 18       /* [L]    bar of b (Poisson right-hand side) */
 };
 
-/* DALM_CG: Jacobi-preconditioned conjugate gradients in the cell-volume inner
- * product -- no counterpart in the reference (which has SOR and BiCGStab); valid
- * because V A is symmetric and definite.  Jacobi preconditioner only. */
-enum dalm_method { DALM_SOR = 0, DALM_BICGSTAB = 1, DALM_CG = 2 };
+enum dalm_method { DALM_SOR = 0, DALM_BICGSTAB = 1 };
 enum dalm_pc { DALM_PC_JACOBI = 0, DALM_PC_SOR = 1 };
 
 /* Turbine / actuator-point records (flat FP64 rows).
@@ -89,7 +86,7 @@ typedef struct {
     double Uin[3];         /* inflow.value */
     double Re, Cs;         /* cfd.Re, cfd.Cs */
     double dt;             /* runtime.dt */
-    int64_t method;        /* eq.method: DALM_SOR | DALM_BICGSTAB (| DALM_CG) */
+    int64_t method;        /* eq.method: DALM_SOR | DALM_BICGSTAB */
     int64_t pc_kind;       /* eq.preconditioner.method */
     double omega;          /* eq.relaxation rate (SOR) */
     double pc_omega;       /* eq.preconditioner.relaxation rate */
@@ -162,10 +159,6 @@ int dalm_run_sor(dalm_ctx *ctx, double omega, int64_t max_it, double tol, int64_
 /* run_pbicgstab, Jacobi PC (src/eq.h:333-402) or SOR PC (src/eq.h:404-473) */
 int dalm_run_bicgstab(dalm_ctx *ctx, int64_t pc_kind, double pc_omega, int64_t pc_it,
                       int64_t max_it, double tol, int64_t *it, double *err);
-/* Jacobi(pc_it)-preconditioned CG on p with right-hand side b, the reference's
- * stopping test |r| / sqrt(N) <= tol.  No counterpart in the reference (see
- * DALM_CG); the alternative to run_pbicgstab for this symmetric operator. */
-int dalm_run_cg(dalm_ctx *ctx, int64_t pc_it, int64_t max_it, double tol, int64_t *it, double *err);
 /* calc_residual r = b - A p (src/mv.h:90-182), result in field B's twin: r is
  * written to `out` (host, [L]) when out != NULL */
 int dalm_residual(dalm_ctx *ctx, double *out);
@@ -225,12 +218,10 @@ enum dalm_kernel {
     DALM_K_SUPDATE = 8,   /* BiCGStab s update + first PC sweep       */
     /* the precision form's kernels of its FP32-preconditioned BiCGStab */
     DALM_K_JACOBI32 = 9,  /* FP32 Jacobi preconditioner sweep         */
-    DALM_K_SPMV_MIXED = 10,    /* A z (FP64 from FP32 z) + dot products */
-    DALM_K_UPDATE_MIXED = 11,  /* x/r update + norms                    */
-    DALM_K_PUPDATE_MIXED = 12, /* p update, FP32 copy, first PC sweep   */
-    DALM_K_SUPDATE_MIXED = 13, /* s update, FP32 copy, first PC sweep   */
-    DALM_K_CG_SPMV = 14,       /* CG p update + A p + <p, Ap>_V          */
-    DALM_K_CG_UPDATE = 15      /* CG x/r update + first Jacobi sweep     */
+    DALM_K_SPMV_MIXED = 10,    /* A z (FP64 arithmetic, FP32 result) + dots */
+    DALM_K_UPDATE_MIXED = 11,  /* x (FP64) / r update + norms           */
+    DALM_K_PUPDATE_MIXED = 12, /* p update, first PC sweep (FP32)       */
+    DALM_K_SUPDATE_MIXED = 13  /* s update, first PC sweep (FP32)      */
 };
 /* Time every launch of `kernel` with HIP events on the context's stream from now
  * on (DALM_K_NONE stops); resets the totals. */

@@ -138,7 +138,7 @@ def test_nut_adj(mesh, ok):
     close(c.download("U_b"), Ub)
 
 
-@pytest.mark.parametrize("method,pc", [("BiCG", "Jacobi"), ("BiCG", "SOR"), ("SOR", "Jacobi"), ("CG", "Jacobi")])
+@pytest.mark.parametrize("method,pc", [("BiCG", "Jacobi"), ("BiCG", "SOR"), ("SOR", "Jacobi")])
 def test_poisson_adj(mesh, ok, method, pc):
     """b_b = A^-T p_b against the oracle's, and the transpose identity
     <p_b, A^-1 b> = <b_b, b> with the GPU's own forward solve.  Both solves stop
@@ -161,15 +161,13 @@ def test_poisson_adj(mesh, ok, method, pc):
     c.set_params(solver.params_for(case))
     its, _ = c.poisson_adj(1e-13, 20000)
     print(f"adjoint solve {method}/{pc}: {its} iterations (oracle {it.value})", flush=True)
-    tol = 2e-5 if method == "SOR" else 1e-9
+    tol = 1e-9 if method == "BiCG" else 2e-5
     close(c.download("b_b")[m], bb[m], tol)
     b = rand_field(mesh, 61, 1, 1.0) * m
     c.upload("b", b)
     c.upload("p", np.zeros(L))
     if method == "BiCG":
         c.run_bicgstab(hip.PC_SOR, 1.2, 2, 20000, 1e-13)
-    elif method == "CG":
-        c.run_cg(2, 20000, 1e-13)
     else:
         c.run_sor(1.4, 20000, 1e-13)
     x = c.download("p")

@@ -30,20 +30,15 @@ po = importlib.import_module("pyoracle")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("method", ["BiCG", "CG"])
 @pytest.mark.parametrize("builder,steps", [
     (lambda: cases.case_c2(32, 16, 16, steps=6, ap_per_blade=8), 6),
     (lambda: cases.case_c2(64, 32, 32, steps=4, ap_per_blade=16), 4),
 ])
-def test_mixed_against_fp64_oracle(builder, steps, method):
-    """The FP64 oracle solves with the case's BiCGStab; the precision form with
-    BiCGStab or with its CG (FP32 vectors), to the same residual test."""
+def test_mixed_against_fp64_oracle(builder, steps):
     case = builder()
-    print(f"[mixed {method} {case.mesh.interior} steps={steps}]", flush=True)
+    print(f"[mixed {case.mesh.interior} steps={steps}]", flush=True)
     o = po.OracleSolver(case)
-    mcase = builder()
-    mcase.method = method
-    s = solver.Solver(mcase, hip.MIXED)
+    s = solver.Solver(case, hip.MIXED)
     try:
         for n in range(steps):
             so, sm = o.step(), s.step()
@@ -84,26 +79,3 @@ def test_mixed_sor_pc_and_c1():
     finally:
         s.close()
 
-
-def test_fp64_cg_steps_match_oracle():
-    """Whole FP64 steps with the pressure solved by CG instead of BiCGStab: with
-    both solves driven to a 1e-13 residual the steps agree with the oracle's to
-    the solution error that tolerance leaves."""
-    case = cases.case_c2(32, 16, 16, steps=3, ap_per_blade=8)
-    case.tol, case.max_it = 1e-13, 20000
-    print(f"[fp64 cg {case.mesh.interior}]", flush=True)
-    o = po.OracleSolver(case)
-    ccase = cases.case_c2(32, 16, 16, steps=3, ap_per_blade=8)
-    ccase.tol, ccase.max_it, ccase.method = 1e-13, 20000, "CG"
-    s = solver.Solver(ccase, hip.FP64)
-    try:
-        for _ in range(3):
-            so, sg = o.step(), s.step()
-            assert sg["status"] == hip.OK and sg["err"] <= 1e-13
-        m = interior_mask(case.mesh)
-        eu = rel_l2(s.field("U")[m], o.field("U")[m])
-        ep = rel_l2(s.field("p")[m], o.field("p")[m])
-        print(f"fp64 cg vs oracle bicgstab: U {eu:.2e} p {ep:.2e}", flush=True)
-        assert eu <= 1e-10 and ep <= 1e-8
-    finally:
-        s.close()
