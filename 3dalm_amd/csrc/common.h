@@ -67,8 +67,13 @@ __device__ __forceinline__ Cell tile_cell(const Dims& d, int t) {
 // streamed vector is read once per unit (plus one halo plane on each end) instead
 // of three times from the caches.  Units are numbered column-major over i so that
 // the consecutive units a single XCD receives (virtual_block) share halo planes.
+//
+// Every unit walks exactly ni planes, [i0, i1): the last unit of a column starts
+// inside its predecessor's range when ni does not divide nx, and owns (writes,
+// accumulates) only the planes from own0 on.
+constexpr int kMaxPlanes = 8;  // planes per column unit at most (the unrolled walk)
 struct Col {
-    int i0, i1, j, k;
+    int i0, i1, own0, j, k;
     bool ok;  // (j, k) inside the interior (ragged tiles have idle lanes)
 };
 __device__ __forceinline__ Col col_unit(const Dims& d) {
@@ -78,9 +83,10 @@ __device__ __forceinline__ Col col_unit(const Dims& d) {
     Col r;
     r.k = d.gc + kt * TK + (int)(threadIdx.x % TK);
     r.j = d.gc + jt * TJ + (int)(threadIdx.x / TK);
-    r.i0 = d.gc + ic * d.ni;
-    r.i1 = r.i0 + d.ni;
+    r.own0 = d.gc + ic * d.ni;
+    r.i1 = r.own0 + d.ni;
     if (r.i1 > d.gc + d.nx) r.i1 = d.gc + d.nx;
+    r.i0 = r.i1 - d.ni;
     r.ok = r.k < d.gc + d.nz && r.j < d.gc + d.ny;
     return r;
 }
@@ -171,14 +177,25 @@ __device__ __forceinline__ void store_partials(const double (&v)[NV], double* pa
 }
 
 // Reduce partials[q*nb + b] over b in a fixed order (one block).  Result in all
-// threads' out[] after the call.
+// threads' out[] after the call.  Each thread requests its values in groups of
+// eight independent loads (the partials were just written by other XCDs, so every
+// load is a trip past the local L2) and sums them in a fixed tree.
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* partials, int nb, double (&out)[NV]) {
+    constexpr int U = 8;
     double v[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double s = 0;
-        for (int b = (int)threadIdx.x; b < nb; b += BLOCK) s += partials[q * nb + b];
+        for (int b0 = (int)threadIdx.x; b0 < nb; b0 += U * BLOCK) {
+            double w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int b = b0 + u * BLOCK;
+                w[u] = b < nb ? partials[q * nb + b] : 0.0;
+            }
+            s += ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+        }
         v[q] = s;
     }
     block_sum<NV>(v);

@@ -19,27 +19,12 @@
 #include <float.h>
 
 #include "ctx.h"
+#include "stencil.h"
 
 namespace {
 
 __device__ __forceinline__ double diag_at(const PCoef& A, int i, int j, int k) {
     return ((((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k]) / A.max_diag;
-}
-
-// a_c x_c + a_e x_e + a_w x_w + a_n x_n + a_s x_s + a_t x_t + a_b x_b, left to
-// right (src/mv.h:87)
-__device__ __forceinline__ double apply7(const Dims& d, const PCoef& A, const double* x, int c,
-                                         int i, int j, int k, double ac) {
-    return ac * x[c] + A.ae[i] * x[c + d.si] + A.aw[i] * x[c - d.si] + A.an[j] * x[c + d.sj] +
-           A.as[j] * x[c - d.sj] + A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
-}
-
-// the same product with the i-neighbours supplied from registers (column
-// streaming) -- identical operands in identical order
-__device__ __forceinline__ double apply7s(const Dims& d, const PCoef& A, const double* x, int c, int i, int j,
-                                          int k, double ac, double x0, double xp, double xm) {
-    return ac * x0 + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * x[c + d.sj] + A.as[j] * x[c - d.sj] +
-           A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
 }
 
 __device__ __forceinline__ bool finished(const SolveState* st) {
@@ -54,20 +39,17 @@ __global__ __launch_bounds__(BLOCK) void k_sor(Dims d, PCoef A, double* __restri
     const Col cl = col_unit(d);
     if (!cl.ok) return;
     const int j = cl.j, k = cl.k;
-    int c = d.at(cl.i0, j, k);
-    // the i-neighbours of a cell of this colour have the other colour, which this
-    // sweep does not change
-    double xm = x[c - d.si], x0 = x[c];
-    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-        const double xp = x[c + d.si];
-        if (((i + j + k) & 1) == color) {
-            double ac = diag_at(A, i, j, k);
-            double rel = (b[c] - apply7s(d, A, x, c, i, j, k, ac, x0, xp, xm)) / ac;
-            x[c] = x0 + omega * rel;
-        }
-        xm = x0;
-        x0 = xp;
-    }
+    const OpD op(A, j, k);
+    // every neighbour of a cell of this colour has the other colour, which this
+    // sweep does not change, so values requested ahead of the updates stay valid
+    walk<double, double>(d, cl, x, [&](int c) { return b[c]; },
+                         [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& q, double bc) {
+                             if (((i + j + k) & 1) == color) {
+                                 const double ac = op.diag(A, i);
+                                 const double rel = (bc - op.apply(A, i, ac, x0, xp, xm, q)) / ac;
+                                 x[c] = x0 + omega * rel;
+                             }
+                         });
 }
 
 // r = b - A x (stored if r != null) and the stopping test of run_sor
@@ -81,18 +63,13 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Dims d, PCoef A, const double
     const Col cl = col_unit(d);
     double v[1] = {0.0};
     if (cl.ok) {
-        const int j = cl.j, k = cl.k;
-        int c = d.at(cl.i0, j, k);
-        double xm = x[c - d.si], x0 = x[c];
-        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-            const double xp = x[c + d.si];
-            double ac = diag_at(A, i, j, k);
-            double rv = b[c] - apply7s(d, A, x, c, i, j, k, ac, x0, xp, xm);
-            if (r) r[c] = rv;
-            v[0] += rv * rv;
-            xm = x0;
-            x0 = xp;
-        }
+        const OpD op(A, cl.j, cl.k);
+        walk<double, double>(d, cl, x, [&](int c) { return b[c]; },
+                             [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& q, double bc) {
+                                 const double rv = bc - op.apply(A, i, op.diag(A, i), x0, xp, xm, q);
+                                 if (r) r[c] = rv;
+                                 v[0] += rv * rv;
+                             });
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
@@ -123,19 +100,14 @@ __global__ __launch_bounds__(BLOCK) void k_bi_init(Dims d, PCoef A, const double
     const Col cl = col_unit(d);
     double v[1] = {0.0};
     if (cl.ok) {
-        const int j = cl.j, k = cl.k;
-        int c = d.at(cl.i0, j, k);
-        double xm = x[c - d.si], x0 = x[c];
-        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-            const double xp = x[c + d.si];
-            double ac = diag_at(A, i, j, k);
-            double rv = b[c] - apply7s(d, A, x, c, i, j, k, ac, x0, xp, xm);
-            r[c] = rv;
-            r0[c] = rv;
-            v[0] += rv * rv;
-            xm = x0;
-            x0 = xp;
-        }
+        const OpD op(A, cl.j, cl.k);
+        walk<double, double>(d, cl, x, [&](int c) { return b[c]; },
+                             [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& q, double bc) {
+                                 const double rv = bc - op.apply(A, i, op.diag(A, i), x0, xp, xm, q);
+                                 r[c] = rv;
+                                 r0[c] = rv;
+                                 v[0] += rv * rv;
+                             });
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
@@ -171,17 +143,13 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dims d, PCoef A, const double*
     if (finished(st)) return;
     const Col cl = col_unit(d);
     if (!cl.ok) return;
-    const int j = cl.j, k = cl.k;
-    int c = d.at(cl.i0, j, k);
-    double xm = zin[c - d.si], x0 = zin[c];
-    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-        const double xp = zin[c + d.si];
-        double ac = diag_at(A, i, j, k);
-        double rel = (v[c] - apply7s(d, A, zin, c, i, j, k, ac, x0, xp, xm)) / ac;
-        zout[c] = x0 + rel;
-        xm = x0;
-        x0 = xp;
-    }
+    const OpD op(A, cl.j, cl.k);
+    walk<double, double>(d, cl, zin, [&](int c) { return v[c]; },
+                         [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& q, double vc) {
+                             const double ac = op.diag(A, i);
+                             const double rel = (vc - op.apply(A, i, ac, x0, xp, xm, q)) / ac;
+                             zout[c] = x0 + rel;
+                         });
 }
 
 // y = A z and the partial sums of <a1, y> (and <y, a2> if a2): with which = 0 the
@@ -196,23 +164,18 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv(Dims d, PCoef A, const double
     const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
     if (cl.ok) {
-        const int j = cl.j, k = cl.k;
-        int c = d.at(cl.i0, j, k);
-        double xm = z[c - d.si], x0 = z[c];
-        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-            const double xp = z[c + d.si];
-            double ac = diag_at(A, i, j, k);
-            double yv = apply7s(d, A, z, c, i, j, k, ac, x0, xp, xm);
-            y[c] = yv;
-            if (WHICH == 0) {
-                v[0] += a1[c] * yv;   // <r0, q>
-            } else {
-                v[0] += yv * a1[c];   // <t, s>
-                v[1] += yv * yv;      // <t, t>
-            }
-            xm = x0;
-            x0 = xp;
-        }
+        const OpD op(A, cl.j, cl.k);
+        walk<double, double>(d, cl, z, [&](int c) { return a1[c]; },
+                             [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& q, double ac1) {
+                                 const double yv = op.apply(A, i, op.diag(A, i), x0, xp, xm, q);
+                                 y[c] = yv;
+                                 if (WHICH == 0) {
+                                     v[0] += ac1 * yv;  // <r0, q>
+                                 } else {
+                                     v[0] += yv * ac1;  // <t, s>
+                                     v[1] += yv * yv;   // <t, t>
+                                 }
+                             });
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
@@ -310,15 +273,11 @@ int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r) 
 __global__ void k_Ax_plain(Dims d, PCoef A, const double* __restrict__ z, double* __restrict__ y) {
     const Col cl = col_unit(d);
     if (!cl.ok) return;
-    const int j = cl.j, k = cl.k;
-    int c = d.at(cl.i0, j, k);
-    double xm = z[c - d.si], x0 = z[c];
-    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-        const double xp = z[c + d.si];
-        y[c] = apply7s(d, A, z, c, i, j, k, diag_at(A, i, j, k), x0, xp, xm);
-        xm = x0;
-        x0 = xp;
-    }
+    const OpD op(A, cl.j, cl.k);
+    walk<double, NoAux>(d, cl, z, [&](int) { return NoAux{}; },
+                        [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& q, NoAux) {
+                            y[c] = op.apply(A, i, op.diag(A, i), x0, xp, xm, q);
+                        });
 }
 
 int launch_Ax(dalm_ctx* ctx, const double* x, double* y) {

@@ -37,21 +37,12 @@
 #include <cstdlib>
 
 #include "ctx.h"
+#include "stencil.h"
 
 namespace {
 
 __device__ __forceinline__ float diag_f(const PCoefF& A, int i, int j, int k) {
     return (((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k];
-}
-// 7-point product in FP32 in the reference's order, the i-neighbours from
-// registers (column streaming)
-__device__ __forceinline__ float apply7fs(const Dims& d, const PCoefF& A, const float* x, int c, int i, int j,
-                                          int k, float ac, float xc, float xp, float xm) {
-    return ac * xc + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * x[c + d.sj] + A.as[j] * x[c - d.sj] +
-           A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
-}
-__device__ __forceinline__ double diag64(const PCoef& A, int i, int j, int k) {
-    return ((((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k]) / A.max_diag;
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float* __restrict__ zin,
@@ -60,16 +51,12 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float
     if (st->done) return;
     const Col cl = col_unit(d);
     if (!cl.ok) return;
-    const int j = cl.j, k = cl.k;
-    int c = d.at(cl.i0, j, k);
-    float xm = zin[c - d.si], x0 = zin[c];
-    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-        const float xp = zin[c + d.si];
-        float ac = diag_f(A, i, j, k);
-        zout[c] = x0 + (v[c] - apply7fs(d, A, zin, c, i, j, k, ac, x0, xp, xm)) * __builtin_amdgcn_rcpf(ac);
-        xm = x0;
-        x0 = xp;
-    }
+    const OpF op(A, cl.j, cl.k);
+    walk<float, float>(d, cl, zin, [&](int c) { return v[c]; },
+                       [&](int i, int c, float x0, float xp, float xm, const Nb4<float>& q, float vc) {
+                           const float ac = op.diag(A, i);
+                           zout[c] = x0 + (vc - op.apply(A, i, ac, x0, xp, xm, q)) * __builtin_amdgcn_rcpf(ac);
+                       });
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restrict__ x,
@@ -79,17 +66,15 @@ __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restr
     const Col cl = col_unit(d);
     if (!cl.ok) return;
     const int j = cl.j, k = cl.k;
-    int c = d.at(cl.i0, j, k);
-    float xm = x[c - d.si], x0 = x[c];
-    for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-        const float xp = x[c + d.si];
-        if (((i + j + k) & 1) == color) {
-            float ac = diag_f(A, i, j, k);
-            x[c] = x0 + omega * (b[c] - apply7fs(d, A, x, c, i, j, k, ac, x0, xp, xm)) * __builtin_amdgcn_rcpf(ac);
-        }
-        xm = x0;
-        x0 = xp;
-    }
+    const OpF op(A, j, k);
+    // neighbours of a cell of this colour have the other colour: unchanged here
+    walk<float, float>(d, cl, x, [&](int c) { return b[c]; },
+                       [&](int i, int c, float x0, float xp, float xm, const Nb4<float>& q, float bc) {
+                           if (((i + j + k) & 1) == color) {
+                               const float ac = op.diag(A, i);
+                               x[c] = x0 + omega * (bc - op.apply(A, i, ac, x0, xp, xm, q)) * __builtin_amdgcn_rcpf(ac);
+                           }
+                       });
 }
 
 __global__ void kg_state_reset(SolveState* st, double tol, long long max_it, double root_n) {
@@ -120,29 +105,24 @@ __global__ __launch_bounds__(BLOCK) void kg_resid(Dims d, PCoef A, const double*
     const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
     if (cl.ok) {
-        const int j = cl.j, k = cl.k;
-        int c = d.at(cl.i0, j, k);
-        double xm = x[c - d.si], x0 = x[c];
-        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-            const double xp = x[c + d.si];
-            double ac = diag64(A, i, j, k);
-            double s = ac * x0 + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * x[c + d.sj] + A.as[j] * x[c - d.sj] +
-                       A.at[k] * x[c + 1] + A.ab[k] * x[c - 1];
-            double rv = b[c] - s;
-            float rf = (float)rv;
-            r[c] = rf;
-            float r0v;
-            if (INIT) {
-                r0[c] = rf;
-                r0v = rf;
-            } else {
-                r0v = r0[c];
-            }
-            v[0] += (double)r0v * (double)rf;
-            v[1] += rv * rv;
-            xm = x0;
-            x0 = xp;
-        }
+        const OpD op(A, cl.j, cl.k);
+        struct BR {
+            double b;
+            float r0;
+        };
+        walk<double, BR>(d, cl, x, [&](int c) { return BR{b[c], INIT ? 0.f : r0[c]}; },
+                         [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& q, const BR& a) {
+                             const double rv = a.b - op.apply(A, i, op.diag(A, i), x0, xp, xm, q);
+                             const float rf = (float)rv;
+                             r[c] = rf;
+                             float r0v = a.r0;
+                             if (INIT) {
+                                 r0[c] = rf;
+                                 r0v = rf;
+                             }
+                             v[0] += (double)r0v * (double)rf;
+                             v[1] += rv * rv;
+                         });
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
@@ -195,26 +175,21 @@ __global__ __launch_bounds__(BLOCK) void kg_spmv(Dims d, PCoef A, const float* _
     const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
     if (cl.ok) {
-        const int j = cl.j, k = cl.k;
-        int c = d.at(cl.i0, j, k);
-        double xm = z[c - d.si], x0 = z[c];
-        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
-            const double xp = z[c + d.si];
-            double ac = diag64(A, i, j, k);
-            double yv = ac * x0 + A.ae[i] * xp + A.aw[i] * xm + A.an[j] * (double)z[c + d.sj] +
-                        A.as[j] * (double)z[c - d.sj] + A.at[k] * (double)z[c + 1] + A.ab[k] * (double)z[c - 1];
-            const float yf = (float)yv;
-            y[c] = yf;
-            const double ys = (double)yf;
-            if (WHICH == 0) {
-                v[0] += (double)a1[c] * ys;
-            } else {
-                v[0] += ys * (double)a1[c];
-                v[1] += ys * ys;
-            }
-            xm = x0;
-            x0 = xp;
-        }
+        const OpD op(A, cl.j, cl.k);
+        walk<float, float>(d, cl, z, [&](int c) { return a1[c]; },
+                           [&](int i, int c, float x0, float xp, float xm, const Nb4<float>& q, float ac1) {
+                               const double yv = op.apply(A, i, op.diag(A, i), (double)x0, (double)xp,
+                                                          (double)xm, q);
+                               const float yf = (float)yv;
+                               y[c] = yf;
+                               const double ys = (double)yf;
+                               if (WHICH == 0) {
+                                   v[0] += (double)ac1 * ys;
+                               } else {
+                                   v[0] += ys * (double)ac1;
+                                   v[1] += ys * ys;
+                               }
+                           });
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};

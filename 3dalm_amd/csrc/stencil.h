@@ -1,0 +1,103 @@
+// stencil.h -- the matrix-free 7-point Poisson operator on one (j, k) column and
+// the software-pipelined walk along i shared by the column-streaming kernels.
+//
+// A column unit (col_unit) owns one (j, k) line per thread and walks ni planes
+// along i, keeping x[i-1], x[i], x[i+1] in registers.  Each plane needs the j and
+// k neighbours of the streamed vector plus the values of the other vectors at the
+// cell; walk() requests all of plane i+1's values (and x[i+2]) before plane i is
+// processed, so every wavefront keeps two planes of loads in flight instead of
+// waiting on each plane's loads in turn.
+//
+// The operator: a_c x_c + a_e x_e + a_w x_w + a_n x_n + a_s x_s + a_t x_t + a_b x_b,
+// left to right (src/mv.h:87), with the diagonal rebuilt per cell in build_A's
+// accumulation order (src/eq.h:701-798); the j and k coefficients of the column are
+// loaded once.  OpD is the FP64 operator (exact: identical operands in identical
+// order to the reference), OpF its FP32 pieces for the precision form's sweeps.
+#pragma once
+#include "ctx.h"
+
+// j- and k-neighbours of one cell of a vector
+template <typename T>
+struct Nb4 {
+    T n, s, t, b;
+};
+template <typename T>
+__device__ __forceinline__ Nb4<T> nb4(const Dims& d, const T* x, int c) {
+    return Nb4<T>{x[c + d.sj], x[c - d.sj], x[c + 1], x[c - 1]};
+}
+
+struct OpD {
+    double an, as, at, ab, anu, asu, atu, abu;
+    __device__ __forceinline__ OpD(const PCoef& A, int j, int k)
+        : an(A.an[j]), as(A.as[j]), at(A.at[k]), ab(A.ab[k]), anu(A.anu[j]), asu(A.asu[j]), atu(A.atu[k]),
+          abu(A.abu[k]) {}
+    __device__ __forceinline__ double diag(const PCoef& A, int i) const {
+        return ((((A.acx[i] - anu) - asu) - atu) - abu) / A.max_diag;
+    }
+    template <typename V>
+    __device__ __forceinline__ double apply(const PCoef& A, int i, double ac, double x0, double xp, double xm,
+                                            const Nb4<V>& q) const {
+        return ac * x0 + A.ae[i] * xp + A.aw[i] * xm + an * (double)q.n + as * (double)q.s + at * (double)q.t +
+               ab * (double)q.b;
+    }
+};
+
+struct OpF {
+    float an, as, at, ab, anu, asu, atu, abu;
+    __device__ __forceinline__ OpF(const PCoefF& A, int j, int k)
+        : an(A.an[j]), as(A.as[j]), at(A.at[k]), ab(A.ab[k]), anu(A.anu[j]), asu(A.asu[j]), atu(A.atu[k]),
+          abu(A.abu[k]) {}
+    __device__ __forceinline__ float diag(const PCoefF& A, int i) const {
+        return (((A.acx[i] - anu) - asu) - atu) - abu;
+    }
+    __device__ __forceinline__ float apply(const PCoefF& A, int i, float ac, float x0, float xp, float xm,
+                                           const Nb4<float>& q) const {
+        return ac * x0 + A.ae[i] * xp + A.aw[i] * xm + an * q.n + as * q.s + at * q.t + ab * q.b;
+    }
+};
+
+struct NoAux {};
+
+// Walk the column of cl along i over the streamed vector x.  load(c) returns the
+// other vectors' values at cell c (an Aux); f(i, c, x0, xp, xm, q, aux) processes
+// plane i, for the planes the unit owns.  Every value of plane i+1 is requested
+// before plane i is processed.  With the full kMaxPlanes planes per unit the walk
+// is straight-line code (unrolled, no per-plane branches around the loads), so
+// the compiler's wait counts leave the next plane's loads in flight; the look-ahead
+// of the last plane re-reads the current plane (cache hits) instead of branching.
+template <typename T, typename Aux, typename Load, typename F>
+__device__ __forceinline__ void walk(const Dims& d, const Col& cl, const T* x, Load&& load, F&& f) {
+    int c = d.at(cl.i0, cl.j, cl.k);
+    T xm = x[c - d.si], x0 = x[c], xp = x[c + d.si];
+    Nb4<T> q0 = nb4(d, x, c);
+    Aux a0 = load(c);
+    if (d.ni == kMaxPlanes) {
+#pragma unroll
+        for (int u = 0; u < kMaxPlanes; ++u) {
+            const int nxt = u + 1 < kMaxPlanes ? d.si : 0;
+            const T xpp = x[c + d.si + nxt];
+            const Nb4<T> q1 = nb4(d, x, c + nxt);
+            const Aux a1 = load(c + nxt);
+            if (cl.i0 + u >= cl.own0) f(cl.i0 + u, c, x0, xp, xm, q0, a0);
+            xm = x0;
+            x0 = xp;
+            xp = xpp;
+            q0 = q1;
+            a0 = a1;
+            c += d.si;
+        }
+    } else {
+        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+            const int nxt = i + 1 < cl.i1 ? d.si : 0;
+            const T xpp = x[c + d.si + nxt];
+            const Nb4<T> q1 = nb4(d, x, c + nxt);
+            const Aux a1 = load(c + nxt);
+            if (i >= cl.own0) f(i, c, x0, xp, xm, q0, a0);
+            xm = x0;
+            x0 = xp;
+            xp = xpp;
+            q0 = q1;
+            a0 = a1;
+        }
+    }
+}
