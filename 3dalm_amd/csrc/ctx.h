@@ -116,6 +116,7 @@ struct KernelProbe {
     // iteration and left out of the totals once the solve's count is known.
     std::vector<long long> tag;    // per recorded launch: solver iteration, -1 outside a solve
     std::vector<char> idle;        // per recorded launch: enqueued past the stopping point
+    std::vector<int> count;        // per event pair: launches it brackets
     long long cur_tag = -1;        // iteration being enqueued by the running solve
     size_t solve_first = 0;        // first launch recorded by the running solve
     double total_ms = 0;
@@ -275,7 +276,8 @@ int ensure_adjoint_buffers(dalm_ctx* ctx);
 int sync_stream(dalm_ctx* ctx);
 int zero_device(dalm_ctx* ctx, double* a, long long n);
 void probe_begin(dalm_ctx* ctx, int kernel);
-void probe_end(dalm_ctx* ctx, int kernel);
+// launches: back-to-back launches of the kernel between the two events
+void probe_end(dalm_ctx* ctx, int kernel, int launches = 1);
 void probe_solve_begin(dalm_ctx* ctx);
 void probe_iteration(dalm_ctx* ctx, long long i);
 void probe_solve_end(dalm_ctx* ctx, long long iterations);

@@ -42,13 +42,14 @@ void probe_begin(dalm_ctx* ctx, int kernel) {
     (void)hipEventRecord(P.ev[P.used], ctx->stream);
 }
 
-void probe_end(dalm_ctx* ctx, int kernel) {
+void probe_end(dalm_ctx* ctx, int kernel, int launches) {
     KernelProbe& P = ctx->probe;
     if (P.kernel == 0 || P.kernel != kernel || P.used + 2 > P.ev.size()) return;
     (void)hipEventRecord(P.ev[P.used + 1], ctx->stream);
     P.used += 2;
     P.tag.push_back(P.cur_tag);
     P.idle.push_back(0);
+    P.count.push_back(launches);
 }
 
 void probe_solve_begin(dalm_ctx* ctx) {
@@ -75,12 +76,13 @@ int probe_collect(dalm_ctx* ctx) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, P.ev[q], P.ev[q + 1]));
         P.total_ms += ms;
-        P.launches += 1;
+        P.launches += P.count[q / 2];
     }
     P.used = 0;
     P.solve_first = 0;
     P.tag.clear();
     P.idle.clear();
+    P.count.clear();
     return DALM_OK;
 }
 

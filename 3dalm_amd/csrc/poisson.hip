@@ -23,10 +23,6 @@
 
 namespace {
 
-__device__ __forceinline__ double diag_at(const PCoef& A, int i, int j, int k) {
-    return ((((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k]) / A.max_diag;
-}
-
 __device__ __forceinline__ bool finished(const SolveState* st) {
     return st != nullptr && st->done;
 }
@@ -124,15 +120,18 @@ __global__ __launch_bounds__(BLOCK) void k_bi_p(Dims d, PCoef A, const double* _
     if (st->done) return;
     const bool first = st->it == 0;
     const double beta = st->beta, om = st->omega;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double pv = first ? r[c] : r[c] + beta * (p[c] - om * q[c]);
-        p[c] = pv;
-        z[c] = pc_first ? pv / diag_at(A, cc.i, cc.j, cc.k) : 0.0;
-    }
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const OpD op(A, cl.j, cl.k);
+    struct RPQ {
+        double r, p, q;
+    };
+    walk_points<RPQ>(d, cl, [&](int c) { return RPQ{r[c], p[c], q[c]}; },
+                     [&](int i, int c, const RPQ& v) {
+                         const double pv = first ? v.r : v.r + beta * (v.p - om * v.q);
+                         p[c] = pv;
+                         z[c] = pc_first ? pv / op.diag(A, i) : 0.0;
+                     });
 }
 
 // One Jacobi sweep z_out = z_in + (v - A z_in) / a_c (src/eq.h:6-39)
@@ -189,15 +188,18 @@ __global__ __launch_bounds__(BLOCK) void k_bi_s(Dims d, PCoef A, const double* _
                                                 const SolveState* st) {
     if (st->done) return;
     const double alpha = st->alpha;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        double sv = r[c] - alpha * q[c];
-        s[c] = sv;
-        z[c] = pc_first ? sv / diag_at(A, cc.i, cc.j, cc.k) : 0.0;
-    }
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const OpD op(A, cl.j, cl.k);
+    struct RQ {
+        double r, q;
+    };
+    walk_points<RQ>(d, cl, [&](int c) { return RQ{r[c], q[c]}; },
+                    [&](int i, int c, const RQ& v) {
+                        const double sv = v.r - alpha * v.q;
+                        s[c] = sv;
+                        z[c] = pc_first ? sv / op.diag(A, i) : 0.0;
+                    });
 }
 
 // x += alpha pp + omega ss; r = s - omega t; err = |r| / sqrt(N); next rho =
@@ -213,17 +215,20 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr(Dims d, double* __restrict__ x,
                                                  unsigned* counter, SolveState* st) {
     if (st->done) return;
     const double alpha = st->alpha, om = st->omega;
-    TileRange tr = block_tiles(d);
+    const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        x[c] = x[c] + alpha * pp[c] + om * ss[c];
-        double rv = s[c] - om * t_[c];
-        r[c] = rv;
-        v[0] += rv * rv;
-        v[1] += r0[c] * rv;
+    if (cl.ok) {
+        struct XR {
+            double x, pp, ss, s, t, r0;
+        };
+        walk_points<XR>(d, cl, [&](int c) { return XR{x[c], pp[c], ss[c], s[c], t_[c], r0[c]}; },
+                        [&](int, int c, const XR& a) {
+                            x[c] = a.x + alpha * a.pp + om * a.ss;
+                            const double rv = a.s - om * a.t;
+                            r[c] = rv;
+                            v[0] += rv * rv;
+                            v[1] += a.r0 * rv;
+                        });
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
@@ -345,15 +350,16 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
         double* cur = z0;
         if (jac) {
             double* other = spare;
+            // the sweeps of one application run back to back: one event pair times them
+            if (pc_it > 1) probe_begin(ctx, DALM_K_JACOBI);
             for (int m = 1; m < pc_it; ++m) {
-                probe_begin(ctx, DALM_K_JACOBI);
                 hipLaunchKernelGGL(k_jacobi, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
                                    other, v, (const SolveState*)ctx->solve);
-                probe_end(ctx, DALM_K_JACOBI);
                 double* tsw = cur;
                 cur = other;
                 other = tsw;
             }
+            if (pc_it > 1) probe_end(ctx, DALM_K_JACOBI, pc_it - 1);
         } else {
             for (int m = 0; m < pc_it; ++m) {
                 hipLaunchKernelGGL(k_sor, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur, v,
@@ -373,7 +379,7 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
         for (long long c = 0; c < chunk; ++c) {
             probe_iteration(ctx, launched + c);
             probe_begin(ctx, DALM_K_PUPDATE);
-            hipLaunchKernelGGL(k_bi_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
+            hipLaunchKernelGGL(k_bi_p, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
                                ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
             probe_end(ctx, DALM_K_PUPDATE);
             double* ppz = precondition(ctx->pp, ctx->tmp, ctx->kp);
@@ -383,7 +389,7 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             probe_end(ctx, DALM_K_SPMV);
             if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, ctx->solve, nullptr)) return rc1;
             probe_begin(ctx, DALM_K_SUPDATE);
-            hipLaunchKernelGGL(k_bi_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
+            hipLaunchKernelGGL(k_bi_s, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
                                ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
             probe_end(ctx, DALM_K_SUPDATE);
             double* ssz = precondition(ctx->ss, ctx->tmp2, ctx->s);
@@ -393,10 +399,10 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             probe_end(ctx, DALM_K_SPMV);
             if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, ctx->solve, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
-            hipLaunchKernelGGL(k_bi_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
+            hipLaunchKernelGGL(k_bi_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
                                ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_UPDATE);
-            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->grid, ctx->solve, nullptr)) return rc3;
+            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->sgrid, ctx->solve, nullptr)) return rc3;
         }
         LAUNCH_CHECK();
         launched += chunk;

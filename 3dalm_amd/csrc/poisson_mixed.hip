@@ -23,8 +23,8 @@
 // and its finalize are enqueued every iteration and return at once when no
 // replacement is due.
 //
-// Safety net: a watchdog on the residual (no 30% gain in 192 iterations, or a
-// rise above 1e3 x the best, or a breakdown) abandons the FP32 iteration and
+// Safety net: a watchdog on the residual (no 30% gain in 96 iterations, or a
+// rise above 100 x the best, or a breakdown) abandons the FP32 iteration and
 // finishes the solve with the FP64 BiCGStab from the better of the pressures
 // before and after, so every step meets the tolerance whatever the conditioning.
 //
@@ -41,9 +41,14 @@
 
 namespace {
 
-__device__ __forceinline__ float diag_f(const PCoefF& A, int i, int j, int k) {
-    return (((A.acx[i] - A.anu[j]) - A.asu[j]) - A.atu[k]) - A.abu[k];
-}
+// Watchdog of the FP32 iteration: a rise above kWatchRise x the lowest residual,
+// or no 30% gain within kWatchWindow iterations.  Left to run longer, the FP32
+// iteration on the stretched C2-size operators mostly wanders or diverges rather
+// than recovering (measured: windows of 192 and 400 iterations cost more FP64
+// finish iterations than they saved).
+constexpr double kWatchRise = 1e2;
+constexpr long long kWatchWindow = 96;
+
 
 __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float* __restrict__ zin,
                                                    float* __restrict__ zout, const float* __restrict__ v,
@@ -137,15 +142,19 @@ __global__ __launch_bounds__(BLOCK) void kg_p(Dims d, PCoefF Af, const float* __
     if (st->done) return;
     const bool first = st->it == 0;
     const double beta = st->beta, om = st->omega;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        float pf = first ? r[c] : (float)((double)r[c] + beta * ((double)p[c] - om * (double)q[c]));
-        p[c] = pf;
-        z[c] = pc_first ? pf * __builtin_amdgcn_rcpf(diag_f(Af, cc.i, cc.j, cc.k)) : 0.f;
-    }
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const OpF op(Af, cl.j, cl.k);
+    struct RPQ {
+        float r, p, q;
+    };
+    walk_points<RPQ>(d, cl, [&](int c) { return RPQ{r[c], p[c], q[c]}; },
+                     [&](int i, int c, const RPQ& v) {
+                         const float pf =
+                             first ? v.r : (float)((double)v.r + beta * ((double)v.p - om * (double)v.q));
+                         p[c] = pf;
+                         z[c] = pc_first ? pf * __builtin_amdgcn_rcpf(op.diag(Af, i)) : 0.f;
+                     });
 }
 
 // s = r - alpha q and the first preconditioner sweep
@@ -154,15 +163,18 @@ __global__ __launch_bounds__(BLOCK) void kg_s(Dims d, PCoefF Af, const float* __
                                               float* __restrict__ z, int pc_first, const SolveState* st) {
     if (st->done) return;
     const double alpha = st->alpha;
-    TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        float sf = (float)((double)r[c] - alpha * (double)q[c]);
-        s[c] = sf;
-        z[c] = pc_first ? sf * __builtin_amdgcn_rcpf(diag_f(Af, cc.i, cc.j, cc.k)) : 0.f;
-    }
+    const Col cl = col_unit(d);
+    if (!cl.ok) return;
+    const OpF op(Af, cl.j, cl.k);
+    struct RQ {
+        float r, q;
+    };
+    walk_points<RQ>(d, cl, [&](int c) { return RQ{r[c], q[c]}; },
+                    [&](int i, int c, const RQ& v) {
+                        const float sf = (float)((double)v.r - alpha * (double)v.q);
+                        s[c] = sf;
+                        z[c] = pc_first ? sf * __builtin_amdgcn_rcpf(op.diag(Af, i)) : 0.f;
+                    });
 }
 
 // y = A z in FP64 arithmetic, stored rounded to FP32, with <a1, y> (WHICH 0:
@@ -204,18 +216,22 @@ __global__ __launch_bounds__(BLOCK) void kg_xr(Dims d, double* __restrict__ x, c
                                                const SolveState* st) {
     if (st->done) return;
     const double alpha = st->alpha, om = st->omega;
-    TileRange tr = block_tiles(d);
+    const Col cl = col_unit(d);
     double v[2] = {0.0, 0.0};
-    for (int t = tr.begin; t < tr.end; ++t) {
-        Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        int c = d.at(cc.i, cc.j, cc.k);
-        x[c] = x[c] + alpha * (double)zp[c] + om * (double)zs[c];
-        const float rf = (float)((double)s[c] - om * (double)t_[c]);
-        r[c] = rf;
-        const double rv = (double)rf;
-        v[0] += rv * rv;
-        v[1] += (double)r0[c] * rv;
+    if (cl.ok) {
+        struct XR {
+            double x;
+            float zp, zs, s, t, r0;
+        };
+        walk_points<XR>(d, cl, [&](int c) { return XR{x[c], zp[c], zs[c], s[c], t_[c], r0[c]}; },
+                        [&](int, int c, const XR& a) {
+                            x[c] = a.x + alpha * (double)a.zp + om * (double)a.zs;
+                            const float rf = (float)((double)a.s - om * (double)a.t);
+                            r[c] = rf;
+                            const double rv = (double)rf;
+                            v[0] += rv * rv;
+                            v[1] += (double)a.r0 * rv;
+                        });
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
@@ -256,15 +272,16 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         float* cur = z0;
         if (jac) {
             float* other = spare;
+            // the sweeps of one application run back to back: one event pair times them
+            if (pc_it > 1) probe_begin(ctx, DALM_K_JACOBI32);
             for (int m = 1; m < pc_it; ++m) {
-                probe_begin(ctx, DALM_K_JACOBI32);
                 hipLaunchKernelGGL(kf_jacobi, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, cur, other, v32,
                                    (const SolveState*)st);
-                probe_end(ctx, DALM_K_JACOBI32);
                 float* tsw = cur;
                 cur = other;
                 other = tsw;
             }
+            if (pc_it > 1) probe_end(ctx, DALM_K_JACOBI32, pc_it - 1);
         } else {
             for (int m = 0; m < pc_it; ++m)
                 for (int color = 0; color < 2; ++color)
@@ -296,7 +313,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             for (long long c = 0; c < chunk; ++c) {
                 probe_iteration(ctx, launched + c);
                 probe_begin(ctx, DALM_K_PUPDATE_MIXED);
-                hipLaunchKernelGGL(kg_p, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fp,
+                hipLaunchKernelGGL(kg_p, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fp,
                                    ctx->fq, ctx->fz, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_PUPDATE_MIXED);
                 float* zp = precondition(ctx->fz, ctx->ftmp, ctx->fp);
@@ -306,7 +323,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 probe_end(ctx, DALM_K_SPMV_MIXED);
                 if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_SUPDATE_MIXED);
-                hipLaunchKernelGGL(kg_s, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fq,
+                hipLaunchKernelGGL(kg_s, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fq,
                                    ctx->fs, ctx->fz2, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SUPDATE_MIXED);
                 float* zs = precondition(ctx->fz2, ctx->ftmp2, ctx->fs);
@@ -316,10 +333,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 probe_end(ctx, DALM_K_SPMV_MIXED);
                 if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_UPDATE_MIXED);
-                hipLaunchKernelGGL(kg_xr, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, zp, zs, ctx->fs,
+                hipLaunchKernelGGL(kg_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, zp, zs, ctx->fs,
                                    ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_UPDATE_MIXED);
-                if ((rc = launch_finalize(ctx, FIN_MX_XR, ctx->grid, st, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_MX_XR, ctx->sgrid, st, nullptr))) return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<false>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d,
                                    A, ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                 if ((rc = launch_finalize(ctx, FIN_MX_RU, ctx->sgrid, st, nullptr))) return rc;
@@ -338,7 +355,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 anchor = e;
                 anchor_it = h.it;
             }
-            if (!(e == e) || e > 1e3 * best || h.it - anchor_it >= 192) {
+            if (!(e == e) || e > kWatchRise * best || h.it - anchor_it >= kWatchWindow) {
                 stalled = true;
                 break;
             }

@@ -101,3 +101,27 @@ __device__ __forceinline__ void walk(const Dims& d, const Col& cl, const T* x, L
         }
     }
 }
+
+// Pointwise kernels in the column layout: load(c) returns every value cell c
+// needs, f(i, c, aux) processes it, for the planes the unit owns; plane i+1's
+// values are requested before plane i is processed (the same straight-line walk).
+template <typename Aux, typename Load, typename F>
+__device__ __forceinline__ void walk_points(const Dims& d, const Col& cl, Load&& load, F&& f) {
+    int c = d.at(cl.i0, cl.j, cl.k);
+    Aux a0 = load(c);
+    if (d.ni == kMaxPlanes) {
+#pragma unroll
+        for (int u = 0; u < kMaxPlanes; ++u) {
+            const Aux a1 = load(c + (u + 1 < kMaxPlanes ? d.si : 0));
+            if (cl.i0 + u >= cl.own0) f(cl.i0 + u, c, a0);
+            a0 = a1;
+            c += d.si;
+        }
+    } else {
+        for (int i = cl.i0; i < cl.i1; ++i, c += d.si) {
+            const Aux a1 = load(c + (i + 1 < cl.i1 ? d.si : 0));
+            if (i >= cl.own0) f(i, c, a0);
+            a0 = a1;
+        }
+    }
+}

@@ -1,0 +1,130 @@
+"""Summaries of rocprofv3 output for the hot-path kernels.
+
+    python 3dalm_amd/profile_summary.py trace <run>_kernel_trace.csv [--cells N]
+    python 3dalm_amd/profile_summary.py traffic <fetch>_counter_collection.csv \\
+        <write>_counter_collection.csv --cells N [--copy-mib 256]
+
+``trace`` tabulates every kernel of a ``rocprofv3 --kernel-trace --output-format csv``
+run.  The Krylov solvers enqueue iterations ahead of the device's stopping test,
+and the launches past it return at once; they are counted apart ("idle": shorter
+than --idle-us) so that the mean duration is that of launches that did the work.
+With --cells, the algorithmic bytes per interior cell of each known kernel give
+its achieved bandwidth and fraction of the 8 TB/s HBM peak.
+
+``traffic`` turns two ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs
+(kilobytes per dispatch) into bytes per interior cell per launch, scaled by what
+the same counters report for the device-to-device copy of ``--copy-mib`` MiB that
+``bench.py --copy-calibration`` issues first (a copy reads and writes exactly that
+much).
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import re
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per interior cell and launch (compulsory reads + writes,
+# operator matrix-free); the FP32 kernels are those of the precision form
+ALGO_BYTES = {
+    "k_jacobi": 24, "k_bi_spmv<0>": 24, "k_bi_spmv<1>": 24, "k_bi_p": 40, "k_bi_s": 32, "k_bi_xr": 64,
+    "k_bi_init": 32, "k_sor": 16, "k_resnorm": 16, "k_predictor": 104, "k_project": 104, "k_post": 64,
+    "k_flux": 48, "k_rhs": 32,
+    "kf_jacobi": 12, "kg_spmv<0>": 12, "kg_spmv<1>": 12, "kg_p": 20, "kg_s": 16, "kg_xr": 40,
+    "kg_resid<true>": 28, "kg_resid<false>": 24, "km_predictor": 64, "km_project": 80, "km_post": 48,
+    "km_flux_rhs": 44,
+}
+
+
+def short_name(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    m = re.match(r"([\w:]+(<[^>]*>)?)", name)
+    return m.group(1) if m else name[:40]
+
+
+def trace_table(path: str, cells: int | None, idle_us: float) -> str:
+    by = collections.defaultdict(list)
+    for row in csv.DictReader(open(path)):
+        us = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
+        by[short_name(row["Kernel_Name"])].append(us)
+    total = sum(sum(v) for v in by.values())
+    lines = ["| kernel | launches | idle | mean µs (working) | median µs | total ms | share | B/cell | GB/s | of 8 TB/s |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
+    for name, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+        v = np.asarray(v)
+        work = v[v >= idle_us] if name in ALGO_BYTES and (v >= idle_us).any() else v
+        mean = float(work.mean())
+        bpc = ALGO_BYTES.get(name)
+        gbs = frac = ""
+        if cells and bpc:
+            g = bpc * cells / (mean * 1e-6) / 1e9
+            gbs, frac = f"{g:.0f}", f"{g / HBM_PEAK_GBS:.3f}"
+        lines.append(f"| {name} | {len(v)} | {len(v) - len(work)} | {mean:.2f} | {float(np.median(work)):.2f} | "
+                     f"{v.sum() / 1e3:.1f} | {100 * v.sum() / total:.1f}% | {bpc or ''} | {gbs} | {frac} |")
+        if 100 * v.sum() / total < 0.05:
+            break
+    return "\n".join(lines)
+
+
+def _counter(path: str, counter: str):
+    by = collections.defaultdict(list)
+    for row in csv.DictReader(open(path)):
+        if row.get("Counter_Name") != counter:
+            continue
+        by[short_name(row["Kernel_Name"])].append(float(row["Counter_Value"]))
+    return by
+
+
+def traffic_table(fetch: str, write: str, cells: int, copy_mib: float) -> str:
+    f, w = _counter(fetch, "FETCH_SIZE"), _counter(write, "WRITE_SIZE")
+    copy_kib = copy_mib * 1024
+    # the calibration copy is the largest copy dispatch of each run
+    cf = max(f.get("__amd_rocclr_copyBuffer", [0]))
+    cw = max(w.get("__amd_rocclr_copyBuffer", [0]))
+    sf = copy_kib / cf if cf else 1.0
+    sw = copy_kib / cw if cw else 1.0
+    lines = [f"FETCH_SIZE scale {sf:.3f}, WRITE_SIZE scale {sw:.3f} (a {copy_mib:g} MiB copy reads and writes "
+             f"{copy_kib:.0f} KiB; the counters report {cf:.0f} and {cw:.0f} KiB)", "",
+             "| kernel | dispatches | read B/cell | write B/cell | total B/cell | algorithmic B/cell | ratio |",
+             "|---|---|---|---|---|---|---|"]
+    for name in sorted(f, key=lambda k: -np.median(f[k])):
+        if name not in ALGO_BYTES:
+            continue
+        vf, vw = np.asarray(f[name]), np.asarray(w.get(name, [0.0]))
+        # launches past a solver's stopping test move almost nothing: use the
+        # median of the dispatches that did work
+        vf = vf[vf > 0.2 * vf.max()] if vf.max() > 0 else vf
+        vw = vw[vw > 0.2 * vw.max()] if vw.max() > 0 else vw
+        rd = float(np.median(vf)) * 1024 * sf / cells
+        wr = float(np.median(vw)) * 1024 * sw / cells
+        algo = ALGO_BYTES[name]
+        lines.append(f"| {name} | {len(f[name])} | {rd:.1f} | {wr:.1f} | {rd + wr:.1f} | {algo} | "
+                     f"{(rd + wr) / algo:.2f} |")
+    return "\n".join(lines)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("csv")
+    t.add_argument("--cells", type=int)
+    t.add_argument("--idle-us", type=float, default=6.0)
+    r = sub.add_parser("traffic")
+    r.add_argument("fetch_csv")
+    r.add_argument("write_csv")
+    r.add_argument("--cells", type=int, required=True)
+    r.add_argument("--copy-mib", type=float, default=256)
+    a = ap.parse_args()
+    if a.cmd == "trace":
+        print(trace_table(a.csv, a.cells, a.idle_us))
+    else:
+        print(traffic_table(a.fetch_csv, a.write_csv, a.cells, a.copy_mib))
+
+
+if __name__ == "__main__":
+    main()
