@@ -231,7 +231,7 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU };
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP };
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
 
 int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
@@ -259,8 +259,10 @@ int run_bicgstab(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long lo
 // the same solvers on any (x, b) pair; `slot` indexes prev_it
 int run_sor_on(dalm_ctx* ctx, double* x, const double* b, int slot, double omega, long long max_it, double tol,
                long long* it, double* err);
+// resume: continue the iteration held in the device state and in r, r0, kp, q
+// (the precision form's hand-over to FP64) instead of starting one
 int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_kind, double pc_omega, int pc_it,
-                    long long max_it, double tol, long long* it, double* err);
+                    long long max_it, double tol, long long* it, double* err, bool resume = false);
 int launch_residual(dalm_ctx* ctx, const double* x, const double* b, double* r);
 int launch_Ax(dalm_ctx* ctx, const double* x, double* y);
 int launch_sor_sweep(dalm_ctx* ctx, double* x, const double* b, double omega, int color);

@@ -334,15 +334,17 @@ int run_sor(dalm_ctx* ctx, double omega, long long max_it, double tol, long long
 }
 
 int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_kind, double pc_omega, int pc_it,
-                    long long max_it, double tol, long long* it, double* err) {
+                    long long max_it, double tol, long long* it, double* err, bool resume) {
     const double root_n = sqrt((double)ctx->N);
     PCoef A = ctx->coef();
     const Dims& d = ctx->d;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it, root_n);
-    hipLaunchKernelGGL(k_bi_init, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, x, b, ctx->r, ctx->r0,
-                       ctx->partials, ctx->counter, ctx->solve);
-    LAUNCH_CHECK();
-    if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->sgrid, ctx->solve, nullptr)) return rc0;
+    if (!resume) {
+        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it, root_n);
+        hipLaunchKernelGGL(k_bi_init, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, x, b, ctx->r, ctx->r0,
+                           ctx->partials, ctx->counter, ctx->solve);
+        LAUNCH_CHECK();
+        if (int rc0 = launch_finalize(ctx, FIN_BI_INIT, ctx->sgrid, ctx->solve, nullptr)) return rc0;
+    }
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     // preconditioner application for one vector; returns the buffer holding z

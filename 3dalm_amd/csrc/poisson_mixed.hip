@@ -23,10 +23,19 @@
 // and its finalize are enqueued every iteration and return at once when no
 // replacement is due.
 //
-// Safety net: a watchdog on the residual (no 30% gain in 96 iterations, or a
-// rise above 100 x the best, or a breakdown) abandons the FP32 iteration and
-// finishes the solve with the FP64 BiCGStab from the better of the pressures
-// before and after, so every step meets the tolerance whatever the conditioning.
+// Passes: FP32 rounding of the directions perturbs BiCGStab's short recurrences,
+// and on the stretched C2-size operators the FP32 iteration stagnates a few
+// decades below where it started (or wanders off).  A watchdog on the residual (no
+// 30% gain in 48 iterations, or a rise above 100x the lowest) then restarts the
+// FP32 iteration from the true residual with a new shadow residual and direction:
+// a pass that stalls far above the tolerance goes on from there.  A pass that
+// stalls within 30x the tolerance (where a new FP32 pass would stagnate again) is
+// handed over to FP64 without restarting: the true residual replaces r, r0, p and
+// q are widened to FP64 (exactly), rho and beta are recomputed as after a
+// replacement, and the FP64 BiCGStab continues from that state.  After a pass that
+// gains less than a factor 2, too many passes, a breakdown or a non-finite
+// residual, the FP64 BiCGStab restarts from the better of the pressures before and
+// after.  Either way every step meets the tolerance.
 //
 // Algorithmic bytes per iteration, Jacobi m = 3, per cell: p update 20, Jacobi
 // sweeps 4 x 12, q = A z 12, s update 16, t = A z' 12, x/r update 40 = 148,
@@ -45,9 +54,16 @@ namespace {
 // or no 30% gain within kWatchWindow iterations.  Left to run longer, the FP32
 // iteration on the stretched C2-size operators mostly wanders or diverges rather
 // than recovering (measured: windows of 192 and 400 iterations cost more FP64
-// finish iterations than they saved).
+// iterations than they saved; with 96 the FP32 phase idles near the tolerance).
 constexpr double kWatchRise = 1e2;
-constexpr long long kWatchWindow = 96;
+constexpr long long kWatchWindow = 48;
+constexpr long long kChunk = 16;  // iterations enqueued between watchdog checks
+// A stalled FP32 pass is restarted from the true residual (a new shadow residual
+// and direction) while each pass gains at least kPassGain.
+constexpr double kPassGain = 2.0;
+constexpr double kPassFloor = 30.0;
+constexpr double kNearTol = 3.0;  // near the tolerance a stall is called after one chunk
+constexpr int kMaxPasses = 6;
 
 
 __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float* __restrict__ zin,
@@ -238,6 +254,37 @@ __global__ __launch_bounds__(BLOCK) void kg_xr(Dims d, double* __restrict__ x, c
     store_partials<2>(pv, partials);
 }
 
+// Hand-over to FP64: r = b - A x (FP64), and r0, p, q widened to FP64 (exactly);
+// partials <r0, r>, |r|^2
+__global__ __launch_bounds__(BLOCK) void kg_upgrade(Dims d, PCoef A, const double* __restrict__ x,
+                                                    const double* __restrict__ b, const float* __restrict__ r0f,
+                                                    const float* __restrict__ pf, const float* __restrict__ qf,
+                                                    double* __restrict__ r, double* __restrict__ r0,
+                                                    double* __restrict__ p, double* __restrict__ q, double* partials) {
+    const Col cl = col_unit(d);
+    double v[2] = {0.0, 0.0};
+    if (cl.ok) {
+        const OpD op(A, cl.j, cl.k);
+        struct U4 {
+            double b;
+            float r0, p, q;
+        };
+        walk<double, U4>(d, cl, x, [&](int c) { return U4{b[c], r0f[c], pf[c], qf[c]}; },
+                         [&](int i, int c, double x0, double xp, double xm, const Nb4<double>& nb, const U4& a) {
+                             const double rv = a.b - op.apply(A, i, op.diag(A, i), x0, xp, xm, nb);
+                             r[c] = rv;
+                             r0[c] = (double)a.r0;
+                             p[c] = (double)a.p;
+                             q[c] = (double)a.q;
+                             v[0] += (double)a.r0 * rv;
+                             v[1] += rv * rv;
+                         });
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    store_partials<2>(pv, partials);
+}
+
 int read_states(dalm_ctx* ctx) {
     HIP_TRY(hipMemcpyAsync(ctx->h_solve, ctx->solve, 2 * sizeof(SolveState), hipMemcpyDeviceToHost,
                            ctx->stream));
@@ -248,7 +295,7 @@ int read_states(dalm_ctx* ctx) {
 long long chunk_for(long long done_it, long long predicted, long long max_it) {
     long long left = predicted - done_it;
     long long c = left > 8 ? left : 8;
-    if (c > 64) c = 64;
+    if (c > kChunk) c = kChunk;
     if (done_it + c > max_it) c = max_it - done_it;
     return c < 1 ? 1 : c;
 }
@@ -292,24 +339,47 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     };
     ctx->outer_it = 0;
     ctx->fp64_finish_it = 0;
-    hipLaunchKernelGGL(kg_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it, root_n);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<true>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->p,
-                       ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
-    LAUNCH_CHECK();
-    if (int rc = launch_finalize(ctx, FIN_MX_INIT, ctx->sgrid, st, nullptr)) return rc;
-    int rc = read_states(ctx);
-    if (rc) return rc;
-    const double start_err = ctx->h_solve[0].err;
-    if (trace) fprintf(stderr, "[mixed] fp64 residual %.3e at the start\n", start_err);
-    bool stalled = false;
-    if (!ctx->h_solve[0].done) {
-        // the pressure at the start, restored if the FP32 iteration makes it worse
-        HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
-        double best = start_err, anchor = start_err;  // lowest residual; last 30% milestone
+    // the pressure at the start, restored if the FP32 passes end worse
+    HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
+    long long total = 0, n_ru = 0;
+    double start_err = -1, prev_pass = -1, e_out = 0;
+    int status = DALM_NONCONVERGED;
+    bool fp64 = false;
+    int rc;
+    for (int pass = 0;; ++pass) {
+        // (re)start from the true residual: r = b - A x, r0 = r
+        hipLaunchKernelGGL(kg_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it - total, root_n);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<true>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
+                           ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+        LAUNCH_CHECK();
+        if ((rc = launch_finalize(ctx, FIN_MX_INIT, ctx->sgrid, st, nullptr))) return rc;
+        if ((rc = read_states(ctx))) return rc;
+        const double e0 = ctx->h_solve[0].err;
+        e_out = e0;
+        if (pass == 0) start_err = e0;
+        if (trace) fprintf(stderr, "[mixed] pass %d: fp64 residual %.3e after %lld iterations\n", pass, e0, total);
+        if (ctx->h_solve[0].done) {
+            status = ctx->h_solve[0].status;
+            fp64 = status == DALM_BREAKDOWN || status == DALM_NONFINITE;
+            break;
+        }
+        if (total >= max_it) {
+            status = DALM_NONCONVERGED;
+            break;
+        }
+        // FP64 from here after a pass that gained less than kPassGain or after too
+        // many passes
+        if (pass > 0 && (!(e0 < prev_pass / kPassGain) || pass >= kMaxPasses)) {
+            fp64 = true;
+            break;
+        }
+        prev_pass = e0;
+        double best = e0, anchor = e0;  // lowest residual; last 30% milestone
         long long anchor_it = 0, launched = 0;
+        bool stalled = false;
         probe_solve_begin(ctx);
-        while (!ctx->h_solve[0].done && launched < max_it) {
-            const long long chunk = chunk_for(launched, ctx->prev_it[1], max_it);
+        while (launched < max_it - total) {
+            const long long chunk = chunk_for(launched, ctx->prev_it[1], max_it - total);
             for (long long c = 0; c < chunk; ++c) {
                 probe_iteration(ctx, launched + c);
                 probe_begin(ctx, DALM_K_PUPDATE_MIXED);
@@ -346,8 +416,8 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             if ((rc = read_states(ctx))) return rc;
             const SolveState& h = ctx->h_solve[0];
             if (trace)
-                fprintf(stderr, "[mixed]   it %lld: residual %.3e (%lld replacements) alpha %.3e omega %.3e\n", h.it,
-                        h.err, h.n_ru, h.alpha, h.omega);
+                fprintf(stderr, "[mixed]   it %lld: residual %.3e (%lld replacements) alpha %.3e omega %.3e\n",
+                        total + h.it, h.err, h.n_ru, h.alpha, h.omega);
             if (h.done) break;
             const double e = h.err;
             if (e < best) best = e;
@@ -355,22 +425,60 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 anchor = e;
                 anchor_it = h.it;
             }
-            if (!(e == e) || e > kWatchRise * best || h.it - anchor_it >= kWatchWindow) {
+            if (!(e == e) || e > kWatchRise * best || h.it - anchor_it >= kWatchWindow ||
+                (best < kNearTol * tol && h.it - anchor_it >= kChunk)) {
                 stalled = true;
                 break;
             }
         }
+        const SolveState& h = ctx->h_solve[0];
+        probe_solve_end(ctx, h.it);
+        total += h.it;
+        n_ru += h.n_ru;
+        status = h.status;
+        e_out = h.err;
+        if (h.done) {
+            fp64 = status == DALM_BREAKDOWN || status == DALM_NONFINITE;
+            break;  // met the test on the true residual, or reached max_it
+        }
+        if (!stalled) {
+            status = DALM_NONCONVERGED;
+            break;
+        }
+        if (e_out < kPassFloor * tol && total < max_it) {
+            // stalled near the tolerance: continue this iteration in FP64
+            hipLaunchKernelGGL(kg_upgrade, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, (const double*)ctx->p,
+                               (const double*)ctx->b, (const float*)ctx->fr0, (const float*)ctx->fp,
+                               (const float*)ctx->fq, ctx->r, ctx->r0, ctx->kp, ctx->q, ctx->partials);
+            LAUNCH_CHECK();
+            if ((rc = launch_finalize(ctx, FIN_MX_UP, ctx->sgrid, st, nullptr))) return rc;
+            if ((rc = read_states(ctx))) return rc;
+            const long long it32 = ctx->h_solve[0].it;
+            if (trace)
+                fprintf(stderr, "[mixed] stalled near the tolerance (%.3e): FP64 continuation\n", ctx->h_solve[0].err);
+            if (ctx->h_solve[0].done) {
+                status = ctx->h_solve[0].status;
+                e_out = ctx->h_solve[0].err;
+                fp64 = status == DALM_BREAKDOWN || status == DALM_NONFINITE;
+                break;
+            }
+            long long it_all = 0;
+            status = run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it_all,
+                                     &e_out, true);
+            if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
+            ctx->fp64_finish_it = it_all - it32;
+            total += it_all - it32;
+            if (trace)
+                fprintf(stderr, "[mixed] FP64 continuation: %lld iterations, residual %.3e\n", ctx->fp64_finish_it,
+                        e_out);
+            fp64 = status == DALM_BREAKDOWN || status == DALM_NONFINITE;
+            break;
+        }
     }
-    const SolveState& h = ctx->h_solve[0];
-    probe_solve_end(ctx, h.it);
-    long long total = h.it;
-    int status = h.status;
-    double e_out = h.err;
-    ctx->outer_it = h.n_ru;
-    if (!h.done && !stalled) status = DALM_NONCONVERGED;
+    ctx->outer_it = n_ru;
     ctx->prev_it[1] = total;
-    if (stalled || status == DALM_BREAKDOWN || status == DALM_NONFINITE) {
-        // true residual at the end; keep the better of start and end, then finish in FP64
+    if (fp64 && total < max_it) {
+        // true residual now; keep the better of the start and now, then finish in FP64
         hipLaunchKernelGGL(kg_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, 1, root_n);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<true>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
                            ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
@@ -380,19 +488,16 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         const double end_err = ctx->h_solve[0].err;
         if (!(end_err <= start_err))
             HIP_TRY(hipMemcpyAsync(ctx->p, ctx->tmp2, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
+        long long it64 = 0;
+        status = run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
+        if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
+        ctx->fp64_finish_it = it64;
+        total += it64;
         if (trace)
-            fprintf(stderr, "[mixed] FP32 iteration stalled at %.3e (start %.3e): FP64 finish\n", end_err, start_err);
-        if (total < max_it) {
-            long long it64 = 0;
-            status =
-                run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
-            if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
-            ctx->fp64_finish_it = it64;
-            total += it64;
-            if (trace) fprintf(stderr, "[mixed] FP64 finish: %lld iterations, residual %.3e\n", it64, e_out);
-        } else {
-            status = DALM_NONCONVERGED;
-        }
+            fprintf(stderr, "[mixed] FP64 finish from %.3e: %lld iterations, residual %.3e\n",
+                    end_err <= start_err ? end_err : start_err, it64, e_out);
+    } else if (fp64) {
+        status = DALM_NONCONVERGED;
     }
     *it = total;
     *err = e_out;

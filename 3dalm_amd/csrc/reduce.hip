@@ -182,6 +182,37 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         }
         break;
     }
+    case FIN_MX_UP: {
+        // precision form handing its iteration over to FP64: s[0] = <r0, r>,
+        // s[1] = |b - A x|^2 with the true residual; the iteration goes on from the
+        // current direction, as after a residual replacement
+        if (st->done) break;
+        st->ru = 0;
+        double rn = sqrt(s[1]);
+        double err = rn / st->root_n;
+        st->err = err;
+        if (!(err == err) || isinf(err)) {
+            st->done = 1;
+            st->status = DALM_NONFINITE;
+        } else if (err <= st->tol) {
+            st->done = 1;
+            st->status = DALM_OK;
+        } else if (st->it >= st->max_it) {
+            st->done = 1;
+            st->status = DALM_NONCONVERGED;
+        } else {
+            double rho_new = s[0];
+            if (fabs(rho_new) < (double)FLT_MIN) {
+                st->err = rho_new;
+                st->done = 1;
+                st->status = DALM_BREAKDOWN;
+            } else {
+                st->beta = (rho_new / st->rho_prev) * (st->alpha / st->omega);
+                st->rho = rho_new;
+            }
+        }
+        break;
+    }
     default:
         break;
     }

@@ -8,7 +8,9 @@ Workload (one "step" = one Solver::main_loop_once): the single-turbine ALM case 
 256x128x128 cells (box 16x8x8, y/z tanh beta=1.5, rotor D=1 at x=3, 3 blades x 64
 actuator points, TSR 6, eps = 2 dx, 5% seeded noise, dt = 0.005), Poisson by
 BiCGStab with 3 Jacobi sweeps as preconditioner to 1e-8.  Synthetic inputs,
-resident in HBM before the timed region.
+resident in HBM before the timed region.  The run is 1000 steps long; by default
+steps 11-110 are timed (the first steps after the impulsive start need two to
+three times the Poisson iterations of the rest and are warm-up).
 
 Forms: "fp64" is the FP64 drop-in; "mixed" is the precision form (FP32 stencils,
 FP32 BiCGStab vectors with FP64 dot products, norms, pressure and residual
@@ -252,8 +254,8 @@ def run_adjoint(steps, every, warmup=2):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--grid", default="256,128,128")
     ap.add_argument("--probe", default="auto")
