@@ -2,7 +2,7 @@
 
     python 3dalm_amd/profile_summary.py trace <run>_kernel_trace.csv [--cells N]
     python 3dalm_amd/profile_summary.py traffic <fetch>_counter_collection.csv \\
-        <write>_counter_collection.csv --cells N [--copy-mib 256]
+        <write>_counter_collection.csv --cells N [--copy-mib 256] [--json out.json]
 
 ``trace`` tabulates every kernel of a ``rocprofv3 --kernel-trace --output-format csv``
 run.  The Krylov solvers enqueue iterations ahead of the device's stopping test,
@@ -15,13 +15,15 @@ its achieved bandwidth and fraction of the 8 TB/s HBM peak.
 (kilobytes per dispatch) into bytes per interior cell per launch, scaled by what
 the same counters report for the device-to-device copy of ``--copy-mib`` MiB that
 ``bench.py --copy-calibration`` issues first (a copy reads and writes exactly that
-much).
+much).  With --json the per-kernel figures are also written as JSON, which
+bench.py reads to fill roofline.traffic.
 """
 from __future__ import annotations
 
 import argparse
 import collections
 import csv
+import json
 import re
 
 import numpy as np
@@ -79,8 +81,9 @@ def _counter(path: str, counter: str):
     return by
 
 
-def traffic_table(fetch: str, write: str, cells: int, copy_mib: float) -> str:
+def traffic_table(fetch: str, write: str, cells: int, copy_mib: float, json_out: str | None = None) -> str:
     f, w = _counter(fetch, "FETCH_SIZE"), _counter(write, "WRITE_SIZE")
+    record = {"cells": cells, "copy_mib": copy_mib, "kernels": {}}
     copy_kib = copy_mib * 1024
     # the calibration copy is the largest copy dispatch of each run
     cf = max(f.get("__amd_rocclr_copyBuffer", [0]))
@@ -104,6 +107,12 @@ def traffic_table(fetch: str, write: str, cells: int, copy_mib: float) -> str:
         algo = ALGO_BYTES[name]
         lines.append(f"| {name} | {len(f[name])} | {rd:.1f} | {wr:.1f} | {rd + wr:.1f} | {algo} | "
                      f"{(rd + wr) / algo:.2f} |")
+        record["kernels"][name] = {"read_B_per_cell": round(rd, 2), "write_B_per_cell": round(wr, 2),
+                                   "algorithmic_B_per_cell": algo}
+    record["fetch_scale"], record["write_scale"] = round(sf, 4), round(sw, 4)
+    if json_out:
+        with open(json_out, "w") as fh:
+            json.dump(record, fh, indent=1)
     return "\n".join(lines)
 
 
@@ -119,11 +128,12 @@ def main():
     r.add_argument("write_csv")
     r.add_argument("--cells", type=int, required=True)
     r.add_argument("--copy-mib", type=float, default=256)
+    r.add_argument("--json")
     a = ap.parse_args()
     if a.cmd == "trace":
         print(trace_table(a.csv, a.cells, a.idle_us))
     else:
-        print(traffic_table(a.fetch_csv, a.write_csv, a.cells, a.copy_mib))
+        print(traffic_table(a.fetch_csv, a.write_csv, a.cells, a.copy_mib, a.json))
 
 
 if __name__ == "__main__":

@@ -45,6 +45,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-steps/s mixed vs FP64 at div<=1e-6; adjoint/forward step-cost ratio"
 HBM_PEAK_GBS = 8000.0
+# per-kernel HBM traffic of this workload from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# (3dalm_amd/profile_summary.py traffic --json; the command is in the .md beside it)
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_alm_256x128x128.json")
+PROBE_KERNEL = {"jacobi": "k_jacobi", "jacobi32": "kf_jacobi", "spmv": "k_bi_spmv<0>", "update": "k_bi_xr",
+                "pupdate": "k_bi_p", "supdate": "k_bi_s", "predictor": "k_predictor", "spmv_mixed": "kg_spmv<0>",
+                "update_mixed": "kg_xr", "pupdate_mixed": "kg_p", "supdate_mixed": "kg_s"}
 
 # algorithmic bytes per interior cell for one launch of each probed kernel (FP64);
 # see DESIGN.md for the derivation (compulsory reads + writes, matrix-free A)
@@ -317,10 +323,19 @@ def main():
         mean_s = pr["total_ms"] / pr["launches"] / 1e3
         bytes_per_launch = PROBE_BYTES[pr["kernel"]] * N
         ach = bytes_per_launch / mean_s / 1e9
+        traffic = None
+        try:
+            rec = json.load(open(TRAFFIC_JSON))
+            k = rec["kernels"].get(PROBE_KERNEL.get(pr["kernel"], ""))
+            if k and rec["cells"] == N:
+                traffic = round((k["read_B_per_cell"] + k["write_B_per_cell"]) * N)
+        except (OSError, ValueError, KeyError):
+            pass
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": pr["kernel"],
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": pr["kernel"],
                 "launches": pr["launches"], "mean_us": round(mean_s * 1e6, 2),
-                "bytes_per_launch": bytes_per_launch}
+                "bytes_per_launch": bytes_per_launch,
+                "traffic_source": os.path.relpath(TRAFFIC_JSON, ROOT) if traffic else None}
     roof = roofline(R)
     roof_fp64 = roofline(results["fp64"]) if "fp64" in results and main_form != "fp64" else None
     adj = None
