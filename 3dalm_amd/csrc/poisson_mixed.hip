@@ -313,6 +313,11 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     static const bool trace = getenv("DALM_TRACE") != nullptr;
+    // testing aid: DALM_TEST_FP32_ITERATIONS=n ends every FP32 pass after n
+    // iterations as if it had stalled, so that small cases exercise the restarted
+    // passes and the FP64 hand-over (read per solve)
+    const char* cap_env = getenv("DALM_TEST_FP32_ITERATIONS");
+    const long long test_cap = cap_env ? atoll(cap_env) : 0;
     const double root_n = sqrt((double)ctx->N);
     // FP32 preconditioner application on v32 into z0 (spare: the other Jacobi buffer)
     auto precondition = [&](float* z0, float* spare, const float* v32) -> float* {
@@ -426,7 +431,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 anchor_it = h.it;
             }
             if (!(e == e) || e > kWatchRise * best || h.it - anchor_it >= kWatchWindow ||
-                (best < kNearTol * tol && h.it - anchor_it >= kChunk)) {
+                (best < kNearTol * tol && h.it - anchor_it >= kChunk) || (test_cap > 0 && h.it >= test_cap)) {
                 stalled = true;
                 break;
             }

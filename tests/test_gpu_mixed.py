@@ -79,3 +79,30 @@ def test_mixed_sor_pc_and_c1():
     finally:
         s.close()
 
+
+
+def test_mixed_passes_and_fp64_hand_over(monkeypatch):
+    """The precision form's recovery paths: with every FP32 pass cut short after
+    40 iterations (DALM_TEST_FP32_ITERATIONS, read per solve), a pass that ends far
+    above the tolerance is restarted from the true residual and one that ends
+    within 30x of it is handed over to FP64; every solve must still end on the
+    true residual with the step matching the FP64 oracle."""
+    monkeypatch.setenv("DALM_TEST_FP32_ITERATIONS", "40")
+    case = cases.case_c2(48, 24, 24, steps=3, ap_per_blade=8)
+    case.tol, case.max_it = 1e-10, 4000
+    print(f"[mixed recovery paths {case.mesh.interior}]", flush=True)
+    o = po.OracleSolver(case)
+    s = solver.Solver(case, hip.MIXED)
+    try:
+        for _ in range(3):
+            so, sm = o.step(), s.step()
+            print(f"step {sm['step']}: it {sm['it']} err {sm['err']:.3e}", flush=True)
+            assert sm["status"] == hip.OK, sm
+            assert sm["err"] <= case.tol
+            assert sm["it"] > 40  # the cut-short pass was followed by more work
+            assert abs(sm["power"][0] - case.sites[0].rot_speed * so["torque"][0]) <= \
+                1e-3 * abs(case.sites[0].rot_speed * so["torque"][0])
+        m = interior_mask(case.mesh)
+        assert rel_l2(s.field("U")[m], o.field("U")[m]) <= 1e-4
+    finally:
+        s.close()
