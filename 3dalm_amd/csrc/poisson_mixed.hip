@@ -24,18 +24,21 @@
 // replacement is due.
 //
 // Passes: FP32 rounding of the directions perturbs BiCGStab's short recurrences,
-// and on the stretched C2-size operators the FP32 iteration stagnates a few
-// decades below where it started (or wanders off).  A watchdog on the residual (no
-// 30% gain in 48 iterations, or a rise above 100x the lowest) then restarts the
-// FP32 iteration from the true residual with a new shadow residual and direction:
-// a pass that stalls far above the tolerance goes on from there.  A pass that
-// stalls within 30x the tolerance (where a new FP32 pass would stagnate again) is
-// handed over to FP64 without restarting: the true residual replaces r, r0, p and
-// q are widened to FP64 (exactly), rho and beta are recomputed as after a
-// replacement, and the FP64 BiCGStab continues from that state.  After a pass that
-// gains less than a factor 2, too many passes, a breakdown or a non-finite
-// residual, the FP64 BiCGStab restarts from the better of the pressures before and
-// after.  Either way every step meets the tolerance.
+// and on the stretched C2-size operators the FP32 iteration gains three to five
+// decades quickly and then stagnates near a floor (about 1e-8 at 256x128x128,
+// 2e-8 at 512x256x256) or wanders off.  A watchdog on the residual (no 30% gain
+// in 48 iterations; near the tolerance none in 16; a rise above 100x the lowest)
+// ends the pass, and the FP32 iteration restarts from the true residual with a
+// new shadow residual and direction: the floor is relative to where a pass
+// starts, and a restarted pass gets a longer window (128 iterations) because it
+// starts on a smooth residual.  The best pressure seen is kept, and a pass that
+// ended well above it restarts from it.  A restarted pass that stalls near the
+// tolerance, gains less than a factor 2 or is the last allowed is handed over to
+// FP64 without restarting: the true residual replaces r, r0, p and q are widened
+// to FP64 (exactly), rho and beta are recomputed as after a replacement, and the
+// FP64 BiCGStab continues from that state.  A breakdown or a non-finite residual
+// restarts the FP64 solver from the better of the current and the best pressure.
+// Either way every step meets the tolerance on the true residual.
 //
 // Algorithmic bytes per iteration, Jacobi m = 3, per cell: p update 20, Jacobi
 // sweeps 4 x 12, q = A z 12, s update 16, t = A z' 12, x/r update 40 = 148,
@@ -44,6 +47,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <utility>
 
 #include "ctx.h"
 #include "stencil.h"
@@ -62,7 +66,11 @@ constexpr long long kChunk = 16;  // iterations enqueued between watchdog checks
 // and direction) while each pass gains at least kPassGain.
 constexpr double kPassGain = 2.0;
 constexpr double kPassFloor = 30.0;
-constexpr double kNearTol = 3.0;  // near the tolerance a stall is called after one chunk
+constexpr double kNearTol = 3.0;  // first pass: near the tolerance a stall is called after one chunk
+// A restarted pass starts from a smooth residual and gains slowly at first (in
+// FP64 as well): it gets a longer window.
+constexpr long long kRestartWindow = 128;
+constexpr double kRestoreRise = 4.0;  // a pass ending this far above its best restarts from the best
 constexpr int kMaxPasses = 6;
 
 
@@ -344,24 +352,41 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     };
     ctx->outer_it = 0;
     ctx->fp64_finish_it = 0;
-    // the pressure at the start, restored if the FP32 passes end worse
+    // tmp2 keeps the best pressure seen (by the running residual estimate): an FP32
+    // pass that wanders off after a good stretch is restarted from it
     HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
     long long total = 0, n_ru = 0;
-    double start_err = -1, prev_pass = -1, e_out = 0;
+    double prev_pass = -1, e_out = 0, best_saved = -1;
     int status = DALM_NONCONVERGED;
     bool fp64 = false;
     int rc;
-    for (int pass = 0;; ++pass) {
-        // (re)start from the true residual: r = b - A x, r0 = r
+    // (re)start from the true residual of the pressure: r = b - A x, r0 = r
+    auto restart = [&]() -> int {
         hipLaunchKernelGGL(kg_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, max_it - total, root_n);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<true>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
                            ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
-        LAUNCH_CHECK();
-        if ((rc = launch_finalize(ctx, FIN_MX_INIT, ctx->sgrid, st, nullptr))) return rc;
-        if ((rc = read_states(ctx))) return rc;
-        const double e0 = ctx->h_solve[0].err;
+        HIP_TRY(hipGetLastError());
+        if (int r = launch_finalize(ctx, FIN_MX_INIT, ctx->sgrid, st, nullptr)) return r;
+        return read_states(ctx);
+    };
+    for (int pass = 0;; ++pass) {
+        if ((rc = restart())) return rc;
+        double e0 = ctx->h_solve[0].err;
+        if (pass > 0 && e0 > kRestoreRise * best_saved) {
+            // the last pass ended well above its best: restart from the best pressure
+            // if it is really better (the saved value was a running estimate)
+            std::swap(ctx->p, ctx->tmp2);
+            if ((rc = restart())) return rc;
+            if (!(ctx->h_solve[0].err < e0)) {
+                std::swap(ctx->p, ctx->tmp2);
+                if ((rc = restart())) return rc;
+            } else if (trace) {
+                fprintf(stderr, "[mixed] restored the best pressure (%.3e instead of %.3e)\n", ctx->h_solve[0].err, e0);
+            }
+            e0 = ctx->h_solve[0].err;
+        }
         e_out = e0;
-        if (pass == 0) start_err = e0;
+        if (pass == 0) best_saved = e0;
         if (trace) fprintf(stderr, "[mixed] pass %d: fp64 residual %.3e after %lld iterations\n", pass, e0, total);
         if (ctx->h_solve[0].done) {
             status = ctx->h_solve[0].status;
@@ -426,12 +451,19 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             if (h.done) break;
             const double e = h.err;
             if (e < best) best = e;
+            if (e < 0.5 * best_saved) {
+                HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+                best_saved = e;
+            }
             if (e < 0.7 * anchor) {
                 anchor = e;
                 anchor_it = h.it;
             }
-            if (!(e == e) || e > kWatchRise * best || h.it - anchor_it >= kWatchWindow ||
-                (best < kNearTol * tol && h.it - anchor_it >= kChunk) || (test_cap > 0 && h.it >= test_cap)) {
+            const long long window = pass == 0 ? kWatchWindow : kRestartWindow;
+            if (!(e == e) || e > kWatchRise * best || h.it - anchor_it >= window ||
+                (pass == 0 && best < kNearTol * tol && h.it - anchor_it >= kChunk) ||
+                (test_cap > 0 && h.it >= test_cap)) {
                 stalled = true;
                 break;
             }
@@ -450,8 +482,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             status = DALM_NONCONVERGED;
             break;
         }
-        if (e_out < kPassFloor * tol && total < max_it) {
-            // stalled near the tolerance: continue this iteration in FP64
+        // a restarted pass that stalled near the tolerance, gained too little or was
+        // the last allowed: continue it in FP64; otherwise start another FP32 pass
+        if (pass > 0 && (e_out < kPassFloor * tol || !(e_out < e0 / kPassGain) || pass + 1 >= kMaxPasses) &&
+            e_out <= kRestoreRise * best_saved && total < max_it) {
             hipLaunchKernelGGL(kg_upgrade, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, (const double*)ctx->p,
                                (const double*)ctx->b, (const float*)ctx->fr0, (const float*)ctx->fp,
                                (const float*)ctx->fq, ctx->r, ctx->r0, ctx->kp, ctx->q, ctx->partials);
@@ -483,24 +517,24 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     ctx->outer_it = n_ru;
     ctx->prev_it[1] = total;
     if (fp64 && total < max_it) {
-        // true residual now; keep the better of the start and now, then finish in FP64
-        hipLaunchKernelGGL(kg_state_reset, dim3(1), dim3(64), 0, ctx->stream, st, tol, 1, root_n);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<true>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
-                           ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
-        LAUNCH_CHECK();
-        if ((rc = launch_finalize(ctx, FIN_MX_INIT, ctx->sgrid, st, nullptr))) return rc;
-        if ((rc = read_states(ctx))) return rc;
+        // the better (true residual) of the current and the best saved pressure,
+        // then the FP64 BiCGStab from there
+        if ((rc = restart())) return rc;
         const double end_err = ctx->h_solve[0].err;
-        if (!(end_err <= start_err))
-            HIP_TRY(hipMemcpyAsync(ctx->p, ctx->tmp2, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
+        std::swap(ctx->p, ctx->tmp2);
+        if ((rc = restart())) return rc;
+        double from = ctx->h_solve[0].err;
+        if (!(from < end_err)) {
+            std::swap(ctx->p, ctx->tmp2);
+            from = end_err;
+        }
         long long it64 = 0;
         status = run_bicgstab_on(ctx, ctx->p, ctx->b, 2, pc_kind, pc_omega, pc_it, max_it - total, tol, &it64, &e_out);
         if (status == DALM_HIP_ERROR || status == DALM_BAD_ARG) return status;
         ctx->fp64_finish_it = it64;
         total += it64;
         if (trace)
-            fprintf(stderr, "[mixed] FP64 finish from %.3e: %lld iterations, residual %.3e\n",
-                    end_err <= start_err ? end_err : start_err, it64, e_out);
+            fprintf(stderr, "[mixed] FP64 finish from %.3e: %lld iterations, residual %.3e\n", from, it64, e_out);
     } else if (fp64) {
         status = DALM_NONCONVERGED;
     }
