@@ -231,6 +231,13 @@ class Case:
     sites: List[TurbineSite] = field(default_factory=list)
     ap_per_blade: int = 64
     width: float = 0.125
+    # output path (setup.json "output" and "probe", src/solver.h:569-598, 656-700):
+    # times of the first snapshot, between snapshots and of the first averaged step,
+    # and probe positions; the defaults switch it off
+    output_start: float = 1e9
+    output_interval: float = 1e9
+    tavg_start: float = 1e9
+    probes: List[tuple] = field(default_factory=list)
 
     def U0(self) -> np.ndarray:
         return initial_velocity(self.mesh, self.uin, self.noise * abs(self.uin[0]), self.seed)
@@ -277,6 +284,19 @@ def case_adjoint_small(nx=20, ny=12, nz=12, steps=4, ap_per_blade=6) -> Case:
                 steps=steps, turbine=t, sites=[site], ap_per_blade=ap_per_blade, width=4.0 / nx * 2)
 
 
+def case_output_small(steps=4) -> Case:
+    """A 10x6x6 rotor case with every output switched on: snapshots at t = 0, 2 dt,
+    4 dt, ..., the time average from t = dt, three probes (one outside the interior,
+    so inactive).  SOR to 1e-8 keeps the device and CPU paths identical."""
+    m = make_mesh(10, 6, 6, ((0.0, 4.0), (-1.5, 1.5), (0.0, 3.0)), 1.3, 1.3)
+    t = default_turbine(R=0.5, tower=1.5)
+    dt = 0.005
+    return Case("out", m, method="SOR", omega=1.2, tol=1e-8, max_it=2000, steps=steps, dt=dt, turbine=t,
+                sites=[TurbineSite((1.2, 0.0, 0.0), 12.0)], ap_per_blade=6, width=0.8,
+                output_start=0.0, output_interval=2 * dt, tavg_start=dt,
+                probes=[(1.7, 0.1, 1.4), (3.1, -0.6, 0.5), (9.0, 0.0, 1.0)])
+
+
 def turbine_records(turbine: TurbineModel, sites: List[TurbineSite]) -> np.ndarray:
     """Turbine rows (include/dalm_hip.h layout), src/alm.h:144-206: rotation centre
     (-overhang, 0, tower); the angle type is the first non-zero or object entry of
@@ -318,8 +338,11 @@ def write_case_dir(case: Case, path: str, time_steps: Optional[int] = None) -> N
     else:
         eq["relaxation rate"] = case.omega
     setup = {"runtime": {"dt": case.dt, "time": case.dt * (steps + 0.5)},
-             "output": {"output start time": 1e9, "output interval time": 1e9, "time avg start time": 1e9},
+             "output": {"output start time": case.output_start, "output interval time": case.output_interval,
+                        "time avg start time": case.tavg_start},
              "inflow": {"value": list(case.uin)}, "cfd": {"Re": case.Re, "Cs": case.Cs}, "eq": eq}
+    if case.probes:
+        setup["probe"] = [list(p) for p in case.probes]
     if case.turbine is not None and case.sites:
         setup["windturbine array"] = [s.to_dict() for s in case.sites]
         setup["alm"] = {"blade points": case.ap_per_blade, "projection width": case.width}

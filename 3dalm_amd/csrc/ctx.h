@@ -217,6 +217,15 @@ struct dalm_ctx {
     KernelProbe probe;
     AdjointState adj;
 
+    // output-path fields (diag.hip), allocated zero-filled on first use: the
+    // Q-criterion and the time-averaged velocity (FP64 in both forms), and the
+    // probe-cell gather buffers
+    double* qcrit = nullptr;
+    double* Utavg[3] = {};
+    long long* gather_idx = nullptr;
+    double* gather_out = nullptr;
+    long long gather_cap = 0;
+
 
     std::string err;
     int hip_err = 0;
@@ -284,6 +293,13 @@ void probe_solve_begin(dalm_ctx* ctx);
 void probe_iteration(dalm_ctx* ctx, long long i);
 void probe_solve_end(dalm_ctx* ctx, long long iterations);
 int probe_collect(dalm_ctx* ctx);
+
+// output-path fields (diag.hip)
+int ensure_diag_buffers(dalm_ctx* ctx);
+void free_diag_buffers(dalm_ctx* ctx);
+int launch_calc_q(dalm_ctx* ctx);
+int launch_div_store(dalm_ctx* ctx);
+int launch_time_average(dalm_ctx* ctx, double weight);
 
 // precision form (cfd_mixed.hip, poisson_mixed.hip, alm.hip)
 int setup_mixed(dalm_ctx* ctx);

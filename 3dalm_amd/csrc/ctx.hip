@@ -181,6 +181,17 @@ int field_ptrs(dalm_ctx* ctx, int64_t field, void** c, int* ncomp, bool* fp32) {
     case DALM_F_DIV: c[0] = ctx->div; break;
     case DALM_F_SOLID: c[0] = ctx->solid; break;
     case DALM_F_B: c[0] = ctx->b; break;
+    case DALM_F_Q:
+    case DALM_F_UTAVG: {
+        if (int rc = ensure_diag_buffers(ctx)) return rc;
+        if (field == DALM_F_Q) {
+            c[0] = ctx->qcrit;
+        } else {
+            *ncomp = 3;
+            for (int q = 0; q < 3; ++q) c[q] = ctx->Utavg[q];
+        }
+        break;
+    }
     default: {
         if (field < DALM_F_U_B || field > DALM_F_B_B) return DALM_BAD_ARG;
         if (int rc = ensure_adjoint_buffers(ctx)) return rc;
@@ -433,6 +444,7 @@ int dalm_destroy(dalm_ctx* ctx) {
     if (!ctx) return DALM_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     adjoint_free(ctx);
+    free_diag_buffers(ctx);
     for (hipEvent_t e : ctx->probe.ev) (void)hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
@@ -626,8 +638,9 @@ int dalm_calc_nut(dalm_ctx* ctx, double Cs) {
     return sync_stream(ctx);
 }
 int dalm_calc_divergence(dalm_ctx* ctx) {
-    CHECK_FP64();
-    TRY(launch_divergence(ctx));
+    CHECK_CTX();
+    // the precision form's face fluxes are FP64 too: same arithmetic, store only
+    TRY(ctx->precision == DALM_MIXED ? launch_div_store(ctx) : launch_divergence(ctx));
     return sync_stream(ctx);
 }
 int dalm_l2_norm(dalm_ctx* ctx, int64_t field, double* out) {
@@ -780,7 +793,7 @@ int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
 
 int dalm_probe_kernel(dalm_ctx* ctx, int64_t kernel) {
     CHECK_CTX();
-    if (kernel < 0 || kernel > DALM_K_SUPDATE_MIXED) return DALM_BAD_ARG;
+    if (kernel < 0 || kernel > DALM_K_TIME_AVERAGE) return DALM_BAD_ARG;
     TRY(probe_collect(ctx));
     ctx->probe.kernel = (int)kernel;
     ctx->probe.total_ms = 0;

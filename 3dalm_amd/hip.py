@@ -25,11 +25,14 @@ MIXED_BUILT = True  # the precision form is part of this library
 F_U, F_UOLD, F_JU, F_P, F_NUT, F_DIV, F_F, F_SOLID, F_B = range(9)
 # sensitivities of the discrete adjoint (FP64 contexts)
 (F_U_B, F_UOLD_B, F_JU_B, F_JUOLD_B, F_NUT_B, F_NUTOLD_B, F_F_B, F_FOLD_B, F_P_B, F_B_B) = range(9, 19)
+# output path (either form)
+F_Q, F_UTAVG = 19, 20
 FIELDS = {"U": F_U, "Uold": F_UOLD, "JU": F_JU, "p": F_P, "nut": F_NUT, "div": F_DIV, "f": F_F,
           "solid": F_SOLID, "b": F_B,
           "U_b": F_U_B, "Uold_b": F_UOLD_B, "JU_b": F_JU_B, "JUold_b": F_JUOLD_B, "nut_b": F_NUT_B,
-          "nutold_b": F_NUTOLD_B, "f_b": F_F_B, "fold_b": F_FOLD_B, "p_b": F_P_B, "b_b": F_B_B}
-VECTOR_FIELDS = {F_U, F_UOLD, F_JU, F_F, F_U_B, F_UOLD_B, F_JU_B, F_JUOLD_B, F_F_B, F_FOLD_B}
+          "nutold_b": F_NUTOLD_B, "f_b": F_F_B, "fold_b": F_FOLD_B, "p_b": F_P_B, "b_b": F_B_B,
+          "q": F_Q, "Utavg": F_UTAVG}
+VECTOR_FIELDS = {F_U, F_UOLD, F_JU, F_F, F_U_B, F_UOLD_B, F_JU_B, F_JUOLD_B, F_F_B, F_FOLD_B, F_UTAVG}
 ADJ_KEEP_PRESSURE = 1
 SOR, BICGSTAB = 0, 1
 PC_JACOBI, PC_SOR = 0, 1
@@ -108,6 +111,9 @@ def lib():
         "dalm_calc_divergence": [V],
         "dalm_l2_norm": [V, i64, P],
         "dalm_max_cfl": [V, dbl, P],
+        "dalm_calc_q": [V],
+        "dalm_time_average": [V, dbl],
+        "dalm_sample_U": [V, PI64, i64, P],
         "dalm_build_ap_props": [P, P, P, P, P, i64, i64, dbl, dbl, i64, i64, i64, P, P, P],
         "dalm_alm_setup": [V, P, i64, P, i64, i64, i64, P, P, P, i64, dbl],
         "dalm_run_alm": [V, dbl, P, P],
@@ -273,6 +279,20 @@ class Context:
         v = dbl()
         self.check(self.L.dalm_max_cfl(self.h, dt, C.byref(v)), "max_cfl")
         return v.value
+
+    # ---- output path
+    def calc_q(self):
+        self.check(self.L.dalm_calc_q(self.h), "calc_q")
+
+    def time_average(self, weight):
+        self.check(self.L.dalm_time_average(self.h, weight), "time_average")
+
+    def sample_U(self, cells):
+        """U at padded linear cell indices, [n][3]."""
+        c = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+        out = np.zeros((len(c), 3))
+        self.check(self.L.dalm_sample_U(self.h, c.ctypes.data_as(PI64), len(c), ptr(out)), "sample_U")
+        return out
 
     # ---- ALM
     def alm_setup(self, wt, ap, blade_per_wt, ap_per_blade, cd, cl, atk, width):

@@ -7,16 +7,21 @@
 // time loop on the GPU and prints the reference's per-step line
 // "step time it err avg_div max_cfl" (src/solver.h:1188-1190).
 //
+// Output (src/solver.h:569-598, 656-700, 950-999, 1193-1250, host/output.h): the
+// folder <case_dir>/output (-c clears it first), partition.json, ap_info.log,
+// probe<m>.csv, inst_/tavg_ snapshot files and snapshot.json on setup.json's
+// "output" schedule, the time average from "time avg start time" on.
+//
 // Beyond the reference (needed by the measured workloads): --steps N replaces
 // time/dt, --perturb A --seed S adds the seeded uniform perturbation of the
 // interior velocity, --precision mixed selects the FP32 + FP64-refined form, and
-// --power prints per-turbine power after each step line.  Snapshot output, probes,
-// time averaging and the turbulence-generating grid are diagnostics / features
-// outside this build; a case that asks for the grid is refused.
+// --power prints per-turbine power after each step line.  The turbulence-generating
+// grid is not implemented; a case that asks for it is refused.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -24,6 +29,7 @@
 
 #include "dalm_hip.h"
 #include "json_lite.h"
+#include "output.h"
 
 namespace {
 
@@ -65,6 +71,7 @@ struct Args {
     double perturb = 0;
     uint64_t seed = 1;
     bool power = false;
+    bool clear = false;
 };
 
 Args parse_args(int argc, char** argv) {
@@ -75,7 +82,7 @@ Args parse_args(int argc, char** argv) {
             if (q + 1 >= argc) throw std::runtime_error("missing value after " + s);
             return argv[++q];
         };
-        if (s == "-c" || s == "--clear") continue;  // output folder handling is not part of this build
+        if (s == "-c" || s == "--clear") a.clear = true;
         else if (s == "--steps") a.steps = std::stoll(next());
         else if (s == "--precision") a.precision = next() == "mixed" ? DALM_MIXED : DALM_FP64;
         else if (s == "--perturb") a.perturb = std::stod(next());
@@ -88,6 +95,60 @@ Args parse_args(int argc, char** argv) {
                                                 "[--precision fp64|mixed] [--perturb A --seed S] [--power]");
     return a;
 }
+
+// The output side of Solver (src/solver.h:1193-1250) on a context.
+struct Output {
+    dalm_ctx* ctx;
+    std::filesystem::path dir;
+    int64_t size[3], gc;
+    const std::vector<double>*x, *y, *z;
+    double dt, start, interval, tavg_start, next;
+    long long tavg_count = 0;
+    std::vector<dalm_out::Probe> probes;
+    std::vector<int64_t> cells;
+    std::vector<dalm_out::SnapshotEntry> snaps;
+
+    void after_step(long long step) {
+        const double t = dt * (double)step;
+        if (t >= tavg_start) {
+            ++tavg_count;
+            check(dalm_time_average(ctx, 1. / (double)tavg_count), ctx, "dalm_time_average");
+        }
+        if (t >= start && !probes.empty()) {
+            std::vector<double> u(3 * probes.size());
+            check(dalm_sample_U(ctx, cells.data(), (int64_t)cells.size(), u.data()), ctx, "dalm_sample_U");
+            for (size_t m = 0; m < probes.size(); ++m) probes[m].add((float)t, &u[3 * m]);
+        }
+        if (t - 0.5 * dt < next && next <= t + 0.5 * dt) {
+            next += interval;
+            check(dalm_calc_q(ctx), ctx, "dalm_calc_q");
+            check(dalm_calc_divergence(ctx), ctx, "dalm_calc_divergence");
+            const int64_t L = size[0] * size[1] * size[2];
+            std::vector<double> U(3 * L), p(L), dv(L), q(L), f(3 * L);
+            check(dalm_download(ctx, DALM_F_U, U.data()), ctx, "download U");
+            check(dalm_download(ctx, DALM_F_P, p.data()), ctx, "download p");
+            check(dalm_download(ctx, DALM_F_DIV, dv.data()), ctx, "download div");
+            check(dalm_download(ctx, DALM_F_Q, q.data()), ctx, "download q");
+            check(dalm_download(ctx, DALM_F_F, f.data()), ctx, "download f");
+            dalm_out::write_binary((dir / dalm_out::rank_binary_filename("inst", 0, step)).string(), size, gc,
+                                   {{"U", 3, &U}, {"p", 1, &p}, {"div", 1, &dv}, {"q", 1, &q}, {"f", 3, &f}}, *x,
+                                   *y, *z);
+            dalm_out::SnapshotEntry e{step, t, false};
+            if (t >= tavg_start) {
+                check(dalm_download(ctx, DALM_F_UTAVG, U.data()), ctx, "download Utavg");
+                dalm_out::write_binary((dir / dalm_out::rank_binary_filename("tavg", 0, step)).string(), size, gc,
+                                       {{"U", 3, &U}}, *x, *y, *z);
+                e.tavg = true;
+            }
+            snaps.push_back(e);
+            dalm_out::write_text((dir / "snapshot.json").string(), dalm_out::snapshot_json(snaps));
+        }
+    }
+    void finish() {
+        dalm_out::write_text((dir / "snapshot.json").string(), dalm_out::snapshot_json(snaps));
+        for (auto& pr : probes) pr.flush();
+    }
+};
 
 }  // namespace
 
@@ -138,6 +199,41 @@ int main(int argc, char** argv) {
         }
         check(dalm_set_params(ctx, &P), ctx, "dalm_set_params");
         long long max_step = a.steps >= 0 ? a.steps : (long long)(setup["runtime"]["time"].d() / P.dt);
+
+        // output folder and schedule (src/solver.h:569-598)
+        namespace fs = std::filesystem;
+        Output out{ctx, fs::path(a.dir) / "output", {size[0], size[1], size[2]}, gc, &x, &y, &z, P.dt,
+                   1e300, 1e300, 1e300, 1e300};
+        if (a.clear && fs::exists(out.dir) && fs::remove_all(out.dir))
+            std::printf("delete %s\n", out.dir.c_str());
+        if (!fs::exists(out.dir) && fs::create_directories(out.dir)) std::printf("create %s\n", out.dir.c_str());
+        if (setup.has("output")) {
+            const jl::Value& o = setup["output"];
+            out.start = out.next = o["output start time"].d();
+            out.interval = o["output interval time"].d();
+            out.tavg_start = o["time avg start time"].d();
+        }
+        dalm_out::write_text((out.dir / "partition.json").string(), dalm_out::partition_json(size, gc));
+        // probes at the nearest cell centre among the interior and the first ghost
+        // layer, kept only when that cell is interior (src/solver.h:656-700)
+        if (setup.has("probe")) {
+            const jl::Value& pr = setup["probe"];
+            for (size_t m = 0; m < pr.size(); ++m) {
+                const std::vector<double>* c[3] = {&x, &y, &z};
+                long long idx[3];
+                bool inside = true;
+                for (int ax = 0; ax < 3; ++ax) {
+                    idx[ax] = dalm_out::find_nearest_index(c[ax]->data() + gc - 1, pr[m][ax].d(),
+                                                           size[ax] - 2 * gc + 2) + gc - 1;
+                    inside = inside && idx[ax] >= gc && idx[ax] < size[ax] - gc;
+                }
+                if (!inside) continue;
+                out.probes.emplace_back((out.dir / ("probe" + std::to_string(m) + ".csv")).string(),
+                                        (idx[0] * size[1] + idx[1]) * size[2] + idx[2], x[idx[0]], y[idx[1]],
+                                        z[idx[2]]);
+                out.cells.push_back(out.probes.back().cell);
+            }
+        }
 
         // initial fields: U = Uold = Uin everywhere, p = 0 (src/solver.h:737-740),
         // optionally a seeded perturbation of the interior velocity
@@ -215,26 +311,32 @@ int main(int argc, char** argv) {
             check(dalm_alm_setup(ctx, wt.data(), wt_count, ap.data(), n_ap, blades, per_blade, cd.data(),
                                  cl.data(), atk.data(), (int64_t)atk.size(), width),
                   ctx, "dalm_alm_setup");
+            if (n_ap > 0)
+                dalm_out::write_ap_info((out.dir / "ap_info.log").string(), ap, DALM_AP_STRIDE, n_ap, cd, cl, atk);
         }
 
         dalm_step_stats st;
         check(dalm_initialize(ctx, &st), ctx, "dalm_initialize");
         std::printf("# initial div(U) = %e, max cfl = %e\n", st.avg_div, st.max_cfl);
+        out.after_step(0);
         for (long long s = 1; s <= max_step; ++s) {
             int rc = dalm_step(ctx, &st);
             if (rc != DALM_OK && rc != DALM_BREAKDOWN && rc != DALM_NONFINITE) check(rc, ctx, "dalm_step");
             std::printf("%ld %e %ld %e %e %e\n", (long)st.step, st.time, (long)st.it, st.err, st.avg_div,
                         st.max_cfl);
+            out.after_step(s);
             if (a.power)
                 for (int64_t w = 0; w < st.wt_count; ++w)
                     std::printf("# wt %ld torque %.17g thrust %.17g power %.17g\n", (long)w, st.torque[w],
                                 st.thrust[w], st.power[w]);
             if (st.status == DALM_NONFINITE) {
                 std::fprintf(stderr, "non-finite residual at step %ld\n", (long)st.step);
+                out.finish();
                 dalm_destroy(ctx);
                 return 3;
             }
         }
+        out.finish();
         dalm_destroy(ctx);
         return 0;
     } catch (const std::exception& e) {

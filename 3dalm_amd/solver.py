@@ -28,9 +28,13 @@ def params_for(case: Case) -> hip.Params:
 
 class Solver:
     """``ap_hook(ap, wt)``, when given, may change the actuator-point and turbine
-    records (chord, twist, pitch, angle formula) before the start-up pass."""
+    records (chord, twist, pitch, angle formula) before the start-up pass.  With
+    ``output_dir`` the reference's output path runs after the start-up pass and
+    after every step (snapshots, time average, probes, on the case's schedule;
+    3dalm_amd/output.py), as main_loop_once does (src/solver.h:1193-1250)."""
 
-    def __init__(self, case: Case, precision: int = hip.FP64, U0: np.ndarray | None = None, ap_hook=None):
+    def __init__(self, case: Case, precision: int = hip.FP64, U0: np.ndarray | None = None, ap_hook=None,
+                 output_dir: str | None = None):
         self.case = case
         self.ctx = hip.Context(case.mesh, precision)
         L = case.mesh.L
@@ -52,11 +56,22 @@ class Solver:
                 ap_hook(ap, wt)
             self.ctx.alm_setup(wt, ap, case.turbine.blades, case.ap_per_blade, cd, cl, atk, case.width)
             self.ap = ap
+            alm_tables = (ap, cd, cl, atk)
+        else:
+            alm_tables = None
+        self.output = None
+        if output_dir is not None:
+            from .output import OutputWriter
+            self.output = OutputWriter(self.ctx, case, output_dir, alm_tables)
         self.stats = self.ctx.initialize()
         self.max_step = case.steps
+        if self.output:
+            self.output.after_step(0)
 
     def main_loop_once(self):
         self.stats = self.ctx.step()
+        if self.output:
+            self.output.after_step(int(self.stats["step"]))
         return self.stats
 
     step = main_loop_once
@@ -68,4 +83,7 @@ class Solver:
         return self.ctx.download(name)
 
     def close(self):
+        if self.output:
+            self.output.close()
+            self.output = None
         self.ctx.close()

@@ -63,7 +63,10 @@ enum dalm_field {
     DALM_F_F_B = 15,      /* [L][3] bar of f                        */
     DALM_F_FOLD_B = 16,   /* [L][3] bar of the previous step's f    */
     DALM_F_P_B = 17,      /* [L]    bar of p                        */
-    DALM_F_B_B = 18       /* [L]    bar of b (Poisson right-hand side) */
+    DALM_F_B_B = 18,      /* [L]    bar of b (Poisson right-hand side) */
+    /* output-path fields, FP64 in both forms, zero until first computed */
+    DALM_F_Q = 19,        /* [L]    Q-criterion            (Cfd::q)     */
+    DALM_F_UTAVG = 20     /* [L][3] time-averaged velocity (Cfd::Utavg) */
 };
 
 enum dalm_method { DALM_SOR = 0, DALM_BICGSTAB = 1 };
@@ -173,12 +176,27 @@ int dalm_apply_Ubc(dalm_ctx *ctx, const double Uin[3], double dt, int64_t retrac
                    double floor_start_x);
 /* calc_nut (src/cfd.h:827-946) */
 int dalm_calc_nut(dalm_ctx *ctx, double Cs);
-/* calc_divergence (src/cfd.h:1079-1107) */
+/* calc_divergence (src/cfd.h:1079-1107): stores the field DALM_F_DIV (either form) */
 int dalm_calc_divergence(dalm_ctx *ctx);
 /* calc_l2_norm (src/mv.h:33-55) of a scalar field */
 int dalm_l2_norm(dalm_ctx *ctx, int64_t field, double *out);
 /* calc_max_cfl (src/cfd.h:1109-1141) */
 int dalm_max_cfl(dalm_ctx *ctx, double dt, double *out);
+
+/* ---- output path (diagnostics) ---------------------------------------------------- */
+
+/* calc_q (src/cfd.h:948-1077): Q-criterion of U over the interior into the field
+ * DALM_F_Q (either form). */
+int dalm_calc_q(dalm_ctx *ctx);
+/* The time average of Solver::main_loop_once (src/solver.h:1194-1204):
+ * Utavg = (1 - weight) Utavg + weight U over every padded cell, the reference's
+ * calc_ax_plus_by(1. - weight, Utavg, weight, U, Utavg, len) (src/mv.h:321-335);
+ * weight = 1 / (number of averaged steps), in (0, 1]. */
+int dalm_time_average(dalm_ctx *ctx, double weight);
+/* U at n padded linear cell indices (src/util.h:10-13) into out[n][3]: the probe
+ * records' `acc update host(cfd.U[id:1])` (src/solver.h:1205-1216), one gather and
+ * one copy for all probes. */
+int dalm_sample_U(dalm_ctx *ctx, const int64_t *cell, int64_t n, double *out);
 
 /* ---- actuator-line model ------------------------------------------------------ */
 
@@ -221,7 +239,10 @@ enum dalm_kernel {
     DALM_K_SPMV_MIXED = 10,    /* A z (FP64 arithmetic, FP32 result) + dots */
     DALM_K_UPDATE_MIXED = 11,  /* x (FP64) / r update + norms           */
     DALM_K_PUPDATE_MIXED = 12, /* p update, first PC sweep (FP32)       */
-    DALM_K_SUPDATE_MIXED = 13  /* s update, first PC sweep (FP32)      */
+    DALM_K_SUPDATE_MIXED = 13, /* s update, first PC sweep (FP32)      */
+    /* output path */
+    DALM_K_CALC_Q = 14,        /* Q-criterion                          */
+    DALM_K_TIME_AVERAGE = 15   /* time average of U                    */
 };
 /* Time every launch of `kernel` with HIP events on the context's stream from now
  * on (DALM_K_NONE stops); resets the totals. */

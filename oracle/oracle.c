@@ -17,6 +17,11 @@
  *     avg_div = dt*max_diag*err, single-AP force in closed form, Gaussian weights
  *     summing to ~1, and the reference's AP-property fixture test/wtprop.json.
  *
+ * Output path: calc_q and the time average of U (src/cfd.h:948-1077,
+ *   src/solver.h:1194-1204), pinned the same way plus closed forms (solid-body
+ *   rotation q = omega^2, pure strain q = -a^2, shear q = 0; the average of a
+ *   constant is that constant, of n fields their mean).
+ *
  * Reference files restated: src/cfd.h, src/eq.h, src/mv.h, src/bc.h, src/alm.h,
  * src/euler_angle.h, src/util.h, src/io.h:98-164, src/solver.h:727-833,1034-1190.
  */
@@ -444,6 +449,34 @@ void or_nut(const or_grid *g, const double *U, double *nut, double Cs) {
                         sq(G3[0][1] + G3[1][0]) + sq(G3[0][2] + G3[2][0]) + sq(G3[1][2] + G3[2][1]));
         nut[c] = sq(Cs * cbrt(vol)) * S;
     }
+}
+
+/* Q-criterion: src/cfd.h:948-976 (calc_q_at_cell), over the interior as calc_q
+ * does on one rank (src/cfd.h:978-1077); other cells are left as they are. */
+void or_calc_q(const or_grid *g, const double *U, double *q) {
+    const double *X = g->c[0], *Y = g->c[1], *Z = g->c[2];
+    const i64 si = g->n[1] * g->n[2], sj = g->n[2];
+    INTERIOR(g, i, j, k) {
+        i64 c = lin(g, i, j, k);
+        double hx = X[i + 1] - X[i - 1], hy = Y[j + 1] - Y[j - 1], hz = Z[k + 1] - Z[k - 1];
+        double G3[3][3]; /* G3[m][a] = d U_m / d x_a */
+        for (int m = 0; m < 3; ++m) {
+            G3[m][0] = (U[3 * (c + si) + m] - U[3 * (c - si) + m]) / hx;
+            G3[m][1] = (U[3 * (c + sj) + m] - U[3 * (c - sj) + m]) / hy;
+            G3[m][2] = (U[3 * (c + 1) + m] - U[3 * (c - 1) + m]) / hz;
+        }
+        double diag = G3[0][0] * G3[0][0] + G3[1][1] * G3[1][1] + G3[2][2] * G3[2][2];
+        double cross = G3[0][1] * G3[1][0] + G3[0][2] * G3[2][0] + G3[1][2] * G3[2][1];
+        q[c] = -0.5 * (diag + 2 * cross);
+    }
+}
+
+/* Time average of U over every padded cell, src/solver.h:1194-1204 with
+ * calc_ax_plus_by<3> (src/mv.h:321-335): Utavg = (1 - w) Utavg + w U. */
+void or_time_average(const or_grid *g, double *Utavg, const double *U, double weight) {
+    const double a = 1. - weight, b = weight;
+    const i64 n = 3 * g->n[0] * g->n[1] * g->n[2];
+    for (i64 e = 0; e < n; ++e) Utavg[e] = a * Utavg[e] + b * U[e];
 }
 
 /* src/cfd.h:1109-1141 */

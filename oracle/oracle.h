@@ -75,6 +75,10 @@ void or_nut(const or_grid *g, const double *U, double *nut, double Cs);
 void or_divergence(const or_grid *g, const double *JU, double *div);
 double or_max_cfl(const or_grid *g, const double *U, double dt);
 
+/* ---- output path --------------------------------------------------------- */
+void or_calc_q(const or_grid *g, const double *U, double *q);
+void or_time_average(const or_grid *g, double *Utavg, const double *U, double weight);
+
 /* ---- actuator-line model ------------------------------------------------- */
 /* Turbine record: 16 doubles
  *   [0..2] base, [3..5] rot_center, [6] angle_type (0 roll,1 pitch,2 yaw,3 none),

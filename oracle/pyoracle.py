@@ -85,6 +85,8 @@ def lib(kind="oracle"):
             "nut": (None, [G, P, P, dbl]),
             "divergence": (None, [G, P, P]),
             "max_cfl": (dbl, [G, P, dbl]),
+            "calc_q": (None, [G, P, P]),
+            "time_average": (None, [G, P, P, dbl]),
             "update_ap_position": (None, [P, i64, P, i64, i64, i64, dbl]),
             "calc_ap_force": (None, [G, P, P, i64, P, i64, i64, i64, P, P, P, i64]),
             "distribute_ap_force": (None, [G, P, P, i64, dbl]),
@@ -126,6 +128,7 @@ def lib(kind="oracle"):
             L.ref_solver_create.argtypes = [C.c_char_p, P, dbl, dbl, dbl, i64, i64, dbl, dbl, i64, i64,
                                             dbl, C.c_char_p, C.c_char_p, i64, dbl]
             L.ref_solver_perturb.argtypes = [C.c_void_p, P]
+            L.ref_solver_output.argtypes = [C.c_void_p, C.c_char_p, dbl, dbl, dbl, P, i64]
             L.ref_solver_initialize.argtypes = [C.c_void_p]
             L.ref_solver_step.argtypes = [C.c_void_p]
             L.ref_solver_get.restype = C.c_int
@@ -278,7 +281,10 @@ class OracleSolver:
 class RefSolver:
     """The reference's own Solver driven through ref_harness.cpp."""
 
-    def __init__(self, case, workdir=None):
+    def __init__(self, case, workdir=None, output_dir=None):
+        """With output_dir, the reference's own output path is switched on with the
+        case's schedule and probes (files appear there; probe files are completed
+        by close())."""
         L, _ = lib("ref")
         self.L = L
         self.case = case
@@ -297,6 +303,10 @@ class RefSolver:
         self.nwt = len(case.sites) if wt_prop else 0
         U0 = np.ascontiguousarray(case.U0())
         L.ref_solver_perturb(self.h, ptr(U0))
+        if output_dir is not None:
+            pr = np.ascontiguousarray(np.array(case.probes, dtype=np.float64).reshape(-1, 3))
+            L.ref_solver_output(self.h, str(output_dir).encode(), case.output_start, case.output_interval,
+                                case.tavg_start, ptr(pr), len(pr))
         L.ref_solver_initialize(self.h)
 
     def step(self):
@@ -316,7 +326,7 @@ class RefSolver:
 
     def field(self, name):
         n = self.case.mesh.L
-        w = 3 if name in ("U", "Uold", "JU", "f") else 1
+        w = 3 if name in ("U", "Uold", "JU", "f", "Utavg") else 1
         out = np.zeros(n * w)
         assert self.L.ref_solver_get(self.h, name.encode(), ptr(out)) == 0
         return out.reshape(n, 3) if w == 3 else out

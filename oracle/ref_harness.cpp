@@ -256,6 +256,17 @@ void ref_divergence(const or_grid *g, const double *JU, double *div) {
     MpiInfo m = one_rank();
     calc_divergence(v3(JU), div, a.dx, a.dy, a.dz, a.size, a.gc, &m);
 }
+void ref_calc_q(const or_grid *g, const double *U, double *q) {
+    G a(g);
+    MpiInfo m = one_rank();
+    calc_q(v3(U), q, a.x, a.y, a.z, a.size, a.gc, &m);
+}
+// the time average as Solver::main_loop_once computes it (src/solver.h:1197-1203)
+void ref_time_average(const or_grid *g, double *Utavg, const double *U, double weight) {
+    G a(g);
+    const Int len = a.size[0] * a.size[1] * a.size[2];
+    calc_ax_plus_by(1. - weight, v3(Utavg), weight, v3(U), v3(Utavg), len);
+}
 double ref_max_cfl(const or_grid *g, const double *U, double dt) {
     G a(g);
     MpiInfo m = one_rank();
@@ -387,6 +398,69 @@ void *ref_solver_create(const char *mesh_path, const double Uin[3], double Re, d
     return R;
 }
 
+// The output part of Solver::initialize (src/solver.h:569-598, 656-700, 835-846):
+// output folder, output / time-average schedule, the inst and tavg variable lists
+// and the probes at the nearest cell centres.  Call before ref_solver_initialize
+// (whose step-0 pass already records and writes).
+void ref_solver_output(void *h, const char *dir, double start, double interval, double tavg_start,
+                       const double *probe_xyz, i64 nprobe) {
+    Solver &S = static_cast<RefSolver *>(h)->s;
+    S.output_dir = dir;
+    S.rt.output_start_time = start;
+    S.rt.output_interval_time = interval;
+    S.rt.next_output_time = start;
+    S.rt.tavg_start_time = tavg_start;
+    S.out_handler.set_size(S.size, S.gc);
+    S.out_handler.set_var({S.cfd.U[0], S.cfd.p, S.cfd.div, S.cfd.q, S.cfd.f[0]}, {3, 1, 1, 1, 3},
+                          {"U", "p", "div", "q", "f"});
+    S.tavg_out_handler.set_size(S.size, S.gc);
+    S.tavg_out_handler.set_var({S.cfd.Utavg[0]}, {3}, {"U"});
+    {   // partition.json (src/solver.h:982-999), one rank
+        nlohmann::json part;
+        part["size"] = {S.gsize[0], S.gsize[1], S.gsize[2]};
+        part["gc"] = S.gc;
+        part["partition"] = {};
+        Int rs[3], ro[3];
+        calc_partition(S.gsize, rs, ro, S.gc, &S.mpi);
+        nlohmann::json r;
+        r["size"] = {rs[0], rs[1], rs[2]};
+        r["offset"] = {ro[0], ro[1], ro[2]};
+        r["rank"] = 0;
+        part["partition"].push_back(r);
+        std::ofstream po(S.output_dir / "partition.json");
+        po << std::setw(2) << part;
+    }
+    if (S.alm.ap_count > 0) {  // ap_info.log (src/solver.h:950-977)
+        auto &alm = S.alm;
+        std::string path = S.output_dir / "ap_info.log";
+        FILE *f = fopen(path.c_str(), "w");
+        fprintf(f, "ap count = %ld, atk acount = %ld\n", alm.ap_count, alm.atk_count);
+        for (Int i = 0; i < alm.ap_count; i++) {
+            auto &ap = alm.ap_lst[i];
+            fprintf(f, "ap %ld\n\tr = %lf\n\tdr = %lf\n\tchord = %lf\n\ttwist = %lf\n\tcd = \n", i, ap.r, ap.dr,
+                    ap.chord, ap.twist);
+            for (Int j = 0; j < alm.atk_count; j++) fprintf(f, "\t\t%lf\n", alm.cd_tbl[i][j]);
+            fprintf(f, "\tcl = \n");
+            for (Int j = 0; j < alm.atk_count; j++) fprintf(f, "\t\t%lf\n", alm.cl_tbl[i][j]);
+        }
+        fprintf(f, "atk\n");
+        for (Int j = 0; j < alm.atk_count; j++) fprintf(f, "\t%lf\n", alm.atk_lst[j]);
+        fclose(f);
+    }
+    S.probes = std::vector<Probe>(nprobe);
+    const Int gc = S.gc;
+    Real *x = S.mesh.x, *y = S.mesh.y, *z = S.mesh.z;
+    for (i64 m = 0; m < nprobe; ++m) {
+        Int ni = find_nearest_index(x + gc - 1, probe_xyz[3 * m], S.size[0] - 2 * gc + 2) + gc - 1;
+        Int nj = find_nearest_index(y + gc - 1, probe_xyz[3 * m + 1], S.size[1] - 2 * gc + 2) + gc - 1;
+        Int nk = find_nearest_index(z + gc - 1, probe_xyz[3 * m + 2], S.size[2] - 2 * gc + 2) + gc - 1;
+        if (ni >= gc && ni < S.size[0] - gc && nj >= gc && nj < S.size[1] - gc && nk >= gc &&
+            nk < S.size[2] - gc)
+            S.probes[m].initialize(S.output_dir / ("probe" + std::to_string(m) + ".csv"), ni, nj, nk, x[ni],
+                                   y[nj], z[nk]);
+    }
+}
+
 // Overwrite the interior of U with U0 (an [L][3] array); ghosts keep Uin.
 void ref_solver_perturb(void *h, const double *U0) {
     Solver &S = static_cast<RefSolver *>(h)->s;
@@ -424,7 +498,7 @@ void ref_solver_initialize(void *h) {
 
 void ref_solver_step(void *h) { static_cast<RefSolver *>(h)->s.main_loop_once(); }
 
-// name: U Uold JU f (L*3), p nut div b (L)
+// name: U Uold JU f Utavg (L*3), p nut div b q (L)
 int ref_solver_get(void *h, const char *name, double *out) {
     Solver &S = static_cast<RefSolver *>(h)->s;
     Int len = S.size[0] * S.size[1] * S.size[2];
@@ -439,6 +513,8 @@ int ref_solver_get(void *h, const char *name, double *out) {
     else if (n == "nut") src = S.cfd.nut;
     else if (n == "div") src = S.cfd.div;
     else if (n == "b") src = S.eq.b;
+    else if (n == "q") src = S.cfd.q;
+    else if (n == "Utavg") { src = S.cfd.Utavg[0]; w = 3; }
     else return -1;
     std::memcpy(out, src, sizeof(double) * len * w);
     return 0;

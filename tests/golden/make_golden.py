@@ -17,6 +17,8 @@ Fixtures:
   wtprop.npz        the reference's test/wtprop.json fixture (expected output of
                     build_ap_props(test/turbine.json, 2, 20)) with its inputs
   mesh.npz          build_mesh_from_json on a generated mesh.json
+  output_small.npz  the reference's output files (snapshots, snapshot.json,
+                    partition.json, probes, ap_info.log) for a 10x6x6 rotor case
 """
 import argparse
 import importlib
@@ -257,13 +259,32 @@ def mesh_fixture(out):
                         dy=arrs[4], dz=arrs[5], size=np.array(list(sz)), gc=np.array([gc.value]))
 
 
+def output_small(out):
+    """The reference's own output files for cases.case_output_small(): snapshot
+    files, snapshot.json, partition.json, probe CSVs and ap_info.log, stored as
+    bytes under their file names (dots as "__")."""
+    case = cases.case_output_small()
+    d = {}
+    with tempfile.TemporaryDirectory() as t:
+        r = po.RefSolver(case, output_dir=t)
+        for _ in range(case.steps):
+            r.step()
+        r.close()  # completes the probe files
+        for name in sorted(os.listdir(t)):
+            with open(os.path.join(t, name), "rb") as fh:
+                d["file_" + name.replace(".", "__")] = np.frombuffer(fh.read(), dtype=np.uint8)
+    d["case"] = np.array([*case.mesh.interior, case.steps, case.dt, case.output_start, case.output_interval,
+                          case.tavg_start])
+    np.savez_compressed(os.path.join(out, "output_small.npz"), **d)
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--full-c1", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     todo = {"kernels": kernels_tiny, "alm": alm_tiny, "steps": steps_small, "wtprop": wtprop,
-            "mesh": mesh_fixture}
+            "mesh": mesh_fixture, "output": output_small}
     for k, fn in todo.items():
         if a.only and k not in a.only.split(","):
             continue

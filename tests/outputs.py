@@ -1,0 +1,115 @@
+"""Checks of the output path shared by the CPU and GPU tests: the reference's own
+output files for cases.case_output_small() (tests/golden/output_small.npz, written
+by tests/golden/make_golden.py from the reference build), an oracle run that
+writes the same files through 3dalm_amd/output.py, and a file-by-file comparison."""
+import importlib
+import os
+
+import numpy as np
+
+cases = importlib.import_module("3dalm_amd.cases")
+output = importlib.import_module("3dalm_amd.output")
+po = importlib.import_module("pyoracle")
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "output_small.npz")
+
+
+def golden_files():
+    """file name -> bytes of the reference's output folder."""
+    d = np.load(GOLDEN)
+    case = cases.case_output_small()
+    want = np.array([*case.mesh.interior, case.steps, case.dt, case.output_start, case.output_interval,
+                     case.tavg_start])
+    assert np.array_equal(d["case"], want), "output_small.npz was written for another case"
+    return {k[5:].replace("__", "."): d[k].tobytes() for k in d.files if k.startswith("file_")}
+
+
+def oracle_output(case, out_dir):
+    """The oracle's whole steps with the output path of main_loop_once
+    (src/solver.h:1193-1250) restated over its fields: q and the time average
+    from the oracle's calc_q / time average, files through 3dalm_amd/output.py."""
+    os.makedirs(out_dir, exist_ok=True)
+    K = po.Kernels("oracle")
+    o = po.OracleSolver(case)
+    m = case.mesh
+    size, gc = tuple(m.size), m.gc
+    g = po.make_grid(m)
+    q = np.zeros(m.L)
+    Ut = np.zeros((m.L, 3))
+    with open(os.path.join(out_dir, "partition.json"), "w") as fh:
+        fh.write(output.partition_json(size, gc))
+    if case.turbine is not None and case.sites:
+        output.write_ap_info(os.path.join(out_dir, "ap_info.log"), o.ap, o.cd, o.cl, o.atk)
+    probes = []
+    for n, pos in enumerate(case.probes):
+        cell = output.probe_cell(m.x, m.y, m.z, gc, pos)
+        if cell is not None:
+            probes.append(output.Probe(os.path.join(out_dir, f"probe{n}.csv"), cell,
+                                       (m.x[cell[0]], m.y[cell[1]], m.z[cell[2]])))
+    sched = output.OutputSchedule(case.dt, case.output_start, case.output_interval, case.tavg_start)
+    snaps = []
+    for step in range(case.steps + 1):
+        if step:
+            o.step()
+        t = case.dt * step
+        w = sched.tavg_weight(t)
+        if w is not None:
+            K.time_average(g, Ut, o.U, w)
+        if sched.probes_due(t):
+            for p in probes:
+                i, j, k = p.cell
+                p.add(t, o.U[(i * size[1] + j) * size[2] + k])
+        if sched.snapshot_due(t):
+            K.calc_q(g, o.U, q)
+            output.write_binary(os.path.join(out_dir, output.rank_binary_filename("inst", 0, step)), size, gc,
+                                [("U", 3, o.U), ("p", 1, o.p), ("div", 1, o.div), ("q", 1, q), ("f", 3, o.f)],
+                                m.x, m.y, m.z)
+            e = {"step": step, "time": t}
+            if t >= sched.tavg_start:
+                output.write_binary(os.path.join(out_dir, output.rank_binary_filename("tavg", 0, step)), size, gc,
+                                    [("U", 3, Ut)], m.x, m.y, m.z)
+                e["tavg"] = "yes"
+            snaps.append(e)
+    with open(os.path.join(out_dir, "snapshot.json"), "w") as fh:
+        fh.write(output.json_dump(snaps))
+    for p in probes:
+        p.flush()
+
+
+def compare_output(got_dir, expected, rtol, atol, exact_text=True):
+    """Every file of `expected` (name -> bytes) must exist in got_dir and agree:
+    text files byte for byte (probe files, with exact_text False, value by value
+    within 1e-5 relative), snapshot files with identical headers and mesh and
+    every variable within rtol (relative to the variable's largest magnitude) +
+    atol.  Returns the largest relative difference per snapshot variable."""
+    got_names = set(os.listdir(got_dir))
+    assert set(expected) <= got_names, sorted(set(expected) - got_names)
+    worst = {}
+    for name, want in sorted(expected.items()):
+        got = open(os.path.join(got_dir, name), "rb").read()
+        if name.startswith(("inst_", "tavg_")):
+            path_w = os.path.join(got_dir, "_expected_" + name)
+            with open(path_w, "wb") as fh:
+                fh.write(want)
+            a, b = output.read_binary(os.path.join(got_dir, name)), output.read_binary(path_w)
+            os.remove(path_w)
+            assert (a.size, a.gc, a.dims) == (b.size, b.gc, b.dims), name
+            assert list(a.var) == list(b.var), name
+            for c in ("x", "y", "z"):
+                assert np.array_equal(getattr(a, c), getattr(b, c)), (name, c)
+            for v in b.var:
+                s = np.max(np.abs(b.var[v]))
+                d = np.max(np.abs(a.var[v] - b.var[v]))
+                rel = d / s if s > 0 else d
+                worst[f"{name[:4]}.{v}"] = max(worst.get(f"{name[:4]}.{v}", 0.0), rel)
+                assert d <= rtol * s + atol, (name, v, d, s)
+        elif name.endswith(".csv") and not exact_text:
+            ga, wa = got.decode().splitlines(), want.decode().splitlines()
+            assert ga[:2] == wa[:2] and len(ga) == len(wa), name
+            for lg, lw in zip(ga[2:], wa[2:]):
+                vg = np.array([float(v) for v in lg.split(",")])
+                vw = np.array([float(v) for v in lw.split(",")])
+                assert np.allclose(vg, vw, rtol=1e-5, atol=1e-7), (name, lg, lw)
+        else:
+            assert got == want, name
+    return worst

@@ -48,7 +48,7 @@ def test_driver_matches_oracle(tmp_path, builder):
     cases.write_case_dir(case, str(tmp_path))
     r = run([str(tmp_path), "--perturb", str(case.noise * case.uin[0]), "--seed", str(case.seed), "--power"])
     assert r.returncode == 0, r.stderr
-    lines = [l.split() for l in r.stdout.splitlines() if l and not l.startswith("#")]
+    lines = [l.split() for l in r.stdout.splitlines() if l and l.split()[0].isdigit()]
     power = [float(l.split()[-1]) for l in r.stdout.splitlines() if l.startswith("# wt")]
     o = po.OracleSolver(case)
     assert len(lines) == case.steps
