@@ -240,7 +240,7 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP };
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP, FIN_MX_SOR };
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
 
 int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
@@ -312,3 +312,4 @@ int launch_solid_m(dalm_ctx* ctx);
 int launch_post_m(dalm_ctx* ctx, double Cs, double dt, bool store_div);
 int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, long long max_it, double tol,
                       long long* it, double* err);
+int run_sor_mixed(dalm_ctx* ctx, double omega, long long max_it, double tol, long long* it, double* err);

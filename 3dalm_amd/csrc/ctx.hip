@@ -762,7 +762,9 @@ static int step_mixed(dalm_ctx* ctx, double t, dalm_step_stats* stats) {
     TRY(launch_flux_rhs_m(ctx, P.Uin, P.dt, ctx->pc.max_diag, true));
     long long it = 0;
     double err = 0;
-    int status = run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+    int status = P.method == DALM_SOR
+                     ? run_sor_mixed(ctx, P.omega, P.max_it, P.tol, &it, &err)
+                     : run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
     if (status == DALM_HIP_ERROR) return status;
     TRY(launch_pbc(ctx));
     TRY(launch_project_m(ctx, P.dt, P.Uin));

@@ -213,6 +213,27 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         }
         break;
     }
+    case FIN_MX_SOR: {
+        // precision form of run_sor: s[0] = |b - A x|^2 in FP64 after the last
+        // refinement; aux1 holds the SOR iterations of one refinement.  At least one
+        // refinement runs, as the reference's do-while does at least one iteration.
+        if (st->done) break;
+        double err = sqrt(s[0]) / st->root_n;
+        st->err = err;
+        if (!(err == err) || isinf(err)) {
+            st->done = 1;
+            st->status = DALM_NONFINITE;
+        } else if (st->it > 0 && err <= st->tol) {
+            st->done = 1;
+            st->status = DALM_OK;
+        } else if (st->it >= st->max_it) {
+            st->done = 1;
+            st->status = DALM_NONCONVERGED;
+        } else {
+            st->it += (long long)st->aux1;
+        }
+        break;
+    }
     default:
         break;
     }

@@ -76,12 +76,14 @@ def oracle_output(case, out_dir):
         p.flush()
 
 
-def compare_output(got_dir, expected, rtol, atol, exact_text=True):
+def compare_output(got_dir, expected, rtol, atol, exact_text=True, div_atol=0.0):
     """Every file of `expected` (name -> bytes) must exist in got_dir and agree:
     text files byte for byte (probe files, with exact_text False, value by value
     within 1e-5 relative), snapshot files with identical headers and mesh and
     every variable within rtol (relative to the variable's largest magnitude) +
-    atol.  Returns the largest relative difference per snapshot variable."""
+    atol.  The divergence is what is left after the projection cancels O(1) face
+    fluxes, so its bar is absolute: div_atol + atol.  Returns the largest
+    relative difference per snapshot variable."""
     got_names = set(os.listdir(got_dir))
     assert set(expected) <= got_names, sorted(set(expected) - got_names)
     worst = {}
@@ -102,7 +104,8 @@ def compare_output(got_dir, expected, rtol, atol, exact_text=True):
                 d = np.max(np.abs(a.var[v] - b.var[v]))
                 rel = d / s if s > 0 else d
                 worst[f"{name[:4]}.{v}"] = max(worst.get(f"{name[:4]}.{v}", 0.0), rel)
-                assert d <= rtol * s + atol, (name, v, d, s)
+                bar = div_atol + atol if v == "div" else rtol * s + atol
+                assert d <= bar, (name, v, d, s)
         elif name.endswith(".csv") and not exact_text:
             ga, wa = got.decode().splitlines(), want.decode().splitlines()
             assert ga[:2] == wa[:2] and len(ga) == len(wa), name

@@ -80,6 +80,31 @@ def test_mixed_sor_pc_and_c1():
         s.close()
 
 
+@pytest.mark.parametrize("interior,tol", [((20, 12, 12), 1e-6), ((48, 24, 24), 1e-8)])
+def test_mixed_sor_method(interior, tol):
+    """Method SOR in the precision form: FP32 red-black sweeps on the correction,
+    refined on the FP64 residual every 8 iterations.  Every solve meets the
+    reference's test on the true FP64 residual, needs at most one refinement
+    (8 iterations) more than the FP64 oracle's SOR, and U stays within 1e-4 of it."""
+    c = cases.case_c1(*interior, steps=3)
+    c.tol = tol
+    print(f"[mixed SOR {interior} tol={tol}]", flush=True)
+    o = po.OracleSolver(c)
+    s = solver.Solver(c, hip.MIXED)
+    try:
+        for _ in range(3):
+            so, sm = o.step(), s.step()
+            assert sm["it"] % 8 == 0 and sm["it"] <= so["it"] + 8, (sm["it"], so["it"])
+            if so["it"] >= c.max_it:  # the FP64 solve stopped at the cap: so must this one, as close
+                assert sm["it"] == c.max_it and sm["err"] <= 1.001 * so["err"], (sm, so)
+            else:
+                assert sm["status"] == hip.OK, sm
+                assert sm["err"] <= tol and sm["avg_div"] <= 1e-6
+        m = interior_mask(c.mesh)
+        assert rel_l2(s.field("U")[m], o.field("U")[m]) <= 1e-4
+    finally:
+        s.close()
+
 
 def test_mixed_passes_and_fp64_hand_over(monkeypatch):
     """The precision form's recovery paths: with every FP32 pass cut short after
