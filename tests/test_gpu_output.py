@@ -137,9 +137,10 @@ def test_driver_output_precision_form(tmp_path):
     out = tmp_path / "output"
     for name in ("snapshot.json", "partition.json", "ap_info.log"):
         assert open(out / name, "rb").read() == gold[name], name
-    # tolerances relative to each variable's largest magnitude; the divergence is
-    # solver-tolerance noise in both forms, so only its size is checked
-    tol = {"U": 1e-5, "p": 1e-4, "q": 1e-3, "f": 1e-4}
+    # tolerances relative to each variable's largest magnitude; after a projection
+    # the divergence is solver-tolerance noise in both forms, so only its size is
+    # checked (step 0 holds the divergence of the initial field)
+    tol = {"U": 1e-5, "p": 1e-4, "q": 1e-3, "f": 1e-4, "div": 1e-5}
     for name in sorted(gold):
         if not name.startswith(("inst_", "tavg_")):
             continue
@@ -148,7 +149,7 @@ def test_driver_output_precision_form(tmp_path):
         a, b = output.read_binary(str(out / name)), output.read_binary(str(ref_path))
         assert (a.size, a.dims) == (b.size, b.dims), name
         for v in b.var:
-            if v == "div":
+            if v == "div" and not name.endswith("_0000000000"):
                 assert np.max(np.abs(a.var[v])) < 1e-6, name
                 continue
             s = np.max(np.abs(b.var[v]))
