@@ -350,11 +350,12 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     ctx->precision = (int)precision;
     int g = d.ntiles < 2048 ? d.ntiles : 2048;
     ctx->grid = g >= 64 ? (g / 8) * 8 : g;
-    // column units: about 2048 of them (more on large grids), 4 to kMaxPlanes planes each
+    // column units: about 2048 of them (more on large grids), 1 to kMaxPlanes planes
+    // each -- small grids get short columns rather than too few workgroups
     {
         long long per_plane = (long long)d.tj * d.tk;
         long long ni = (d.nx * per_plane + 2047) / 2048;
-        if (ni < 4) ni = 4;
+        if (ni < 1) ni = 1;
         if (ni > kMaxPlanes) ni = kMaxPlanes;
         if (ni > d.nx) ni = d.nx;
         d.ni = (int)ni;

@@ -9,7 +9,8 @@ run.  The Krylov solvers enqueue iterations ahead of the device's stopping test,
 and the launches past it return at once; they are counted apart ("idle": shorter
 than --idle-us) so that the mean duration is that of launches that did the work.
 With --cells, the algorithmic bytes per interior cell of each known kernel give
-its achieved bandwidth and fraction of the 8 TB/s HBM peak.
+its achieved bandwidth and fraction of the 8 TB/s HBM peak (--padded-cells for the
+kernels that sweep the whole padded box).
 
 ``traffic`` turns two ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs
 (kilobytes per dispatch) into bytes per interior cell per launch, scaled by what
@@ -39,7 +40,12 @@ ALGO_BYTES = {
     "kf_jacobi": 12, "kg_spmv<0>": 12, "kg_spmv<1>": 12, "kg_p": 20, "kg_s": 16, "kg_xr": 40,
     "kg_resid<true>": 28, "kg_resid<false>": 24, "km_predictor": 64, "km_project": 80, "km_post": 48,
     "km_flux_rhs": 44,
+    "k_calc_q<double>": 32, "k_calc_q<float>": 20,
+    "kf_sor": 10, "ks_resid": 24, "ks_add": 20,
 }
+# kernels that sweep every padded cell: algorithmic bytes per padded cell
+# (needs --padded-cells)
+ALGO_BYTES_PADDED = {"k_time_average<double>": 72, "k_time_average<float>": 60}
 
 
 def short_name(name: str) -> str:
@@ -48,7 +54,7 @@ def short_name(name: str) -> str:
     return m.group(1) if m else name[:40]
 
 
-def trace_table(path: str, cells: int | None, idle_us: float) -> str:
+def trace_table(path: str, cells: int | None, idle_us: float, padded: int | None = None) -> str:
     by = collections.defaultdict(list)
     for row in csv.DictReader(open(path)):
         us = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
@@ -60,10 +66,12 @@ def trace_table(path: str, cells: int | None, idle_us: float) -> str:
         v = np.asarray(v)
         work = v[v >= idle_us] if name in ALGO_BYTES and (v >= idle_us).any() else v
         mean = float(work.mean())
-        bpc = ALGO_BYTES.get(name)
+        bpc, n = ALGO_BYTES.get(name), cells
+        if bpc is None and padded and name in ALGO_BYTES_PADDED:
+            bpc, n = ALGO_BYTES_PADDED[name], padded
         gbs = frac = ""
-        if cells and bpc:
-            g = bpc * cells / (mean * 1e-6) / 1e9
+        if n and bpc:
+            g = bpc * n / (mean * 1e-6) / 1e9
             gbs, frac = f"{g:.0f}", f"{g / HBM_PEAK_GBS:.3f}"
         lines.append(f"| {name} | {len(v)} | {len(v) - len(work)} | {mean:.2f} | {float(np.median(work)):.2f} | "
                      f"{v.sum() / 1e3:.1f} | {100 * v.sum() / total:.1f}% | {bpc or ''} | {gbs} | {frac} |")
@@ -123,6 +131,7 @@ def main():
     t.add_argument("csv")
     t.add_argument("--cells", type=int)
     t.add_argument("--idle-us", type=float, default=6.0)
+    t.add_argument("--padded-cells", type=int, help="padded cell count, for the kernels that sweep it")
     r = sub.add_parser("traffic")
     r.add_argument("fetch_csv")
     r.add_argument("write_csv")
@@ -131,7 +140,7 @@ def main():
     r.add_argument("--json")
     a = ap.parse_args()
     if a.cmd == "trace":
-        print(trace_table(a.csv, a.cells, a.idle_us))
+        print(trace_table(a.csv, a.cells, a.idle_us, a.padded_cells))
     else:
         print(traffic_table(a.fetch_csv, a.write_csv, a.cells, a.copy_mib, a.json))
 

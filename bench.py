@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed]
                     [--grid 256,128,128] [--no-cpu] [--probe jacobi]
-                    [--adjoint-steps 20] [--adjoint-every 10] [--no-adjoint]
+                    [--adjoint-steps 20] [--adjoint-every 10] [--no-adjoint] [--no-extras]
 
 Workload (one "step" = one Solver::main_loop_once): the single-turbine ALM case on
 256x128x128 cells (box 16x8x8, y/z tanh beta=1.5, rotor D=1 at x=3, 3 blades x 64
@@ -23,6 +23,9 @@ launch over its mean launch time, timed with HIP events on the library's stream.
 cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) when
 that library is present and loads, else the C restatement in oracle/ (one core),
 on a bounded sample of this workload (see "sample").
+output_path: mean device time and achieved bandwidth of the output-path kernels
+(Q-criterion, time average of U) at the bench grid, both forms.  sor_forms: the
+64x48x48 SOR case (no turbine) in both forms.
 adjoint_forward_ratio: on the differentiation-form workload (192x96x96, three
 rotors, FP64, BiCGStab(Jacobi 3) to 1e-8), the wall time per step of the reverse
 sweep (segment replays from checkpoints plus reverse steps, adjoint solves to 1e-8
@@ -257,6 +260,64 @@ def run_adjoint(steps, every, warmup=2):
             "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]]}
 
 
+# output path kernels: id, algorithmic bytes per cell (FP64 form, precision form) and
+# whether the count is over interior cells (calc_q) or every padded cell (the average)
+OUTPUT_KERNELS = {"calc_q": (14, 32, 20, "interior"),        # read U(3); write q
+                  "time_average": (15, 72, 60, "padded")}    # read Utavg(3), U(3); write Utavg(3)
+
+
+def run_output_path(grid, reps=20):
+    """Mean device time of the output-path kernels (Q-criterion, time average of U)
+    at the bench grid, both forms, HIP events on the library's stream."""
+    import ctypes as C
+    hip = importlib.import_module("3dalm_amd.hip")
+    case = make_case(grid, 1)
+    m = case.mesh
+    out = {}
+    for form, prec in (("fp64", hip.FP64), ("mixed", hip.MIXED)):
+        c = hip.Context(m, prec)
+        try:
+            c.upload("U", case.U0())
+            c.calc_q()
+            c.time_average(1.0)
+            res = {}
+            for name, (kid, b64, b32, over) in OUTPUT_KERNELS.items():
+                c.check(c.L.dalm_probe_kernel(c.h, kid), "probe")
+                for n in range(reps):
+                    c.calc_q() if name == "calc_q" else c.time_average(1.0 / (n + 2))
+                launches, ms = C.c_int64(), C.c_double()
+                c.check(c.L.dalm_probe_read(c.h, C.byref(launches), C.byref(ms)), "probe_read")
+                c.check(c.L.dalm_probe_kernel(c.h, 0), "probe off")
+                cells = m.N if over == "interior" else m.L
+                nbytes = (b64 if form == "fp64" else b32) * cells
+                mean_s = ms.value / launches.value / 1e3
+                res[name] = {"mean_us": round(mean_s * 1e6, 2), "bytes_per_launch": nbytes,
+                             "GB/s": round(nbytes / mean_s / 1e9, 1),
+                             "frac": round(nbytes / mean_s / 1e9 / HBM_PEAK_GBS, 4), "launches": launches.value}
+            out[form] = res
+        finally:
+            c.close()
+    log(f"[bench] output path: {out}")
+    return out
+
+
+def run_sor_forms(steps=5, warmup=2):
+    """The reference's own CPU-runnable case (64x48x48, no turbine, SOR 1.2 to 1e-6,
+    at most 1000 iterations) in both forms: Mcell-steps/s and iterations per step."""
+    cases = importlib.import_module("3dalm_amd.cases")
+    hip = importlib.import_module("3dalm_amd.hip")
+    res = {}
+    for form, prec in (("fp64", hip.FP64), ("mixed", hip.MIXED)):
+        case = cases.case_c1(steps=warmup + steps)
+        r, s = run_form(case, prec, steps, warmup, None, f"sor-{form}")
+        s.close()
+        res[form] = {"mcell_steps_s": round(case.mesh.N * steps / r["seconds"] / 1e6, 3),
+                     "it_per_step": round(r["it_mean"], 1), "avg_div_max": r["avg_div_max"]}
+    res["mixed_vs_fp64"] = round(res["mixed"]["mcell_steps_s"] / res["fp64"]["mcell_steps_s"], 3)
+    res["workload"] = "64x48x48, no turbine, SOR 1.2 to 1e-6 (max 1000 it)"
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +330,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the output-path and SOR legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a kernel of known traffic for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -344,6 +406,13 @@ def main():
             adj = run_adjoint(a.adjoint_steps, a.adjoint_every)
         except Exception as e:
             log(f"[bench] adjoint leg failed: {e}")
+    output_path = sor = None
+    if not a.no_extras:
+        try:
+            output_path = run_output_path(grid)
+            sor = run_sor_forms()
+        except Exception as e:
+            log(f"[bench] output-path / SOR legs failed: {e}")
     cpu = None
     if not a.no_cpu:
         try:
@@ -382,6 +451,8 @@ def main():
         "roofline": roof,
         "roofline_fp64": roof_fp64,
         "cpu_baseline": cpu,
+        "output_path": output_path,
+        "sor_forms": sor,
     }
     print(json.dumps(line), flush=True)
 
