@@ -238,6 +238,11 @@ class Case:
     output_interval: float = 1e9
     tavg_start: float = 1e9
     probes: List[tuple] = field(default_factory=list)
+    # turbulence-generating grid (setup.json "turbulence grid", src/solver.h:598-612):
+    # {"bar": width, "mesh": spacing, "thick": x-thickness of a layer, "placement": [x, y, z]}
+    tgg: Optional[dict] = None
+    # retracted floor (setup.json "floor start", src/solver.h:620-635): slip floor for x < floor_start
+    floor_start: Optional[float] = None
 
     def U0(self) -> np.ndarray:
         return initial_velocity(self.mesh, self.uin, self.noise * abs(self.uin[0]), self.seed)
@@ -297,6 +302,16 @@ def case_output_small(steps=4) -> Case:
                 probes=[(1.7, 0.1, 1.4), (3.1, -0.6, 0.5), (9.0, 0.0, 1.0)])
 
 
+def case_tgg_small(nx=32, ny=12, nz=12, steps=3) -> Case:
+    """A grid of bars across the inflow (the reference's tgg cases, scaled): box
+    8 x 3 x 3, bars 0.18 wide every 0.6, layers 0.15 thick ending at x = 2,
+    floor retracted up to x = 1.5 (the option those cases use), SOR 1.5 to 1e-6."""
+    m = make_mesh(nx, ny, nz, ((0.0, 8.0), (-1.5, 1.5), (0.0, 3.0)), 1.2, 1.2)
+    c = Case("tgg", m, method="SOR", omega=1.5, tol=1e-6, max_it=2000, steps=steps,
+             tgg={"bar": 0.18, "mesh": 0.6, "thick": 0.15, "placement": [2.0, 0.3, 1.2]}, floor_start=1.5)
+    return c
+
+
 def turbine_records(turbine: TurbineModel, sites: List[TurbineSite]) -> np.ndarray:
     """Turbine rows (include/dalm_hip.h layout), src/alm.h:144-206: rotation centre
     (-overhang, 0, tower); the angle type is the first non-zero or object entry of
@@ -343,6 +358,11 @@ def write_case_dir(case: Case, path: str, time_steps: Optional[int] = None) -> N
              "inflow": {"value": list(case.uin)}, "cfd": {"Re": case.Re, "Cs": case.Cs}, "eq": eq}
     if case.probes:
         setup["probe"] = [list(p) for p in case.probes]
+    if case.floor_start is not None:
+        setup["floor start"] = case.floor_start
+    if case.tgg is not None:
+        setup["turbulence grid"] = {"bar": case.tgg["bar"], "mesh": case.tgg["mesh"], "thick": case.tgg["thick"],
+                                    "placement": list(case.tgg["placement"])}
     if case.turbine is not None and case.sites:
         setup["windturbine array"] = [s.to_dict() for s in case.sites]
         setup["alm"] = {"blade points": case.ap_per_blade, "projection width": case.width}

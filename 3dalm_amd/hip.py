@@ -112,6 +112,7 @@ def lib():
         "dalm_l2_norm": [V, i64, P],
         "dalm_max_cfl": [V, dbl, P],
         "dalm_calc_q": [V],
+        "dalm_set_turbulence_grid": [V, dbl, dbl, dbl, P],
         "dalm_time_average": [V, dbl],
         "dalm_sample_U": [V, PI64, i64, P],
         "dalm_build_ap_props": [P, P, P, P, P, i64, i64, dbl, dbl, i64, i64, i64, P, P, P],
@@ -279,6 +280,10 @@ class Context:
         v = dbl()
         self.check(self.L.dalm_max_cfl(self.h, dt, C.byref(v)), "max_cfl")
         return v.value
+
+    def set_turbulence_grid(self, thick, mesh, bar, placement):
+        plc = np.array(placement, dtype=np.float64)
+        self.check(self.L.dalm_set_turbulence_grid(self.h, thick, mesh, bar, ptr(plc)), "set_turbulence_grid")
 
     # ---- output path
     def calc_q(self):

@@ -7,6 +7,9 @@
 // time loop on the GPU and prints the reference's per-step line
 // "step time it err avg_div max_cfl" (src/solver.h:1188-1190).
 //
+// A "turbulence grid" sets the solid fraction of its bars (src/solver.h:52-159,
+// 740-764); set_solid_U then runs every step.
+//
 // Output (src/solver.h:569-598, 656-700, 950-999, 1193-1250, host/output.h): the
 // folder <case_dir>/output (-c clears it first), partition.json, ap_info.log,
 // probe<m>.csv, inst_/tavg_ snapshot files and snapshot.json on setup.json's
@@ -15,8 +18,7 @@
 // Beyond the reference (needed by the measured workloads): --steps N replaces
 // time/dt, --perturb A --seed S adds the seeded uniform perturbation of the
 // interior velocity, --precision mixed selects the FP32 + FP64-refined form, and
-// --power prints per-turbine power after each step line.  The turbulence-generating
-// grid is not implemented; a case that asks for it is refused.
+// --power prints per-turbine power after each step line.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -157,9 +159,18 @@ int main(int argc, char** argv) {
         Args a = parse_args(argc, argv);
         jl::Value setup = jl::parse(slurp(a.dir + "/setup.json"));
         jl::Value mesh = jl::parse(slurp(a.dir + "/mesh.json"));
-        if (setup.has("turbulence grid"))
-            throw std::runtime_error("setup.json asks for a turbulence-generating grid, which this build "
-                                     "does not implement");
+        // turbulence-generating grid parameters (src/solver.h:598-612)
+        const bool tgg = setup.has("turbulence grid");
+        double tgg_thick = 0, tgg_mesh = 0, tgg_bar = 0, tgg_plc[3] = {0, 0, 0};
+        if (tgg) {
+            const jl::Value& t = setup["turbulence grid"];
+            tgg_thick = t["thick"].d();
+            tgg_mesh = t["mesh"].d();
+            tgg_bar = t["bar"].d();
+            for (int m = 0; m < 3; ++m) tgg_plc[m] = t["placement"][m].d();
+            if (!(tgg_mesh > 0) || !(tgg_thick >= 0) || !(tgg_bar >= 0))
+                throw std::runtime_error("turbulence grid: need mesh > 0, thick >= 0, bar >= 0");
+        }
 
         // mesh (src/io.h:105-151)
         int64_t gc = mesh["guide cell"].i();
@@ -214,6 +225,16 @@ int main(int argc, char** argv) {
             out.tavg_start = o["time avg start time"].d();
         }
         dalm_out::write_text((out.dir / "partition.json").string(), dalm_out::partition_json(size, gc));
+        // turbulence-generating grid (src/solver.h:598-612, 740-764): the solid fraction,
+        // written once as solid_00000_0000000000
+        if (tgg) {
+            check(dalm_set_turbulence_grid(ctx, tgg_thick, tgg_mesh, tgg_bar, tgg_plc), ctx,
+                  "dalm_set_turbulence_grid");
+            std::vector<double> solid(L);
+            check(dalm_download(ctx, DALM_F_SOLID, solid.data()), ctx, "download solid");
+            dalm_out::write_binary((out.dir / dalm_out::rank_binary_filename("solid", 0, 0)).string(), size, gc,
+                                   {{"solid", 1, &solid}}, x, y, z);
+        }
         // probes at the nearest cell centre among the interior and the first ghost
         // layer, kept only when that cell is interior (src/solver.h:656-700)
         if (setup.has("probe")) {

@@ -22,7 +22,8 @@ def params_for(case: Case) -> hip.Params:
     p.pc_kind = hip.PC_SOR if case.pc == "SOR" else hip.PC_JACOBI
     p.omega, p.pc_omega, p.pc_it = case.omega, case.pc_omega, case.pc_it
     p.max_it, p.tol = case.max_it, case.tol
-    p.retracted_floor, p.floor_start_x = 0, 0.0
+    p.retracted_floor = 0 if case.floor_start is None else 1
+    p.floor_start_x = 0.0 if case.floor_start is None else case.floor_start
     return p
 
 
@@ -44,6 +45,9 @@ class Solver:
         self.ctx.upload("U", U)
         self.ctx.upload("Uold", Uold)
         self.ctx.set_params(params_for(case))
+        if case.tgg is not None:  # src/solver.h:740-750
+            t = case.tgg
+            self.ctx.set_turbulence_grid(t["thick"], t["mesh"], t["bar"], t["placement"])
         self.ap = None
         if case.turbine is not None and case.sites:
             nwt = len(case.sites)

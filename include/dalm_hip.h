@@ -171,6 +171,11 @@ int dalm_apply_pbc(dalm_ctx *ctx);
 int dalm_project_p(dalm_ctx *ctx, double dt);
 /* set_solid_U (src/cfd.h:1143-1160) */
 int dalm_set_solid_U(dalm_ctx *ctx);
+/* set_turbulence_generating_grid (src/solver.h:52-159): the solid fraction of a
+ * grid of bars (setup.json "turbulence grid": thick, mesh, bar, placement) into the
+ * interior of DALM_F_SOLID, either form; set_solid_U then runs every step. */
+int dalm_set_turbulence_grid(dalm_ctx *ctx, double thick, double mesh, double bar,
+                             const double placement[3]);
 /* apply_Ubc (src/bc.h:5-233) */
 int dalm_apply_Ubc(dalm_ctx *ctx, const double Uin[3], double dt, int64_t retracted_floor,
                    double floor_start_x);

@@ -451,6 +451,29 @@ void or_nut(const or_grid *g, const double *U, double *nut, double Cs) {
     }
 }
 
+/* Solid fraction of the turbulence-generating grid: src/solver.h:52-159, with
+ * get_intersection (src/util.h:84-90) as the overlap of two intervals. */
+static double overlap1(double h1, double t1, double h2, double t2) {
+    double l1 = t1 - h1, l2 = t2 - h2, x1 = t1 - h2, x2 = t2 - h1;
+    return fmax(0., fmin(l1, fmin(l2, fmin(x1, x2))));
+}
+void or_tgg_solid(const or_grid *g, double *solid, double thick, double mesh, double bar, const double plc[3]) {
+    const double *X = g->c[0], *Y = g->c[1], *Z = g->c[2];
+    const double *DX = g->d[0], *DY = g->d[1], *DZ = g->d[2];
+    INTERIOR(g, i, j, k) {
+        double xc = X[i], yc = Y[j], zc = Z[k], hx = DX[i], hy = DY[j], hz = DZ[k];
+        double xv = overlap1(xc - 0.5 * hx, xc + 0.5 * hx, plc[0] - thick, plc[0]);
+        i64 nj = (i64)round((yc - plc[1]) / mesh);
+        double by = nj * mesh + plc[1];
+        double yi = overlap1(yc - 0.5 * hy, yc + 0.5 * hy, by - 0.5 * bar, by + 0.5 * bar);
+        double xh = overlap1(xc - 0.5 * hx, xc + 0.5 * hx, plc[0] - 2 * thick, plc[0] - thick);
+        i64 nk = (i64)round((zc - plc[2]) / mesh);
+        double bz = nk * mesh + plc[2];
+        double zi = overlap1(zc - 0.5 * hz, zc + 0.5 * hz, bz - 0.5 * bar, bz + 0.5 * bar);
+        solid[lin(g, i, j, k)] = (xv * yi * hz + xh * zi * hy) / (hx * hy * hz);
+    }
+}
+
 /* Q-criterion: src/cfd.h:948-976 (calc_q_at_cell), over the interior as calc_q
  * does on one rank (src/cfd.h:978-1077); other cells are left as they are. */
 void or_calc_q(const or_grid *g, const double *U, double *q) {

@@ -77,6 +77,8 @@ double or_max_cfl(const or_grid *g, const double *U, double dt);
 
 /* ---- output path --------------------------------------------------------- */
 void or_calc_q(const or_grid *g, const double *U, double *q);
+/* turbulence-generating grid: solid fraction over the interior (plc = placement) */
+void or_tgg_solid(const or_grid *g, double *solid, double thick, double mesh, double bar, const double plc[3]);
 void or_time_average(const or_grid *g, double *Utavg, const double *U, double weight);
 
 /* ---- actuator-line model ------------------------------------------------- */

@@ -86,6 +86,7 @@ def lib(kind="oracle"):
             "divergence": (None, [G, P, P]),
             "max_cfl": (dbl, [G, P, dbl]),
             "calc_q": (None, [G, P, P]),
+            "tgg_solid": (None, [G, P, dbl, dbl, dbl, P]),
             "time_average": (None, [G, P, P, dbl]),
             "update_ap_position": (None, [P, i64, P, i64, i64, i64, dbl]),
             "calc_ap_force": (None, [G, P, P, i64, P, i64, i64, i64, P, P, P, i64]),
@@ -129,6 +130,8 @@ def lib(kind="oracle"):
                                             dbl, C.c_char_p, C.c_char_p, i64, dbl]
             L.ref_solver_perturb.argtypes = [C.c_void_p, P]
             L.ref_solver_output.argtypes = [C.c_void_p, C.c_char_p, dbl, dbl, dbl, P, i64]
+            L.ref_solver_tgg.argtypes = [C.c_void_p, dbl, dbl, dbl, P]
+            L.ref_solver_floor.argtypes = [C.c_void_p, dbl]
             L.ref_solver_initialize.argtypes = [C.c_void_p]
             L.ref_solver_step.argtypes = [C.c_void_p]
             L.ref_solver_get.restype = C.c_int
@@ -231,6 +234,8 @@ class OracleSolver:
         s.method = 1 if case.method == "BiCG" else 0
         s.pc_kind = 1 if case.pc == "SOR" else 0
         s.omega, s.pc_omega, s.pc_it, s.max_it, s.tol = case.omega, case.pc_omega, case.pc_it, case.max_it, case.tol
+        if case.floor_start is not None:
+            s.retracted_floor, s.floor_start_x = 1, case.floor_start
         if case.turbine is not None and case.sites:
             self.ap, self.cd, self.cl, self.atk = ap_props_oracle(case.turbine, len(case.sites), case.ap_per_blade)
             for a in range(len(self.ap)):
@@ -245,6 +250,10 @@ class OracleSolver:
                 ap_hook(self.ap, self.wt)
         self.s = s
         self.L, _ = lib("oracle")
+        if case.tgg is not None:
+            plc = np.array(case.tgg["placement"], dtype=np.float64)
+            self.L.or_tgg_solid(C.byref(self.g), ptr(self.solid), case.tgg["thick"], case.tgg["mesh"],
+                                case.tgg["bar"], ptr(plc))
         self.L.or_state_initialize(C.byref(self.g), C.byref(self.s))
 
     def step(self):
@@ -303,6 +312,11 @@ class RefSolver:
         self.nwt = len(case.sites) if wt_prop else 0
         U0 = np.ascontiguousarray(case.U0())
         L.ref_solver_perturb(self.h, ptr(U0))
+        if case.floor_start is not None:
+            L.ref_solver_floor(self.h, case.floor_start)
+        if case.tgg is not None:
+            plc = np.array(case.tgg["placement"], dtype=np.float64)
+            L.ref_solver_tgg(self.h, case.tgg["thick"], case.tgg["mesh"], case.tgg["bar"], ptr(plc))
         if output_dir is not None:
             pr = np.ascontiguousarray(np.array(case.probes, dtype=np.float64).reshape(-1, 3))
             L.ref_solver_output(self.h, str(output_dir).encode(), case.output_start, case.output_interval,
@@ -326,7 +340,7 @@ class RefSolver:
 
     def field(self, name):
         n = self.case.mesh.L
-        w = 3 if name in ("U", "Uold", "JU", "f", "Utavg") else 1
+        w = 3 if name in ("U", "Uold", "JU", "f", "Utavg") else 1  # p nut div b q solid: scalars
         out = np.zeros(n * w)
         assert self.L.ref_solver_get(self.h, name.encode(), ptr(out)) == 0
         return out.reshape(n, 3) if w == 3 else out

@@ -256,6 +256,11 @@ void ref_divergence(const or_grid *g, const double *JU, double *div) {
     MpiInfo m = one_rank();
     calc_divergence(v3(JU), div, a.dx, a.dy, a.dz, a.size, a.gc, &m);
 }
+void ref_tgg_solid(const or_grid *g, double *solid, double thick, double mesh, double bar, const double plc[3]) {
+    G a(g);
+    set_turbulence_generating_grid(solid, a.x, a.y, a.z, a.dx, a.dy, a.dz, thick, mesh, bar, plc[0], plc[1],
+                                   plc[2], a.size, a.gc);
+}
 void ref_calc_q(const or_grid *g, const double *U, double *q) {
     G a(g);
     MpiInfo m = one_rank();
@@ -461,6 +466,20 @@ void ref_solver_output(void *h, const char *dir, double start, double interval, 
     }
 }
 
+// The retracted floor of Solver::initialize (src/solver.h:620-635).
+void ref_solver_floor(void *h, double floor_start_x) {
+    Solver &S = static_cast<RefSolver *>(h)->s;
+    S.cfd.retracted_floor = true;
+    S.cfd.floor_start_x = floor_start_x;
+}
+
+// The turbulence-generating grid of Solver::initialize (src/solver.h:740-750).
+void ref_solver_tgg(void *h, double thick, double mesh, double bar, const double *plc) {
+    Solver &S = static_cast<RefSolver *>(h)->s;
+    set_turbulence_generating_grid(S.cfd.solid, S.mesh.x, S.mesh.y, S.mesh.z, S.mesh.dx, S.mesh.dy, S.mesh.dz,
+                                   thick, mesh, bar, plc[0], plc[1], plc[2], S.size, S.gc);
+}
+
 // Overwrite the interior of U with U0 (an [L][3] array); ghosts keep Uin.
 void ref_solver_perturb(void *h, const double *U0) {
     Solver &S = static_cast<RefSolver *>(h)->s;
@@ -514,6 +533,7 @@ int ref_solver_get(void *h, const char *name, double *out) {
     else if (n == "div") src = S.cfd.div;
     else if (n == "b") src = S.eq.b;
     else if (n == "q") src = S.cfd.q;
+    else if (n == "solid") src = S.cfd.solid;
     else if (n == "Utavg") { src = S.cfd.Utavg[0]; w = 3; }
     else return -1;
     std::memcpy(out, src, sizeof(double) * len * w);

@@ -13,20 +13,21 @@ import pytest
 cases = importlib.import_module("3dalm_amd.cases")
 build = importlib.import_module("3dalm_amd.build")
 po = importlib.import_module("pyoracle")
+output = importlib.import_module("3dalm_amd.output")
 
 
 def run(args, timeout=300):
     return subprocess.run([build.DRIVER, *args], capture_output=True, text=True, timeout=timeout)
 
 
-def test_refuses_turbulence_grid(tmp_path):
-    case = cases.case_c1(8, 6, 6, steps=1)
+def test_bad_turbulence_grid(tmp_path):
+    """A grid spacing of zero is refused while the case is read, before any
+    device call."""
+    case = cases.case_tgg_small(steps=1)
+    case.tgg = dict(case.tgg, mesh=0.0)
     cases.write_case_dir(case, str(tmp_path))
-    setup = json.load(open(tmp_path / "setup.json"))
-    setup["turbulence grid"] = {"bar": 0.1, "mesh": 0.5, "thick": 0.05, "placement": [0, 0, 0]}
-    json.dump(setup, open(tmp_path / "setup.json", "w"))
     r = run([str(tmp_path)])
-    assert r.returncode == 1 and "turbulence-generating grid" in r.stderr
+    assert r.returncode == 1 and "turbulence grid: need mesh > 0" in r.stderr, r.stderr
 
 
 def test_missing_case_file(tmp_path):
@@ -41,7 +42,8 @@ def test_usage():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("builder", [lambda: cases.case_c1(12, 10, 10, steps=4),
-                                     lambda: cases.case_c2(24, 12, 12, steps=3, ap_per_blade=6)])
+                                     lambda: cases.case_c2(24, 12, 12, steps=3, ap_per_blade=6),
+                                     lambda: cases.case_tgg_small(steps=3)])
 def test_driver_matches_oracle(tmp_path, builder):
     case = builder()
     print(f"[driver case {case.name} {case.mesh.interior}]", flush=True)
@@ -69,6 +71,9 @@ def test_driver_matches_oracle(tmp_path, builder):
         for q, key, tol in checks:
             ref = s["step"] * case.dt if key is None else s[key]
             assert abs(float(l[q]) - ref) <= tol * abs(ref) + 1e-300, (n, key, l[q], ref)
+        if case.tgg is not None and n == 0:  # the solid fraction written at start-up
+            sol = output.read_binary(str(tmp_path / "output" / "solid_00000_0000000000"))
+            assert list(sol.var) == ["solid"] and np.array_equal(sol.var["solid"], o.field("solid"))
         if power:
             rp = 1e-10 if case.method == "SOR" else 1e-7
             assert abs(power[n] - case.sites[0].rot_speed * s["torque"][0]) <= rp * abs(power[n])
