@@ -310,6 +310,17 @@ int dalm_set_device(int64_t device) {
     return hipSetDevice((int)device) == hipSuccess ? DALM_OK : DALM_HIP_ERROR;
 }
 
+int dalm_device_count(int64_t* n) {
+    if (!n) return DALM_BAD_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        *n = 0;
+        return DALM_HIP_ERROR;
+    }
+    *n = c;
+    return DALM_OK;
+}
+
 int dalm_mesh_widths(const double* c, int64_t n, double* d) {
     if (!c || !d || n < 3) return DALM_BAD_ARG;
     for (int64_t i = 1; i < n - 1; ++i) d[i] = 0.5 * (c[i + 1] - c[i - 1]);

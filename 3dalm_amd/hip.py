@@ -90,6 +90,7 @@ def lib():
         "dalm_create": [C.POINTER(V), PI64, i64, P, P, P, i64],
         "dalm_destroy": [V],
         "dalm_set_device": [i64],
+        "dalm_device_count": [PI64],
         "dalm_mesh_widths": [P, i64, P],
         "dalm_upload": [V, i64, P],
         "dalm_download": [V, i64, P],
@@ -280,6 +281,13 @@ class Context:
         v = dbl()
         self.check(self.L.dalm_max_cfl(self.h, dt, C.byref(v)), "max_cfl")
         return v.value
+
+    @staticmethod
+    def device_count():
+        n = C.c_int64()
+        if lib().dalm_device_count(C.byref(n)) != OK:
+            raise HipError("dalm_device_count failed")
+        return n.value
 
     def set_turbulence_grid(self, thick, mesh, bar, placement):
         plc = np.array(placement, dtype=np.float64)

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <filesystem>
 #include <fstream>
 #include <sstream>
@@ -152,9 +153,24 @@ struct Output {
     }
 };
 
+const char* angle_name(double type) {
+    switch ((int)type) {
+    case 0: return "roll";
+    case 1: return "pitch";
+    case 2: return "yaw";
+    default: return "none";
+    }
+}
+
+void print_time(const char* tag) {
+    std::time_t t = std::time(nullptr);
+    std::printf("%s %s\n", tag, std::ctime(&t));  // ctime's text ends in a newline, as in src/solver.cpp:8-9
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+    print_time("START");
     try {
         Args a = parse_args(argc, argv);
         jl::Value setup = jl::parse(slurp(a.dir + "/setup.json"));
@@ -237,6 +253,11 @@ int main(int argc, char** argv) {
         }
         // probes at the nearest cell centre among the interior and the first ghost
         // layer, kept only when that cell is interior (src/solver.h:656-700)
+        struct ProbeInfo {
+            double x, y, z;
+            long long i, j, k;
+        };
+        std::vector<ProbeInfo> probe_info;
         if (setup.has("probe")) {
             const jl::Value& pr = setup["probe"];
             for (size_t m = 0; m < pr.size(); ++m) {
@@ -253,6 +274,7 @@ int main(int argc, char** argv) {
                                         (idx[0] * size[1] + idx[1]) * size[2] + idx[2], x[idx[0]], y[idx[1]],
                                         z[idx[2]]);
                 out.cells.push_back(out.probes.back().cell);
+                probe_info.push_back({x[idx[0]], y[idx[1]], z[idx[2]], idx[0], idx[1], idx[2]});
             }
         }
 
@@ -274,7 +296,8 @@ int main(int argc, char** argv) {
         check(dalm_upload(ctx, DALM_F_UOLD, Uold.data()), ctx, "upload Uold");
 
         // turbines (src/solver.h:702-708, src/alm.h:35-206)
-        int64_t wt_count = 0;
+        int64_t wt_count = 0, n_ap_all = 0, per_blade_all = 0, blades_all = 0;
+        double width_all = 0;
         std::vector<double> wt;
         if (setup.has("windturbine array")) {
             const jl::Value& arr = setup["windturbine array"];
@@ -334,11 +357,73 @@ int main(int argc, char** argv) {
                   ctx, "dalm_alm_setup");
             if (n_ap > 0)
                 dalm_out::write_ap_info((out.dir / "ap_info.log").string(), ap, DALM_AP_STRIDE, n_ap, cd, cl, atk);
+            n_ap_all = n_ap;
+            per_blade_all = per_blade;
+            blades_all = blades;
+            width_all = width;
         }
 
         dalm_step_stats st;
         check(dalm_initialize(ctx, &st), ctx, "dalm_initialize");
-        std::printf("# initial div(U) = %e, max cfl = %e\n", st.avg_div, st.max_cfl);
+
+        // the start-up report of Solver::initialize (src/solver.h:855-1010), one rank
+        double max_diag = 0;
+        check(dalm_build_A(ctx, &max_diag), ctx, "dalm_build_A");
+        int64_t ngpu = 0;
+        check(dalm_device_count(&ngpu), ctx, "dalm_device_count");
+        const fs::path case_dir(a.dir);
+        std::printf("SETUP INFO\n\tcase directory = %s\n", case_dir.c_str());
+        std::printf("DEVICE INFO\n\tnumber of GPUs = %d\n", (int)ngpu);
+        std::printf("CHARACTERISTIC SCALE INFO\n\tU = %lf\n\tL = %lf\n\tt = %lf\n\tp = %lf\n", 1.0, 1.0, 1.0, 1.0);
+        std::printf("MESH INFO\n\tpath = %s\n\tglobal size = (%ld %ld %ld)\n\tguide cell = %ld\n",
+                    (case_dir / "mesh.json").c_str(), (long)size[0], (long)size[1], (long)size[2], (long)gc);
+        std::printf("RUNTIME INFO\n\tdt = %lf\n\tmax step = %ld\n", P.dt, (long)max_step);
+        std::printf("OUTPUT INFO\n\tdirectory = %s\n\toutput start time = %lf\n\toutput interval time = %lf\n"
+                    "\ttime avg start time = %lf\n\toutput vars = U p div q f\n\ttime avg vars = U\n",
+                    out.dir.c_str(), out.start, out.interval, out.tavg_start);
+        std::printf("TURBULENCE GENERATING GRID INFO\n");
+        if (tgg)
+            std::printf("\tthickness = %lf\n\tspacing = %lf\n\tplacement = (%lf %lf %lf)\n", tgg_thick, tgg_mesh,
+                        tgg_plc[0], tgg_plc[1], tgg_plc[2]);
+        else
+            std::printf("\tno turbulence generating grid\n");
+        std::printf("CFD INFO\n\tRe = %lf\n\tCs = %lf\n\tinitial div(U) = %e\n\tinitial max cfl = %e\n", P.Re, P.Cs,
+                    st.avg_div, st.max_cfl);
+        if (P.retracted_floor)
+            std::printf("\tretracted floor boundary applied\n\tfloor boundary start from x = %lf\n",
+                        P.floor_start_x);
+        std::printf("EQ INFO\n\tmax iteration = %ld\n\ttolerance = %lf\n\tmax A diag = %lf\n\tmethod = %s\n",
+                    (long)P.max_it, P.tol, max_diag, method.c_str());
+        if (P.method == DALM_SOR) {
+            std::printf("\trelaxation rate = %lf\n", P.omega);
+        } else {
+            std::printf("PRECONDITIONER INFO\n\tmethod = %s\n\tmax iteration = %ld\n",
+                        P.pc_kind == DALM_PC_SOR ? "SOR" : "Jacobi", (long)P.pc_it);
+            if (P.pc_kind == DALM_PC_SOR) std::printf("\trelaxation rate = %lf\n", P.pc_omega);
+        }
+        if (n_ap_all > 0) {
+            std::printf("ALM INFO\n\tpoints per blade = %ld\n\tprojection width = %lf\n", (long)per_blade_all,
+                        width_all);
+            std::printf("WINDTURBINE INFO\n");
+            for (int64_t w = 0; w < wt_count; ++w) {
+                const double* T = wt.data() + DALM_WT_STRIDE * w;
+                std::printf("\twt %ld\n\t\tblades = %ld\n\t\tbase = (%lf %lf %lf)\n\t\trotation speed = %lf\n"
+                            "\t\trotation center = (%lf %lf %lf)\n\t\tangle type = %s\n"
+                            "\t\tangle formula = %lf sin(%lf t + %lf) + %lf\n",
+                            (long)w, (long)blades_all, T[0], T[1], T[2], T[13], T[3], T[4], T[5], angle_name(T[6]),
+                            T[9], T[10], T[11], T[12]);
+            }
+            std::printf("ACTUATOR POINT INFO\n\toutput to %s\n", (out.dir / "ap_info.log").c_str());
+        } else {
+            std::printf("WINDTURBINE INFO\n\tno windturbine\n");
+        }
+        std::printf("PROC INFO 0/1\n\tgsize = (%ld %ld %ld)\n\tsize = (%ld %ld %ld)\n\toffset = (0 0 0)\n"
+                    "\tGPU id = 0\n\tprobes = \n",
+                    (long)size[0], (long)size[1], (long)size[2], (long)size[0], (long)size[1], (long)size[2]);
+        for (const ProbeInfo& q : probe_info)
+            std::printf("\t\tposition = (%lf %lf %lf), local index = (%lld %lld %lld)\n", q.x, q.y, q.z, q.i, q.j,
+                        q.k);
+        std::fflush(stdout);
         out.after_step(0);
         for (long long s = 1; s <= max_step; ++s) {
             int rc = dalm_step(ctx, &st);
@@ -359,6 +444,7 @@ int main(int argc, char** argv) {
         }
         out.finish();
         dalm_destroy(ctx);
+        print_time("END");
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "dalm_solver: %s\n", e.what());

@@ -129,6 +129,9 @@ int dalm_destroy(dalm_ctx *ctx);
 const char *dalm_last_error(const dalm_ctx *ctx);
 /* Device selection for subsequent dalm_create calls (src/solver.h:560-562). */
 int dalm_set_device(int64_t device);
+/* Number of visible GPUs: acc_get_num_devices(acc_device_nvidia)
+ * (src/solver.h:560-562), printed in the start-up report. */
+int dalm_device_count(int64_t *n);
 
 /* Mesh width rule, dx[i] = (x[i+1]-x[i-1])/2 with one-sided 2nd-order ends
  * (build_mesh_from_json, src/io.h:115-125).  Host only. */

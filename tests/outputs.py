@@ -116,3 +116,75 @@ def compare_output(got_dir, expected, rtol, atol, exact_text=True, div_atol=0.0)
         else:
             assert got == want, name
     return worst
+
+
+def expected_report(case, case_dir, max_diag, ngpu):
+    """The start-up report that Solver::initialize prints on one rank
+    (src/solver.h:855-1010) for a case written by cases.write_case_dir, as lines.
+    The two %e lines (initial div(U), initial max cfl) end in "<value>"."""
+    m = case.mesh
+    n = tuple(m.size)
+    out_dir = os.path.join(case_dir, "output")
+    max_step = int(case.dt * (case.steps + 0.5) / case.dt)
+    L = ["SETUP INFO", f"\tcase directory = {case_dir}", "DEVICE INFO", f"\tnumber of GPUs = {ngpu}",
+         "CHARACTERISTIC SCALE INFO", "\tU = 1.000000", "\tL = 1.000000", "\tt = 1.000000", "\tp = 1.000000",
+         "MESH INFO", f"\tpath = {os.path.join(case_dir, 'mesh.json')}", "\tglobal size = (%d %d %d)" % n,
+         f"\tguide cell = {m.gc}", "RUNTIME INFO", "\tdt = %f" % case.dt, f"\tmax step = {max_step}",
+         "OUTPUT INFO", f"\tdirectory = {out_dir}", "\toutput start time = %f" % case.output_start,
+         "\toutput interval time = %f" % case.output_interval, "\ttime avg start time = %f" % case.tavg_start,
+         "\toutput vars = U p div q f", "\ttime avg vars = U", "TURBULENCE GENERATING GRID INFO"]
+    if case.tgg is not None:
+        t = case.tgg
+        L += ["\tthickness = %f" % t["thick"], "\tspacing = %f" % t["mesh"],
+              "\tplacement = (%f %f %f)" % tuple(t["placement"])]
+    else:
+        L += ["\tno turbulence generating grid"]
+    L += ["CFD INFO", "\tRe = %f" % case.Re, "\tCs = %f" % case.Cs, "\tinitial div(U) = <value>",
+          "\tinitial max cfl = <value>"]
+    if case.floor_start is not None:
+        L += ["\tretracted floor boundary applied", "\tfloor boundary start from x = %f" % case.floor_start]
+    L += ["EQ INFO", f"\tmax iteration = {case.max_it}", "\ttolerance = %f" % case.tol, "\tmax A diag = %f" % max_diag,
+          f"\tmethod = {'BiCG' if case.method == 'BiCG' else 'SOR'}"]
+    if case.method == "BiCG":
+        L += ["PRECONDITIONER INFO", f"\tmethod = {case.pc}", f"\tmax iteration = {case.pc_it}"]
+        if case.pc == "SOR":
+            L += ["\trelaxation rate = %f" % case.pc_omega]
+    else:
+        L += ["\trelaxation rate = %f" % case.omega]
+    if case.turbine is not None and case.sites:
+        L += ["ALM INFO", f"\tpoints per blade = {case.ap_per_blade}", "\tprojection width = %f" % case.width,
+              "WINDTURBINE INFO"]
+        wt = cases.turbine_records(case.turbine, case.sites)
+        names = {0: "roll", 1: "pitch", 2: "yaw"}
+        for w, row in enumerate(wt):
+            L += [f"\twt {w}", f"\t\tblades = {case.turbine.blades}", "\t\tbase = (%f %f %f)" % tuple(row[0:3]),
+                  "\t\trotation speed = %f" % row[13], "\t\trotation center = (%f %f %f)" % tuple(row[3:6]),
+                  f"\t\tangle type = {names.get(int(row[6]), 'none')}",
+                  "\t\tangle formula = %f sin(%f t + %f) + %f" % tuple(row[9:13])]
+        L += ["ACTUATOR POINT INFO", f"\toutput to {os.path.join(out_dir, 'ap_info.log')}"]
+    else:
+        L += ["WINDTURBINE INFO", "\tno windturbine"]
+    L += ["PROC INFO 0/1", "\tgsize = (%d %d %d)" % n, "\tsize = (%d %d %d)" % n, "\toffset = (0 0 0)",
+          "\tGPU id = 0", "\tprobes = "]
+    for pos in case.probes:
+        c = output.probe_cell(m.x, m.y, m.z, m.gc, pos)
+        if c is not None:
+            L.append("\t\tposition = (%f %f %f), local index = (%d %d %d)" % (m.x[c[0]], m.y[c[1]], m.z[c[2]], *c))
+    return L
+
+
+def check_report(stdout, want):
+    """stdout's start-up report equals `want` line for line (values of the
+    "<value>" lines: any %e number)."""
+    lines = stdout.splitlines()
+    first = lines.index("SETUP INFO")
+    got = lines[first:first + len(want)]
+    for g, w in zip(got, want):
+        if w.endswith("<value>"):
+            head = w[:-len("<value>")]
+            assert g.startswith(head), (g, w)
+            float(g[len(head):])
+        else:
+            assert g == w, (g, w)
+    assert len(got) == len(want)
+    assert lines[0].startswith("START ") and lines[-2].startswith("END "), (lines[0], lines[-2:])

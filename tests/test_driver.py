@@ -13,6 +13,8 @@ import pytest
 cases = importlib.import_module("3dalm_amd.cases")
 build = importlib.import_module("3dalm_amd.build")
 po = importlib.import_module("pyoracle")
+hip = importlib.import_module("3dalm_amd.hip")
+from outputs import check_report, expected_report  # noqa: E402
 output = importlib.import_module("3dalm_amd.output")
 
 
@@ -53,6 +55,7 @@ def test_driver_matches_oracle(tmp_path, builder):
     lines = [l.split() for l in r.stdout.splitlines() if l and l.split()[0].isdigit()]
     power = [float(l.split()[-1]) for l in r.stdout.splitlines() if l.startswith("# wt")]
     o = po.OracleSolver(case)
+    check_report(r.stdout, expected_report(case, str(tmp_path), o.stats()["max_diag"], hip.Context.device_count()))
     assert len(lines) == case.steps
     for n, l in enumerate(lines):
         s = o.step()

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from helpers import interior_mask, rand_field, small_mesh
-from outputs import compare_output, golden_files
+from outputs import check_report, compare_output, expected_report, golden_files
 
 hip = importlib.import_module("3dalm_amd.hip")
 cases = importlib.import_module("3dalm_amd.cases")
@@ -102,6 +102,10 @@ def test_driver_output_matches_reference(tmp_path):
     cases.write_case_dir(case, str(tmp_path))
     r = _run_driver(str(tmp_path), case)
     assert f"create {tmp_path}/output" in r.stdout
+    # the start-up report of Solver::initialize, line for line
+    o = po.OracleSolver(case)
+    ngpu = hip.Context.device_count()
+    check_report(r.stdout, expected_report(case, str(tmp_path), o.stats()["max_diag"], ngpu))
     worst = compare_output(str(tmp_path / "output"), golden_files(), rtol=1e-11, atol=1e-300, exact_text=False,
                            div_atol=1e-13)
     print(worst)
