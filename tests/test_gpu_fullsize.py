@@ -1,0 +1,69 @@
+"""The configurations at their full sizes, through properties that hold at any
+size (the CPU oracle would take minutes per step here):
+
+  * every FP64 step ends on the reference's stopping test (err <= tol, or the
+    iteration cap) and satisfies the projection identity
+    avg_div = dt * max_diag * err (the projected face fluxes are the operator's);
+  * the precision form ends every step on the same test, with avg div <= 1e-6
+    (where the FP64 solve stops at the iteration cap, it must too, at most twice
+    as far from the tolerance), turbine power within 0.1 % of the FP64 form and U
+    within 1e-4 relative L2 of it.
+
+256x128x128 with one rotor (the measured workload), 192x96x96 with three rotors
+(the differentiation-form geometry) and 512x256x256 with one rotor (one step)."""
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import interior_mask, rel_l2
+
+hip = importlib.import_module("3dalm_amd.hip")
+cases = importlib.import_module("3dalm_amd.cases")
+solver = importlib.import_module("3dalm_amd.solver")
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = {
+    "256x128x128": (lambda: cases.case_c2(256, 128, 128, steps=3), 3),
+    "192x96x96": (lambda: cases.case_c4(192, 96, 96, steps=2), 2),
+    "512x256x256": (lambda: cases.case_c2(512, 256, 256, steps=1), 1),
+}
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_full_size_properties(name):
+    builder, steps = CONFIGS[name]
+    case = builder()
+    print(f"[full size {name} steps={steps}]", flush=True)
+    s64 = solver.Solver(case)
+    try:
+        md = s64.ctx.build_A()
+        st64 = []
+        for _ in range(steps):
+            st = s64.step()
+            print(f"  fp64 step {st['step']} it={st['it']} err={st['err']:.3e} div={st['avg_div']:.3e}", flush=True)
+            assert st["status"] == hip.OK or st["it"] >= case.max_it, st
+            if st["status"] == hip.OK:
+                assert st["err"] <= case.tol
+            assert abs(st["avg_div"] - case.dt * md * st["err"]) <= 1e-5 * case.dt * md * st["err"], (st, md)
+            st64.append(st)
+        U64 = s64.field("U")
+    finally:
+        s64.close()
+    sm = solver.Solver(case, hip.MIXED)
+    try:
+        for n in range(steps):
+            st = sm.step()
+            print(f"  mixed step {st['step']} it={st['it']} err={st['err']:.3e} div={st['avg_div']:.3e}", flush=True)
+            if st64[n]["status"] == hip.OK:
+                assert st["status"] == hip.OK, st
+                assert st["err"] <= case.tol and st["avg_div"] <= 1e-6
+            else:  # the FP64 solve stopped at the cap (impulsive start): so must this one, as close
+                assert st["it"] >= case.max_it and st["err"] <= 2 * st64[n]["err"], (st, st64[n])
+            p64, pm = st64[n]["power"], st["power"]
+            assert np.all(np.abs(pm - p64) <= 1e-3 * np.abs(p64)), (pm, p64)
+        m = interior_mask(case.mesh)
+        assert rel_l2(sm.field("U")[m], U64[m]) <= 1e-4
+    finally:
+        sm.close()
