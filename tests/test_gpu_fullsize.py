@@ -5,7 +5,7 @@ size (the CPU oracle would take minutes per step here):
     iteration cap) and satisfies the projection identity
     avg_div = dt * max_diag * err (the projected face fluxes are the operator's);
   * the precision form ends every step on the same test, with avg div <= 1e-6
-    (where the FP64 solve stops at the iteration cap, it must too, at most twice
+    or, where the tolerance allows more, dt * max_diag * tol (where the FP64 solve stops at the iteration cap, it must too, at most twice
     as far from the tolerance), turbine power within 0.1 % of the FP64 form and U
     within 1e-4 relative L2 of it.
 
@@ -51,6 +51,9 @@ def test_full_size_properties(name):
         U64 = s64.field("U")
     finally:
         s64.close()
+    # the divergence a solve to tol leaves: dt * max_diag * tol (the projection
+    # identity); above 1e-6 on the finest grid, where the tolerance decides it
+    div_bar = max(1e-6, case.dt * md * case.tol * (1 + 1e-5))
     sm = solver.Solver(case, hip.MIXED)
     try:
         for n in range(steps):
@@ -58,7 +61,7 @@ def test_full_size_properties(name):
             print(f"  mixed step {st['step']} it={st['it']} err={st['err']:.3e} div={st['avg_div']:.3e}", flush=True)
             if st64[n]["status"] == hip.OK:
                 assert st["status"] == hip.OK, st
-                assert st["err"] <= case.tol and st["avg_div"] <= 1e-6
+                assert st["err"] <= case.tol and st["avg_div"] <= div_bar
             else:  # the FP64 solve stopped at the cap (impulsive start): so must this one, as close
                 assert st["it"] >= case.max_it and st["err"] <= 2 * st64[n]["err"], (st, st64[n])
             p64, pm = st64[n]["power"], st["power"]
