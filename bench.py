@@ -23,6 +23,8 @@ launch over its mean launch time, timed with HIP events on the library's stream.
 cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) when
 that library is present and loads, else the C restatement in oracle/ (one core),
 on a bounded sample of this workload (see "sample").
+forms.<form>.step_roofline: the whole step's algorithmic bytes (fixed kernels + bytes
+per BiCGStab iteration x iterations per step) over the time per step.
 output_path: mean device time and achieved bandwidth of the output-path kernels
 (Q-criterion, time average of U) at the bench grid, both forms.  sor_forms: the
 64x48x48 SOR case (no turbine) in both forms.
@@ -73,6 +75,11 @@ PROBE_BYTES = {
     "pupdate_mixed": 20,  # read r, p, q; write p, z (FP32)
     "supdate_mixed": 16,  # read r, q; write s, z (FP32)
 }
+# algorithmic bytes per interior cell of one whole step: the fixed kernels (predictor,
+# flux + right-hand side, projection, post with div stored) and one BiCGStab(Jacobi 3)
+# iteration, plus the solve's start (residual and shadow residual)
+STEP_BYTES = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 280, "per_solve": 32},
+              "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 148, "per_solve": 28}}
 PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
              "supdate": 8, "jacobi32": 9, "spmv_mixed": 10, "update_mixed": 11, "pupdate_mixed": 12,
              "supdate_mixed": 13}
@@ -421,10 +428,21 @@ def main():
             cpu = cpu_baseline(make_case(grid, 1), it_ref)
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
+    def step_roofline(form, r):
+        """whole-step algorithmic bytes (fixed + per iteration x iterations) over the
+        time per step (the ALM projection's box and the precision form's residual
+        replacements are left out: a lower bound)"""
+        b = STEP_BYTES[form]
+        nbytes = (b["fixed"] + b["per_solve"] + b["per_it"] * r["it_mean"]) * N
+        ach = nbytes / (r["ms_per_step"] * 1e-3) / 1e9
+        return {"bytes_per_step": round(nbytes), "achieved": round(ach, 1), "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4)}
+
     forms_out = {f: {"mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
                      "it_per_step": round(r["it_mean"], 2), "avg_div_max": r["avg_div_max"],
                      "div_within_1e-6": bool(r["avg_div_max"] <= 1e-6),
-                     "power_last": r["power_last"]} for f, r in results.items()}
+                     "power_last": r["power_last"], "step_roofline": step_roofline(f, r)}
+                 for f, r in results.items()}
 
     line = {
         "metric": METRIC,
