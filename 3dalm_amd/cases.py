@@ -278,6 +278,18 @@ def case_c4(nx=192, ny=96, nz=96, steps=200, ap_per_blade=64, yaw_deg=(5.0, 0.0,
                 turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
 
 
+def case_c5(nx=768, ny=384, nz=384, steps=2000, ap_per_blade=96) -> Case:
+    """The largest configuration: 768x384x384 on a 32x16x8 box, a 3x3 array of
+    rotors (D = 1) at 6 D along x and 4 D along y from (4, -4, 0), TSR 6,
+    eps = 2 dx, 8 % seeded noise, BiCGStab(Jacobi 3) to 1e-8."""
+    m = make_mesh(nx, ny, nz, ((0.0, 32.0), (-8.0, 8.0), (0.0, 8.0)), 1.5, 1.5)
+    dx = 32.0 / nx
+    t = default_turbine(R=0.5, tower=4.0)
+    sites = [TurbineSite((4.0 + 6.0 * a, -4.0 + 4.0 * b, 0.0), 12.0) for a in range(3) for b in range(3)]
+    return Case("c5", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000, steps=steps, noise=0.08,
+                turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
+
+
 def case_adjoint_small(nx=20, ny=12, nz=12, steps=4, ap_per_blade=6) -> Case:
     """A few-thousand-cell rotor case for finite-difference checks of the adjoint:
     one rotor (R = 0.5) at x = 1 on a 4x3x3 box, yaw 10 deg, pitch 2 deg,

@@ -10,7 +10,8 @@ size (the CPU oracle would take minutes per step here):
     within 1e-4 relative L2 of it.
 
 256x128x128 with one rotor (the measured workload), 192x96x96 with three rotors
-(the differentiation-form geometry) and 512x256x256 with one rotor (one step)."""
+(the differentiation-form geometry), 512x256x256 with one rotor and 768x384x384
+with a 3x3 rotor array (one step each; the largest size, 113 M cells)."""
 import importlib
 
 import numpy as np
@@ -28,6 +29,7 @@ CONFIGS = {
     "256x128x128": (lambda: cases.case_c2(256, 128, 128, steps=3), 3),
     "192x96x96": (lambda: cases.case_c4(192, 96, 96, steps=2), 2),
     "512x256x256": (lambda: cases.case_c2(512, 256, 256, steps=1), 1),
+    "768x384x384": (lambda: cases.case_c5(steps=1), 1),
 }
 
 
