@@ -90,10 +90,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_case(grid, steps):
+def make_case(grid, steps, tol=1e-8):
     cases = importlib.import_module("3dalm_amd.cases")
     nx, ny, nz = grid
-    return cases.case_c2(nx, ny, nz, steps=steps)
+    c = cases.case_c2(nx, ny, nz, steps=steps)
+    c.tol = tol
+    return c
 
 
 def run_form(case, precision, steps, warmup, probe, form):
@@ -332,6 +334,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--grid", default="256,128,128")
+    ap.add_argument("--tol", type=float, default=1e-8, help="Poisson tolerance (the workload's is 1e-8)")
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=20)
@@ -370,7 +373,7 @@ def main():
     results = {}
     N = None
     for form in forms:
-        case = make_case(grid, a.steps)
+        case = make_case(grid, a.steps, a.tol)
         N = case.mesh.N
         if a.probe == "auto":
             probe = "jacobi" if form == "fp64" else "jacobi32"
@@ -458,9 +461,9 @@ def main():
         "dtype": "f64" if main_form == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": (f"single 3-blade turbine ALM, {grid[0]}x{grid[1]}x{grid[2]} cells, "
-                                f"BiCGStab(Jacobi x3) to 1e-8, form={main_form}"),
+                                f"BiCGStab(Jacobi x3) to {a.tol:g}, form={main_form}"),
                    "cells": N, "turbines": 1, "actuator_points": 192, "poisson": "BiCGStab+Jacobi3",
-                   "tol": 1e-8},
+                   "tol": a.tol},
         "forms": forms_out,
         "mixed_vs_fp64": (round(results["mixed"]["mcell_steps_s"] / results["fp64"]["mcell_steps_s"], 3)
                           if "mixed" in results and "fp64" in results else None),
