@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed]
                     [--grid 256,128,128] [--no-cpu] [--probe jacobi]
                     [--adjoint-steps 20] [--adjoint-every 10] [--no-adjoint] [--no-extras]
+                    [--no-target] [--tol 1e-8]
 
 Workload (one "step" = one Solver::main_loop_once): the single-turbine ALM case on
 256x128x128 cells (box 16x8x8, y/z tanh beta=1.5, rotor D=1 at x=3, 3 blades x 64
@@ -23,6 +24,9 @@ launch over its mean launch time, timed with HIP events on the library's stream.
 cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) when
 that library is present and loads, else the C restatement in oracle/ (one core),
 on a bounded sample of this workload (see "sample").
+target_size: both forms at 512x256x256 (the size the precision form's targets are
+stated at), steps 11-60, with the Poisson tolerance 4e-9 that keeps avg div <= 1e-6
+there; ratio, iterations, whole-step bandwidth and the turbine-power difference.
 forms.<form>.step_roofline: the whole step's algorithmic bytes (fixed kernels + bytes
 per BiCGStab iteration x iterations per step) over the time per step.
 output_path: mean device time and achieved bandwidth of the output-path kernels
@@ -310,6 +314,38 @@ def run_output_path(grid, reps=20):
     return out
 
 
+# the size the precision form's targets are stated at, with the Poisson tolerance
+# that keeps avg div <= 1e-6 there: avg div = dt * max_diag * err and dt * max_diag =
+# 230 at 512x256x256 (err <= 4.35e-9), where 1e-8 would leave 2.3e-6 in both forms
+TARGET_GRID, TARGET_TOL = (512, 256, 256), 4e-9
+
+
+def run_target_size(steps=50, warmup=10):
+    """Both forms at the target size: Mcell-steps/s, iterations, avg div, the
+    whole-step algorithmic bandwidth and the turbine power after the last step."""
+    hip = importlib.import_module("3dalm_amd.hip")
+    res = {}
+    for form, prec in (("fp64", hip.FP64), ("mixed", hip.MIXED)):
+        case = make_case(TARGET_GRID, warmup + steps, TARGET_TOL)
+        r, s = run_form(case, prec, steps, warmup, None, f"target-{form}")
+        s.close()
+        N = case.mesh.N
+        b = STEP_BYTES[form]
+        ms = 1e3 * r["seconds"] / steps
+        bw = (b["fixed"] + b["per_solve"] + b["per_it"] * r["it_mean"]) * N / (ms * 1e-3) / 1e9
+        res[form] = {"mcell_steps_s": round(N * steps / r["seconds"] / 1e6, 3), "ms_per_step": round(ms, 2),
+                     "it_per_step": round(r["it_mean"], 1), "avg_div_max": r["avg_div_max"],
+                     "power_last": r["power_last"], "step_bandwidth_GBs": round(bw, 1),
+                     "step_frac": round(bw / HBM_PEAK_GBS, 4)}
+    res["mixed_vs_fp64"] = round(res["mixed"]["mcell_steps_s"] / res["fp64"]["mcell_steps_s"], 3)
+    p64, pm = res["fp64"]["power_last"], res["mixed"]["power_last"]
+    res["power_rel_diff_last"] = abs(pm - p64) / abs(p64) if p64 else None
+    res["workload"] = (f"{TARGET_GRID[0]}x{TARGET_GRID[1]}x{TARGET_GRID[2]}, one rotor, BiCGStab(Jacobi x3) to "
+                       f"{TARGET_TOL:g}, steps {warmup + 1}-{warmup + steps}")
+    log(f"[bench] target size: {res}")
+    return res
+
+
 def run_sor_forms(steps=5, warmup=2):
     """The reference's own CPU-runnable case (64x48x48, no turbine, SOR 1.2 to 1e-6,
     at most 1000 iterations) in both forms: Mcell-steps/s and iterations per step."""
@@ -341,6 +377,7 @@ def main():
     ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the output-path and SOR legs")
+    ap.add_argument("--no-target", action="store_true", help="skip the 512x256x256 leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a kernel of known traffic for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -423,6 +460,12 @@ def main():
             sor = run_sor_forms()
         except Exception as e:
             log(f"[bench] output-path / SOR legs failed: {e}")
+    target = None
+    if not a.no_target:
+        try:
+            target = run_target_size()
+        except Exception as e:
+            log(f"[bench] target-size leg failed: {e}")
     cpu = None
     if not a.no_cpu:
         try:
@@ -472,6 +515,7 @@ def main():
         "roofline": roof,
         "roofline_fp64": roof_fp64,
         "cpu_baseline": cpu,
+        "target_size": target,
         "output_path": output_path,
         "sor_forms": sor,
     }
