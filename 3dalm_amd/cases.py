@@ -264,6 +264,19 @@ def case_c2(nx=256, ny=128, nz=128, steps=1000, ap_per_blade=64) -> Case:
                 ap_per_blade=ap_per_blade, width=2 * dx)
 
 
+def case_c3(nx=512, ny=256, nz=256, steps=500, ap_per_blade=64, tol=4e-9) -> Case:
+    """The precision form at scale: 512x256x256 on a 16x8x8 box, two rotors (D = 1)
+    5 D apart at x = 3 and 8, TSR 6, eps = 2 dx, 10 % seeded initial noise (the
+    reference's inflow boundary is constant), BiCGStab(Jacobi 3) to 4e-9, the
+    tolerance that keeps avg div <= 1e-6 at this size (dt * max_diag = 230)."""
+    m = make_mesh(nx, ny, nz, ((0.0, 16.0), (-4.0, 4.0), (0.0, 8.0)), 1.5, 1.5)
+    dx = 16.0 / nx
+    t = default_turbine(R=0.5, tower=4.0)
+    sites = [TurbineSite((3.0, 0.0, 0.0), 12.0), TurbineSite((8.0, 0.0, 0.0), 12.0)]
+    return Case("c3", m, method="BiCG", pc="Jacobi", pc_it=3, tol=tol, max_it=1000, steps=steps, noise=0.10,
+                turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
+
+
 def case_c4(nx=192, ny=96, nz=96, steps=200, ap_per_blade=64, yaw_deg=(5.0, 0.0, -5.0),
             pitch_deg=(0.0, 0.0, 0.0)) -> Case:
     """Differentiation form: three rotors (D=1) in tandem at x = 3, 8, 13 on a 24x8x8
