@@ -25,7 +25,7 @@ cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) w
 that library is present and loads, else the C restatement in oracle/ (one core),
 on a bounded sample of this workload (see "sample").
 target_size: both forms at 512x256x256 (the size the precision form's targets are
-stated at; two rotors 5 D apart, 10 % seeded noise), steps 11-60, with the Poisson tolerance 4e-9 that keeps avg div <= 1e-6
+stated at; two rotors 5 D apart, 10 % seeded noise), steps 11-110, with the Poisson tolerance 4e-9 that keeps avg div <= 1e-6
 there; ratio, iterations, whole-step bandwidth and the turbine-power difference.
 forms.<form>.step_roofline: the whole step's algorithmic bytes (fixed kernels + bytes
 per BiCGStab iteration x iterations per step) over the time per step.
@@ -320,7 +320,7 @@ def run_output_path(grid, reps=20):
 TARGET_GRID, TARGET_TOL = (512, 256, 256), 4e-9
 
 
-def run_target_size(steps=50, warmup=10):
+def run_target_size(steps=100, warmup=10):
     """Both forms at the target size (two rotors 5 D apart, 10 % seeded noise):
     Mcell-steps/s, iterations, avg div, the whole-step algorithmic bandwidth and
     the turbine power after the last step."""
