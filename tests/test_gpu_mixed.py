@@ -33,6 +33,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("builder,steps", [
     (lambda: cases.case_c2(32, 16, 16, steps=6, ap_per_blade=8), 6),
     (lambda: cases.case_c2(64, 32, 32, steps=4, ap_per_blade=16), 4),
+    # rows of 256 cells: the four-cells-per-lane Jacobi sweep
+    (lambda: cases.case_c2(16, 8, 256, steps=4, ap_per_blade=4), 4),
 ])
 def test_mixed_against_fp64_oracle(builder, steps):
     case = builder()
