@@ -166,6 +166,193 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi4(Dims d, PCoefF A, const floa
     }
 }
 
+// Unit of a four-per-lane kernel: a lane owns k0 .. k0+3 of row j over the unit's
+// planes [i0, i1), of which it writes [own0, i1).  ok is false on a ragged j row.
+struct Col4 {
+    int i0, i1, own0, j, k0;
+    bool ok;
+};
+__device__ __forceinline__ Col4 col_unit4(const Dims& d) {
+    const int tkv = d.nz / (TK * kVecK);
+    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
+    const int ic = u % d.nch, col = u / d.nch;
+    const int kt = col % tkv, jt = col / tkv;
+    Col4 r;
+    r.k0 = d.gc + (kt * TK + (int)(threadIdx.x % TK)) * kVecK;
+    r.j = d.gc + jt * TJ + (int)(threadIdx.x / TK);
+    r.own0 = d.gc + ic * d.ni;
+    r.i1 = r.own0 + d.ni;
+    if (r.i1 > d.gc + d.nx) r.i1 = d.gc + d.nx;
+    r.i0 = r.i1 - d.ni;
+    r.ok = r.j < d.gc + d.ny;
+    return r;
+}
+
+// kg_spmv with four cells per lane: y = A z in FP64 arithmetic (OpD's order), stored
+// rounded, and the partial dot products of the stored values, cells in k order
+template <int WHICH>
+__global__ __launch_bounds__(BLOCK) void kg_spmv4(Dims d, PCoef A, const float* __restrict__ z,
+                                                  float* __restrict__ y, const float* __restrict__ a1,
+                                                  double* partials, const SolveState* st) {
+    if (st->done) return;
+    const Col4 cl = col_unit4(d);
+    double v[2] = {0.0, 0.0};
+    if (cl.ok) {
+        const int j = cl.j, k0 = cl.k0;
+        const double an = A.an[j], as = A.as[j], anu = A.anu[j], asu = A.asu[j];
+        double at[kVecK], ab[kVecK], atu[kVecK], abu[kVecK];
+#pragma unroll
+        for (int q = 0; q < kVecK; ++q) {
+            at[q] = A.at[k0 + q];
+            ab[q] = A.ab[k0 + q];
+            atu[q] = A.atu[k0 + q];
+            abu[q] = A.abu[k0 + q];
+        }
+        int c = d.at(cl.i0, j, k0);
+        f4k xm = *(const f4k*)(z + c - d.si), x0 = *(const f4k*)(z + c), xp = *(const f4k*)(z + c + d.si);
+        f4k xn = *(const f4k*)(z + c + d.sj), xs = *(const f4k*)(z + c - d.sj), av = *(const f4k*)(a1 + c);
+        float kb = z[c - 1], kf = z[c + kVecK];
+        auto plane = [&](int i, int cc) {
+            const double acx = A.acx[i], ae = A.ae[i], aw = A.aw[i];
+            f4k out;
+#pragma unroll
+            for (int q = 0; q < kVecK; ++q) {
+                const double ac = ((((acx - anu) - asu) - atu[q]) - abu[q]) / A.max_diag;
+                const double t = q + 1 < kVecK ? (double)x0[q + 1] : (double)kf;
+                const double b = q > 0 ? (double)x0[q - 1] : (double)kb;
+                const double yv = ac * (double)x0[q] + ae * (double)xp[q] + aw * (double)xm[q] + an * (double)xn[q] +
+                                  as * (double)xs[q] + at[q] * t + ab[q] * b;
+                const float yf = (float)yv;
+                out[q] = yf;
+                const double ys = (double)yf;
+                if (WHICH == 0) {
+                    v[0] += (double)av[q] * ys;
+                } else {
+                    v[0] += ys * (double)av[q];
+                    v[1] += ys * ys;
+                }
+            }
+            *(f4k*)(y + cc) = out;
+        };
+        auto step = [&](int i, int nxt) {
+            const int cn = c + nxt;
+            const f4k xpp = *(const f4k*)(z + cn + d.si);
+            const f4k xn1 = *(const f4k*)(z + cn + d.sj), xs1 = *(const f4k*)(z + cn - d.sj);
+            const f4k av1 = *(const f4k*)(a1 + cn);
+            const float kb1 = z[cn - 1], kf1 = z[cn + kVecK];
+            if (i >= cl.own0) plane(i, c);
+            xm = x0;
+            x0 = xp;
+            xp = xpp;
+            xn = xn1;
+            xs = xs1;
+            av = av1;
+            kb = kb1;
+            kf = kf1;
+            c += d.si;
+        };
+        if (d.ni == kMaxPlanes) {
+#pragma unroll
+            for (int w = 0; w < kMaxPlanes; ++w) step(cl.i0 + w, w + 1 < kMaxPlanes ? d.si : 0);
+        } else {
+            for (int i = cl.i0; i < cl.i1; ++i) step(i, i + 1 < cl.i1 ? d.si : 0);
+        }
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    store_partials<2>(pv, partials);
+}
+
+// Pointwise walk of a four-per-lane unit: load(c) returns the vectors' values at
+// c .. c+3, f(i, c, values) processes plane i for the owned planes; plane i+1 is
+// requested before plane i is processed.
+template <typename Aux, typename Load, typename F>
+__device__ __forceinline__ void walk_points4(const Dims& d, const Col4& cl, Load&& load, F&& f) {
+    int c = d.at(cl.i0, cl.j, cl.k0);
+    Aux a0 = load(c);
+    auto step = [&](int i, int nxt) {
+        const Aux a1 = load(c + nxt);
+        if (i >= cl.own0) f(i, c, a0);
+        a0 = a1;
+        c += d.si;
+    };
+    if (d.ni == kMaxPlanes) {
+#pragma unroll
+        for (int w = 0; w < kMaxPlanes; ++w) step(cl.i0 + w, w + 1 < kMaxPlanes ? d.si : 0);
+    } else {
+        for (int i = cl.i0; i < cl.i1; ++i) step(i, i + 1 < cl.i1 ? d.si : 0);
+    }
+}
+
+// kg_p and kg_s with four cells per lane (the same arithmetic per cell)
+__global__ __launch_bounds__(BLOCK) void kg_p4(Dims d, PCoefF Af, const float* __restrict__ r,
+                                               float* __restrict__ p, const float* __restrict__ q,
+                                               float* __restrict__ z, int pc_first, const SolveState* st) {
+    if (st->done) return;
+    const bool first = st->it == 0;
+    const double beta = st->beta, om = st->omega;
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    const float anu = Af.anu[cl.j], asu = Af.asu[cl.j];
+    float atu[kVecK], abu[kVecK];
+#pragma unroll
+    for (int m = 0; m < kVecK; ++m) {
+        atu[m] = Af.atu[cl.k0 + m];
+        abu[m] = Af.abu[cl.k0 + m];
+    }
+    struct RPQ {
+        f4k r, p, q;
+    };
+    walk_points4<RPQ>(d, cl, [&](int c) { return RPQ{*(const f4k*)(r + c), *(const f4k*)(p + c), *(const f4k*)(q + c)}; },
+                      [&](int i, int c, const RPQ& v) {
+                          const float acx = Af.acx[i];
+                          f4k pv, zv;
+#pragma unroll
+                          for (int m = 0; m < kVecK; ++m) {
+                              const float pf = first ? v.r[m]
+                                                     : (float)((double)v.r[m] + beta * ((double)v.p[m] - om * (double)v.q[m]));
+                              pv[m] = pf;
+                              const float ac = (((acx - anu) - asu) - atu[m]) - abu[m];
+                              zv[m] = pc_first ? pf * __builtin_amdgcn_rcpf(ac) : 0.f;
+                          }
+                          *(f4k*)(p + c) = pv;
+                          *(f4k*)(z + c) = zv;
+                      });
+}
+
+__global__ __launch_bounds__(BLOCK) void kg_s4(Dims d, PCoefF Af, const float* __restrict__ r,
+                                               const float* __restrict__ q, float* __restrict__ s,
+                                               float* __restrict__ z, int pc_first, const SolveState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    const float anu = Af.anu[cl.j], asu = Af.asu[cl.j];
+    float atu[kVecK], abu[kVecK];
+#pragma unroll
+    for (int m = 0; m < kVecK; ++m) {
+        atu[m] = Af.atu[cl.k0 + m];
+        abu[m] = Af.abu[cl.k0 + m];
+    }
+    struct RQ {
+        f4k r, q;
+    };
+    walk_points4<RQ>(d, cl, [&](int c) { return RQ{*(const f4k*)(r + c), *(const f4k*)(q + c)}; },
+                     [&](int i, int c, const RQ& v) {
+                         const float acx = Af.acx[i];
+                         f4k sv, zv;
+#pragma unroll
+                         for (int m = 0; m < kVecK; ++m) {
+                             const float sf = (float)((double)v.r[m] - alpha * (double)v.q[m]);
+                             sv[m] = sf;
+                             const float ac = (((acx - anu) - asu) - atu[m]) - abu[m];
+                             zv[m] = pc_first ? sf * __builtin_amdgcn_rcpf(ac) : 0.f;
+                         }
+                         *(f4k*)(s + c) = sv;
+                         *(f4k*)(z + c) = zv;
+                     });
+}
+
 __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restrict__ x,
                                                 const float* __restrict__ b, float omega, int color,
                                                 const SolveState* st) {
@@ -597,25 +784,41 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             for (long long c = 0; c < chunk; ++c) {
                 probe_iteration(ctx, launched + c);
                 probe_begin(ctx, DALM_K_PUPDATE_MIXED);
-                hipLaunchKernelGGL(kg_p, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fp,
-                                   ctx->fq, ctx->fz, pc_first, (const SolveState*)st);
+                if (vec4)
+                    hipLaunchKernelGGL(kg_p4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fp,
+                                       ctx->fq, ctx->fz, pc_first, (const SolveState*)st);
+                else
+                    hipLaunchKernelGGL(kg_p, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fp,
+                                       ctx->fq, ctx->fz, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_PUPDATE_MIXED);
                 float* zp = precondition(ctx->fz, ctx->ftmp, ctx->fp);
                 probe_begin(ctx, DALM_K_SPMV_MIXED);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv<0>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
-                                   zp, ctx->fq, ctx->fr0, ctx->partials, (const SolveState*)st);
+                if (vec4)
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv4<0>), dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A,
+                                       zp, ctx->fq, ctx->fr0, ctx->partials, (const SolveState*)st);
+                else
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv<0>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d,
+                                       A, zp, ctx->fq, ctx->fr0, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SPMV_MIXED);
-                if ((rc = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, st, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_ALPHA, vec4 ? grid4 : ctx->sgrid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_SUPDATE_MIXED);
-                hipLaunchKernelGGL(kg_s, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fq,
-                                   ctx->fs, ctx->fz2, pc_first, (const SolveState*)st);
+                if (vec4)
+                    hipLaunchKernelGGL(kg_s4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fq,
+                                       ctx->fs, ctx->fz2, pc_first, (const SolveState*)st);
+                else
+                    hipLaunchKernelGGL(kg_s, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fq,
+                                       ctx->fs, ctx->fz2, pc_first, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SUPDATE_MIXED);
                 float* zs = precondition(ctx->fz2, ctx->ftmp2, ctx->fs);
                 probe_begin(ctx, DALM_K_SPMV_MIXED);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv<1>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A,
-                                   zs, ctx->ft, ctx->fs, ctx->partials, (const SolveState*)st);
+                if (vec4)
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv4<1>), dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A,
+                                       zs, ctx->ft, ctx->fs, ctx->partials, (const SolveState*)st);
+                else
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_spmv<1>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d,
+                                       A, zs, ctx->ft, ctx->fs, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_SPMV_MIXED);
-                if ((rc = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, st, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_OMEGA, vec4 ? grid4 : ctx->sgrid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_UPDATE_MIXED);
                 hipLaunchKernelGGL(kg_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, zp, zs, ctx->fs,
                                    ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
