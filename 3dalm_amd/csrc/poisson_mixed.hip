@@ -353,6 +353,54 @@ __global__ __launch_bounds__(BLOCK) void kg_s4(Dims d, PCoefF Af, const float* _
                      });
 }
 
+// kg_xr with four cells per lane: x (FP64) as two 16-byte halves, the FP32 vectors
+// as one 16-byte load each; partial sums cell by cell in k order
+typedef double d2k __attribute__((ext_vector_type(2), aligned(8)));
+__global__ __launch_bounds__(BLOCK) void kg_xr4(Dims d, double* __restrict__ x, const float* __restrict__ zp,
+                                                const float* __restrict__ zs, const float* __restrict__ s,
+                                                const float* __restrict__ t_, float* __restrict__ r,
+                                                const float* __restrict__ r0, double* partials,
+                                                const SolveState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha, om = st->omega;
+    const Col4 cl = col_unit4(d);
+    double v[2] = {0.0, 0.0};
+    if (cl.ok) {
+        struct XR {
+            d2k xa, xb;
+            f4k zp, zs, s, t, r0;
+        };
+        walk_points4<XR>(
+            d, cl,
+            [&](int c) {
+                return XR{*(const d2k*)(x + c), *(const d2k*)(x + c + 2), *(const f4k*)(zp + c),
+                          *(const f4k*)(zs + c), *(const f4k*)(s + c), *(const f4k*)(t_ + c), *(const f4k*)(r0 + c)};
+            },
+            [&](int, int c, const XR& a) {
+                d2k xa, xb;
+                f4k rv4;
+#pragma unroll
+                for (int m = 0; m < kVecK; ++m) {
+                    const double xo = m < 2 ? a.xa[m] : a.xb[m - 2];
+                    const double xn = xo + alpha * (double)a.zp[m] + om * (double)a.zs[m];
+                    if (m < 2) xa[m] = xn;
+                    else xb[m - 2] = xn;
+                    const float rf = (float)((double)a.s[m] - om * (double)a.t[m]);
+                    rv4[m] = rf;
+                    const double rv = (double)rf;
+                    v[0] += rv * rv;
+                    v[1] += (double)a.r0[m] * rv;
+                }
+                *(d2k*)(x + c) = xa;
+                *(d2k*)(x + c + 2) = xb;
+                *(f4k*)(r + c) = rv4;
+            });
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    store_partials<2>(pv, partials);
+}
+
 __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restrict__ x,
                                                 const float* __restrict__ b, float omega, int color,
                                                 const SolveState* st) {
@@ -820,10 +868,14 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 probe_end(ctx, DALM_K_SPMV_MIXED);
                 if ((rc = launch_finalize(ctx, FIN_OMEGA, vec4 ? grid4 : ctx->sgrid, st, nullptr))) return rc;
                 probe_begin(ctx, DALM_K_UPDATE_MIXED);
-                hipLaunchKernelGGL(kg_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, zp, zs, ctx->fs,
-                                   ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+                if (vec4)
+                    hipLaunchKernelGGL(kg_xr4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, ctx->p, zp, zs, ctx->fs,
+                                       ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+                else
+                    hipLaunchKernelGGL(kg_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, ctx->p, zp, zs,
+                                       ctx->fs, ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                 probe_end(ctx, DALM_K_UPDATE_MIXED);
-                if ((rc = launch_finalize(ctx, FIN_MX_XR, ctx->sgrid, st, nullptr))) return rc;
+                if ((rc = launch_finalize(ctx, FIN_MX_XR, vec4 ? grid4 : ctx->sgrid, st, nullptr))) return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<false>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d,
                                    A, ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                 if ((rc = launch_finalize(ctx, FIN_MX_RU, ctx->sgrid, st, nullptr))) return rc;
