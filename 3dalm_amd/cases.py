@@ -17,6 +17,8 @@ golden vectors) sees bit-identical inputs:
   the ends);
 * noise: counter-based splitmix64 of (seed, component, interior index), top 53 bits
   mapped to [-1, 1);
+* Fourier-mode turbulence (the at-scale workloads): a sum of seeded random
+  divergence-free modes, ``fourier_mode_velocity``;
 * polar: every 1 degree on [-180, 180], ``Cl = clip(2 pi alpha, +-1.2)``,
   ``Cd = 0.01 + 0.05 alpha^2`` (alpha in radians).
 """
@@ -147,6 +149,62 @@ def initial_velocity(mesh: Mesh, uin, amp: float, seed: int) -> np.ndarray:
     return U
 
 
+def fourier_mode_velocity(mesh: Mesh, uin, intensity: float, seed: int, modes: int = 200) -> np.ndarray:
+    """[L][3] velocity: Uin everywhere, the interior plus a seeded synthetic isotropic
+    turbulence of `modes` random Fourier modes,
+
+        u'(x) = sum_n a_n sigma_n cos(k_n . x + psi_n),   sigma_n . k_n = 0,
+
+    so every mode is divergence-free.  Directions of k_n are uniform on the sphere,
+    |k_n| log-uniform between 2 pi / (smallest box extent) and 2 pi / (8 x the widest
+    interior cell), so every mode spans at least 8 cells; sigma_n is the unit vector
+    along k_n x w_n for a random unit w_n; a_n ~ |k_n|^(-1/3) (a -5/3 spectrum sampled
+    log-uniformly); psi_n uniform in [0, 2 pi).  The field is scaled so that its
+    per-component RMS over the interior is intensity * |Uin_x|.  Each random quantity
+    comes from its own counter-based stream of the seed (streams 16-23, apart from
+    the uniform noise's 0-2).  The sum is separable: per component, one complex
+    (nx ny x modes) @ (modes x nz) product."""
+    U = np.empty((mesh.L, 3))
+    U[:] = np.asarray(uin, dtype=np.float64)
+    if not intensity or not modes:
+        return U
+    g = mesh.gc
+    nx, ny, nz = mesh.size
+    x, y, z = mesh.x[g:nx - g], mesh.y[g:ny - g], mesh.z[g:nz - g]
+    r = [uniform_pm1(seed, modes, stream=16 + q) for q in range(8)]
+    ct = r[0]
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    ph = math.pi * (r[1] + 1.0)
+    d = np.stack([st * np.cos(ph), st * np.sin(ph), ct], axis=1)
+    ext = min(x[-1] - x[0], y[-1] - y[0], z[-1] - z[0])
+    wmax = max(mesh.dx[g:nx - g].max(), mesh.dy[g:ny - g].max(), mesh.dz[g:nz - g].max())
+    kmin, kmax = 2 * math.pi / ext, 2 * math.pi / (8 * wmax)
+    kmag = kmin * (kmax / kmin) ** (0.5 * (r[2] + 1.0))
+    k = d * kmag[:, None]
+    w = np.stack(r[3:6], axis=1)
+    sig = np.cross(d, w)
+    nrm = np.linalg.norm(sig, axis=1)
+    bad = nrm < 1e-3  # w (almost) along k: use any vector normal to k instead
+    alt = np.where(np.abs(d[:, :1]) < 0.9, np.array([[1.0, 0.0, 0.0]]), np.array([[0.0, 1.0, 0.0]]))
+    sig[bad] = np.cross(d[bad], alt[bad])
+    sig /= np.linalg.norm(sig, axis=1)[:, None]
+    amp = kmag ** (-1.0 / 3.0)
+    psi = math.pi * (r[6] + 1.0)
+    Ey = np.exp(1j * np.outer(k[:, 1], y))   # [modes][ny]
+    Ez = np.exp(1j * np.outer(k[:, 2], z))   # [modes][nz]
+    Ex = np.exp(1j * (np.outer(x, k[:, 0]) + psi[None, :]))  # [nx][modes]
+    idx = np.arange(mesh.L).reshape(nx, ny, nz)[g:nx - g, g:ny - g, g:nz - g].ravel()
+    up = np.empty((len(idx), 3))
+    ni, nj, nk = len(x), len(y), len(z)
+    for c in range(3):
+        T = (Ex * (amp * sig[:, c])[None, :])[:, None, :] * Ey.T[None, :, :]  # [nx][ny][modes]
+        up[:, c] = (T.reshape(ni * nj, modes) @ Ez).real.ravel()
+        del T
+    rms = math.sqrt(float(np.mean(up * up)))
+    U[idx] += up * (intensity * abs(float(uin[0])) / rms)
+    return U
+
+
 # ----------------------------------------------------------------------------- turbines
 def polar_table(step_deg: float = 1.0):
     atk = np.arange(-180.0, 180.0 + 0.5 * step_deg, step_deg)
@@ -226,6 +284,9 @@ class Case:
     tol: float = 1e-6
     noise: float = 0.05
     seed: int = 1
+    # > 0: the initial perturbation is `modes` random Fourier modes of RMS intensity
+    # `noise` (fourier_mode_velocity) instead of uniform per-cell noise
+    modes: int = 0
     steps: int = 100
     turbine: Optional[TurbineModel] = None
     sites: List[TurbineSite] = field(default_factory=list)
@@ -245,6 +306,8 @@ class Case:
     floor_start: Optional[float] = None
 
     def U0(self) -> np.ndarray:
+        if self.modes:
+            return fourier_mode_velocity(self.mesh, self.uin, self.noise, self.seed, self.modes)
         return initial_velocity(self.mesh, self.uin, self.noise * abs(self.uin[0]), self.seed)
 
 
@@ -266,15 +329,17 @@ def case_c2(nx=256, ny=128, nz=128, steps=1000, ap_per_blade=64) -> Case:
 
 def case_c3(nx=512, ny=256, nz=256, steps=500, ap_per_blade=64, tol=4e-9) -> Case:
     """The precision form at scale: 512x256x256 on a 16x8x8 box, two rotors (D = 1)
-    5 D apart at x = 3 and 8, TSR 6, eps = 2 dx, 10 % seeded initial noise (the
-    reference's inflow boundary is constant), BiCGStab(Jacobi 3) to 4e-9, the
-    tolerance that keeps avg div <= 1e-6 at this size (dt * max_diag = 230)."""
+    5 D apart at x = 3 and 8, TSR 6, eps = 2 dx, an initial seeded isotropic
+    turbulence of 200 random Fourier modes at 10 % intensity (the reference's inflow
+    boundary is constant, so the turbulence is in the initial field), BiCGStab(Jacobi 3)
+    to 4e-9, the tolerance that keeps avg div <= 1e-6 at this size
+    (dt * max_diag = 230)."""
     m = make_mesh(nx, ny, nz, ((0.0, 16.0), (-4.0, 4.0), (0.0, 8.0)), 1.5, 1.5)
     dx = 16.0 / nx
     t = default_turbine(R=0.5, tower=4.0)
     sites = [TurbineSite((3.0, 0.0, 0.0), 12.0), TurbineSite((8.0, 0.0, 0.0), 12.0)]
     return Case("c3", m, method="BiCG", pc="Jacobi", pc_it=3, tol=tol, max_it=1000, steps=steps, noise=0.10,
-                turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
+                modes=200, turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
 
 
 def case_c4(nx=192, ny=96, nz=96, steps=200, ap_per_blade=64, yaw_deg=(5.0, 0.0, -5.0),
@@ -294,12 +359,14 @@ def case_c4(nx=192, ny=96, nz=96, steps=200, ap_per_blade=64, yaw_deg=(5.0, 0.0,
 def case_c5(nx=768, ny=384, nz=384, steps=2000, ap_per_blade=96) -> Case:
     """The largest configuration: 768x384x384 on a 32x16x8 box, a 3x3 array of
     rotors (D = 1) at 6 D along x and 4 D along y from (4, -4, 0), TSR 6,
-    eps = 2 dx, 8 % seeded noise, BiCGStab(Jacobi 3) to 1e-8."""
+    eps = 2 dx, an initial turbulence of 200 random Fourier modes at 8 % intensity,
+    BiCGStab(Jacobi 3) to 1e-8."""
     m = make_mesh(nx, ny, nz, ((0.0, 32.0), (-8.0, 8.0), (0.0, 8.0)), 1.5, 1.5)
     dx = 32.0 / nx
     t = default_turbine(R=0.5, tower=4.0)
     sites = [TurbineSite((4.0 + 6.0 * a, -4.0 + 4.0 * b, 0.0), 12.0) for a in range(3) for b in range(3)]
     return Case("c5", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000, steps=steps, noise=0.08,
+                modes=200,
                 turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
 
 

@@ -25,7 +25,8 @@ cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) w
 that library is present and loads, else the C restatement in oracle/ (one core),
 on a bounded sample of this workload (see "sample").
 target_size: both forms at 512x256x256 (the size the precision form's targets are
-stated at; two rotors 5 D apart, 10 % seeded noise), steps 11-110, with the Poisson tolerance 4e-9 that keeps avg div <= 1e-6
+stated at; two rotors 5 D apart, an initial turbulence of 200 seeded Fourier modes at
+10 % intensity), steps 11-110, with the Poisson tolerance 4e-9 that keeps avg div <= 1e-6
 there; ratio, iterations, whole-step bandwidth and the turbine-power difference.
 forms.<form>.step_roofline: the whole step's algorithmic bytes (fixed kernels + bytes
 per BiCGStab iteration x iterations per step) over the time per step.
@@ -321,7 +322,8 @@ TARGET_GRID, TARGET_TOL = (512, 256, 256), 4e-9
 
 
 def run_target_size(steps=100, warmup=10):
-    """Both forms at the target size (two rotors 5 D apart, 10 % seeded noise):
+    """Both forms at the target size (two rotors 5 D apart, an initial turbulence of
+    200 seeded Fourier modes at 10 % intensity):
     Mcell-steps/s, iterations, avg div, the whole-step algorithmic bandwidth and
     the turbine power after the last step."""
     hip = importlib.import_module("3dalm_amd.hip")
@@ -342,8 +344,8 @@ def run_target_size(steps=100, warmup=10):
     res["mixed_vs_fp64"] = round(res["mixed"]["mcell_steps_s"] / res["fp64"]["mcell_steps_s"], 3)
     p64, pm = res["fp64"]["power_last"], res["mixed"]["power_last"]
     res["power_rel_diff_last"] = abs(pm - p64) / abs(p64) if p64 else None
-    res["workload"] = (f"{TARGET_GRID[0]}x{TARGET_GRID[1]}x{TARGET_GRID[2]}, two rotors 5 D apart, 10 % seeded "
-                       f"noise, BiCGStab(Jacobi x3) to {TARGET_TOL:g}, steps {warmup + 1}-{warmup + steps}")
+    res["workload"] = (f"{TARGET_GRID[0]}x{TARGET_GRID[1]}x{TARGET_GRID[2]}, two rotors 5 D apart, 200-mode "
+                       f"Fourier turbulence at 10 %, BiCGStab(Jacobi x3) to {TARGET_TOL:g}, steps {warmup + 1}-{warmup + steps}")
     log(f"[bench] target size: {res}")
     return res
 

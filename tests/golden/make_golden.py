@@ -19,6 +19,9 @@ Fixtures:
   mesh.npz          build_mesh_from_json on a generated mesh.json
   output_small.npz  the reference's output files (snapshots, snapshot.json,
                     partition.json, probes, ap_info.log) for a 10x6x6 rotor case
+  fullsize_c2c4.npz (--only fullsize, about 15 minutes) the first 3 steps of the
+                    256x128x128 one-rotor and 192x96x96 three-rotor cases: per-step
+                    log and turbine loads, final field norms and a U.x mid-plane
 """
 import argparse
 import importlib
@@ -259,6 +262,37 @@ def mesh_fixture(out):
                         dy=arrs[4], dz=arrs[5], size=np.array(list(sz)), gc=np.array([gc.value]))
 
 
+def fullsize(out, steps=3):
+    """The reference's own run of the 256x128x128 one-rotor case and the 192x96x96
+    three-rotor case, first steps: per step it, err, avg_div, max_cfl and turbine
+    torque/thrust, and at the end the interior L2 norms of U (per component), p and
+    JU, and the k mid-plane of U.x (to check full-size runs against the reference
+    without its fields)."""
+    d = {}
+    for name, case in (("c2", cases.case_c2(256, 128, 128, steps=steps)),
+                       ("c4", cases.case_c4(192, 96, 96, steps=steps))):
+        r = po.RefSolver(case)
+        rows, tq, th = [], [], []
+        for _ in range(steps):
+            s = r.step()
+            rows.append([s["step"], s["it"], s["err"], s["avg_div"], s["max_cfl"]])
+            tq.append(s["torque"])
+            th.append(s["thrust"])
+            print(name, rows[-1], flush=True)
+        m = case.mesh
+        mi = interior_mask(m)
+        U, p, JU = r.field("U"), r.field("p"), r.field("JU")
+        d[name + "_steps"] = np.array(rows)
+        d[name + "_torque"] = np.array(tq)
+        d[name + "_thrust"] = np.array(th)
+        d[name + "_norms"] = np.array([np.linalg.norm(U[mi, 0]), np.linalg.norm(U[mi, 1]), np.linalg.norm(U[mi, 2]),
+                                      np.linalg.norm(p[mi]), np.linalg.norm(JU[mi].reshape(-1))])
+        n = m.size
+        d[name + "_Ux_midplane"] = U[:, 0].reshape(n)[:, :, n[2] // 2].copy()
+        r.close()
+    np.savez_compressed(os.path.join(out, "fullsize_c2c4.npz"), **d)
+
+
 def output_small(out):
     """The reference's own output files for cases.case_output_small(): snapshot
     files, snapshot.json, partition.json, probe CSVs and ap_info.log, stored as
@@ -284,9 +318,11 @@ if __name__ == "__main__":
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     todo = {"kernels": kernels_tiny, "alm": alm_tiny, "steps": steps_small, "wtprop": wtprop,
-            "mesh": mesh_fixture, "output": output_small}
+            "mesh": mesh_fixture, "output": output_small, "fullsize": fullsize}
     for k, fn in todo.items():
         if a.only and k not in a.only.split(","):
+            continue
+        if not a.only and k == "fullsize":  # long: only when named
             continue
         fn(HERE)
         print("wrote", k)

@@ -206,3 +206,40 @@ def test_gaussian_weights_integrate_to_one(K):
     V = np.einsum("i,j,k->ijk", m.dx, m.dy, m.dz).reshape(-1)
     tot = (f * V[:, None]).sum(axis=0)
     assert np.allclose(tot, ap[0, 8:11], rtol=1e-6)
+
+
+def test_fourier_mode_turbulence_is_solenoidal_and_scaled():
+    """The initial turbulence of the at-scale workloads: its per-component RMS over
+    the interior is the intensity (as a whole exactly, per component roughly: the
+    modes' directions are isotropic), it is reproducible from the seed, and it is
+    divergence-free: central differences on a uniform mesh leave a divergence of a
+    few per cent of the velocity gradients (the truncation error at 8 cells per
+    shortest wavelength), where amplitudes not normal to k would leave one of their
+    order."""
+    m = cases.make_mesh(64, 32, 32, ((0.0, 8.0), (-2.0, 2.0), (0.0, 4.0)), 0, 0)
+    U = cases.fourier_mode_velocity(m, (1.0, 0.0, 0.0), 0.1, 3, 200)
+    assert np.array_equal(U, cases.fourier_mode_velocity(m, (1.0, 0.0, 0.0), 0.1, 3, 200))
+    assert not np.array_equal(U, cases.fourier_mode_velocity(m, (1.0, 0.0, 0.0), 0.1, 4, 200))
+    g = m.gc
+    nx, ny, nz = m.size
+    full = U.reshape(nx, ny, nz, 3)
+    assert np.all(full[:g] == [1.0, 0.0, 0.0]) and np.all(full[:, :, -g:] == [1.0, 0.0, 0.0])
+    V = full[g:-g, g:-g, g:-g] - np.array([1.0, 0.0, 0.0])
+    assert abs(np.sqrt(np.mean(V * V)) - 0.1) <= 1e-12
+    for c in range(3):
+        assert 0.06 <= np.sqrt(np.mean(V[..., c] ** 2)) <= 0.14, c
+    h = [m.x[3] - m.x[2], m.y[3] - m.y[2], m.z[3] - m.z[2]]
+    c = (slice(1, -1),) * 3
+    dudx = (V[2:, 1:-1, 1:-1, 0] - V[:-2, 1:-1, 1:-1, 0]) / (2 * h[0])
+    dvdy = (V[1:-1, 2:, 1:-1, 1] - V[1:-1, :-2, 1:-1, 1]) / (2 * h[1])
+    dwdz = (V[1:-1, 1:-1, 2:, 2] - V[1:-1, 1:-1, :-2, 2]) / (2 * h[2])
+    div = dudx + dvdy + dwdz
+    scale = np.sqrt(np.mean(dudx ** 2 + dvdy ** 2 + dwdz ** 2))
+    assert np.sqrt(np.mean(div ** 2)) <= 0.05 * scale
+    # the same measure on an arbitrary field of the same size: of order one
+    rng = np.random.default_rng(0)
+    R = V[c][..., [1, 2, 0]] * rng.choice([-1.0, 1.0], size=3)
+    dR = ((R[2:, 1:-1, 1:-1, 0] - R[:-2, 1:-1, 1:-1, 0]) / (2 * h[0])
+          + (R[1:-1, 2:, 1:-1, 1] - R[1:-1, :-2, 1:-1, 1]) / (2 * h[1])
+          + (R[1:-1, 1:-1, 2:, 2] - R[1:-1, 1:-1, :-2, 2]) / (2 * h[2]))
+    assert np.sqrt(np.mean(dR ** 2)) >= 0.3 * scale
