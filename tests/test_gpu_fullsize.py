@@ -11,8 +11,12 @@ size (the CPU oracle would take minutes per step here):
 
 256x128x128 with one rotor (the measured workload), 192x96x96 with three rotors
 (the differentiation-form geometry), 512x256x256 with one rotor and 768x384x384
-with a 3x3 rotor array (one step each; the largest size, 113 M cells)."""
+with a 3x3 rotor array (one step each; the largest size, 113 M cells).
+
+The first two also run against the reference's own first three steps of them
+(test_full_size_against_reference_trace), in both forms."""
 import importlib
+import os
 
 import numpy as np
 import pytest
@@ -75,3 +79,62 @@ def test_full_size_properties(name):
             assert rel_l2(sm.field("U")[m], U64[m]) <= 1e-4
     finally:
         sm.close()
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fullsize_c2c4.npz")
+TRACES = {"256x128x128": ("c2", lambda: cases.case_c2(256, 128, 128, steps=3)),
+          "192x96x96": ("c4", lambda: cases.case_c4(192, 96, 96, steps=3))}
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD), reason="full-size reference trace not generated")
+@pytest.mark.parametrize("name", list(TRACES))
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_full_size_against_reference_trace(name, form):
+    """The first three steps of the 256x128x128 one-rotor and 192x96x96 three-rotor
+    cases against the reference's own run of them (tests/golden/fullsize_c2c4.npz).
+    At these sizes BiCGStab's iteration path depends on the order in which the dot
+    products are summed, so the FP64 iteration counts differ from the reference's
+    (measured: by up to 16 %; the precision form's own policy takes up to 1.6x the
+    reference's count at the impulsive start) and the fields agree to what a solve
+    to 1e-8 leaves, not to rounding.  Every solve ends on the reference's stopping
+    test; the CFL number, turbine torque and thrust, the norms of U and JU and the
+    U.x mid-plane agree with the reference's to about 10x the differences measured
+    on MI355X (at most 3e-6 per step, 1.1e-6 for the fields), in both forms.  The
+    pressure norm differs by up to 2.2e-2: a residual of 1e-8 pins the smoothest
+    pressure mode, which only the outlet's Dirichlet condition fixes, to no better
+    (the velocity takes the pressure's gradient, and agrees to 1e-6)."""
+    tag, builder = TRACES[name]
+    d = np.load(GOLD)
+    case = builder()
+    prec = hip.FP64 if form == "fp64" else hip.MIXED
+    print(f"[full-size trace {name} {form}]", flush=True)
+    s = solver.Solver(case, prec)
+    try:
+        steps = d[tag + "_steps"]
+        for n in range(len(steps)):
+            st = s.step()
+            row = steps[n]
+            dq = np.max(np.abs(st["torque"] - d[tag + "_torque"][n]) / np.abs(d[tag + "_torque"][n]))
+            dt_ = np.max(np.abs(st["thrust"] - d[tag + "_thrust"][n]) / np.abs(d[tag + "_thrust"][n]))
+            print(f"  step {st['step']}: it {st['it']} vs {int(row[1])}, err {st['err']:.3e} vs {row[2]:.3e}, "
+                  f"cfl {abs(st['max_cfl'] - row[4]) / row[4]:.2e}, torque {dq:.2e}, thrust {dt_:.2e}", flush=True)
+            assert st["step"] == row[0]
+            assert st["status"] == hip.OK and st["err"] <= case.tol and row[2] <= case.tol, (st, row)
+            if form == "fp64":
+                assert abs(st["it"] - row[1]) <= 0.3 * row[1] + 10, (st["it"], row[1])
+            assert abs(st["max_cfl"] - row[4]) <= 3e-5 * row[4]
+            assert dq <= 5e-5 and dt_ <= 5e-5, (dq, dt_)
+        m = interior_mask(case.mesh)
+        U, p, JU = s.field("U"), s.field("p"), s.field("JU")
+        got = np.array([np.linalg.norm(U[m, 0]), np.linalg.norm(U[m, 1]), np.linalg.norm(U[m, 2]),
+                        np.linalg.norm(p[m]), np.linalg.norm(JU[m].reshape(-1))])
+        ref = d[tag + "_norms"]
+        dn = np.abs(got - ref) / np.abs(ref)
+        n3 = case.mesh.size
+        plane = U[:, 0].reshape(n3)[:, :, n3[2] // 2]
+        dp = rel_l2(plane, d[tag + "_Ux_midplane"])
+        print(f"  norms (Ux Uy Uz p JU) {np.array2string(dn, precision=2)}; U.x mid-plane {dp:.2e}", flush=True)
+        assert np.all(dn[[0, 1, 2, 4]] <= 1e-5) and dn[3] <= 0.1, dn
+        assert dp <= 1e-5, dp
+    finally:
+        s.close()
