@@ -9,8 +9,8 @@
 // Arithmetic: FP32 with FMA contraction and precomputed reciprocal metrics (no
 // divisions in the cell loops) for the predictor and nu_t; FP64 for everything
 // that writes JU, b or div.  Same HBM-bound streaming structure as the FP64 form,
-// with about half the bytes per cell (predictor 64 B, flux + rhs 44 B, projection
-// 56 B, post 40 B).
+// with about half the bytes per cell (compulsory reads + writes per interior cell:
+// predictor 64 B, flux + rhs 44 B, projection 56 B, post 40 B).
 #include <math.h>
 
 #include "ctx.h"

@@ -62,8 +62,8 @@ PROBE_KERNEL = {"jacobi": "k_jacobi", "jacobi32": "kf_jacobi", "spmv": "k_bi_spm
                 "pupdate": "k_bi_p", "supdate": "k_bi_s", "predictor": "k_predictor", "spmv_mixed": "kg_spmv<0>",
                 "update_mixed": "kg_xr", "pupdate_mixed": "kg_p", "supdate_mixed": "kg_s"}
 
-# algorithmic bytes per interior cell for one launch of each probed kernel (FP64);
-# see DESIGN.md for the derivation (compulsory reads + writes, matrix-free A)
+# algorithmic bytes per interior cell for one launch of each probed kernel (FP64):
+# compulsory reads + writes, with the operator applied matrix-free from 1-D coefficients
 PROBE_BYTES = {
     "jacobi": 24,     # read z_in, v; write z_out
     "spmv": 24,       # read z, a1 (r0 or s); write y
@@ -387,8 +387,10 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the whole domain lives on one MI355X and this build has "
-              "no multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the whole domain lives on one MI355X and there is "
+              "no multi-GPU path (it would shard as x-slabs with one- and two-plane halo exchanges). Run with "
+              "--gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
     hip = importlib.import_module("3dalm_amd.hip")
     if a.copy_calibration:

@@ -3,7 +3,7 @@ sweeps; FP64 face fluxes, pressure, dot products, norms and the residual
 replacements that end every solve on the true FP64 residual) against the FP64
 oracle.
 
-Stated tolerances (the north star asks for them to be stated against FP64):
+Stated tolerances, against FP64:
   * every step ends with the FP64 residual test met: err <= tol, and therefore
     avg div <= 1e-6 (it is dt * max_diag * err by the projection identity);
   * turbine power within 0.1 % of the FP64 oracle;
