@@ -335,11 +335,16 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
                 const double* z, int64_t precision) {
     if (!out || !size || !x || !y || !z) return DALM_BAD_ARG;
     *out = nullptr;
-    if (gc < 2) return DALM_BAD_ARG;  // the reference asserts gc >= 2 (src/solver.h:615)
+    if (gc < 2 || gc > 16) return DALM_BAD_ARG;  // the reference asserts gc >= 2 (src/solver.h:615)
+    // 32-bit cell indices: the padded box stays below 2^31 - 2^20 cells (checked
+    // factor by factor, so that no product overflows)
+    constexpr long long kMaxCells = (1LL << 31) - (1LL << 20);
     for (int a = 0; a < 3; ++a)
-        if (size[a] < 2 * gc + 3) return DALM_BAD_ARG;
-    long long L = size[0] * size[1] * size[2];
-    if (L >= (1LL << 31) - (1LL << 20)) return DALM_BAD_ARG;  // 32-bit cell indices
+        if (size[a] < 2 * gc + 3 || size[a] >= kMaxCells) return DALM_BAD_ARG;
+    long long L = size[0] * size[1];
+    if (L >= kMaxCells) return DALM_BAD_ARG;
+    L *= size[2];
+    if (L >= kMaxCells) return DALM_BAD_ARG;
     if (precision != DALM_FP64 && precision != DALM_MIXED) return DALM_BAD_ARG;
 
     dalm_ctx* ctx = new dalm_ctx();

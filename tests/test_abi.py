@@ -38,4 +38,13 @@ def test_bad_arguments_rejected_without_gpu():
     x = np.zeros(8)
     # gc < 2 is refused before any device work (src/solver.h:615 asserts gc >= 2)
     assert L.dalm_create(ctypes.byref(h), size, 1, hip.ptr(x), hip.ptr(x), hip.ptr(x), 0) == hip.BAD_ARG
+    # fewer than 3 interior cells along an axis, a padded box past 32-bit cell
+    # indices (also where the product of the sizes would overflow 64 bits), an
+    # unknown precision: refused before any device work
+    for bad in ((6, 8, 8), (8, 8, 6), (2048, 2048, 512), (1 << 40, 1 << 40, 1 << 40), (8, 1 << 62, 8)):
+        size = (ctypes.c_int64 * 3)(*bad)
+        assert L.dalm_create(ctypes.byref(h), size, 2, hip.ptr(x), hip.ptr(x), hip.ptr(x), 0) == hip.BAD_ARG, bad
+    size = (ctypes.c_int64 * 3)(8, 8, 8)
+    assert L.dalm_create(ctypes.byref(h), size, 2, hip.ptr(x), hip.ptr(x), hip.ptr(x), 7) == hip.BAD_ARG
+    assert L.dalm_create(ctypes.byref(h), size, 1 << 62, hip.ptr(x), hip.ptr(x), hip.ptr(x), 0) == hip.BAD_ARG
     assert L.dalm_mesh_widths(hip.ptr(x), 2, hip.ptr(x)) == hip.BAD_ARG
