@@ -680,10 +680,12 @@ int dalm_alm_setup(dalm_ctx* ctx, const double* wt, int64_t wt_count, const doub
                    int64_t ap_count, int64_t blade_per_wt, int64_t ap_per_blade, const double* cd,
                    const double* cl, const double* atk, int64_t atk_count, double width) {
     CHECK_CTX();
-    if (wt_count < 0 || wt_count > DALM_MAX_TURBINES || ap_count < 0 ||
-        ap_count != wt_count * blade_per_wt * ap_per_blade || (ap_count > 0 && atk_count < 1) ||
-        blade_per_wt * ap_per_blade > 1024 ||
-        (ap_count > 0 && !(width > 0)))
+    // each factor is bounded before any product is formed
+    if (wt_count < 0 || wt_count > DALM_MAX_TURBINES || blade_per_wt < 1 || ap_per_blade < 1 ||
+        blade_per_wt > 1024 || ap_per_blade > 1024 || blade_per_wt * ap_per_blade > 1024 ||
+        ap_count != wt_count * blade_per_wt * ap_per_blade ||
+        (ap_count > 0 && (atk_count < 1 || atk_count > (1LL << 24) || !(width > 0) || !wt || !ap || !cd ||
+                          !cl || !atk)))
         return dalm_fail_msg(ctx, DALM_BAD_ARG, "invalid ALM set-up");
     AlmTables& A = ctx->alm;
     A.wt_count = (int)wt_count;
