@@ -1345,7 +1345,8 @@ int dalm_run_alm_adj(dalm_ctx* ctx, double t, const double* tq_w, double* ap_gra
 // ---- recorded run and reverse sweep --------------------------------------------------
 int dalm_adjoint_begin(dalm_ctx* ctx, int64_t steps, int64_t every, const double* tq_w, int64_t flags) {
     CHECK_ADJ();
-    if (steps < 1 || every < 1) return DALM_BAD_ARG;
+    // bounded so that the recording's byte counts below cannot overflow
+    if (steps < 1 || every < 1 || steps > (1LL << 24)) return DALM_BAD_ARG;
     if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
     AdjointState& S = ctx->adj;
     double** big[4] = {&S.ck, &S.ptape, &S.seg, &S.fin};

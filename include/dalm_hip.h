@@ -303,7 +303,7 @@ int dalm_run_alm_adj(dalm_ctx *ctx, double t, const double *tq_w, double *ap_gra
  * step, so the replay needs no Poisson solve).  dalm_adjoint_run then replays
  * each segment from its checkpoint and runs the reverse steps, and restores the
  * final state.  ap_grad [ap][4] as above; wt_grad [wt][2]: dJ/d(turbine angle
- * offset D), dJ/d(collective pitch). */
+ * offset D), dJ/d(collective pitch).  1 <= steps <= 2^24, every >= 1. */
 #define DALM_ADJ_KEEP_PRESSURE 1
 typedef struct {
     int64_t steps, every;  /* recorded steps and checkpoint interval */
