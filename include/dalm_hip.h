@@ -124,6 +124,8 @@ typedef struct {
  * src/io.h:98-164). */
 int dalm_create(dalm_ctx **out, const int64_t size[3], int64_t gc, const double *x,
                 const double *y, const double *z, int64_t precision);
+/* Free the context and its device memory: the `finalize` methods' `acc exit data`
+ * (src/solver.h:272-280, 328-341, 386). */
 int dalm_destroy(dalm_ctx *ctx);
 /* Human-readable text of the last failure on this context ("" if none). */
 const char *dalm_last_error(const dalm_ctx *ctx);
