@@ -151,6 +151,69 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dims d, PCoef A, const double*
                          });
 }
 
+// k_jacobi with four consecutive k per lane: the same per-cell arithmetic (OpD's
+// diagonal and operator, in its order), so the sweep is bit-identical; each vector
+// moves with two 16-byte accesses per lane and plane.
+typedef double d4k __attribute__((ext_vector_type(kVecK), aligned(8)));
+
+__global__ __launch_bounds__(BLOCK) void k_jacobi4(Dims d, PCoef A, const double* __restrict__ zin,
+                                                   double* __restrict__ zout, const double* __restrict__ v,
+                                                   const SolveState* st) {
+    if (finished(st)) return;
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    const double an = A.an[cl.j], as = A.as[cl.j], anu = A.anu[cl.j], asu = A.asu[cl.j];
+    double at[kVecK], ab[kVecK], atu[kVecK], abu[kVecK];
+#pragma unroll
+    for (int q = 0; q < kVecK; ++q) {
+        at[q] = A.at[cl.k0 + q];
+        ab[q] = A.ab[cl.k0 + q];
+        atu[q] = A.atu[cl.k0 + q];
+        abu[q] = A.abu[cl.k0 + q];
+    }
+    int c = d.at(cl.i0, cl.j, cl.k0);
+    d4k xm = *(const d4k*)(zin + c - d.si), x0 = *(const d4k*)(zin + c), xp = *(const d4k*)(zin + c + d.si);
+    d4k xn = *(const d4k*)(zin + c + d.sj), xs = *(const d4k*)(zin + c - d.sj), vc = *(const d4k*)(v + c);
+    double kb = zin[c - 1], kf = zin[c + kVecK];
+    auto plane = [&](int i, int cc) {
+        const double acx = A.acx[i], ae = A.ae[i], aw = A.aw[i];
+        d4k out;
+#pragma unroll
+        for (int q = 0; q < kVecK; ++q) {
+            const double ac = ((((acx - anu) - asu) - atu[q]) - abu[q]) / A.max_diag;
+            const double t = q + 1 < kVecK ? x0[q + 1] : kf;
+            const double b = q > 0 ? x0[q - 1] : kb;
+            const double ax = ac * x0[q] + ae * xp[q] + aw * xm[q] + an * xn[q] + as * xs[q] + at[q] * t + ab[q] * b;
+            out[q] = x0[q] + (vc[q] - ax) / ac;
+        }
+        *(d4k*)(zout + cc) = out;
+    };
+    // every value of plane i+1 is requested before plane i is computed
+    auto step = [&](int i, int nxt) {
+        const int cn = c + nxt;
+        const d4k xpp = *(const d4k*)(zin + cn + d.si);
+        const d4k xn1 = *(const d4k*)(zin + cn + d.sj), xs1 = *(const d4k*)(zin + cn - d.sj);
+        const d4k vc1 = *(const d4k*)(v + cn);
+        const double kb1 = zin[cn - 1], kf1 = zin[cn + kVecK];
+        if (i >= cl.own0) plane(i, c);
+        xm = x0;
+        x0 = xp;
+        xp = xpp;
+        xn = xn1;
+        xs = xs1;
+        vc = vc1;
+        kb = kb1;
+        kf = kf1;
+        c += d.si;
+    };
+    if (d.ni == kMaxPlanes) {
+#pragma unroll
+        for (int w = 0; w < kMaxPlanes; ++w) step(cl.i0 + w, w + 1 < kMaxPlanes ? d.si : 0);
+    } else {
+        for (int i = cl.i0; i < cl.i1; ++i) step(i, i + 1 < cl.i1 ? d.si : 0);
+    }
+}
+
 // y = A z and the partial sums of <a1, y> (and <y, a2> if a2): with which = 0 the
 // last block sets alpha = rho / <r0, q>; with which = 1 it sets
 // omega = <t, s> / <t, t> (src/eq.h:369-383).
@@ -347,6 +410,9 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     }
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
+    // rows of a multiple of 256 interior cells: the four-per-lane Jacobi sweep
+    const bool vec4 = d.nz % (TK * kVecK) == 0;
+    const int grid4 = d.tj * (d.nz / (TK * kVecK)) * d.nch;
     // preconditioner application for one vector; returns the buffer holding z
     auto precondition = [&](double* z0, double* spare, const double* v) -> double* {
         double* cur = z0;
@@ -355,8 +421,12 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             // the sweeps of one application run back to back: one event pair times them
             if (pc_it > 1) probe_begin(ctx, DALM_K_JACOBI);
             for (int m = 1; m < pc_it; ++m) {
-                hipLaunchKernelGGL(k_jacobi, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
-                                   other, v, (const SolveState*)ctx->solve);
+                if (vec4)
+                    hipLaunchKernelGGL(k_jacobi4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, cur, other, v,
+                                       (const SolveState*)ctx->solve);
+                else
+                    hipLaunchKernelGGL(k_jacobi, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, cur,
+                                       other, v, (const SolveState*)ctx->solve);
                 double* tsw = cur;
                 cur = other;
                 other = tsw;

@@ -93,7 +93,6 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float
 // and stores, which stream from HBM about 20 % faster than 4-byte ones at
 // 512x256x256; k-neighbours inside the four come from registers.  Identical
 // arithmetic per cell.  Launched over tj x (nz / 256) x nch units.
-constexpr int kVecK = 4;
 typedef float f4k __attribute__((ext_vector_type(kVecK), aligned(4)));
 
 __global__ __launch_bounds__(BLOCK) void kf_jacobi4(Dims d, PCoefF A, const float* __restrict__ zin,
@@ -164,28 +163,6 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi4(Dims d, PCoefF A, const floa
     } else {
         for (int i = i0; i < i1; ++i) step(i, i + 1 < i1 ? d.si : 0);
     }
-}
-
-// Unit of a four-per-lane kernel: a lane owns k0 .. k0+3 of row j over the unit's
-// planes [i0, i1), of which it writes [own0, i1).  ok is false on a ragged j row.
-struct Col4 {
-    int i0, i1, own0, j, k0;
-    bool ok;
-};
-__device__ __forceinline__ Col4 col_unit4(const Dims& d) {
-    const int tkv = d.nz / (TK * kVecK);
-    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
-    const int ic = u % d.nch, col = u / d.nch;
-    const int kt = col % tkv, jt = col / tkv;
-    Col4 r;
-    r.k0 = d.gc + (kt * TK + (int)(threadIdx.x % TK)) * kVecK;
-    r.j = d.gc + jt * TJ + (int)(threadIdx.x / TK);
-    r.own0 = d.gc + ic * d.ni;
-    r.i1 = r.own0 + d.ni;
-    if (r.i1 > d.gc + d.nx) r.i1 = d.gc + d.nx;
-    r.i0 = r.i1 - d.ni;
-    r.ok = r.j < d.gc + d.ny;
-    return r;
 }
 
 // kg_spmv with four cells per lane: y = A z in FP64 arithmetic (OpD's order), stored

@@ -125,3 +125,31 @@ __device__ __forceinline__ void walk_points(const Dims& d, const Col& cl, Load&&
         }
     }
 }
+
+// ---- four cells per lane (rows of a multiple of TK * kVecK interior cells) ----
+// A wavefront covers 256 consecutive k of one row; each vector moves with one
+// 16-byte (FP32) or two 16-byte (FP64) accesses per lane and plane.  Launch with
+// tj * (nz / (TK * kVecK)) * nch blocks.
+constexpr int kVecK = 4;
+
+// Unit of a four-per-lane kernel: a lane owns k0 .. k0+3 of row j over the unit's
+// planes [i0, i1), of which it writes [own0, i1).  ok is false on a ragged j row.
+struct Col4 {
+    int i0, i1, own0, j, k0;
+    bool ok;
+};
+__device__ __forceinline__ Col4 col_unit4(const Dims& d) {
+    const int tkv = d.nz / (TK * kVecK);
+    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
+    const int ic = u % d.nch, col = u / d.nch;
+    const int kt = col % tkv, jt = col / tkv;
+    Col4 r;
+    r.k0 = d.gc + (kt * TK + (int)(threadIdx.x % TK)) * kVecK;
+    r.j = d.gc + jt * TJ + (int)(threadIdx.x / TK);
+    r.own0 = d.gc + ic * d.ni;
+    r.i1 = r.own0 + d.ni;
+    if (r.i1 > d.gc + d.nx) r.i1 = d.gc + d.nx;
+    r.i0 = r.i1 - d.ni;
+    r.ok = r.j < d.gc + d.ny;
+    return r;
+}

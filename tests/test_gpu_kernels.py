@@ -125,6 +125,32 @@ def test_bicgstab(mesh, ok, pc):
     assert rel_l2(c.download("p")[m], x[m]) <= 1e-10
 
 
+def test_bicgstab_rows256_first_iterations(ok):
+    """Rows of 256 interior cells run the FP64 Jacobi sweeps with four cells per
+    lane.  Over the first two iterations the Krylov path has not yet amplified the
+    dot products' summation order, so the iterate must agree with the oracle's to
+    rounding (the sweeps themselves are bit-identical to the one-cell ones)."""
+    mesh = small_mesh(5, 6, 256, 1.2)
+    g = po.make_grid(mesh)
+    A = np.zeros((mesh.L, 7))
+    ok.build_A(g, A)
+    p0 = rand_field(mesh, 31, 1, 0.1)
+    b = rand_field(mesh, 32, 1, 1.0) * interior_mask(mesh)
+    c = ctx_with(mesh, p=p0, b=b)
+    x = p0.copy()
+    work = np.zeros(9 * mesh.L)
+    it, err = po.C.c_int64(), po.C.c_double()
+    po.lib("oracle")[0].or_run_bicgstab(po.C.byref(g), po.ptr(A), po.ptr(x), po.ptr(b), po.ptr(work),
+                                        hip.PC_JACOBI, 1.0, 3, po.C.byref(it), 2, po.C.byref(err), 1e-30)
+    git, gerr, st = c.run_bicgstab(hip.PC_JACOBI, 1.0, 3, 2, 1e-30)
+    print(f"bicgstab rows256, 2 iterations: err {gerr:.6e} vs {err.value:.6e}", flush=True)
+    assert git == it.value == 2 and st == hip.NONCONVERGED
+    assert abs(gerr - err.value) <= 1e-12 * err.value
+    m = interior_mask(mesh)
+    assert rel_l2(c.download("p")[m], x[m]) <= 1e-13
+    c.close()
+
+
 def test_pbc_project_ubc(mesh, ok):
     g = po.make_grid(mesh)
     p = rand_field(mesh, 31, 1, 0.2)
