@@ -298,6 +298,178 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr(Dims d, double* __restrict__ x,
     store_partials<2>(pv, partials);
 }
 
+// ---- the BiCGStab kernels with four cells per lane (rows of 256 interior cells):
+// the per-cell arithmetic of the one-cell kernels above, so every stored vector is
+// bit-identical; the dot products add the four cells of a lane in k order.
+__device__ __forceinline__ d4k ld4(const double* a, int c) { return *(const d4k*)(a + c); }
+
+// the diagonal of OpD for the four cells of a lane at plane i
+struct Diag4 {
+    double anu, asu, atu[kVecK], abu[kVecK];
+    __device__ __forceinline__ Diag4(const PCoef& A, const Col4& cl) : anu(A.anu[cl.j]), asu(A.asu[cl.j]) {
+#pragma unroll
+        for (int m = 0; m < kVecK; ++m) {
+            atu[m] = A.atu[cl.k0 + m];
+            abu[m] = A.abu[cl.k0 + m];
+        }
+    }
+    __device__ __forceinline__ double at(const PCoef& A, double acx, int m) const {
+        return ((((acx - anu) - asu) - atu[m]) - abu[m]) / A.max_diag;
+    }
+};
+
+__global__ __launch_bounds__(BLOCK) void k_bi_p4(Dims d, PCoef A, const double* __restrict__ r,
+                                                 double* __restrict__ p, const double* __restrict__ q,
+                                                 double* __restrict__ z, int pc_first, const SolveState* st) {
+    if (st->done) return;
+    const bool first = st->it == 0;
+    const double beta = st->beta, om = st->omega;
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    const Diag4 dg(A, cl);
+    struct RPQ {
+        d4k r, p, q;
+    };
+    walk_points4<RPQ>(d, cl, [&](int c) { return RPQ{ld4(r, c), ld4(p, c), ld4(q, c)}; },
+                      [&](int i, int c, const RPQ& v) {
+                          const double acx = A.acx[i];
+                          d4k pv, zv;
+#pragma unroll
+                          for (int m = 0; m < kVecK; ++m) {
+                              pv[m] = first ? v.r[m] : v.r[m] + beta * (v.p[m] - om * v.q[m]);
+                              zv[m] = pc_first ? pv[m] / dg.at(A, acx, m) : 0.0;
+                          }
+                          *(d4k*)(p + c) = pv;
+                          *(d4k*)(z + c) = zv;
+                      });
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bi_s4(Dims d, PCoef A, const double* __restrict__ r,
+                                                 const double* __restrict__ q, double* __restrict__ s,
+                                                 double* __restrict__ z, int pc_first, const SolveState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    const Diag4 dg(A, cl);
+    struct RQ {
+        d4k r, q;
+    };
+    walk_points4<RQ>(d, cl, [&](int c) { return RQ{ld4(r, c), ld4(q, c)}; },
+                     [&](int i, int c, const RQ& v) {
+                         const double acx = A.acx[i];
+                         d4k sv, zv;
+#pragma unroll
+                         for (int m = 0; m < kVecK; ++m) {
+                             sv[m] = v.r[m] - alpha * v.q[m];
+                             zv[m] = pc_first ? sv[m] / dg.at(A, acx, m) : 0.0;
+                         }
+                         *(d4k*)(s + c) = sv;
+                         *(d4k*)(z + c) = zv;
+                     });
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x, const double* __restrict__ pp,
+                                                  const double* __restrict__ ss, const double* __restrict__ s,
+                                                  const double* __restrict__ t_, double* __restrict__ r,
+                                                  const double* __restrict__ r0, double* partials,
+                                                  SolveState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha, om = st->omega;
+    const Col4 cl = col_unit4(d);
+    double v[2] = {0.0, 0.0};
+    if (cl.ok) {
+        struct XR {
+            d4k x, pp, ss, s, t, r0;
+        };
+        walk_points4<XR>(d, cl,
+                         [&](int c) { return XR{ld4(x, c), ld4(pp, c), ld4(ss, c), ld4(s, c), ld4(t_, c), ld4(r0, c)}; },
+                         [&](int, int c, const XR& a) {
+                             d4k xv, rv;
+#pragma unroll
+                             for (int m = 0; m < kVecK; ++m) {
+                                 xv[m] = a.x[m] + alpha * a.pp[m] + om * a.ss[m];
+                                 rv[m] = a.s[m] - om * a.t[m];
+                                 v[0] += rv[m] * rv[m];
+                                 v[1] += a.r0[m] * rv[m];
+                             }
+                             *(d4k*)(x + c) = xv;
+                             *(d4k*)(r + c) = rv;
+                         });
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    store_partials<2>(pv, partials);
+}
+
+template <int WHICH>
+__global__ __launch_bounds__(BLOCK) void k_bi_spmv4(Dims d, PCoef A, const double* __restrict__ z,
+                                                    double* __restrict__ y, const double* __restrict__ a1,
+                                                    double* partials, SolveState* st) {
+    if (st->done) return;
+    const Col4 cl = col_unit4(d);
+    double v[2] = {0.0, 0.0};
+    if (cl.ok) {
+        const Diag4 dg(A, cl);
+        const double an = A.an[cl.j], as = A.as[cl.j];
+        double at[kVecK], ab[kVecK];
+#pragma unroll
+        for (int m = 0; m < kVecK; ++m) {
+            at[m] = A.at[cl.k0 + m];
+            ab[m] = A.ab[cl.k0 + m];
+        }
+        int c = d.at(cl.i0, cl.j, cl.k0);
+        d4k xm = ld4(z, c - d.si), x0 = ld4(z, c), xp = ld4(z, c + d.si);
+        d4k xn = ld4(z, c + d.sj), xs = ld4(z, c - d.sj), av = ld4(a1, c);
+        double kb = z[c - 1], kf = z[c + kVecK];
+        auto plane = [&](int i, int cc) {
+            const double acx = A.acx[i], ae = A.ae[i], aw = A.aw[i];
+            d4k out;
+#pragma unroll
+            for (int m = 0; m < kVecK; ++m) {
+                const double ac = dg.at(A, acx, m);
+                const double t = m + 1 < kVecK ? x0[m + 1] : kf;
+                const double b = m > 0 ? x0[m - 1] : kb;
+                const double yv = ac * x0[m] + ae * xp[m] + aw * xm[m] + an * xn[m] + as * xs[m] + at[m] * t + ab[m] * b;
+                out[m] = yv;
+                if (WHICH == 0) {
+                    v[0] += av[m] * yv;  // <r0, q>
+                } else {
+                    v[0] += yv * av[m];  // <t, s>
+                    v[1] += yv * yv;     // <t, t>
+                }
+            }
+            *(d4k*)(y + cc) = out;
+        };
+        auto step = [&](int i, int nxt) {
+            const int cn = c + nxt;
+            const d4k xpp = ld4(z, cn + d.si);
+            const d4k xn1 = ld4(z, cn + d.sj), xs1 = ld4(z, cn - d.sj);
+            const d4k av1 = ld4(a1, cn);
+            const double kb1 = z[cn - 1], kf1 = z[cn + kVecK];
+            if (i >= cl.own0) plane(i, c);
+            xm = x0;
+            x0 = xp;
+            xp = xpp;
+            xn = xn1;
+            xs = xs1;
+            av = av1;
+            kb = kb1;
+            kf = kf1;
+            c += d.si;
+        };
+        if (d.ni == kMaxPlanes) {
+#pragma unroll
+            for (int w = 0; w < kMaxPlanes; ++w) step(cl.i0 + w, w + 1 < kMaxPlanes ? d.si : 0);
+        } else {
+            for (int i = cl.i0; i < cl.i1; ++i) step(i, i + 1 < cl.i1 ? d.si : 0);
+        }
+    }
+    block_sum<2>(v);
+    double pv[2] = {v[0], v[1]};
+    store_partials<2>(pv, partials);
+}
+
 __global__ void k_zero(long long n, double* __restrict__ a) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
          e += (long long)gridDim.x * blockDim.x)
@@ -410,9 +582,11 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     }
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
-    // rows of a multiple of 256 interior cells: the four-per-lane Jacobi sweep
+    // rows of a multiple of 256 interior cells: the four-per-lane kernels, whose
+    // reductions leave grid4 partials
     const bool vec4 = d.nz % (TK * kVecK) == 0;
     const int grid4 = d.tj * (d.nz / (TK * kVecK)) * d.nch;
+    const int nb = vec4 ? grid4 : ctx->sgrid;
     // preconditioner application for one vector; returns the buffer holding z
     auto precondition = [&](double* z0, double* spare, const double* v) -> double* {
         double* cur = z0;
@@ -451,30 +625,50 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
         for (long long c = 0; c < chunk; ++c) {
             probe_iteration(ctx, launched + c);
             probe_begin(ctx, DALM_K_PUPDATE);
-            hipLaunchKernelGGL(k_bi_p, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
-                               ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
+            if (vec4)
+                hipLaunchKernelGGL(k_bi_p4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp, ctx->q,
+                                   ctx->pp, pc_first, (const SolveState*)ctx->solve);
+            else
+                hipLaunchKernelGGL(k_bi_p, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp,
+                                   ctx->q, ctx->pp, pc_first, (const SolveState*)ctx->solve);
             probe_end(ctx, DALM_K_PUPDATE);
             double* ppz = precondition(ctx->pp, ctx->tmp, ctx->kp);
             probe_begin(ctx, DALM_K_SPMV);
-            hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
-                               ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+            if (vec4)
+                hipLaunchKernelGGL(k_bi_spmv4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ppz, ctx->q,
+                                   ctx->r0, ctx->partials, ctx->solve);
+            else
+                hipLaunchKernelGGL(k_bi_spmv<0>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ppz,
+                                   ctx->q, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
-            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, ctx->sgrid, ctx->solve, nullptr)) return rc1;
+            if (int rc1 = launch_finalize(ctx, FIN_ALPHA, nb, ctx->solve, nullptr)) return rc1;
             probe_begin(ctx, DALM_K_SUPDATE);
-            hipLaunchKernelGGL(k_bi_s, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
-                               ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
+            if (vec4)
+                hipLaunchKernelGGL(k_bi_s4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q, ctx->s,
+                                   ctx->ss, pc_first, (const SolveState*)ctx->solve);
+            else
+                hipLaunchKernelGGL(k_bi_s, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->q,
+                                   ctx->s, ctx->ss, pc_first, (const SolveState*)ctx->solve);
             probe_end(ctx, DALM_K_SUPDATE);
             double* ssz = precondition(ctx->ss, ctx->tmp2, ctx->s);
             probe_begin(ctx, DALM_K_SPMV);
-            hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
-                               ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
+            if (vec4)
+                hipLaunchKernelGGL(k_bi_spmv4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ssz, ctx->t,
+                                   ctx->s, ctx->partials, ctx->solve);
+            else
+                hipLaunchKernelGGL(k_bi_spmv<1>, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, A, ssz,
+                                   ctx->t, ctx->s, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_SPMV);
-            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, ctx->sgrid, ctx->solve, nullptr)) return rc2;
+            if (int rc2 = launch_finalize(ctx, FIN_OMEGA, nb, ctx->solve, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
-            hipLaunchKernelGGL(k_bi_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
-                               ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
+            if (vec4)
+                hipLaunchKernelGGL(k_bi_xr4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz, ctx->s,
+                                   ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->solve);
+            else
+                hipLaunchKernelGGL(k_bi_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
+                                   ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
             probe_end(ctx, DALM_K_UPDATE);
-            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, ctx->sgrid, ctx->solve, nullptr)) return rc3;
+            if (int rc3 = launch_finalize(ctx, FIN_BI_XR, nb, ctx->solve, nullptr)) return rc3;
         }
         LAUNCH_CHECK();
         launched += chunk;

@@ -153,3 +153,24 @@ __device__ __forceinline__ Col4 col_unit4(const Dims& d) {
     r.ok = r.j < d.gc + d.ny;
     return r;
 }
+
+// Pointwise walk of a four-per-lane unit: load(c) returns the vectors' values at
+// c .. c+3, f(i, c, values) processes plane i for the owned planes; plane i+1 is
+// requested before plane i is processed.
+template <typename Aux, typename Load, typename F>
+__device__ __forceinline__ void walk_points4(const Dims& d, const Col4& cl, Load&& load, F&& f) {
+    int c = d.at(cl.i0, cl.j, cl.k0);
+    Aux a0 = load(c);
+    auto step = [&](int i, int nxt) {
+        const Aux a1 = load(c + nxt);
+        if (i >= cl.own0) f(i, c, a0);
+        a0 = a1;
+        c += d.si;
+    };
+    if (d.ni == kMaxPlanes) {
+#pragma unroll
+        for (int w = 0; w < kMaxPlanes; ++w) step(cl.i0 + w, w + 1 < kMaxPlanes ? d.si : 0);
+    } else {
+        for (int i = cl.i0; i < cl.i1; ++i) step(i, i + 1 < cl.i1 ? d.si : 0);
+    }
+}

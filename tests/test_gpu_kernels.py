@@ -126,10 +126,12 @@ def test_bicgstab(mesh, ok, pc):
 
 
 def test_bicgstab_rows256_first_iterations(ok):
-    """Rows of 256 interior cells run the FP64 Jacobi sweeps with four cells per
-    lane.  Over the first two iterations the Krylov path has not yet amplified the
-    dot products' summation order, so the iterate must agree with the oracle's to
-    rounding (the sweeps themselves are bit-identical to the one-cell ones)."""
+    """Rows of 256 interior cells run the FP64 BiCGStab kernels (p and s updates
+    with the first Jacobi sweep, Jacobi sweeps, SpMV with its dot products, x/r
+    update with its norms) with four cells per lane.  Over the first two iterations
+    the Krylov path has not yet amplified the dot products' summation order, so the
+    iterate must agree with the oracle's to rounding (every stored vector is
+    bit-identical to the one-cell kernels' for the same scalars)."""
     mesh = small_mesh(5, 6, 256, 1.2)
     g = po.make_grid(mesh)
     A = np.zeros((mesh.L, 7))

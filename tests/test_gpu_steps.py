@@ -102,13 +102,13 @@ def test_c2_small_bicgstab_tight(pc):
 
 def test_rows256_bicgstab_tight():
     """Rows of 256 interior cells (32x16x256 on a 4x2x8 box, one rotor): the FP64
-    BiCGStab runs its Jacobi sweeps with four cells per lane there, with the same
-    per-cell arithmetic, so the bar is the one-cell kernels' (solve to 1e-15) for
+    BiCGStab runs all its iteration kernels with four cells per lane there, with the
+    same per-cell arithmetic, so the bar is the one-cell kernels' (solve to 1e-15) for
     every field but p.  Along 256 stretched cells the operator is worse conditioned
     than in the 32x16x16 case: two solves to 1e-15 whose dot products are summed in
     different orders leave p 2.6e-10 apart (measured on MI355X), so p is held to
-    1e-9.  The sweep itself is pinned exactly by the first-iterations test in
-    test_gpu_kernels.py."""
+    1e-9.  The kernels themselves are pinned to rounding by the first-iterations
+    test in test_gpu_kernels.py."""
     m = cases.make_mesh(32, 16, 256, ((0.0, 4.0), (-1.0, 1.0), (0.0, 8.0)), 1.2, 1.2)
     c = cases.Case("rows256", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-15, max_it=5000, steps=3,
                    turbine=cases.default_turbine(R=0.5, tower=4.0), sites=[cases.TurbineSite((1.0, 0.0, 0.0), 12.0)],
