@@ -1190,13 +1190,15 @@ double now_ms() {
 }  // namespace
 
 // record one finished forward step (called by dalm_step while taping)
-int adjoint_record(dalm_ctx* ctx) {
+int adjoint_record(dalm_ctx* ctx, int status) {
     AdjointState& S = ctx->adj;
     const long long n = ctx->step - S.start_step;
-    if (n < 1 || n > S.steps) {
+    if (n < 1 || n > S.steps || n != S.recorded + 1) {
         S.taping = false;
-        return dalm_fail_msg(ctx, DALM_BAD_ARG, "more steps than the recording was set up for");
+        S.tape_bad = true;
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "recorded steps out of sequence or beyond the recording's length");
     }
+    if (status == DALM_NONFINITE) S.tape_bad = true;
     const size_t L = (size_t)ctx->L;
     if (S.keep_p)
         HIP_TRY(hipMemcpyAsync(S.ptape + (size_t)(n - 1) * L, ctx->p, L * sizeof(double), hipMemcpyDeviceToDevice,
@@ -1204,8 +1206,8 @@ int adjoint_record(dalm_ctx* ctx) {
     if (n % S.every == 0)
         if (int rc = save_state(ctx, S.ck + (size_t)(n / S.every) * CK_FIELDS * L)) return rc;
     for (int w = 0; w < ctx->alm.wt_count; ++w) S.J += S.h_tqw[w] * ctx->h_scal->torque[w];
-    S.recorded = n;
-    if (n == S.steps) S.taping = false;
+    S.recorded += 1;
+    if (S.recorded == S.steps) S.taping = false;
     return DALM_OK;
 }
 
@@ -1361,6 +1363,7 @@ int dalm_adjoint_begin(dalm_ctx* ctx, int64_t steps, int64_t every, const double
     S.keep_p = (flags & DALM_ADJ_KEEP_PRESSURE) != 0;
     S.start_step = ctx->step;
     S.recorded = 0;
+    S.tape_bad = false;
     S.J = 0;
     S.h_tqw.assign(DALM_MAX_TURBINES, 0.0);
     for (int q = 0; q < ctx->alm.wt_count && tq_w; ++q) S.h_tqw[q] = tq_w[q];
@@ -1391,6 +1394,9 @@ int dalm_adjoint_run(dalm_ctx* ctx, double tol_rel, double* J, double* ap_grad, 
     if (!S.ck || S.recorded != S.steps)
         return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_adjoint_run needs a complete recording (dalm_adjoint_begin "
                                                 "then that many dalm_step calls)");
+    if (S.tape_bad)
+        return dalm_fail_msg(ctx, DALM_NONFINITE, "the recording holds a step that did not finish or left a "
+                                                  "non-finite state; it cannot be differentiated");
     const size_t L = (size_t)ctx->L;
     AlmTables& A = ctx->alm;
     const long long end_step = ctx->step;
@@ -1419,7 +1425,7 @@ int dalm_adjoint_run(dalm_ctx* ctx, double tol_rel, double* J, double* ap_grad, 
         for (long long n = s0 + 1; n <= s1; ++n) {
             const double* pg = S.keep_p ? S.ptape + (size_t)(n - 1) * L : nullptr;
             int rc = step_fp64(ctx, nullptr, pg);
-            if (rc && rc != DALM_NONCONVERGED) return rc;
+            if (rc && rc != DALM_NONCONVERGED && rc != DALM_BREAKDOWN) return rc;
             TRY(save_slot(ctx, n - s0));
         }
         HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -140,6 +140,9 @@ struct AdjointState {
     double *tqw = nullptr;    // [wt] objective weights on the torques
     // recorded forward run
     bool taping = false;
+    // a step of the recording failed (non-finite state or a HIP error): the tape
+    // cannot be replayed and dalm_adjoint_run refuses it
+    bool tape_bad = false;
     long long steps = 0, every = 0, recorded = 0, start_step = 0;
     bool keep_p = false;
     double* ck = nullptr;     // checkpoints [steps/every + 1][11][L]: U, JU, p, nut, f
@@ -196,6 +199,10 @@ struct dalm_ctx {
     MetricF mf{};
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
+    double* pbest = nullptr;      // best pressure seen by the running solve (its own buffer:
+                                  // the FP64 continuation uses tmp / tmp2 as Jacobi spares)
+    long long fp32_pass_cap = 0;  // testing aid (dalm_set_fp32_pass_cap): FP32 passes end
+                                  // after this many iterations as if stalled; 0 = off
     long long outer_it = 0;       // FP64 true-residual passes of the last mixed solve
     long long fp64_finish_it = 0; // iterations of its FP64 finish (0: FP32 preconditioning sufficed)
 
@@ -240,8 +247,16 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP, FIN_MX_SOR };
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP, FIN_MX_SOR, FIN_L2 };
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
+// the finalize on explicit partials (FIN_SUM / FIN_L2 only: no solve or step state)
+int launch_finalize_raw(hipStream_t stream, int mode, int nb, const double* partials, double* out);
+// partial sums of v^2 over the interior, one per block (k_scalar_sum)
+int launch_sumsq_raw(hipStream_t stream, const Dims& d, int grid, const double* v, double* partials);
+// blocks of the cell kernels (the tile loop) for a grid of ntiles tiles
+int cell_grid(int ntiles);
+// interior extents, tiles and strides of a padded box (no column units)
+Dims make_dims(const int64_t size[3], int64_t gc);
 
 int dalm_fail(dalm_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
 int dalm_fail_msg(dalm_ctx* ctx, int status, const std::string& msg);
@@ -281,7 +296,8 @@ int alm_forces(dalm_ctx* ctx, double t, double* const U[3]);
 // one FP64 step; with p_given the Poisson solve is replaced by copying p_given
 int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given);
 // adjoint.hip: record a forward step on the tape, free the adjoint memory
-int adjoint_record(dalm_ctx* ctx);
+// status: the step's dalm_status (OK, BREAKDOWN or NONFINITE)
+int adjoint_record(dalm_ctx* ctx, int status);
 void adjoint_free(dalm_ctx* ctx);
 int ensure_adjoint_buffers(dalm_ctx* ctx);
 int sync_stream(dalm_ctx* ctx);

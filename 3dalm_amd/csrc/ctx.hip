@@ -304,6 +304,28 @@ int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given) {
     return status == DALM_NONCONVERGED ? DALM_OK : status;
 }
 
+Dims make_dims(const int64_t size[3], int64_t gc) {
+    Dims d{};
+    d.n0 = (int)size[0];
+    d.n1 = (int)size[1];
+    d.n2 = (int)size[2];
+    d.gc = (int)gc;
+    d.nx = d.n0 - 2 * d.gc;
+    d.ny = d.n1 - 2 * d.gc;
+    d.nz = d.n2 - 2 * d.gc;
+    d.si = d.n1 * d.n2;
+    d.sj = d.n2;
+    d.tk = (d.nz + TK - 1) / TK;
+    d.tj = (d.ny + TJ - 1) / TJ;
+    d.ntiles = d.nx * d.tj * d.tk;
+    return d;
+}
+
+int cell_grid(int ntiles) {
+    const int g = ntiles < 2048 ? ntiles : 2048;
+    return g >= 64 ? (g / 8) * 8 : g;
+}
+
 extern "C" {
 
 int dalm_set_device(int64_t device) {
@@ -348,24 +370,12 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     if (precision != DALM_FP64 && precision != DALM_MIXED) return DALM_BAD_ARG;
 
     dalm_ctx* ctx = new dalm_ctx();
+    ctx->d = make_dims(size, gc);
     Dims& d = ctx->d;
-    d.n0 = (int)size[0];
-    d.n1 = (int)size[1];
-    d.n2 = (int)size[2];
-    d.gc = (int)gc;
-    d.nx = d.n0 - 2 * d.gc;
-    d.ny = d.n1 - 2 * d.gc;
-    d.nz = d.n2 - 2 * d.gc;
-    d.si = d.n1 * d.n2;
-    d.sj = d.n2;
-    d.tk = (d.nz + TK - 1) / TK;
-    d.tj = (d.ny + TJ - 1) / TJ;
-    d.ntiles = d.nx * d.tj * d.tk;
     ctx->L = L;
     ctx->N = (long long)d.nx * d.ny * d.nz;
     ctx->precision = (int)precision;
-    int g = d.ntiles < 2048 ? d.ntiles : 2048;
-    ctx->grid = g >= 64 ? (g / 8) * 8 : g;
+    ctx->grid = cell_grid(d.ntiles);
     // column units: about 2048 of them (more on large grids), 1 to kMaxPlanes planes
     // each -- small grids get short columns rather than too few workgroups
     {
@@ -431,10 +441,11 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
                            &ctx->ft,    &ctx->fz, &ctx->fz2, &ctx->ftmp, &ctx->ftmp2};
         for (int q = 0; q < 11; ++q)
             if ((rc = dev_alloc(ctx, scf[q], L))) return fail(rc);
-        // FP64 vectors for the FP64 finish of a stalled solve (the FP64 solver's set;
-        // tmp2 also keeps the pressure at the start of the solve)
-        double** sd[9] = {&ctx->kp, &ctx->q, &ctx->s, &ctx->t, &ctx->r0, &ctx->pp, &ctx->ss, &ctx->tmp, &ctx->tmp2};
-        for (int q = 0; q < 9; ++q)
+        // FP64 vectors for the FP64 finish of a stalled solve (the FP64 solver's set),
+        // and the best pressure seen by the running solve
+        double** sd[10] = {&ctx->kp, &ctx->q,  &ctx->s,   &ctx->t,    &ctx->r0,
+                           &ctx->pp, &ctx->ss, &ctx->tmp, &ctx->tmp2, &ctx->pbest};
+        for (int q = 0; q < 10; ++q)
             if ((rc = dev_alloc(ctx, sd[q], L))) return fail(rc);
     } else {
         double** sc[10] = {&ctx->nut, &ctx->r0, &ctx->kp, &ctx->pp, &ctx->q, &ctx->s, &ctx->ss, &ctx->t,
@@ -465,7 +476,7 @@ int dalm_destroy(dalm_ctx* ctx) {
     for (hipEvent_t e : ctx->probe.ev) (void)hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
-                      ctx->t, ctx->tmp, ctx->tmp2, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
+                      ctx->t, ctx->tmp, ctx->tmp2, ctx->pbest, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
                       ctx->pc.an_n, ctx->pc.as_n, ctx->pc.at_n, ctx->pc.ab_n, ctx->pc.acx,
                       ctx->pc.an_u, ctx->pc.as_u, ctx->pc.at_u, ctx->pc.ab_u, ctx->alm.wt, ctx->alm.ap,
                       ctx->alm.cd, ctx->alm.cl, ctx->alm.atk};
@@ -806,9 +817,42 @@ int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
         ctx->step += 1;
         return step_mixed(ctx, ctx->prm.dt * (double)ctx->step, stats);
     }
-    int rc = step_fp64(ctx, stats, nullptr);
-    if (rc) return rc;
-    if (ctx->adj.taping) return adjoint_record(ctx);
+    const int rc = step_fp64(ctx, stats, nullptr);
+    if (ctx->adj.taping) {
+        // a BiCGStab breakdown ends the solve and the step goes on, as in the
+        // reference (src/eq.h:349-353): the step is recorded like any other.  A
+        // non-finite step is recorded as unusable; a HIP error leaves the step
+        // unfinished, and the recording with it.
+        if (rc == DALM_OK || rc == DALM_BREAKDOWN || rc == DALM_NONFINITE) {
+            TRY(adjoint_record(ctx, rc));
+        } else {
+            ctx->adj.tape_bad = true;
+            ctx->adj.taping = false;
+        }
+    }
+    return rc;
+}
+
+int dalm_last_solve_info(dalm_ctx* ctx, int64_t out[3]) {
+    CHECK_CTX();
+    if (!out) return DALM_BAD_ARG;
+    const long long it = ctx->last.it;
+    if (ctx->precision == DALM_MIXED) {
+        out[0] = it - ctx->fp64_finish_it;
+        out[1] = ctx->fp64_finish_it;
+        out[2] = ctx->outer_it;
+    } else {
+        out[0] = 0;
+        out[1] = it;
+        out[2] = 0;
+    }
+    return DALM_OK;
+}
+
+int dalm_set_fp32_pass_cap(dalm_ctx* ctx, int64_t n) {
+    CHECK_CTX();
+    if (n < 0) return DALM_BAD_ARG;
+    ctx->fp32_pass_cap = n;
     return DALM_OK;
 }
 
@@ -833,7 +877,11 @@ int dalm_probe_read(dalm_ctx* ctx, int64_t* launches, double* total_ms) {
 int dalm_run(dalm_ctx* ctx, int64_t n, dalm_step_stats* stats) {
     CHECK_CTX();
     dalm_step_stats st{};
-    for (int64_t s = 0; s < n; ++s) TRY(dalm_step(ctx, &st));
+    // a breakdown ends that step's solve and the run goes on (src/eq.h:349-353)
+    for (int64_t s = 0; s < n; ++s) {
+        const int rc = dalm_step(ctx, &st);
+        if (rc && rc != DALM_BREAKDOWN) return rc;
+    }
     if (stats) *stats = st;
     return DALM_OK;
 }

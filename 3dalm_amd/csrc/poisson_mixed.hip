@@ -647,6 +647,8 @@ int run_sor_mixed(dalm_ctx* ctx, double omega, long long max_it, double tol, lon
     const PCoefF Af = ctx->pcf;
     float* r = ctx->fr;
     float* e = ctx->fz;
+    ctx->outer_it = 0;
+    ctx->fp64_finish_it = 0;
     hipLaunchKernelGGL(ks_state_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->solve, tol, max_it,
                        sqrt((double)ctx->N), (double)kSorInner);
     LAUNCH_CHECK();
@@ -689,11 +691,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     const int jac = pc_kind == DALM_PC_JACOBI;
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     static const bool trace = getenv("DALM_TRACE") != nullptr;
-    // testing aid: DALM_TEST_FP32_ITERATIONS=n ends every FP32 pass after n
+    // testing aid (dalm_set_fp32_pass_cap): every FP32 pass ends after this many
     // iterations as if it had stalled, so that small cases exercise the restarted
-    // passes and the FP64 hand-over (read per solve)
-    const char* cap_env = getenv("DALM_TEST_FP32_ITERATIONS");
-    const long long test_cap = cap_env ? atoll(cap_env) : 0;
+    // passes and the FP64 hand-over; 0 in production
+    const long long test_cap = ctx->fp32_pass_cap;
     const double root_n = sqrt((double)ctx->N);
     // rows of a multiple of 256 interior cells: the four-per-lane Jacobi sweep
     const bool vec4 = d.nz % (TK * kVecK) == 0;
@@ -727,9 +728,9 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     };
     ctx->outer_it = 0;
     ctx->fp64_finish_it = 0;
-    // tmp2 keeps the best pressure seen (by the running residual estimate): an FP32
+    // pbest keeps the best pressure seen (by the running residual estimate): an FP32
     // pass that wanders off after a good stretch is restarted from it
-    HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->pbest, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
     long long total = 0, n_ru = 0;
     double prev_pass = -1, e_out = 0, best_saved = -1;
     int status = DALM_NONCONVERGED;
@@ -750,10 +751,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         if (pass > 0 && e0 > kRestoreRise * best_saved) {
             // the last pass ended well above its best: restart from the best pressure
             // if it is really better (the saved value was a running estimate)
-            std::swap(ctx->p, ctx->tmp2);
+            std::swap(ctx->p, ctx->pbest);
             if ((rc = restart())) return rc;
             if (!(ctx->h_solve[0].err < e0)) {
-                std::swap(ctx->p, ctx->tmp2);
+                std::swap(ctx->p, ctx->pbest);
                 if ((rc = restart())) return rc;
             } else if (trace) {
                 fprintf(stderr, "[mixed] restored the best pressure (%.3e instead of %.3e)\n", ctx->h_solve[0].err, e0);
@@ -847,7 +848,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             const double e = h.err;
             if (e < best) best = e;
             if (e < 0.5 * best_saved) {
-                HIP_TRY(hipMemcpyAsync(ctx->tmp2, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
+                HIP_TRY(hipMemcpyAsync(ctx->pbest, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
                                        ctx->stream));
                 best_saved = e;
             }
@@ -916,11 +917,11 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         // then the FP64 BiCGStab from there
         if ((rc = restart())) return rc;
         const double end_err = ctx->h_solve[0].err;
-        std::swap(ctx->p, ctx->tmp2);
+        std::swap(ctx->p, ctx->pbest);
         if ((rc = restart())) return rc;
         double from = ctx->h_solve[0].err;
         if (!(from < end_err)) {
-            std::swap(ctx->p, ctx->tmp2);
+            std::swap(ctx->p, ctx->pbest);
             from = end_err;
         }
         long long it64 = 0;

@@ -125,6 +125,8 @@ def lib():
         "dalm_run": [V, i64, C.POINTER(StepStats)],
         "dalm_probe_kernel": [V, i64],
         "dalm_probe_read": [V, PI64, P],
+        "dalm_set_fp32_pass_cap": [V, i64],
+        "dalm_last_solve_info": [V, PI64],
         "dalm_predictor_adj": [V, dbl, dbl],
         "dalm_interpolate_JU_adj": [V],
         "dalm_apply_JUbc_adj": [V],
@@ -349,6 +351,16 @@ class Context:
         st = StepStats()
         self.check(self.L.dalm_run(self.h, n, C.byref(st)), "run")
         return st.as_dict()
+
+    def last_solve_info(self):
+        """(FP32 iterations, FP64 iterations, residual replacements) of the last step's solve."""
+        out = np.zeros(3, dtype=np.int64)
+        self.check(self.L.dalm_last_solve_info(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "last_solve_info")
+        return int(out[0]), int(out[1]), int(out[2])
+
+    def set_fp32_pass_cap(self, n):
+        """Testing aid of the precision form: FP32 passes end after n iterations."""
+        self.check(self.L.dalm_set_fp32_pass_cap(self.h, int(n)), "set_fp32_pass_cap")
 
     # ---- discrete adjoint (bars are the *_b fields)
     def predictor_adj(self, Re, dt):

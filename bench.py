@@ -1,43 +1,51 @@
 """Benchmark of the 3dalm per-time-step hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed]
-                    [--grid 256,128,128] [--no-cpu] [--probe jacobi]
-                    [--adjoint-steps 20] [--adjoint-every 10] [--no-adjoint] [--no-extras]
-                    [--no-target] [--tol 1e-8]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed] [--tol 4e-9]
+                    [--no-cpu] [--cpu-steps 1] [--probe auto]
+                    [--adjoint-steps 200] [--adjoint-every 20] [--no-adjoint] [--no-extras]
 
-Workload (one "step" = one Solver::main_loop_once): the single-turbine ALM case on
-256x128x128 cells (box 16x8x8, y/z tanh beta=1.5, rotor D=1 at x=3, 3 blades x 64
-actuator points, TSR 6, eps = 2 dx, 5% seeded noise, dt = 0.005), Poisson by
-BiCGStab with 3 Jacobi sweeps as preconditioner to 1e-8.  Synthetic inputs,
-resident in HBM before the timed region.  The run is 1000 steps long; by default
-steps 11-110 are timed (the first steps after the impulsive start need two to
-three times the Poisson iterations of the rest and are warm-up).
+Headline workload (one "step" = one Solver::main_loop_once): 512x256x256 cells on a
+16x8x8 box (y/z tanh beta=1.5), two 3-blade rotors (D = 1) 5 D apart at x = 3 and
+8 with 3 x 64 actuator points each, TSR 6, eps = 2 dx, a seeded isotropic turbulence
+of 200 random Fourier modes at 10 % intensity, dt = 0.005, Poisson by BiCGStab with
+3 Jacobi sweeps as preconditioner to 4e-9 (the tolerance that keeps avg div <= 1e-6
+at this size).  Synthetic inputs, resident in HBM before the timed region.  W
+untimed warm-up steps from the impulsive start, then steps W+1 .. W+K timed, in
+each form:
 
-Forms: "fp64" is the FP64 drop-in; "mixed" is the precision form (FP32 stencils,
-FP32 BiCGStab vectors with FP64 dot products, norms, pressure and residual
-replacements, to the same true-residual test).
+  fp64   the FP64 drop-in;
+  mixed  the precision form (FP32 stencils, FP32 BiCGStab vectors with FP64 dot
+         products, norms, pressure and residual replacements, to the same
+         true-residual test).
 
-Prints ONE JSON line (rank 0).  value = Mcell-steps/s = N_interior * K / t / 1e6
-of the precision form when it is built, else of the FP64 drop-in; every form is
-reported under "forms".  roofline: the dominant kernel's algorithmic bytes per
-launch over its mean launch time, timed with HIP events on the library's stream.
-cpu_baseline: the reference's own code compiled from its headers (oracle/_ref) when
-that library is present and loads, else the C restatement in oracle/ (one core),
-on a bounded sample of this workload (see "sample").
-target_size: both forms at 512x256x256 (the size the precision form's targets are
-stated at; two rotors 5 D apart, an initial turbulence of 200 seeded Fourier modes at
-10 % intensity), steps 11-110, with the Poisson tolerance 4e-9 that keeps avg div <= 1e-6
-there; ratio, iterations, whole-step bandwidth and the turbine-power difference.
-forms.<form>.step_roofline: the whole step's algorithmic bytes (fixed kernels + bytes
-per BiCGStab iteration x iterations per step) over the time per step.
-output_path: mean device time and achieved bandwidth of the output-path kernels
-(Q-criterion, time average of U) at the bench grid, both forms.  sor_forms: the
-64x48x48 SOR case (no turbine) in both forms.
+Prints ONE JSON line.  value = Mcell-steps/s = N_interior * K / t / 1e6 of the
+precision form; forms holds both (ms/step, iterations per step and how many of them
+ran in FP64, avg div, turbine power, the whole step's algorithmic bandwidth);
+mixed_vs_fp64 is their ratio and mixed_errors_vs_fp64 the relative L2 distance of
+the precision form's U, p and div from the FP64 drop-in's after the last timed step,
+with the turbine-power difference.
+
+roofline: the dominant kernel of the precision form, its FP32 Jacobi sweep
+(kf_jacobi4; roofline_fp64: the FP64 k_jacobi4): algorithmic bytes per launch
+(12 / 24 B per interior cell x N) over the mean launch duration, timed with HIP
+events on the library's stream over the timed steps (each event pair brackets the
+two back-to-back sweeps of one preconditioner application).  traffic: HBM bytes per
+launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
+(profiles/hbm_traffic_alm_512x256x256.json).
+
+cpu_baseline: the oracle's OpenMP build on every host core, full steps of the
+headline workload continuing from the FP64 GPU run's final state.
+
 adjoint_forward_ratio: on the differentiation-form workload (192x96x96, three
 rotors, FP64, BiCGStab(Jacobi 3) to 1e-8), the wall time per step of the reverse
 sweep (segment replays from checkpoints plus reverse steps, adjoint solves to 1e-8
 relative) over the wall time per step of the recorded forward run of the same
-steps (which includes keeping the pressure of every step).
+steps; 200 steps with checkpoints every 20 by default.
+
+c2_256x128x128: the single-rotor 256x128x128 workload (tolerance 1e-8) in both
+forms.  output_path: mean device time and achieved bandwidth of the output-path
+kernels (Q-criterion, time average of U) at 256x128x128, both forms.  sor_forms:
+the 64x48x48 SOR case (no turbine) in both forms.
 """
 from __future__ import annotations
 
@@ -54,13 +62,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-steps/s mixed vs FP64 at div<=1e-6; adjoint/forward step-cost ratio"
+# the headline workload: the size the precision form's targets are stated at, with the
+# Poisson tolerance that keeps avg div <= 1e-6 there (avg div = dt * max_diag * err and
+# dt * max_diag = 230 at 512x256x256, so err <= 4.35e-9)
+HEADLINE_GRID, HEADLINE_TOL = (512, 256, 256), 4e-9
 HBM_PEAK_GBS = 8000.0
 # per-kernel HBM traffic of this workload from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
 # (3dalm_amd/profile_summary.py traffic --json; the command is in the .md beside it)
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_alm_256x128x128.json")
-PROBE_KERNEL = {"jacobi": "k_jacobi", "jacobi32": "kf_jacobi", "spmv": "k_bi_spmv<0>", "update": "k_bi_xr",
-                "pupdate": "k_bi_p", "supdate": "k_bi_s", "predictor": "k_predictor", "spmv_mixed": "kg_spmv<0>",
-                "update_mixed": "kg_xr", "pupdate_mixed": "kg_p", "supdate_mixed": "kg_s"}
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_alm_512x256x256.json")
+# the kernels behind each probe on rows of a multiple of 256 interior cells (the
+# headline grid), where the iteration kernels run four cells per lane
+PROBE_KERNEL = {"jacobi": "k_jacobi4", "jacobi32": "kf_jacobi4", "spmv": "k_bi_spmv4<0>", "update": "k_bi_xr4",
+                "pupdate": "k_bi_p4", "supdate": "k_bi_s4", "predictor": "k_predictor",
+                "spmv_mixed": "kg_spmv4<0>", "update_mixed": "kg_xr4", "pupdate_mixed": "kg_p4",
+                "supdate_mixed": "kg_s4"}
 
 # algorithmic bytes per interior cell for one launch of each probed kernel (FP64):
 # compulsory reads + writes, with the operator applied matrix-free from 1-D coefficients
@@ -96,6 +111,7 @@ def log(msg):
 
 
 def make_case(grid, steps, tol=1e-8):
+    """The single-rotor workload (256x128x128 by default)."""
     cases = importlib.import_module("3dalm_amd.cases")
     nx, ny, nz = grid
     c = cases.case_c2(nx, ny, nz, steps=steps)
@@ -103,7 +119,16 @@ def make_case(grid, steps, tol=1e-8):
     return c
 
 
-def run_form(case, precision, steps, warmup, probe, form):
+def make_headline_case(steps, tol):
+    cases = importlib.import_module("3dalm_amd.cases")
+    return cases.case_c3(*HEADLINE_GRID, steps=steps, tol=tol)
+
+
+def run_form(case, precision, steps, warmup, probe, form, keep_fields=False):
+    """warmup untimed steps, then `steps` timed ones.  With `probe`, every launch of
+    that kernel inside the timed region is timed with HIP events on the library's
+    stream (dalm_probe_kernel).  With keep_fields, U, p and div after the last step
+    are returned (for the comparison of the forms)."""
     solver = importlib.import_module("3dalm_amd.solver")
     import ctypes as C
     log(f"[bench] form={form} grid={case.mesh.interior} warmup={warmup} steps={steps}")
@@ -112,32 +137,33 @@ def run_form(case, precision, steps, warmup, probe, form):
         for w in range(warmup):
             st = s.step()
             log(f"[bench] warmup step {st['step']} it={st['it']} err={st['err']:.3e} div={st['avg_div']:.3e}")
+        if probe:
+            s.ctx.check(s.ctx.L.dalm_probe_kernel(s.ctx.h, PROBE_IDS[probe]), "probe")
         sync()
         t0 = time.perf_counter()
-        its, divs, power = [], [], []
+        its, it64, divs, power = [], [], [], []
         for k in range(steps):
             st = s.step()
             its.append(st["it"])
+            it64.append(s.ctx.last_solve_info()[1])
             divs.append(st["avg_div"])
             if "power" in st:
                 power.append(float(st["power"].sum()))
         sync()
         t1 = time.perf_counter()
-        log(f"[bench] timed {steps} steps in {t1 - t0:.3f} s; it/step mean {np.mean(its):.1f}; "
-            f"last div {divs[-1]:.3e}")
-        res = {"seconds": t1 - t0, "it_mean": float(np.mean(its)), "it": its,
-               "avg_div_max": float(np.max(divs)), "power_last": power[-1] if power else None}
+        log(f"[bench] timed {steps} steps in {t1 - t0:.3f} s; it/step mean {np.mean(its):.1f} "
+            f"(FP64 {np.mean(it64):.1f}); last div {divs[-1]:.3e}")
+        res = {"seconds": t1 - t0, "it_mean": float(np.mean(its)), "it64_mean": float(np.mean(it64)), "it": its,
+               "avg_div_max": float(np.max(divs)), "avg_div_last": float(divs[-1]),
+               "power_last": power[-1] if power else None,
+               "window": [warmup + 1, warmup + steps]}
         if probe:
-            # kernel durations with HIP events on the library's stream, over a few
-            # more steps (launched one by one while the probe is on)
-            s.ctx.check(s.ctx.L.dalm_probe_kernel(s.ctx.h, PROBE_IDS[probe]), "probe")
-            for k in range(3):
-                s.step()
             n, ms = C.c_int64(), C.c_double()
             s.ctx.check(s.ctx.L.dalm_probe_read(s.ctx.h, C.byref(n), C.byref(ms)), "probe_read")
             s.ctx.check(s.ctx.L.dalm_probe_kernel(s.ctx.h, 0), "probe off")
             res["probe"] = {"kernel": probe, "launches": n.value, "total_ms": ms.value}
-        res["U"] = None
+        if keep_fields:
+            res["fields"] = {name: s.field(name) for name in ("U", "p", "div")}
         return res, s
     except Exception:
         s.close()
@@ -176,62 +202,48 @@ class _StdoutToStderr:
         os.close(self.saved)
 
 
-def cpu_baseline(case, gpu_it_mean):
+def cpu_baseline(case, state, steps, fp64_res):
     with _StdoutToStderr():
-        return _cpu_baseline(case, gpu_it_mean)
+        return _cpu_baseline(case, state, steps, fp64_res)
 
 
-def _cpu_baseline(case, gpu_it_mean):
-    """Reference (or restatement) on the host: one initialisation + one step of this
-    workload with the Poisson solve capped, then a separately timed capped solve;
-    a full step at the GPU run's mean iteration count is extrapolated from the fixed
-    and per-iteration costs."""
+def _omp_threads():
+    import ctypes
+    try:
+        return int(ctypes.CDLL("libgomp.so.1").omp_get_max_threads())
+    except OSError:
+        return os.cpu_count() or 1
+
+
+def _cpu_baseline(case, state, steps, fp64_res):
+    """The oracle's OpenMP build (the C restatement of the reference's step with its
+    cell loops shared among threads) on every host core: `steps` full steps of the
+    headline workload, continuing from the FP64 GPU run's state after its timed
+    steps, so the CPU integrates the next step(s) of the same run, at the same point
+    of the flow, with the same BiCGStab.  The reference's Gaussian projection is
+    dense over all cells and actuator points (src/alm.h:416-448), as here."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     po = importlib.import_module("pyoracle")
-    kind = "port"
-    try:
-        if po.ref_available():
-            po.lib("ref")
-            kind = "reference"
-    except OSError as e:
-        log(f"[bench] reference build does not load here ({e}); timing the restatement")
-        kind = "port"
-    cap = 4
-    case.max_it = cap
-    case.tol = 0.0
+    cores = _omp_threads()
     t0 = time.perf_counter()
-    S = po.RefSolver(case) if kind == "reference" else po.OracleSolver(case)
-    t_init = time.perf_counter() - t0
+    S = po.OracleSolver(case, kind="omp", U0=state["U"], initialize=False)
+    S.load_state({k: state[k] for k in ("Uold", "JU", "p", "nut", "f")}, state["step"])
+    t_setup = time.perf_counter() - t0
+    its = []
     t0 = time.perf_counter()
-    S.step()
-    t_step_cap = time.perf_counter() - t0
-    # per-iteration cost: a capped solve on the same data
-    K = po.Kernels("ref" if kind == "reference" else "oracle")
+    for _ in range(steps):
+        st = S.step()
+        its.append(st["it"])
+        log(f"[bench] cpu step {st['step']}: it {st['it']} err {st['err']:.3e} ({time.perf_counter() - t0:.1f} s)")
+    t = time.perf_counter() - t0
     m = case.mesh
-    g = po.make_grid(m)
-    A = np.zeros((m.L, 7))
-    K.build_A(g, A)
-    x = np.zeros(m.L)
-    b = np.ascontiguousarray(S.field("b"))
-    work = np.zeros(9 * m.L)
-    it, err = po.C.c_int64(), po.C.c_double()
-    fn = po.lib(kind if kind == "oracle" else "ref")[0]
-    pref = "ref_" if kind == "reference" else "or_"
-    t0 = time.perf_counter()
-    getattr(fn, pref + "run_bicgstab")(po.C.byref(g), po.ptr(A), po.ptr(x), po.ptr(b), po.ptr(work), 0, 1.0,
-                                       case.pc_it, po.C.byref(it), cap, po.C.byref(err), 0.0)
-    t_solve = time.perf_counter() - t0
-    t_iter = t_solve / max(it.value, 1)
-    t_fixed = max(t_step_cap - cap * t_iter, 0.0)
-    t_step = t_fixed + gpu_it_mean * t_iter
-    if hasattr(S, "close"):
-        S.close()
-    return {"value": m.N / t_step / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": kind,
-            "sample": (f"{m.interior[0]}x{m.interior[1]}x{m.interior[2]}: init {t_init:.1f}s + 1 step with "
-                       f"the solve capped at {cap} it ({t_step_cap:.1f}s) + a {it.value}-it solve "
-                       f"({t_solve:.1f}s); step cost extrapolated to {gpu_it_mean:.1f} it/step "
-                       f"(fixed {t_fixed:.2f}s + {t_iter:.3f}s/it)"),
-            "seconds_per_step": t_step}
+    return {"value": m.N * steps / t / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": (f"{steps} full step(s) of the headline workload ({m.interior[0]}x{m.interior[1]}x"
+                       f"{m.interior[2]}, step {state['step'] + 1}, continuing from the FP64 GPU run's state) "
+                       f"on {cores} OpenMP threads: {t:.1f} s, {np.mean(its):.0f} BiCGStab iterations per step "
+                       f"(the GPU's FP64 form: {fp64_res['it_mean']:.0f} over its timed steps); set-up "
+                       f"{t_setup:.1f} s untimed"),
+            "seconds_per_step": t / steps, "it_per_step": float(np.mean(its))}
 
 
 def run_adjoint(steps, every, warmup=2):
@@ -315,38 +327,22 @@ def run_output_path(grid, reps=20):
     return out
 
 
-# the size the precision form's targets are stated at, with the Poisson tolerance
-# that keeps avg div <= 1e-6 there: avg div = dt * max_diag * err and dt * max_diag =
-# 230 at 512x256x256 (err <= 4.35e-9), where 1e-8 would leave 2.3e-6 in both forms
-TARGET_GRID, TARGET_TOL = (512, 256, 256), 4e-9
-
-
-def run_target_size(steps=100, warmup=10):
-    """Both forms at the target size (two rotors 5 D apart, an initial turbulence of
-    200 seeded Fourier modes at 10 % intensity):
-    Mcell-steps/s, iterations, avg div, the whole-step algorithmic bandwidth and
-    the turbine power after the last step."""
+def run_c2_forms(steps=100, warmup=10):
+    """The single-rotor workload at 256x128x128 (BiCGStab(Jacobi 3) to 1e-8) in both
+    forms: Mcell-steps/s, iterations per step, avg div."""
     hip = importlib.import_module("3dalm_amd.hip")
-    cases = importlib.import_module("3dalm_amd.cases")
     res = {}
     for form, prec in (("fp64", hip.FP64), ("mixed", hip.MIXED)):
-        case = cases.case_c3(*TARGET_GRID, steps=warmup + steps, tol=TARGET_TOL)
-        r, s = run_form(case, prec, steps, warmup, None, f"target-{form}")
+        case = make_case((256, 128, 128), warmup + steps)
+        r, s = run_form(case, prec, steps, warmup, None, f"c2-{form}")
         s.close()
-        N = case.mesh.N
-        b = STEP_BYTES[form]
-        ms = 1e3 * r["seconds"] / steps
-        bw = (b["fixed"] + b["per_solve"] + b["per_it"] * r["it_mean"]) * N / (ms * 1e-3) / 1e9
-        res[form] = {"mcell_steps_s": round(N * steps / r["seconds"] / 1e6, 3), "ms_per_step": round(ms, 2),
-                     "it_per_step": round(r["it_mean"], 1), "avg_div_max": r["avg_div_max"],
-                     "power_last": r["power_last"], "step_bandwidth_GBs": round(bw, 1),
-                     "step_frac": round(bw / HBM_PEAK_GBS, 4)}
+        res[form] = {"mcell_steps_s": round(case.mesh.N * steps / r["seconds"] / 1e6, 3),
+                     "ms_per_step": round(1e3 * r["seconds"] / steps, 3), "it_per_step": round(r["it_mean"], 1),
+                     "fp64_it_per_step": round(r["it64_mean"], 1), "avg_div_max": r["avg_div_max"]}
     res["mixed_vs_fp64"] = round(res["mixed"]["mcell_steps_s"] / res["fp64"]["mcell_steps_s"], 3)
-    p64, pm = res["fp64"]["power_last"], res["mixed"]["power_last"]
-    res["power_rel_diff_last"] = abs(pm - p64) / abs(p64) if p64 else None
-    res["workload"] = (f"{TARGET_GRID[0]}x{TARGET_GRID[1]}x{TARGET_GRID[2]}, two rotors 5 D apart, 200-mode "
-                       f"Fourier turbulence at 10 %, BiCGStab(Jacobi x3) to {TARGET_TOL:g}, steps {warmup + 1}-{warmup + steps}")
-    log(f"[bench] target size: {res}")
+    res["workload"] = (f"256x128x128, one rotor (3 x 64 actuator points), 5 % uniform noise, BiCGStab(Jacobi x3) "
+                       f"to 1e-8, steps {warmup + 1}-{warmup + steps}")
+    log(f"[bench] c2: {res}")
     return res
 
 
@@ -367,24 +363,37 @@ def run_sor_forms(steps=5, warmup=2):
     return res
 
 
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def interior(mesh, a):
+    nx, ny, nz = mesh.size
+    g = mesh.gc
+    return a.reshape((nx, ny, nz) + a.shape[1:])[g:nx - g, g:ny - g, g:nz - g]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--forms", default="fp64,mixed")
-    ap.add_argument("--grid", default="256,128,128")
-    ap.add_argument("--tol", type=float, default=1e-8, help="Poisson tolerance (the workload's is 1e-8)")
+    ap.add_argument("--tol", type=float, default=HEADLINE_TOL,
+                    help="Poisson tolerance (4e-9 keeps avg div <= 1e-6 at 512x256x256)")
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--adjoint-steps", type=int, default=20)
-    ap.add_argument("--adjoint-every", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="full steps of the CPU baseline")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the output-path and SOR legs")
-    ap.add_argument("--no-target", action="store_true", help="skip the 512x256x256 leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the 256x128x128, output-path and SOR legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="first copy MIB MiB device to device (a kernel of known traffic for scaling "
-                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+                    help="first copy MIB MiB device to device and sum 4 x MIB MiB (kernels of known traffic "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the whole domain lives on one MI355X and there is "
@@ -401,9 +410,12 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"[bench] calibration copy of {a.copy_calibration} MiB done")
-        del src, dst
-    grid = tuple(int(v) for v in a.grid.split(","))
+        big = torch.ones(4 * n, dtype=torch.float64, device="cuda")
+        tot = float(big.sum())
+        torch.cuda.synchronize()
+        log(f"[bench] calibration: copy of {a.copy_calibration} MiB, sum over {4 * a.copy_calibration} MiB "
+            f"({tot:.0f})")
+        del src, dst, big
     forms = [f for f in a.forms.split(",") if f]
     unknown = [f for f in forms if f not in FORMS]
     if unknown:
@@ -414,16 +426,21 @@ def main():
         log("[bench] precision form not built in this library; measuring the FP64 drop-in only")
         forms = [f for f in forms if f != "mixed"]
     results = {}
-    N = None
+    fp64_state = None
+    N = mesh = None
     for form in forms:
-        case = make_case(grid, a.steps, a.tol)
-        N = case.mesh.N
+        case = make_headline_case(a.warmup + a.steps, a.tol)
+        N, mesh = case.mesh.N, case.mesh
         if a.probe == "auto":
             probe = "jacobi" if form == "fp64" else "jacobi32"
         else:
             probe = a.probe if a.probe != "none" else None
         prec = hip.FP64 if form == "fp64" else hip.MIXED
-        res, s = run_form(case, prec, a.steps, a.warmup, probe, form)
+        res, s = run_form(case, prec, a.steps, a.warmup, probe, form, keep_fields=len(forms) > 1)
+        if form == "fp64" and not a.no_cpu:
+            # the state after the timed steps: the CPU baseline continues from it
+            fp64_state = {name: s.field(name) for name in ("U", "Uold", "p", "JU", "nut", "f")}
+            fp64_state["step"] = s.stats["step"]
         s.close()
         res["mcell_steps_s"] = N * a.steps / res["seconds"] / 1e6
         res["ms_per_step"] = 1e3 * res["seconds"] / a.steps
@@ -447,53 +464,65 @@ def main():
         except (OSError, ValueError, KeyError):
             pass
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": pr["kernel"],
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": PROBE_KERNEL[pr["kernel"]],
                 "launches": pr["launches"], "mean_us": round(mean_s * 1e6, 2),
-                "bytes_per_launch": bytes_per_launch,
+                "bytes_per_launch": bytes_per_launch, "timing": "HIP events on the library's stream, timed steps",
                 "traffic_source": os.path.relpath(TRAFFIC_JSON, ROOT) if traffic else None}
     roof = roofline(R)
     roof_fp64 = roofline(results["fp64"]) if "fp64" in results and main_form != "fp64" else None
+    errors = None
+    if "fp64" in results and "mixed" in results:
+        f64, fm = results["fp64"].pop("fields"), results["mixed"].pop("fields")
+        errors = {"U_rel_l2": rel_l2(interior(mesh, fm["U"]), interior(mesh, f64["U"])),
+                  "p_rel_l2": rel_l2(interior(mesh, fm["p"]), interior(mesh, f64["p"])),
+                  "div_rel_l2": rel_l2(interior(mesh, fm["div"]), interior(mesh, f64["div"])),
+                  "at_step": a.warmup + a.steps}
+        p64, pm = results["fp64"]["power_last"], results["mixed"]["power_last"]
+        errors["power_rel"] = abs(pm - p64) / abs(p64) if p64 else None
+        del f64, fm
+        log(f"[bench] precision form against FP64: {errors}")
+    for r in results.values():
+        r.pop("fields", None)
     adj = None
     if not a.no_adjoint:
         try:
             adj = run_adjoint(a.adjoint_steps, a.adjoint_every)
         except Exception as e:
             log(f"[bench] adjoint leg failed: {e}")
-    output_path = sor = None
+    c2 = output_path = sor = None
     if not a.no_extras:
         try:
-            output_path = run_output_path(grid)
+            c2 = run_c2_forms(min(a.steps, 100), a.warmup)
+            output_path = run_output_path((256, 128, 128))
             sor = run_sor_forms()
         except Exception as e:
-            log(f"[bench] output-path / SOR legs failed: {e}")
-    target = None
-    if not a.no_target:
-        try:
-            target = run_target_size()
-        except Exception as e:
-            log(f"[bench] target-size leg failed: {e}")
+            log(f"[bench] 256x128x128 / output-path / SOR legs failed: {e}")
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and fp64_state is not None:
         try:
-            # the reference solves in FP64: extrapolate with the FP64 form's iterations
-            it_ref = results["fp64"]["it_mean"] if "fp64" in results else R["it_mean"]
-            cpu = cpu_baseline(make_case(grid, 1), it_ref)
+            cpu = cpu_baseline(make_headline_case(a.warmup + a.steps + a.cpu_steps, a.tol), fp64_state,
+                               a.cpu_steps, results["fp64"])
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
+
     def step_roofline(form, r):
         """whole-step algorithmic bytes (fixed + per iteration x iterations) over the
         time per step (the ALM projection's box and the precision form's residual
-        replacements are left out: a lower bound)"""
+        replacements are left out; its FP64 iterations are counted at FP64 bytes)"""
         b = STEP_BYTES[form]
-        nbytes = (b["fixed"] + b["per_solve"] + b["per_it"] * r["it_mean"]) * N
+        if form == "mixed":
+            per_it = (b["per_it"] * (r["it_mean"] - r["it64_mean"]) + STEP_BYTES["fp64"]["per_it"] * r["it64_mean"])
+        else:
+            per_it = b["per_it"] * r["it_mean"]
+        nbytes = (b["fixed"] + b["per_solve"] + per_it) * N
         ach = nbytes / (r["ms_per_step"] * 1e-3) / 1e9
         return {"bytes_per_step": round(nbytes), "achieved": round(ach, 1), "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4)}
 
     forms_out = {f: {"mcell_steps_s": round(r["mcell_steps_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
-                     "it_per_step": round(r["it_mean"], 2), "avg_div_max": r["avg_div_max"],
-                     "div_within_1e-6": bool(r["avg_div_max"] <= 1e-6),
-                     "power_last": r["power_last"], "step_roofline": step_roofline(f, r)}
+                     "it_per_step": round(r["it_mean"], 2), "fp64_it_per_step": round(r["it64_mean"], 2),
+                     "avg_div_max": r["avg_div_max"], "div_within_1e-6": bool(r["avg_div_max"] <= 1e-6),
+                     "power_last": r["power_last"], "window": r["window"], "step_roofline": step_roofline(f, r)}
                  for f, r in results.items()}
 
     line = {
@@ -509,19 +538,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if main_form == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"single 3-blade turbine ALM, {grid[0]}x{grid[1]}x{grid[2]} cells, "
-                                f"BiCGStab(Jacobi x3) to {a.tol:g}, form={main_form}"),
-                   "cells": N, "turbines": 1, "actuator_points": 192, "poisson": "BiCGStab+Jacobi3",
+        "config": {"workload": (f"two 3-blade rotors 5 D apart (ALM, 2 x 3 x 64 actuator points), "
+                                f"{HEADLINE_GRID[0]}x{HEADLINE_GRID[1]}x{HEADLINE_GRID[2]} cells, 200-mode Fourier "
+                                f"turbulence at 10 %, BiCGStab(Jacobi x3) to {a.tol:g}, steps "
+                                f"{a.warmup + 1}-{a.warmup + a.steps}, form={main_form}"),
+                   "cells": N, "turbines": 2, "actuator_points": 384, "poisson": "BiCGStab+Jacobi3",
                    "tol": a.tol},
         "forms": forms_out,
         "mixed_vs_fp64": (round(results["mixed"]["mcell_steps_s"] / results["fp64"]["mcell_steps_s"], 3)
                           if "mixed" in results and "fp64" in results else None),
+        "mixed_errors_vs_fp64": errors,
         "adjoint_forward_ratio": round(adj["ratio"], 3) if adj else None,
         "adjoint": adj,
         "roofline": roof,
         "roofline_fp64": roof_fp64,
         "cpu_baseline": cpu,
-        "target_size": target,
+        "c2_256x128x128": c2,
         "output_path": output_path,
         "sor_forms": sor,
     }

@@ -193,6 +193,7 @@ int dalm_l2_norm(dalm_ctx *ctx, int64_t field, double *out);
 /* calc_max_cfl (src/cfd.h:1109-1141) */
 int dalm_max_cfl(dalm_ctx *ctx, double dt, double *out);
 
+
 /* ---- output path (diagnostics) ---------------------------------------------------- */
 
 /* calc_q (src/cfd.h:948-1077): Q-criterion of U over the interior into the field
@@ -259,6 +260,15 @@ enum dalm_kernel {
 int dalm_probe_kernel(dalm_ctx *ctx, int64_t kernel);
 /* Launch count and summed device time (ms) of the probed kernel so far. */
 int dalm_probe_read(dalm_ctx *ctx, int64_t *launches, double *total_ms);
+/* Testing aid of the precision form: every FP32 pass of the Poisson solve ends
+ * after n iterations as if it had stalled (0, the default, switches it off), so
+ * that small cases exercise the restarted passes and the FP64 hand-over.  The
+ * result still meets the reference's test on the true FP64 residual. */
+int dalm_set_fp32_pass_cap(dalm_ctx *ctx, int64_t n);
+/* The last step's Poisson solve: out[0] iterations run on FP32 vectors, out[1]
+ * iterations run in FP64 (the precision form's FP64 continuation or finish; every
+ * iteration in an FP64 context), out[2] residual replacements (precision form). */
+int dalm_last_solve_info(dalm_ctx *ctx, int64_t out[3]);
 
 /* ---- whole step ----------------------------------------------------------------- */
 
@@ -267,9 +277,11 @@ int dalm_probe_read(dalm_ctx *ctx, int64_t *launches, double *total_ms);
  * apply_JUbc, calc_nut, calc_divergence, run_alm(t = 0), avg_div, max_cfl. */
 int dalm_initialize(dalm_ctx *ctx, dalm_step_stats *stats);
 /* One Solver::main_loop_once (src/solver.h:1034-1190): advances the step counter
- * and integrates it (first call = step 1, t = dt). */
+ * and integrates it (first call = step 1, t = dt).  DALM_BREAKDOWN and
+ * DALM_NONFINITE report a finished step whose solve ended that way; the
+ * reference continues after a breakdown (src/eq.h:349-353). */
 int dalm_step(dalm_ctx *ctx, dalm_step_stats *stats);
-/* n steps back to back; stats of the last one. */
+/* n steps back to back, continuing after a breakdown; stats of the last one. */
 int dalm_run(dalm_ctx *ctx, int64_t n, dalm_step_stats *stats);
 
 /* ---- differentiation form: discrete adjoint of the FP64 step --------------------

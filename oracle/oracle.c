@@ -33,6 +33,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 
 
@@ -82,7 +85,7 @@ void or_predictor(const or_grid *g, double *U, const double *Uold, const double 
     const double *X = g->c[0], *Y = g->c[1], *Z = g->c[2];
     const double *DX = g->d[0], *DY = g->d[1], *DZ = g->d[2];
     const i64 si = g->n[1] * g->n[2], sj = g->n[2];
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         i64 c = lin(g, i, j, k);
         double J[6] = {JU[3 * c + 0], JU[3 * (c - si) + 0], JU[3 * c + 1],
                        JU[3 * (c - sj) + 1], JU[3 * c + 2], JU[3 * (c - 1) + 2]};
@@ -109,6 +112,7 @@ void or_predictor(const or_grid *g, double *U, const double *Uold, const double 
 void or_interpolate_JU(const or_grid *g, const double *U, double *JU) {
     const double *DX = g->d[0], *DY = g->d[1], *DZ = g->d[2];
     const i64 si = g->n[1] * g->n[2], sj = g->n[2];
+    OMP_FOR
     for (i64 i = g->gc - 1; i < g->n[0] - g->gc; ++i)
         for (i64 j = g->gc - 1; j < g->n[1] - g->gc; ++j)
             for (i64 k = g->gc - 1; k < g->n[2] - g->gc; ++k) {
@@ -150,12 +154,12 @@ static inline double flux_div(const or_grid *g, const double *JU, i64 i, i64 j, 
 
 /* Poisson right-hand side: src/cfd.h:569-598 */
 void or_poisson_rhs(const or_grid *g, const double *JU, double *b, double dt, double scale) {
-    INTERIOR(g, i, j, k) b[lin(g, i, j, k)] = flux_div(g, JU, i, j, k) / (dt * scale);
+    PAR_INTERIOR(g, i, j, k) b[lin(g, i, j, k)] = flux_div(g, JU, i, j, k) / (dt * scale);
 }
 
 /* divergence: src/cfd.h:1079-1107 */
 void or_divergence(const or_grid *g, const double *JU, double *div) {
-    INTERIOR(g, i, j, k) div[lin(g, i, j, k)] = flux_div(g, JU, i, j, k);
+    PAR_INTERIOR(g, i, j, k) div[lin(g, i, j, k)] = flux_div(g, JU, i, j, k);
 }
 
 /* ========================================================================== */
@@ -165,7 +169,7 @@ double or_build_A(const or_grid *g, double *A) {
     const double *DX = g->d[0], *DY = g->d[1], *DZ = g->d[2];
     const i64 G = g->gc;
     double mx = 0;
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR_MAX(g, i, j, k) {
         double *a = A + 7 * lin(g, i, j, k);
         double e = 0, w = 0, nn = 0, s = 0, t = 0, bo = 0, c = 0;
         if (i < g->n[0] - G - 1) {
@@ -182,25 +186,49 @@ double or_build_A(const or_grid *g, double *A) {
         a[0] = c; a[1] = e; a[2] = w; a[3] = nn; a[4] = s; a[5] = t; a[6] = bo;
         if (fabs(c) > mx) mx = fabs(c);
     }
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         double *a = A + 7 * lin(g, i, j, k);
         for (int m = 0; m < 7; ++m) a[m] /= mx;
     }
     return mx;
 }
 
-/* src/mv.h:8-31 */
-double or_dot(const or_grid *g, const double *a, const double *b) {
+/* sum of a[c] * b[c] over the interior.  Serial build: the reference's loop order.
+ * OpenMP build: each thread sums its static share of the i planes in that order,
+ * and the per-thread sums are added in thread order, so a given thread count always
+ * gives the same result (an OpenMP reduction clause adds them in arrival order). */
+static double interior_dot(const or_grid *g, const double *a, const double *b) {
+#ifdef _OPENMP
+    enum { MAX_THREADS = 512 };
+    double part[MAX_THREADS];
+    int nt = omp_get_max_threads();
+    if (nt > MAX_THREADS) nt = MAX_THREADS;
+#pragma omp parallel num_threads(nt)
+    {
+        double s = 0;
+#pragma omp for schedule(static)
+        for (i64 i = g->gc; i < g->n[0] - g->gc; ++i)
+            for (i64 j = g->gc; j < g->n[1] - g->gc; ++j)
+                for (i64 k = g->gc; k < g->n[2] - g->gc; ++k) {
+                    i64 c = lin(g, i, j, k);
+                    s += a[c] * b[c];
+                }
+        part[omp_get_thread_num()] = s;
+    }
+    double s = 0;
+    for (int t = 0; t < nt; ++t) s += part[t];
+    return s;
+#else
     double s = 0;
     INTERIOR(g, i, j, k) { i64 c = lin(g, i, j, k); s += a[c] * b[c]; }
     return s;
+#endif
 }
-/* src/mv.h:33-55 */
-double or_l2_norm(const or_grid *g, const double *v) {
-    double s = 0;
-    INTERIOR(g, i, j, k) s += sq(v[lin(g, i, j, k)]);
-    return sqrt(s);
-}
+
+/* src/mv.h:8-31 */
+double or_dot(const or_grid *g, const double *a, const double *b) { return interior_dot(g, a, b); }
+/* src/mv.h:33-55 (sq(v) = v * v) */
+double or_l2_norm(const or_grid *g, const double *v) { return sqrt(interior_dot(g, v, v)); }
 
 /* 7-point product at one cell in the reference's summation order (src/mv.h:87) */
 static inline double stencil7(const or_grid *g, const double *A, const double *x, i64 c) {
@@ -212,16 +240,16 @@ static inline double stencil7(const or_grid *g, const double *A, const double *x
 
 /* src/mv.h:57-182 */
 void or_residual(const or_grid *g, const double *A, const double *x, const double *b, double *r) {
-    INTERIOR(g, i, j, k) { i64 c = lin(g, i, j, k); r[c] = b[c] - stencil7(g, A, x, c); }
+    PAR_INTERIOR(g, i, j, k) { i64 c = lin(g, i, j, k); r[c] = b[c] - stencil7(g, A, x, c); }
 }
 /* src/mv.h:184-309 */
 void or_Ax(const or_grid *g, const double *A, const double *x, double *y) {
-    INTERIOR(g, i, j, k) { i64 c = lin(g, i, j, k); y[c] = stencil7(g, A, x, c); }
+    PAR_INTERIOR(g, i, j, k) { i64 c = lin(g, i, j, k); y[c] = stencil7(g, A, x, c); }
 }
 /* src/eq.h:145-289 (offset = 0) */
 void or_sor_sweep(const or_grid *g, const double *A, double *x, const double *b,
                   double omega, i64 color) {
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         if ((i + j + k) % 2 != color) continue;
         i64 c = lin(g, i, j, k);
         double rel = (b[c] - stencil7(g, A, x, c)) / A[7 * c];
@@ -231,7 +259,7 @@ void or_sor_sweep(const or_grid *g, const double *A, double *x, const double *b,
 /* src/eq.h:6-143 */
 void or_jacobi_sweep(const or_grid *g, const double *A, double *x, const double *xold,
                      const double *b) {
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         i64 c = lin(g, i, j, k);
         double rel = (b[c] - stencil7(g, A, xold, c)) / A[7 * c];
         x[c] = xold[c] + rel;
@@ -257,10 +285,12 @@ void or_run_sor(const or_grid *g, const double *A, double *x, const double *b, d
  *   Jacobi: src/eq.h:320-331; SOR: src/eq.h:308-318 */
 static void apply_pc(const or_grid *g, const double *A, double *z, const double *v,
                      double *tmp, i64 kind, double pc_omega, i64 pc_it, i64 L) {
-    memset(z, 0, sizeof(double) * (size_t)L);
+    OMP_FOR
+    for (i64 c = 0; c < L; ++c) z[c] = 0;
     for (i64 m = 0; m < pc_it; ++m) {
         if (kind == 0) {
-            memcpy(tmp, z, sizeof(double) * (size_t)L);
+            OMP_FOR
+            for (i64 c = 0; c < L; ++c) tmp[c] = z[c];
             or_jacobi_sweep(g, A, z, tmp, v);
         } else {
             or_sor_sweep(g, A, z, v, pc_omega, 0);
@@ -293,16 +323,20 @@ void or_run_bicgstab(const or_grid *g, const double *A, double *x, const double 
             memcpy(p, r, sizeof(double) * (size_t)L);
         } else {
             beta = (rho / rho_prev) * (alpha / omega);
+            OMP_FOR
             for (i64 c = 0; c < L; ++c) p[c] = r[c] + beta * (p[c] - omega * q[c]);
         }
         apply_pc(g, A, pp, p, tmp, pc_kind, pc_omega, pc_it, L);
         or_Ax(g, A, pp, q);
         alpha = rho / or_dot(g, r0, q);
+        OMP_FOR
         for (i64 c = 0; c < L; ++c) s[c] = r[c] - alpha * q[c];
         apply_pc(g, A, ss, s, tmp, pc_kind, pc_omega, pc_it, L);
         or_Ax(g, A, ss, t);
         omega = or_dot(g, t, s) / or_dot(g, t, t);
+        OMP_FOR
         for (i64 c = 0; c < L; ++c) x[c] = x[c] + alpha * pp[c] + omega * ss[c];
+        OMP_FOR
         for (i64 c = 0; c < L; ++c) r[c] = s[c] - omega * t[c];
         rho_prev = rho;
         *err = or_l2_norm(g, r) / rootN;
@@ -339,12 +373,13 @@ void or_project(const or_grid *g, double *U, double *JU, const double *p, double
     const double *X = g->c[0], *Y = g->c[1], *Z = g->c[2];
     const double *DX = g->d[0], *DY = g->d[1], *DZ = g->d[2];
     const i64 si = g->n[1] * g->n[2], sj = g->n[2];
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         i64 c = lin(g, i, j, k);
         U[3 * c + 0] -= dt * (p[c + si] - p[c - si]) / (X[i + 1] - X[i - 1]);
         U[3 * c + 1] -= dt * (p[c + sj] - p[c - sj]) / (Y[j + 1] - Y[j - 1]);
         U[3 * c + 2] -= dt * (p[c + 1] - p[c - 1]) / (Z[k + 1] - Z[k - 1]);
     }
+    OMP_FOR
     for (i64 i = g->gc - 1; i < g->n[0] - g->gc; ++i)
         for (i64 j = g->gc - 1; j < g->n[1] - g->gc; ++j)
             for (i64 k = g->gc - 1; k < g->n[2] - g->gc; ++k) {
@@ -360,7 +395,7 @@ void or_project(const or_grid *g, double *U, double *JU, const double *p, double
 
 /* src/cfd.h:1143-1160 */
 void or_set_solid(const or_grid *g, double *U, const double *solid) {
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         i64 c = lin(g, i, j, k);
         for (int m = 0; m < 3; ++m) U[3 * c + m] = (1. - solid[c]) * U[3 * c + m];
     }
@@ -435,7 +470,7 @@ void or_nut(const or_grid *g, const double *U, double *nut, double Cs) {
     const double *X = g->c[0], *Y = g->c[1], *Z = g->c[2];
     const double *DX = g->d[0], *DY = g->d[1], *DZ = g->d[2];
     const i64 si = g->n[1] * g->n[2], sj = g->n[2];
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         i64 c = lin(g, i, j, k);
         double hx = X[i + 1] - X[i - 1], hy = Y[j + 1] - Y[j - 1], hz = Z[k + 1] - Z[k - 1];
         double vol = DX[i] * DY[j] * DZ[k];
@@ -505,7 +540,7 @@ void or_time_average(const or_grid *g, double *Utavg, const double *U, double we
 /* src/cfd.h:1109-1141 */
 double or_max_cfl(const or_grid *g, const double *U, double dt) {
     double mx = 0;
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR_MAX(g, i, j, k) {
         i64 c = lin(g, i, j, k);
         double v = fabs(U[3 * c + 0]) * dt / g->d[0][i] + fabs(U[3 * c + 1]) * dt / g->d[1][j] +
                    fabs(U[3 * c + 2]) * dt / g->d[2][k];
@@ -658,7 +693,7 @@ void or_calc_ap_force(const or_grid *g, const double *U, double *wt, i64 wt_coun
 void or_distribute_ap_force(const or_grid *g, double *f, const double *ap, i64 ap_count,
                             double width) {
     const double *X = g->c[0], *Y = g->c[1], *Z = g->c[2];
-    INTERIOR(g, i, j, k) {
+    PAR_INTERIOR(g, i, j, k) {
         i64 c = lin(g, i, j, k);
         double sx = 0, sy = 0, sz = 0;
         double norm = 1 / ((width * sqrt(PI_)) * (width * sqrt(PI_)) * (width * sqrt(PI_)));

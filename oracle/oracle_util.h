@@ -24,6 +24,23 @@ static inline double sq(double v) { return v * v; }
         for (i64 j = (g)->gc; j < (g)->n[1] - (g)->gc; ++j) \
             for (i64 k = (g)->gc; k < (g)->n[2] - (g)->gc; ++k)
 
+/* The OpenMP build of the oracle (_build/liboracle_omp.so), timed as the CPU
+ * baseline on every host core, shares the outer i loop of the cell loops among
+ * threads; the serial build, the checker, ignores these pragmas and keeps the
+ * reference's order.  Maxima use OpenMP's reduction clause on the variable `mx`
+ * (exact in any order); sums use fixed-order per-thread partials (oracle.c,
+ * interior_dot), so the OpenMP build sums in another order than the reference but
+ * always in the same one for a given thread count. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#define OMP_FOR_MAX _Pragma("omp parallel for schedule(static) reduction(max : mx)")
+#else
+#define OMP_FOR
+#define OMP_FOR_MAX
+#endif
+#define PAR_INTERIOR(g, i, j, k) OMP_FOR INTERIOR(g, i, j, k)
+#define PAR_INTERIOR_MAX(g, i, j, k) OMP_FOR_MAX INTERIOR(g, i, j, k)
+
 static inline i64 n_interior(const or_grid *g) {
     return (g->n[0] - 2 * g->gc) * (g->n[1] - 2 * g->gc) * (g->n[2] - 2 * g->gc);
 }

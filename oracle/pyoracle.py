@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+ORACLE_OMP_SO = os.path.join(HERE, "_build", "liboracle_omp.so")
 REF_SO = os.path.join(HERE, "_ref", "libref3dalm.so")
 
 i64 = C.c_int64
@@ -61,8 +62,8 @@ _libs = {}
 
 def lib(kind="oracle"):
     if kind not in _libs:
-        L = _load(ORACLE_SO if kind == "oracle" else REF_SO)
-        pre = "or_" if kind == "oracle" else "ref_"
+        L = _load({"oracle": ORACLE_SO, "omp": ORACLE_OMP_SO, "ref": REF_SO}[kind])
+        pre = "ref_" if kind == "ref" else "or_"
         G = C.POINTER(OrGrid)
         sig = {
             "predictor": (None, [G, P, P, P, P, P, dbl, dbl]),
@@ -96,11 +97,12 @@ def lib(kind="oracle"):
             f = getattr(L, pre + name)
             f.restype = res
             f.argtypes = args
-        if kind == "oracle":
+        if kind in ("oracle", "omp"):
             L.or_mesh_widths.argtypes = [P, i64, P]
             L.or_build_ap_props.argtypes = [P, P, P, P, P, i64, i64, dbl, dbl, i64, i64, i64, P, P, P]
             L.or_state_initialize.argtypes = [G, C.c_void_p]
             L.or_state_step.argtypes = [G, C.c_void_p]
+        if kind == "oracle":
             adj = {
                 "predictor_adj": [G, P, P, P, P, dbl, dbl, P, P, P, P],
                 "interpolate_JU_adj": [G, P, P],
@@ -119,7 +121,7 @@ def lib(kind="oracle"):
                 f.restype = None
                 f.argtypes = args
             L.or_adjoint_run.argtypes = [G, C.c_void_p, i64, P, dbl, P, P, P]
-        else:
+        elif kind == "ref":
             L.ref_build_mesh.argtypes = [C.c_char_p, P, P, P, P, P, P, C.POINTER(i64), C.POINTER(i64)]
             L.ref_build_ap_props.restype = i64
             L.ref_build_ap_props.argtypes = [C.c_char_p, i64, i64, P, P, P, P, C.POINTER(i64)]
@@ -207,14 +209,16 @@ class OrState(C.Structure):
 
 
 class OracleSolver:
-    """Whole-step CPU oracle on a Case (cases.Case)."""
+    """Whole-step CPU oracle on a Case (cases.Case).  kind "omp" runs the OpenMP
+    build (the CPU baseline); with initialize=False the start-up pass is left to
+    load_state (only the operator is built)."""
 
-    def __init__(self, case, ap_hook=None):
+    def __init__(self, case, ap_hook=None, kind="oracle", U0=None, initialize=True):
         self.case = case
         m = case.mesh
         self.g = make_grid(m)
         L = m.L
-        self.U = np.ascontiguousarray(case.U0())
+        self.U = np.ascontiguousarray(case.U0() if U0 is None else U0, dtype=np.float64)
         self.Uold = np.empty((L, 3))
         self.Uold[:] = case.uin
         self.JU = np.zeros((L, 3))
@@ -249,12 +253,22 @@ class OracleSolver:
             if ap_hook is not None:
                 ap_hook(self.ap, self.wt)
         self.s = s
-        self.L, _ = lib("oracle")
+        self.L, _ = lib(kind)
         if case.tgg is not None:
             plc = np.array(case.tgg["placement"], dtype=np.float64)
             self.L.or_tgg_solid(C.byref(self.g), ptr(self.solid), case.tgg["thick"], case.tgg["mesh"],
                                 case.tgg["bar"], ptr(plc))
-        self.L.or_state_initialize(C.byref(self.g), C.byref(self.s))
+        if initialize:
+            self.L.or_state_initialize(C.byref(self.g), C.byref(self.s))
+        else:
+            self.s.max_diag = self.L.or_build_A(C.byref(self.g), ptr(self.A))
+
+    def load_state(self, fields, step):
+        """Continue from a given state (U, Uold, JU, p, nut, f as [L][3] / [L] arrays)
+        after step `step`."""
+        for name, v in fields.items():
+            getattr(self, name)[...] = np.asarray(v, dtype=np.float64).reshape(getattr(self, name).shape)
+        self.s.step = int(step)
 
     def step(self):
         self.L.or_state_step(C.byref(self.g), C.byref(self.s))

@@ -8,6 +8,7 @@ the stated tolerance where the device libm (cbrt, exp, sin, cos, atan) or a
 reduction order enters.
 """
 import importlib
+import os
 
 import numpy as np
 import pytest
@@ -220,3 +221,44 @@ def test_alm(ok):
     assert abs(th[0] - owt[0, 15]) <= 1e-12 * abs(owt[0, 15])
     assert rel_l2(c.download("f"), f) <= 1e-12
 
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ALM_BRANCHES = ("none", "roll", "pitch", "yaw", "yaw_osc", "pitch_osc", "roll_osc")
+
+
+@pytest.mark.parametrize("name", ALM_BRANCHES)
+def test_alm_angle_branches(name, ok):
+    """Every angle branch of update_ap_position and calc_ap_force
+    (src/alm.h:233-414, src/euler_angle.h:34-127) on the GPU against the reference's
+    own outputs in tests/golden/alm_tiny.npz (written by the reference build): no
+    angle, static roll (frame_transform_U's roll algebra, quirk Q1), pitch (the
+    pitch sign, Q3), yaw, and the time-varying {amp, T, phase, offset} formula on
+    yaw (omega_angle != 0: the yaw-rate term, Q2), pitch and roll, at t = 0.37.
+    Positions, angle and angular rate, forces, torque, thrust and the projected
+    force field within 1e-12 (device sin / cos / atan / exp against glibc)."""
+    d = np.load(os.path.join(GOLDEN, "alm_tiny.npz"))
+    mesh = cases.Mesh(d["x"], d["y"], d["z"])
+    n = d["ap0"].shape[0]
+    per_blade = n // 3
+    wt0 = d[f"{name}_wt0"]
+    print(f"[alm branch {name}: angle type {int(wt0[0, 6])}, formula {wt0[0, 9:13]}]", flush=True)
+    c = ctx_with(mesh, U=d["U"], f=np.zeros((mesh.L, 3)))
+    try:
+        c.alm_setup(wt0, d["ap0"], 3, per_blade, d["cd"], d["cl"], d["atk"], float(d["width"][0]))
+        tq, th = c.run_alm(0.37)
+        gap, gwt = c.alm_download()
+        ref_ap, ref_wt = d[f"{name}_ap"], d[f"{name}_wt"]
+        assert max_rel(gap[:, :8], d[f"{name}_pos"]) <= 1e-13
+        assert max_rel(gap[:, 8:13], ref_ap[:, 8:13]) <= 1e-12
+        assert np.allclose(gwt[0, 7:9], ref_wt[0, 7:9], rtol=1e-13, atol=1e-15), (gwt[0, 7:9], ref_wt[0, 7:9])
+        assert abs(tq[0] - ref_wt[0, 14]) <= 1e-12 * abs(ref_wt[0, 14])
+        assert abs(th[0] - ref_wt[0, 15]) <= 1e-12 * abs(ref_wt[0, 15])
+        if f"{name}_f" in d.files:
+            fexp = d[f"{name}_f"]
+        else:  # the oracle's projection (pinned to the reference) of the reference's forces
+            fexp = np.zeros((mesh.L, 3))
+            ok.distribute_ap_force(po.make_grid(mesh), fexp, ref_ap, n, float(d["width"][0]))
+        assert rel_l2(c.download("f"), fexp) <= 1e-12
+    finally:
+        c.close()

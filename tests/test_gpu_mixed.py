@@ -118,20 +118,20 @@ def test_mixed_sor_method(interior, tol):
 
 @pytest.mark.parametrize("builder", [lambda: cases.case_c2(48, 24, 24, steps=3, ap_per_blade=8),
                                      lambda: _rows256()])
-def test_mixed_passes_and_fp64_hand_over(monkeypatch, builder):
+def test_mixed_passes_and_fp64_hand_over(builder):
     """The precision form's recovery paths: with every FP32 pass cut short after
-    40 iterations (DALM_TEST_FP32_ITERATIONS, read per solve), a pass that ends far
+    40 iterations (dalm_set_fp32_pass_cap), a pass that ends far
     above the tolerance is restarted from the true residual and one that ends
     within 30x of it is handed over to FP64; every solve must still end on the
     true residual with the step matching the FP64 oracle.  Also with rows of 256
     cells (the four-cells-per-lane kernels)."""
-    monkeypatch.setenv("DALM_TEST_FP32_ITERATIONS", "40")
     case = builder()
     case.steps = 3
     case.tol, case.max_it = 1e-10, 4000
     print(f"[mixed recovery paths {case.mesh.interior}]", flush=True)
     o = po.OracleSolver(case)
     s = solver.Solver(case, hip.MIXED)
+    s.ctx.set_fp32_pass_cap(40)
     try:
         for _ in range(3):
             so, sm = o.step(), s.step()

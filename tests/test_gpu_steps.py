@@ -168,3 +168,18 @@ def test_c1_full_against_reference_trace():
             assert rel_l2(plane, d[f"{name}_plane_i"]) <= TOL, name
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("which", ["yaw", "pitch"])
+def test_c2_small_time_varying_angle(which):
+    """Whole steps with a time-varying turbine angle, {amp, T, phase, offset}
+    (src/alm.h:195-200, 236-244): yaw or pitch oscillating with a period of 20
+    steps, so the angular rate omega_angle is comparable to the rotor speed and the
+    frame transform's rate terms (quirks Q2, Q3; src/euler_angle.h:64-127) drive the
+    blade forces.  BiCGStab converged to 1e-15; every field within 1e-10."""
+    c = cases.case_c2(32, 16, 16, steps=4, ap_per_blade=8)
+    osc = {"amp": 10.0, "T": 0.1, "phase": 30.0, "offset": 5.0}
+    c.sites[0].angle = (0.0, 0.0, osc) if which == "yaw" else (0.0, osc, 0.0)
+    c.tol, c.max_it = 1e-15, 5000
+    errs, _ = run_pair(c, 4, exact_it=False)
+    assert max(errs.values()) <= TOL, errs
