@@ -150,8 +150,13 @@ def initial_velocity(mesh: Mesh, uin, amp: float, seed: int) -> np.ndarray:
 
 
 def fourier_mode_velocity(mesh: Mesh, uin, intensity: float, seed: int, modes: int = 200) -> np.ndarray:
-    """[L][3] velocity: Uin everywhere, the interior plus a seeded synthetic isotropic
-    turbulence of `modes` random Fourier modes,
+    """[L][3] velocity of fourier_modes (Uin plus the modes on the interior)."""
+    return fourier_modes(mesh, uin, intensity, seed, modes)[0]
+
+
+def fourier_modes(mesh: Mesh, uin, intensity: float, seed: int, modes: int = 200):
+    """([L][3] velocity, [modes][7] mode table): Uin everywhere, the interior plus a
+    seeded synthetic isotropic turbulence of `modes` random Fourier modes,
 
         u'(x) = sum_n a_n sigma_n cos(k_n . x + psi_n),   sigma_n . k_n = 0,
 
@@ -163,11 +168,14 @@ def fourier_mode_velocity(mesh: Mesh, uin, intensity: float, seed: int, modes: i
     per-component RMS over the interior is intensity * |Uin_x|.  Each random quantity
     comes from its own counter-based stream of the seed (streams 16-23, apart from
     the uniform noise's 0-2).  The sum is separable: per component, one complex
-    (nx ny x modes) @ (modes x nz) product."""
+    (nx ny x modes) @ (modes x nz) product.  The table's rows are
+    (kx, ky, kz, psi, ax, ay, az) with a = the scaled amplitude times sigma, so that
+    u'(x) = sum_n a_n cos(k_n . x + psi_n) everywhere (the turbulent inflow
+    evaluates it upstream of the box)."""
     U = np.empty((mesh.L, 3))
     U[:] = np.asarray(uin, dtype=np.float64)
     if not intensity or not modes:
-        return U
+        return U, np.zeros((0, 7))
     g = mesh.gc
     nx, ny, nz = mesh.size
     x, y, z = mesh.x[g:nx - g], mesh.y[g:ny - g], mesh.z[g:nz - g]
@@ -201,8 +209,10 @@ def fourier_mode_velocity(mesh: Mesh, uin, intensity: float, seed: int, modes: i
         up[:, c] = (T.reshape(ni * nj, modes) @ Ez).real.ravel()
         del T
     rms = math.sqrt(float(np.mean(up * up)))
-    U[idx] += up * (intensity * abs(float(uin[0])) / rms)
-    return U
+    scale = intensity * abs(float(uin[0])) / rms
+    U[idx] += up * scale
+    table = np.concatenate([k, psi[:, None], (amp * scale)[:, None] * sig], axis=1)
+    return U, np.ascontiguousarray(table)
 
 
 # ----------------------------------------------------------------------------- turbines
@@ -304,11 +314,30 @@ class Case:
     tgg: Optional[dict] = None
     # retracted floor (setup.json "floor start", src/solver.h:620-635): slip floor for x < floor_start
     floor_start: Optional[float] = None
+    # turbulent inflow (not in the reference, whose x- inflow is the constant Uin): with
+    # modes > 0 the x- ghost cells and face flux take the initial field's Fourier modes
+    # as frozen turbulence convected at Uin_x, Uin + u'(x - Uin_x t, y, z)
+    inflow: bool = False
+    _modes_cache: Optional[tuple] = field(default=None, init=False, repr=False, compare=False)
+
+    def _fourier(self):
+        if self._modes_cache is None:
+            self._modes_cache = fourier_modes(self.mesh, self.uin, self.noise, self.seed, self.modes)
+        return self._modes_cache
 
     def U0(self) -> np.ndarray:
         if self.modes:
-            return fourier_mode_velocity(self.mesh, self.uin, self.noise, self.seed, self.modes)
+            return self._fourier()[0].copy()
         return initial_velocity(self.mesh, self.uin, self.noise * abs(self.uin[0]), self.seed)
+
+    def inflow_modes(self) -> Optional[np.ndarray]:
+        """[n][7] mode table of the turbulent inflow (None: the constant Uin)."""
+        if not (self.inflow and self.modes):
+            return None
+        return self._fourier()[1]
+
+    def inflow_speed(self) -> float:
+        return float(self.uin[0])
 
 
 def case_c1(nx=64, ny=48, nz=48, steps=100) -> Case:
@@ -329,17 +358,16 @@ def case_c2(nx=256, ny=128, nz=128, steps=1000, ap_per_blade=64) -> Case:
 
 def case_c3(nx=512, ny=256, nz=256, steps=500, ap_per_blade=64, tol=4e-9) -> Case:
     """The precision form at scale: 512x256x256 on a 16x8x8 box, two rotors (D = 1)
-    5 D apart at x = 3 and 8, TSR 6, eps = 2 dx, an initial seeded isotropic
-    turbulence of 200 random Fourier modes at 10 % intensity (the reference's inflow
-    boundary is constant, so the turbulence is in the initial field), BiCGStab(Jacobi 3)
-    to 4e-9, the tolerance that keeps avg div <= 1e-6 at this size
-    (dt * max_diag = 230)."""
+    5 D apart at x = 3 and 8, TSR 6, eps = 2 dx, a seeded isotropic turbulence of 200
+    random Fourier modes at 10 % intensity as the initial field and, frozen and
+    convected at Uin, as the inflow; BiCGStab(Jacobi 3) to 4e-9, the tolerance that
+    keeps avg div <= 1e-6 at this size (dt * max_diag = 230)."""
     m = make_mesh(nx, ny, nz, ((0.0, 16.0), (-4.0, 4.0), (0.0, 8.0)), 1.5, 1.5)
     dx = 16.0 / nx
     t = default_turbine(R=0.5, tower=4.0)
     sites = [TurbineSite((3.0, 0.0, 0.0), 12.0), TurbineSite((8.0, 0.0, 0.0), 12.0)]
     return Case("c3", m, method="BiCG", pc="Jacobi", pc_it=3, tol=tol, max_it=1000, steps=steps, noise=0.10,
-                modes=200, turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
+                modes=200, inflow=True, turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
 
 
 def case_c4(nx=192, ny=96, nz=96, steps=200, ap_per_blade=64, yaw_deg=(5.0, 0.0, -5.0),
@@ -359,14 +387,14 @@ def case_c4(nx=192, ny=96, nz=96, steps=200, ap_per_blade=64, yaw_deg=(5.0, 0.0,
 def case_c5(nx=768, ny=384, nz=384, steps=2000, ap_per_blade=96) -> Case:
     """The largest configuration: 768x384x384 on a 32x16x8 box, a 3x3 array of
     rotors (D = 1) at 6 D along x and 4 D along y from (4, -4, 0), TSR 6,
-    eps = 2 dx, an initial turbulence of 200 random Fourier modes at 8 % intensity,
-    BiCGStab(Jacobi 3) to 1e-8."""
+    eps = 2 dx, a turbulence of 200 random Fourier modes at 8 % intensity as the
+    initial field and the inflow, BiCGStab(Jacobi 3) to 1e-8."""
     m = make_mesh(nx, ny, nz, ((0.0, 32.0), (-8.0, 8.0), (0.0, 8.0)), 1.5, 1.5)
     dx = 32.0 / nx
     t = default_turbine(R=0.5, tower=4.0)
     sites = [TurbineSite((4.0 + 6.0 * a, -4.0 + 4.0 * b, 0.0), 12.0) for a in range(3) for b in range(3)]
     return Case("c5", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000, steps=steps, noise=0.08,
-                modes=200,
+                modes=200, inflow=True,
                 turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
 
 

@@ -161,11 +161,51 @@ __device__ __forceinline__ FaceIdx face_idx(const Dims& d, long long t) {
     return f;
 }
 
+// The x- inflow velocity of face (j, k): the constant uin0 (src/bc.h:243-257), or
+// the turbulent inflow's face value when one is set (face != nullptr).
+__device__ __forceinline__ double inflow_u(const Dims& d, const double* face, int j, int k, double uin0) {
+    return face ? face[(j - d.gc) * d.nz + (k - d.gc)] : uin0;
+}
+
+// Turbulent inflow (not in the reference): the x- ghost cells and the x- face
+// velocity of frozen turbulence convected by `shift` = speed * t,
+// Uin + sum_n a_n cos(kx (x - shift) + ky y + kz z + psi_n), summed in mode order;
+// modes[n] = (kx, ky, kz, psi, ax, ay, az).  One thread per (j, k) boundary cell;
+// out: ghost cells [3][gc][ny][nz], then the face [ny][nz] at (x[gc-1] + x[gc]) / 2.
+__global__ void k_inflow(Dims d, M1 m, const double* __restrict__ modes, int n_modes, double shift,
+                         double uin0, double uin1, double uin2, double* __restrict__ out) {
+    const int g = d.gc;
+    const long long nyz = (long long)d.ny * d.nz;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nyz;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(t / d.nz), k = (int)(t % d.nz);
+        const double Y = m.y[g + j], Z = m.z[g + k];
+        const double uin[3] = {uin0, uin1, uin2};
+        for (int i = 0; i <= g; ++i) {
+            const double X = (i < g ? m.x[i] : 0.5 * (m.x[g - 1] + m.x[g])) - shift;
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int n = 0; n < n_modes; ++n) {
+                const double* M = modes + 7 * n;
+                const double c = cos(M[0] * X + M[1] * Y + M[2] * Z + M[3]);
+                s0 += M[4] * c;
+                s1 += M[5] * c;
+                s2 += M[6] * c;
+            }
+            if (i < g) {
+                const double s[3] = {s0, s1, s2};
+                for (int q = 0; q < 3; ++q) out[(q * g + i) * nyz + t] = uin[q] + s[q];
+            } else {
+                out[3 * g * nyz + t] = uin0 + s0;
+            }
+        }
+    }
+}
+
 // Face fluxes (interpolate_JU, src/cfd.h:394-420) with, optionally, the face BCs
 // of apply_JUbc (src/bc.h:235-302) written in place of the interpolated value.
 __global__ void k_flux(Dims d, M1 m, const double* __restrict__ u0, const double* __restrict__ u1,
                        const double* __restrict__ u2, double* __restrict__ J0, double* __restrict__ J1,
-                       double* __restrict__ J2, int apply_bc, double uin0) {
+                       double* __restrict__ J2, int apply_bc, double uin0, const double* __restrict__ face) {
     long long n = (long long)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     const int g = d.gc;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
@@ -176,7 +216,7 @@ __global__ void k_flux(Dims d, M1 m, const double* __restrict__ u0, const double
         const bool ji = j >= g && j < d.n1 - g, ki = k >= g && k < d.n2 - g, ii = i >= g && i < d.n0 - g;
         double ayz = m.dy[j] * m.dz[k];
         double vx = 0.5 * (u0[c] * ayz + u0[c + d.si] * ayz);
-        if (apply_bc && i == g - 1 && ji && ki) vx = ayz * uin0;
+        if (apply_bc && i == g - 1 && ji && ki) vx = ayz * inflow_u(d, face, j, k, uin0);
         J0[c] = vx;
         double axz = m.dx[i] * m.dz[k];
         double vy = 0.5 * (u1[c] * axz + u1[c + d.sj] * axz);
@@ -191,7 +231,7 @@ __global__ void k_flux(Dims d, M1 m, const double* __restrict__ u0, const double
 
 // apply_JUbc alone (src/bc.h:235-302): planes x-, y+-, z+-.
 __global__ void k_jubc(Dims d, M1 m, double* __restrict__ J0, double* __restrict__ J1,
-                       double* __restrict__ J2, double uin0) {
+                       double* __restrict__ J2, double uin0, const double* __restrict__ face) {
     const int g = d.gc;
     long long nyz = (long long)d.ny * d.nz, nxz = (long long)d.nx * d.nz, nxy = (long long)d.nx * d.ny;
     long long n = nyz + 2 * nxz + 2 * nxy;
@@ -200,7 +240,7 @@ __global__ void k_jubc(Dims d, M1 m, double* __restrict__ J0, double* __restrict
         long long s = t;
         if (s < nyz) {
             int j = g + (int)(s / d.nz), k = g + (int)(s % d.nz);
-            J0[d.at(g - 1, j, k)] = m.dy[j] * m.dz[k] * uin0;
+            J0[d.at(g - 1, j, k)] = m.dy[j] * m.dz[k] * inflow_u(d, face, j, k, uin0);
             continue;
         }
         s -= nyz;
@@ -287,7 +327,7 @@ __global__ void k_pbc(Dims d, M1 m, double* __restrict__ p) {
 __global__ void k_project(Dims d, M1 m, const double* __restrict__ p, double* __restrict__ u0,
                           double* __restrict__ u1, double* __restrict__ u2, double* __restrict__ J0,
                           double* __restrict__ J1, double* __restrict__ J2, double dt, int apply_bc,
-                          double uin0) {
+                          double uin0, const double* __restrict__ face) {
     long long n = (long long)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     const int g = d.gc;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
@@ -305,7 +345,7 @@ __global__ void k_project(Dims d, M1 m, const double* __restrict__ p, double* __
         const bool jint = ji && j < d.n1 - g, kint = ki && k < d.n2 - g, iint = ii && i < d.n0 - g;
         double gx = (pe - pc) / (m.x[i + 1] - m.x[i]);
         double vx = J0[c] - dt * (m.dy[j] * m.dz[k]) * gx;
-        if (apply_bc && i == g - 1 && jint && kint) vx = m.dy[j] * m.dz[k] * uin0;
+        if (apply_bc && i == g - 1 && jint && kint) vx = m.dy[j] * m.dz[k] * inflow_u(d, face, j, k, uin0);
         J0[c] = vx;
         double gy = (pn - pc) / (m.y[j + 1] - m.y[j]);
         double vy = J1[c] - dt * (m.dx[i] * m.dz[k]) * gy;
@@ -349,7 +389,8 @@ __device__ __forceinline__ double quad3(double x0, double x1, double x2, double 
 __global__ void k_ubc(Dims d, M1 m, double* __restrict__ u0, double* __restrict__ u1,
                       double* __restrict__ u2, const double* __restrict__ o0,
                       const double* __restrict__ o1, const double* __restrict__ o2, double uin0,
-                      double uin1, double uin2, double dt, int retracted, double floor_x) {
+                      double uin1, double uin2, double dt, int retracted, double floor_x,
+                      const double* __restrict__ cells) {
     const int g = d.gc;
     long long nyz = (long long)d.ny * d.nz, nxz = (long long)d.nx * d.nz, nxy = (long long)d.nx * d.ny;
     long long n = 2 * nyz + 2 * nxz + 2 * nxy;
@@ -365,9 +406,17 @@ __global__ void k_ubc(Dims d, M1 m, double* __restrict__ u0, double* __restrict_
             if (!side) {
                 for (int i = 0; i < g; ++i) {
                     int c = d.at(i, j, k);
-                    u0[c] = uin0;
-                    u1[c] = uin1;
-                    u2[c] = uin2;
+                    if (cells) {
+                        // turbulent inflow: [3][gc][ny][nz]
+                        const long long o = (long long)i * nyz + (long long)(j - g) * d.nz + (k - g);
+                        u0[c] = cells[o];
+                        u1[c] = cells[g * nyz + o];
+                        u2[c] = cells[2 * g * nyz + o];
+                    } else {
+                        u0[c] = uin0;
+                        u1[c] = uin1;
+                        u2[c] = uin2;
+                    }
                 }
             } else {
                 for (int i = d.n0 - g; i < d.n0; ++i) {
@@ -530,7 +579,17 @@ int launch_flux(dalm_ctx* ctx, const double Uin[3], bool apply_bc) {
     long long n = (long long)(ctx->d.nx + 1) * (ctx->d.ny + 1) * (ctx->d.nz + 1);
     hipLaunchKernelGGL(k_flux, dim3(grid_for(n)), dim3(256), 0, ctx->stream, ctx->d, mesh(ctx),
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
-                       apply_bc ? 1 : 0, Uin ? Uin[0] : 0.0);
+                       apply_bc ? 1 : 0, Uin ? Uin[0] : 0.0, ctx->inflow_face());
+    LAUNCH_CHECK();
+    return DALM_OK;
+}
+
+int launch_inflow(dalm_ctx* ctx, double t) {
+    if (!ctx->inflow_n) return DALM_OK;
+    const dalm_params& P = ctx->prm;
+    hipLaunchKernelGGL(k_inflow, dim3(grid_for((long long)ctx->d.ny * ctx->d.nz)), dim3(256), 0, ctx->stream,
+                       ctx->d, mesh(ctx), ctx->inflow_modes, ctx->inflow_n, ctx->inflow_speed * t, P.Uin[0],
+                       P.Uin[1], P.Uin[2], ctx->inflow);
     LAUNCH_CHECK();
     return DALM_OK;
 }
@@ -539,7 +598,7 @@ int launch_jubc(dalm_ctx* ctx, const double Uin[3]) {
     const Dims& d = ctx->d;
     long long n = (long long)d.ny * d.nz + 2LL * d.nx * d.nz + 2LL * d.nx * d.ny;
     hipLaunchKernelGGL(k_jubc, dim3(grid_for(n)), dim3(256), 0, ctx->stream, ctx->d, mesh(ctx),
-                       ctx->JU[0], ctx->JU[1], ctx->JU[2], Uin[0]);
+                       ctx->JU[0], ctx->JU[1], ctx->JU[2], Uin[0], ctx->inflow_face());
     LAUNCH_CHECK();
     return DALM_OK;
 }
@@ -563,7 +622,7 @@ int launch_project(dalm_ctx* ctx, double dt, const double* Uin_bc) {
     long long n = (long long)(ctx->d.nx + 1) * (ctx->d.ny + 1) * (ctx->d.nz + 1);
     hipLaunchKernelGGL(k_project, dim3(grid_for(n)), dim3(256), 0, ctx->stream, ctx->d, mesh(ctx),
                        ctx->p, ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], dt,
-                       Uin_bc ? 1 : 0, Uin_bc ? Uin_bc[0] : 0.0);
+                       Uin_bc ? 1 : 0, Uin_bc ? Uin_bc[0] : 0.0, ctx->inflow_face());
     LAUNCH_CHECK();
     return DALM_OK;
 }
@@ -580,7 +639,7 @@ int launch_ubc(dalm_ctx* ctx, const double Uin[3], double dt, int retracted, dou
     long long n = 2LL * d.ny * d.nz + 2LL * d.nx * d.nz + 2LL * d.nx * d.ny;
     hipLaunchKernelGGL(k_ubc, dim3(grid_for(n)), dim3(256), 0, ctx->stream, ctx->d, mesh(ctx),
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->Uold[0], ctx->Uold[1], ctx->Uold[2],
-                       Uin[0], Uin[1], Uin[2], dt, retracted, floor_x);
+                       Uin[0], Uin[1], Uin[2], dt, retracted, floor_x, ctx->inflow_cells());
     LAUNCH_CHECK();
     return DALM_OK;
 }

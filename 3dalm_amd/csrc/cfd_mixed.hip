@@ -97,12 +97,15 @@ __global__ __launch_bounds__(BLOCK) void km_predictor(
 // Face fluxes (FP64) with their boundary values, and -- for interior cells -- the
 // Poisson right-hand side from the six faces of the cell (the west/south/bottom
 // faces are recomputed here rather than re-read).  src/cfd.h:394-598, src/bc.h:235-302.
+// The x- face takes the constant inflow uin0, or the turbulent inflow's face value
+// (face != nullptr, [ny][nz]).
 __device__ __forceinline__ double face_x(const Dims& d, const M1& m, const float* u0, int i, int j, int k,
-                                         double uin0) {
+                                         double uin0, const double* face) {
     const int g = d.gc;
     const int c = d.at(i, j, k);
     double a = m.dy[j] * m.dz[k];
-    if (i == g - 1 && j >= g && j < d.n1 - g && k >= g && k < d.n2 - g) return a * uin0;
+    if (i == g - 1 && j >= g && j < d.n1 - g && k >= g && k < d.n2 - g)
+        return a * (face ? face[(j - g) * d.nz + (k - g)] : uin0);
     return 0.5 * ((double)u0[c] + (double)u0[c + d.si]) * a;
 }
 __device__ __forceinline__ double face_y(const Dims& d, const M1& m, const float* u1, int i, int j, int k) {
@@ -121,7 +124,7 @@ __device__ __forceinline__ double face_z(const Dims& d, const M1& m, const float
 __global__ void km_flux_rhs(Dims d, M1 m, const float* __restrict__ u0, const float* __restrict__ u1,
                             const float* __restrict__ u2, double* __restrict__ J0, double* __restrict__ J1,
                             double* __restrict__ J2, double* __restrict__ b, double uin0, double inv_dts,
-                            int rhs) {
+                            int rhs, const double* __restrict__ face) {
     long long n = (long long)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     const int g = d.gc;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
@@ -131,14 +134,14 @@ __global__ void km_flux_rhs(Dims d, M1 m, const float* __restrict__ u0, const fl
         int j = g - 1 + (int)(r % (d.ny + 1));
         int i = g - 1 + (int)(r / (d.ny + 1));
         const int c = d.at(i, j, k);
-        double fe = face_x(d, m, u0, i, j, k, uin0);
+        double fe = face_x(d, m, u0, i, j, k, uin0, face);
         double fn = face_y(d, m, u1, i, j, k);
         double ft = face_z(d, m, u2, i, j, k);
         J0[c] = fe;
         J1[c] = fn;
         J2[c] = ft;
         if (rhs && i >= g && j >= g && k >= g) {
-            double fw = face_x(d, m, u0, i - 1, j, k, uin0);
+            double fw = face_x(d, m, u0, i - 1, j, k, uin0, face);
             double fs = face_y(d, m, u1, i, j - 1, k);
             double fb = face_z(d, m, u2, i, j, k - 1);
             double vol = m.dx[i] * m.dy[j] * m.dz[k];
@@ -150,7 +153,8 @@ __global__ void km_flux_rhs(Dims d, M1 m, const float* __restrict__ u0, const fl
 // Projection (src/cfd.h:600-783) + face BCs; U in FP32 updated from FP64 math.
 __global__ void km_project(Dims d, M1 m, MetricF mf, const double* __restrict__ p, float* __restrict__ u0,
                            float* __restrict__ u1, float* __restrict__ u2, double* __restrict__ J0,
-                           double* __restrict__ J1, double* __restrict__ J2, double dt, double uin0) {
+                           double* __restrict__ J1, double* __restrict__ J2, double dt, double uin0,
+                           const double* __restrict__ face) {
     long long n = (long long)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     const int g = d.gc;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
@@ -170,7 +174,7 @@ __global__ void km_project(Dims d, M1 m, MetricF mf, const double* __restrict__ 
         const bool jint = ji && j < d.n1 - g, kint = ki && k < d.n2 - g, iint = ii && i < d.n0 - g;
         double ayz = m.dy[j] * m.dz[k];
         double vx = J0[c] - dt * ayz * (pe - pc) / (m.x[i + 1] - m.x[i]);
-        if (i == g - 1 && jint && kint) vx = ayz * uin0;
+        if (i == g - 1 && jint && kint) vx = ayz * (face ? face[(j - g) * d.nz + (k - g)] : uin0);
         J0[c] = vx;
         double vy = J1[c] - dt * (m.dx[i] * m.dz[k]) * (pn - pc) / (m.y[j + 1] - m.y[j]);
         if (iint && kint && (j == g - 1 || j == d.n1 - g - 1)) vy = 0;
@@ -193,7 +197,7 @@ __device__ __forceinline__ double quad3(double x0, double x1, double x2, double 
 __global__ void km_ubc(Dims d, M1 m, float* __restrict__ u0, float* __restrict__ u1, float* __restrict__ u2,
                        const float* __restrict__ o0, const float* __restrict__ o1,
                        const float* __restrict__ o2, double uin0, double uin1, double uin2, double dt,
-                       int retracted, double floor_x) {
+                       int retracted, double floor_x, const double* __restrict__ cells) {
     const int g = d.gc;
     long long nyz = (long long)d.ny * d.nz, nxz = (long long)d.nx * d.nz, nxy = (long long)d.nx * d.ny;
     long long n = 2 * nyz + 2 * nxz + 2 * nxy;
@@ -209,9 +213,17 @@ __global__ void km_ubc(Dims d, M1 m, float* __restrict__ u0, float* __restrict__
             if (!side) {
                 for (int i = 0; i < g; ++i) {
                     int c = d.at(i, j, k);
-                    u0[c] = (float)uin0;
-                    u1[c] = (float)uin1;
-                    u2[c] = (float)uin2;
+                    if (cells) {
+                        // turbulent inflow: [3][gc][ny][nz]
+                        const long long o = (long long)i * nyz + (long long)(j - g) * d.nz + (k - g);
+                        u0[c] = (float)cells[o];
+                        u1[c] = (float)cells[g * nyz + o];
+                        u2[c] = (float)cells[2 * g * nyz + o];
+                    } else {
+                        u0[c] = (float)uin0;
+                        u1[c] = (float)uin1;
+                        u2[c] = (float)uin2;
+                    }
                 }
             } else {
                 for (int i = d.n0 - g; i < d.n0; ++i) {
@@ -450,7 +462,7 @@ int launch_flux_rhs_m(dalm_ctx* ctx, const double Uin[3], double dt, double scal
     long long n = (long long)(ctx->d.nx + 1) * (ctx->d.ny + 1) * (ctx->d.nz + 1);
     hipLaunchKernelGGL(km_flux_rhs, dim3(grid_for(n)), dim3(256), 0, ctx->stream, ctx->d, mesh(ctx),
                        ctx->U32[0], ctx->U32[1], ctx->U32[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->b, Uin[0],
-                       rhs ? 1.0 / (dt * scale) : 0.0, rhs ? 1 : 0);
+                       rhs ? 1.0 / (dt * scale) : 0.0, rhs ? 1 : 0, ctx->inflow_face());
     LAUNCH_CHECK();
     return DALM_OK;
 }
@@ -459,7 +471,7 @@ int launch_project_m(dalm_ctx* ctx, double dt, const double Uin[3]) {
     long long n = (long long)(ctx->d.nx + 1) * (ctx->d.ny + 1) * (ctx->d.nz + 1);
     hipLaunchKernelGGL(km_project, dim3(grid_for(n)), dim3(256), 0, ctx->stream, ctx->d, mesh(ctx), ctx->mf,
                        ctx->p, ctx->U32[0], ctx->U32[1], ctx->U32[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], dt,
-                       Uin[0]);
+                       Uin[0], ctx->inflow_face());
     LAUNCH_CHECK();
     return DALM_OK;
 }
@@ -469,7 +481,7 @@ int launch_ubc_m(dalm_ctx* ctx, const double Uin[3], double dt, int retracted, d
     long long n = 2LL * d.ny * d.nz + 2LL * d.nx * d.nz + 2LL * d.nx * d.ny;
     hipLaunchKernelGGL(km_ubc, dim3(grid_for(n)), dim3(256), 0, ctx->stream, ctx->d, mesh(ctx), ctx->U32[0],
                        ctx->U32[1], ctx->U32[2], ctx->Uold32[0], ctx->Uold32[1], ctx->Uold32[2], Uin[0], Uin[1],
-                       Uin[2], dt, retracted, floor_x);
+                       Uin[2], dt, retracted, floor_x, ctx->inflow_cells());
     LAUNCH_CHECK();
     return DALM_OK;
 }

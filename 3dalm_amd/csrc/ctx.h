@@ -224,6 +224,18 @@ struct dalm_ctx {
     KernelProbe probe;
     AdjointState adj;
 
+    // turbulent inflow (dalm_set_inflow; not in the reference): the mode table
+    // [n][7] and, per step, the x- boundary values it gives -- ghost cells
+    // [3][gc][ny][nz] (component-major) then the face velocity [ny][nz]
+    double* inflow_modes = nullptr;
+    double* inflow = nullptr;
+    int inflow_n = 0;
+    double inflow_speed = 0;
+    const double* inflow_cells() const { return inflow_n ? inflow : nullptr; }
+    const double* inflow_face() const {
+        return inflow_n ? inflow + 3LL * d.gc * d.ny * d.nz : nullptr;
+    }
+
     // output-path fields (diag.hip), allocated zero-filled on first use: the
     // Q-criterion and the time-averaged velocity (FP64 in both forms), and the
     // probe-cell gather buffers
@@ -265,6 +277,7 @@ int dalm_fail_msg(dalm_ctx* ctx, int status, const std::string& msg);
 int launch_predictor(dalm_ctx* ctx, double Re, double dt);
 int launch_copy_uold(dalm_ctx* ctx);
 int launch_ghost_copy(dalm_ctx* ctx);
+int launch_inflow(dalm_ctx* ctx, double t);
 int launch_flux(dalm_ctx* ctx, const double Uin[3], bool apply_bc);
 int launch_jubc(dalm_ctx* ctx, const double Uin[3]);
 int launch_rhs(dalm_ctx* ctx, double dt, double scale);

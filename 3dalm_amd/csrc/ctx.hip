@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 
 #include "ctx.h"
@@ -272,6 +273,7 @@ int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given) {
     const dalm_params& P = ctx->prm;
     ctx->step += 1;
     const double t = P.dt * (double)ctx->step;
+    TRY(launch_inflow(ctx, t));
     // Uold <- U: swap the buffers, then restore U's ghost shell from Uold
     for (int q = 0; q < 3; ++q) std::swap(ctx->U[q], ctx->Uold[q]);
     TRY(launch_ghost_copy(ctx));
@@ -479,7 +481,7 @@ int dalm_destroy(dalm_ctx* ctx) {
                       ctx->t, ctx->tmp, ctx->tmp2, ctx->pbest, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
                       ctx->pc.an_n, ctx->pc.as_n, ctx->pc.at_n, ctx->pc.ab_n, ctx->pc.acx,
                       ctx->pc.an_u, ctx->pc.as_u, ctx->pc.at_u, ctx->pc.ab_u, ctx->alm.wt, ctx->alm.ap,
-                      ctx->alm.cd, ctx->alm.cl, ctx->alm.atk};
+                      ctx->alm.cd, ctx->alm.cl, ctx->alm.atk, ctx->inflow_modes, ctx->inflow};
     for (double* p : ptrs)
         if (p) (void)hipFree(p);
     for (int q = 0; q < 3; ++q) {
@@ -757,6 +759,39 @@ int dalm_alm_download(dalm_ctx* ctx, double* ap, double* wt) {
     return DALM_OK;
 }
 
+// ---- turbulent inflow ---------------------------------------------------------------
+int dalm_set_inflow(dalm_ctx* ctx, const double* modes, int64_t n_modes, double speed) {
+    CHECK_CTX();
+    if (n_modes < 0 || n_modes > 4096 || (n_modes > 0 && !modes) || !std::isfinite(speed))
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "invalid inflow modes");
+    for (int64_t q = 0; q < 7 * n_modes; ++q)
+        if (!std::isfinite(modes[q])) return dalm_fail_msg(ctx, DALM_BAD_ARG, "non-finite inflow mode");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->inflow_modes) (void)hipFree(ctx->inflow_modes);
+    if (ctx->inflow) (void)hipFree(ctx->inflow);
+    ctx->inflow_modes = ctx->inflow = nullptr;
+    ctx->inflow_n = 0;
+    ctx->inflow_speed = 0;
+    if (n_modes == 0) return DALM_OK;
+    const Dims& d = ctx->d;
+    const size_t cells = (3 * (size_t)d.gc + 1) * (size_t)d.ny * (size_t)d.nz;
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->inflow_modes, 7 * (size_t)n_modes)) || (rc = dev_alloc(ctx, &ctx->inflow, cells)))
+        return rc;
+    HIP_TRY(hipMemcpy(ctx->inflow_modes, modes, sizeof(double) * 7 * (size_t)n_modes, hipMemcpyHostToDevice));
+    ctx->inflow_n = (int)n_modes;
+    ctx->inflow_speed = speed;
+    return DALM_OK;
+}
+
+int dalm_eval_inflow(dalm_ctx* ctx, double t) {
+    CHECK_CTX();
+    if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
+    TRY(launch_inflow(ctx, t));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DALM_OK;
+}
+
 // ---- whole step -----------------------------------------------------------------------
 int dalm_initialize(dalm_ctx* ctx, dalm_step_stats* stats) {
     CHECK_CTX();
@@ -764,6 +799,7 @@ int dalm_initialize(dalm_ctx* ctx, dalm_step_stats* stats) {
     const dalm_params& P = ctx->prm;
     ctx->step = 0;
     if (!ctx->have_A) TRY(build_operator(ctx));
+    TRY(launch_inflow(ctx, 0.0));
     if (ctx->precision == DALM_MIXED) {
         if (ctx->has_solid) TRY(launch_solid_m(ctx));
         TRY(launch_ubc_m(ctx, P.Uin, P.dt, (int)P.retracted_floor, P.floor_start_x));
@@ -786,6 +822,7 @@ int dalm_initialize(dalm_ctx* ctx, dalm_step_stats* stats) {
 // One time step of the precision form (same order as the FP64 step below).
 static int step_mixed(dalm_ctx* ctx, double t, dalm_step_stats* stats) {
     const dalm_params& P = ctx->prm;
+    TRY(launch_inflow(ctx, t));
     for (int q = 0; q < 3; ++q) std::swap(ctx->U32[q], ctx->Uold32[q]);
     TRY(launch_ghost_copy_m(ctx));
     TRY(launch_predictor_m(ctx, P.Re, P.dt));

@@ -111,6 +111,8 @@ def lib():
         "dalm_calc_nut": [V, dbl],
         "dalm_calc_divergence": [V],
         "dalm_l2_norm": [V, i64, P],
+        "dalm_set_inflow": [V, P, i64, dbl],
+        "dalm_eval_inflow": [V, dbl],
         "dalm_max_cfl": [V, dbl, P],
         "dalm_calc_q": [V],
         "dalm_set_turbulence_grid": [V, dbl, dbl, dbl, P],
@@ -334,6 +336,15 @@ class Context:
         return ap, wt
 
     # ---- whole steps
+    def set_inflow(self, modes, speed):
+        """Turbulent inflow: modes [n][7] = (kx, ky, kz, psi, ax, ay, az), frozen and
+        convected at `speed` (an empty table restores the constant inflow)."""
+        m = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1, 7)
+        self.check(self.L.dalm_set_inflow(self.h, ptr(m) if len(m) else None, len(m), float(speed)), "set_inflow")
+
+    def eval_inflow(self, t):
+        self.check(self.L.dalm_eval_inflow(self.h, float(t)), "eval_inflow")
+
     def set_params(self, prm: Params):
         self.check(self.L.dalm_set_params(self.h, C.byref(prm)), "set_params")
 

@@ -48,6 +48,9 @@ class Solver:
         if case.tgg is not None:  # src/solver.h:740-750
             t = case.tgg
             self.ctx.set_turbulence_grid(t["thick"], t["mesh"], t["bar"], t["placement"])
+        modes = case.inflow_modes()
+        if modes is not None and len(modes):  # turbulent inflow (not in the reference)
+            self.ctx.set_inflow(modes, case.inflow_speed())
         self.ap = None
         if case.turbine is not None and case.sites:
             nwt = len(case.sites)

@@ -194,6 +194,20 @@ int dalm_l2_norm(dalm_ctx *ctx, int64_t field, double *out);
 int dalm_max_cfl(dalm_ctx *ctx, double dt, double *out);
 
 
+/* ---- turbulent inflow ------------------------------------------------------------ *
+ * No counterpart in the reference, whose x- inflow is the constant Uin
+ * (apply_Ubc src/bc.h:16-30, apply_JUbc src/bc.h:243-257).  With n_modes > 0 the
+ * x- ghost cells and the x- face flux take frozen turbulence convected at `speed`:
+ * Uin + sum_n a_n cos(kx (x - speed t) + ky y + kz z + psi_n), modes [n][7] =
+ * (kx, ky, kz, psi, ax, ay, az), at the ghost cell centres and at the face
+ * x_f = (x[gc-1] + x[gc]) / 2, with Uin from dalm_set_params.  dalm_step and
+ * dalm_initialize evaluate it at each step's time level; the per-kernel entry
+ * points (dalm_apply_Ubc, dalm_apply_JUbc, dalm_interpolate_JU's callers) use the
+ * values of the last evaluation, made explicit by dalm_eval_inflow.  n_modes = 0
+ * restores the constant inflow (the default).  At most 4096 modes. */
+int dalm_set_inflow(dalm_ctx *ctx, const double *modes, int64_t n_modes, double speed);
+int dalm_eval_inflow(dalm_ctx *ctx, double t);
+
 /* ---- output path (diagnostics) ---------------------------------------------------- */
 
 /* calc_q (src/cfd.h:948-1077): Q-criterion of U over the interior into the field

@@ -402,6 +402,67 @@ void or_set_solid(const or_grid *g, double *U, const double *solid) {
 }
 
 
+/* turbulent inflow: not in the reference (its x- inflow is the constant Uin,
+ * src/bc.h:16-30 and 243-257).  Frozen turbulence convected at `speed`: the x-
+ * ghost cells and the x- face flux take Uin + u'(x - speed t, y, z) with
+ * u'(x) = sum_n a_n cos(k_n . x + psi_n), modes[n] = (kx, ky, kz, psi, ax, ay, az),
+ * summed in mode order.  cells: [gc][ny][nz][3] for ghost planes i = 0..gc-1 at the
+ * cell centres; face: [ny][nz] the face velocity at x_f = (x[gc-1] + x[gc]) / 2.
+ * Pinned by closed forms: each mode divergence-free, the shift property
+ * u(x - s, t) = u(x, t + s / speed), the single-mode value. */
+static void inflow_point(const double *modes, i64 n_modes, double X, double Y, double Z, double out[3]) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (i64 n = 0; n < n_modes; ++n) {
+        const double *M = modes + 7 * n;
+        const double c = cos(M[0] * X + M[1] * Y + M[2] * Z + M[3]);
+        s0 += M[4] * c;
+        s1 += M[5] * c;
+        s2 += M[6] * c;
+    }
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+}
+
+void or_inflow_eval(const or_grid *g, const double *modes, i64 n_modes, const double Uin[3], double shift,
+                    double *cells, double *face) {
+    const i64 G = g->gc, NY = g->n[1], NZ = g->n[2], ny = NY - 2 * G, nz = NZ - 2 * G;
+    const double *X = g->c[0], *Y = g->c[1], *Z = g->c[2];
+    const double xf = 0.5 * (X[G - 1] + X[G]);
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static)
+#endif
+    for (i64 j = 0; j < ny; ++j)
+        for (i64 k = 0; k < nz; ++k) {
+            double u[3];
+            for (i64 i = 0; i < G; ++i) {
+                inflow_point(modes, n_modes, X[i] - shift, Y[G + j], Z[G + k], u);
+                double *c = cells + 3 * ((i * ny + j) * nz + k);
+                for (int m = 0; m < 3; ++m) c[m] = Uin[m] + u[m];
+            }
+            inflow_point(modes, n_modes, xf - shift, Y[G + j], Z[G + k], u);
+            face[j * nz + k] = Uin[0] + u[0];
+        }
+}
+
+/* the x- ghost cells of U from or_inflow_eval's cells (in place of the constant) */
+void or_apply_inflow_U(const or_grid *g, double *U, const double *cells) {
+    const i64 G = g->gc, NY = g->n[1], NZ = g->n[2], ny = NY - 2 * G, nz = NZ - 2 * G;
+    for (i64 i = 0; i < G; ++i)
+        for (i64 j = 0; j < ny; ++j)
+            for (i64 k = 0; k < nz; ++k)
+                for (int m = 0; m < 3; ++m)
+                    U[3 * lin(g, i, G + j, G + k) + m] = cells[3 * ((i * ny + j) * nz + k) + m];
+}
+
+/* the x- face flux JU = dy dz u_face (in place of dy dz Uin[0]) */
+void or_apply_inflow_JU(const or_grid *g, double *JU, const double *face) {
+    const double *DY = g->d[1], *DZ = g->d[2];
+    const i64 G = g->gc, NY = g->n[1], NZ = g->n[2], ny = NY - 2 * G, nz = NZ - 2 * G;
+    for (i64 j = 0; j < ny; ++j)
+        for (i64 k = 0; k < nz; ++k) JU[3 * lin(g, G - 1, G + j, G + k) + 0] = DY[G + j] * DZ[G + k] * face[j * nz + k];
+}
+
 /* velocity ghosts: src/bc.h:5-233 */
 void or_apply_Ubc(const or_grid *g, double *U, const double *Uold, const double Uin[3],
                   double dt, i64 retracted_floor, double floor_start_x) {
@@ -720,14 +781,28 @@ static void run_alm(const or_grid *g, or_state *s, double t) {
     or_distribute_ap_force(g, s->f, s->ap, s->ap_count, s->width);
 }
 
+/* the turbulent inflow of the step's time level (nothing without modes) */
+static void inflow_at(const or_grid *g, or_state *s, double t) {
+    if (s->inflow_n > 0)
+        or_inflow_eval(g, s->inflow_modes, s->inflow_n, s->Uin, s->inflow_speed * t, s->inflow_cells,
+                       s->inflow_face);
+}
+
+static void apply_JUbc_state(const or_grid *g, or_state *s) {
+    or_apply_JUbc(g, s->JU, s->Uin);
+    if (s->inflow_n > 0) or_apply_inflow_JU(g, s->JU, s->inflow_face);
+}
+
 /* ========================================================================== */
 /* start-up pass after the fields are filled: src/solver.h:727-833 */
 void or_state_initialize(const or_grid *g, or_state *s) {
     s->max_diag = or_build_A(g, s->A);
     or_set_solid(g, s->U, s->solid);
+    inflow_at(g, s, s->dt * s->step);
     or_apply_Ubc(g, s->U, s->Uold, s->Uin, s->dt, s->retracted_floor, s->floor_start_x);
+    if (s->inflow_n > 0) or_apply_inflow_U(g, s->U, s->inflow_cells);
     or_interpolate_JU(g, s->U, s->JU);
-    or_apply_JUbc(g, s->JU, s->Uin);
+    apply_JUbc_state(g, s);
     or_nut(g, s->U, s->nut, s->Cs);
     or_divergence(g, s->JU, s->div);
     run_alm(g, s, s->dt * s->step);
@@ -741,9 +816,10 @@ void or_state_step(const or_grid *g, or_state *s) {
     const i64 L = g->n[0] * g->n[1] * g->n[2];
     s->step += 1;
     memcpy(s->Uold, s->U, sizeof(double) * 3 * (size_t)L);
+    inflow_at(g, s, s->dt * s->step);
     or_predictor(g, s->U, s->Uold, s->JU, s->nut, s->f, s->Re, s->dt);
     or_interpolate_JU(g, s->U, s->JU);
-    or_apply_JUbc(g, s->JU, s->Uin);
+    apply_JUbc_state(g, s);
     or_poisson_rhs(g, s->JU, s->b, s->dt, s->max_diag);
     if (s->method == 1) {
         or_run_bicgstab(g, s->A, s->p, s->b, s->work, s->pc_kind, s->pc_omega, s->pc_it, &s->it,
@@ -755,7 +831,8 @@ void or_state_step(const or_grid *g, or_state *s) {
     or_project(g, s->U, s->JU, s->p, s->dt);
     or_set_solid(g, s->U, s->solid);
     or_apply_Ubc(g, s->U, s->Uold, s->Uin, s->dt, s->retracted_floor, s->floor_start_x);
-    or_apply_JUbc(g, s->JU, s->Uin);
+    if (s->inflow_n > 0) or_apply_inflow_U(g, s->U, s->inflow_cells);
+    apply_JUbc_state(g, s);
     or_nut(g, s->U, s->nut, s->Cs);
     or_divergence(g, s->JU, s->div);
     run_alm(g, s, s->dt * s->step);

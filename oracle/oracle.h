@@ -75,6 +75,12 @@ void or_nut(const or_grid *g, const double *U, double *nut, double Cs);
 void or_divergence(const or_grid *g, const double *JU, double *div);
 double or_max_cfl(const or_grid *g, const double *U, double dt);
 
+/* ---- turbulent inflow (not in the reference; see oracle.c) --------------- */
+void or_inflow_eval(const or_grid *g, const double *modes /* [n][7] */, i64 n_modes, const double Uin[3],
+                    double shift, double *cells, double *face);
+void or_apply_inflow_U(const or_grid *g, double *U, const double *cells);
+void or_apply_inflow_JU(const or_grid *g, double *JU, const double *face);
+
 /* ---- output path --------------------------------------------------------- */
 void or_calc_q(const or_grid *g, const double *U, double *q);
 /* turbulence-generating grid: solid fraction over the interior (plc = placement) */
@@ -151,6 +157,12 @@ typedef struct {
     double max_diag;
     i64 step, it;
     double err, avg_div, max_cfl;
+    /* turbulent inflow (inflow_n = 0: the reference's constant Uin); the cells and
+       face buffers are caller-owned, gc*ny*nz*3 and ny*nz doubles */
+    const double *inflow_modes;
+    i64 inflow_n;
+    double inflow_speed;
+    double *inflow_cells, *inflow_face;
 } or_state;
 
 void or_state_initialize(const or_grid *g, or_state *s);

@@ -102,6 +102,9 @@ def lib(kind="oracle"):
             L.or_build_ap_props.argtypes = [P, P, P, P, P, i64, i64, dbl, dbl, i64, i64, i64, P, P, P]
             L.or_state_initialize.argtypes = [G, C.c_void_p]
             L.or_state_step.argtypes = [G, C.c_void_p]
+            L.or_inflow_eval.argtypes = [G, P, i64, P, dbl, P, P]
+            L.or_apply_inflow_U.argtypes = [G, P, P]
+            L.or_apply_inflow_JU.argtypes = [G, P, P]
         if kind == "oracle":
             adj = {
                 "predictor_adj": [G, P, P, P, P, dbl, dbl, P, P, P, P],
@@ -205,7 +208,8 @@ class OrState(C.Structure):
         ("wt", P), ("ap", P), ("cd_tbl", P), ("cl_tbl", P), ("atk", P),
         ("wt_count", i64), ("ap_count", i64), ("blade_per_wt", i64), ("ap_per_blade", i64),
         ("atk_count", i64), ("width", dbl),
-        ("max_diag", dbl), ("step", i64), ("it", i64), ("err", dbl), ("avg_div", dbl), ("max_cfl", dbl)]
+        ("max_diag", dbl), ("step", i64), ("it", i64), ("err", dbl), ("avg_div", dbl), ("max_cfl", dbl),
+        ("inflow_modes", P), ("inflow_n", i64), ("inflow_speed", dbl), ("inflow_cells", P), ("inflow_face", P)]
 
 
 class OracleSolver:
@@ -252,6 +256,16 @@ class OracleSolver:
             s.atk_count, s.width = len(self.atk), case.width
             if ap_hook is not None:
                 ap_hook(self.ap, self.wt)
+        modes = case.inflow_modes() if hasattr(case, "inflow_modes") else None
+        if modes is not None and len(modes):
+            g0 = m.gc
+            ny, nz = m.size[1] - 2 * g0, m.size[2] - 2 * g0
+            self.inflow_modes = np.ascontiguousarray(modes, dtype=np.float64)
+            self.inflow_cells = np.zeros(g0 * ny * nz * 3)
+            self.inflow_face = np.zeros(ny * nz)
+            s.inflow_modes, s.inflow_n = ptr(self.inflow_modes), len(self.inflow_modes)
+            s.inflow_speed = case.inflow_speed()
+            s.inflow_cells, s.inflow_face = ptr(self.inflow_cells), ptr(self.inflow_face)
         self.s = s
         self.L, _ = lib(kind)
         if case.tgg is not None:

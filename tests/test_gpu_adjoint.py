@@ -318,17 +318,19 @@ def test_autograd_harness():
 def test_recording_through_breakdowns():
     """A BiCGStab breakdown (|rho| < FLT_MIN at the first inner product) ends the
     step's solve and the reference carries on (src/eq.h:349-353); such a step must be
-    recorded like any other.  Blades of chord 1e-30 in a uniform inflow give a
-    force field of order 1e-30, so every step's right-hand side has <r0, r> ~ 1e-60
-    and breaks down at once, while dJ/d(chord) stays of order one.  The reverse
-    sweep over three such steps must match the oracle's.  A recording that holds a
-    non-finite step is refused."""
+    recorded like any other.  Blades of chord 1e-30 turning in fluid at rest (zero
+    inflow, no noise: the walls, the inflow and the outflow all hold U = 0 exactly)
+    give a force field of order 1e-30, so every step's right-hand side has
+    <r0, r> ~ 1e-60 and breaks down at once, while dJ/d(chord) stays of order 0.1.
+    The reverse sweep over three such steps must match the oracle's.  A recording
+    that holds a non-finite step is refused."""
     steps = 3
     case = cases.case_adjoint_small(steps=steps)
     case.noise = 0.0
+    case.uin = (0.0, 0.0, 0.0)
     case.turbine.stations_chord = np.full_like(case.turbine.stations_chord, 1e-30)
     tq_w = np.array([case.sites[0].rot_speed / steps])
-    print("[adjoint run through breakdowns: 20x12x12, chord 1e-30, uniform inflow]", flush=True)
+    print("[adjoint run through breakdowns: 20x12x12, chord 1e-30, fluid at rest]", flush=True)
     J_o, g_o = _oracle_adjoint(case, steps, tq_w)
     s = solver.Solver(case)
     try:
@@ -349,7 +351,8 @@ def test_recording_through_breakdowns():
         s.ctx.adjoint_begin(2, 1, tq_w, False)
         s.step()
         U = s.field("U")
-        U[U.shape[0] // 2, 0] = np.nan
+        n0, n1, n2 = case.mesh.size
+        U[((n0 // 2) * n1 + n1 // 2) * n2 + n2 // 2, 0] = np.nan  # an interior cell at the box centre
         s.ctx.upload("U", U)
         st = s.step()
         assert st["status"] == hip.NONFINITE, st
