@@ -361,12 +361,13 @@ def case_c3(nx=512, ny=256, nz=256, steps=500, ap_per_blade=64, tol=4e-9) -> Cas
     5 D apart at x = 3 and 8, TSR 6, eps = 2 dx, a seeded isotropic turbulence of 200
     random Fourier modes at 10 % intensity as the initial field and, frozen and
     convected at Uin, as the inflow; BiCGStab(Jacobi 3) to 4e-9, the tolerance that
-    keeps avg div <= 1e-6 at this size (dt * max_diag = 230)."""
+    keeps avg div <= 1e-6 at this size (dt * max_diag = 230), at most 6000
+    iterations (the first steps after the impulsive start need 1000-3000)."""
     m = make_mesh(nx, ny, nz, ((0.0, 16.0), (-4.0, 4.0), (0.0, 8.0)), 1.5, 1.5)
     dx = 16.0 / nx
     t = default_turbine(R=0.5, tower=4.0)
     sites = [TurbineSite((3.0, 0.0, 0.0), 12.0), TurbineSite((8.0, 0.0, 0.0), 12.0)]
-    return Case("c3", m, method="BiCG", pc="Jacobi", pc_it=3, tol=tol, max_it=1000, steps=steps, noise=0.10,
+    return Case("c3", m, method="BiCG", pc="Jacobi", pc_it=3, tol=tol, max_it=6000, steps=steps, noise=0.10,
                 modes=200, inflow=True, turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
 
 
@@ -388,12 +389,13 @@ def case_c5(nx=768, ny=384, nz=384, steps=2000, ap_per_blade=96) -> Case:
     """The largest configuration: 768x384x384 on a 32x16x8 box, a 3x3 array of
     rotors (D = 1) at 6 D along x and 4 D along y from (4, -4, 0), TSR 6,
     eps = 2 dx, a turbulence of 200 random Fourier modes at 8 % intensity as the
-    initial field and the inflow, BiCGStab(Jacobi 3) to 1e-8."""
+    initial field and the inflow, BiCGStab(Jacobi 3) to 1e-8, at most 6000
+    iterations."""
     m = make_mesh(nx, ny, nz, ((0.0, 32.0), (-8.0, 8.0), (0.0, 8.0)), 1.5, 1.5)
     dx = 32.0 / nx
     t = default_turbine(R=0.5, tower=4.0)
     sites = [TurbineSite((4.0 + 6.0 * a, -4.0 + 4.0 * b, 0.0), 12.0) for a in range(3) for b in range(3)]
-    return Case("c5", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000, steps=steps, noise=0.08,
+    return Case("c5", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=6000, steps=steps, noise=0.08,
                 modes=200, inflow=True,
                 turbine=t, sites=sites, ap_per_blade=ap_per_blade, width=2 * dx)
 

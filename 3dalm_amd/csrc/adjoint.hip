@@ -1115,6 +1115,33 @@ int save_state(dalm_ctx* ctx, double* dst) {
     return DALM_OK;
 }
 
+__global__ void ka_widen(long long n, const float* __restrict__ a, double* __restrict__ b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+// The precision form's state (src, DALM_MIXED) as an FP64 checkpoint in
+// state_ptrs' order: its FP32 U, nu_t and f widened exactly, its FP64 JU and p
+// copied.  Enqueued on src's stream.
+int save_state_from(dalm_ctx* src, double* dst) {
+    dalm_ctx* ctx = src;  // for HIP_TRY's error report
+    const size_t L = (size_t)src->L;
+    const int grid = (int)std::min<long long>(4096, ((long long)L + 255) / 256);
+    const float* w[5] = {src->U32[0], src->U32[1], src->U32[2], src->nut32, nullptr};
+    const int wslot[4] = {0, 1, 2, 7};
+    for (int q = 0; q < 4; ++q)
+        hipLaunchKernelGGL(ka_widen, dim3(grid), dim3(256), 0, src->stream, (long long)L, w[q], dst + wslot[q] * L);
+    for (int q = 0; q < 3; ++q) {
+        hipLaunchKernelGGL(ka_widen, dim3(grid), dim3(256), 0, src->stream, (long long)L, (const float*)src->f32[q],
+                           dst + (8 + q) * L);
+        HIP_TRY(hipMemcpyAsync(dst + (3 + q) * L, src->JU[q], L * sizeof(double), hipMemcpyDeviceToDevice,
+                               src->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(dst + 6 * L, src->p, L * sizeof(double), hipMemcpyDeviceToDevice, src->stream));
+    HIP_TRY(hipGetLastError());
+    return DALM_OK;
+}
+
 int load_state(dalm_ctx* ctx, const double* src) {
     double* v[CK_FIELDS];
     state_ptrs(ctx, v);
@@ -1211,8 +1238,52 @@ int adjoint_record(dalm_ctx* ctx, int status) {
     return DALM_OK;
 }
 
+// One dalm_step of a precision-form context recording into its FP64 sink
+// (dalm_adjoint_begin_from): the same bookkeeping as adjoint_record, with the
+// checkpoints widened from the precision form's state.
+int adjoint_record_from(dalm_ctx* src, int status) {
+    dalm_ctx* sink = src->adj.sink;
+    if (!sink) return DALM_OK;
+    AdjointState& S = sink->adj;
+    dalm_ctx* ctx = src;  // errors are reported on the stepping context
+    const long long n = src->step - S.start_step;
+    if (n < 1 || n > S.steps || n != S.recorded + 1) {
+        S.taping = false;
+        S.tape_bad = true;
+        src->adj.sink = nullptr;
+        S.source = nullptr;
+        return dalm_fail_msg(src, DALM_BAD_ARG, "recorded steps out of sequence or beyond the recording's length");
+    }
+    if (status == DALM_NONFINITE) S.tape_bad = true;
+    const size_t L = (size_t)src->L;
+    if (S.keep_p)
+        HIP_TRY(hipMemcpyAsync(S.ptape + (size_t)(n - 1) * L, src->p, L * sizeof(double), hipMemcpyDeviceToDevice,
+                               src->stream));
+    if (n % S.every == 0)
+        if (int rc = save_state_from(src, S.ck + (size_t)(n / S.every) * CK_FIELDS * L)) return rc;
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    for (int w = 0; w < src->alm.wt_count; ++w) S.J += S.h_tqw[w] * src->h_scal->torque[w];
+    S.recorded += 1;
+    sink->step = src->step;
+    if (S.recorded == S.steps) {
+        S.taping = false;
+        S.source = nullptr;
+        src->adj.sink = nullptr;
+    }
+    return DALM_OK;
+}
+
 void adjoint_free(dalm_ctx* ctx) {
     AdjointState& S = ctx->adj;
+    // a recording between two contexts ends with either of them
+    if (S.sink) {
+        S.sink->adj.source = nullptr;
+        if (S.sink->adj.taping) {
+            S.sink->adj.taping = false;
+            S.sink->adj.tape_bad = true;
+        }
+    }
+    if (S.source) S.source->adj.sink = nullptr;
     double** big[4] = {&S.ck, &S.ptape, &S.seg, &S.fin};
     for (double** p : big)
         if (*p) {
@@ -1345,12 +1416,20 @@ int dalm_run_alm_adj(dalm_ctx* ctx, double t, const double* tq_w, double* ap_gra
 }
 
 // ---- recorded run and reverse sweep --------------------------------------------------
-int dalm_adjoint_begin(dalm_ctx* ctx, int64_t steps, int64_t every, const double* tq_w, int64_t flags) {
-    CHECK_ADJ();
+}  // extern "C"
+
+namespace {
+
+// The tape of a recording of `steps` steps from step `start` (checkpoint every
+// `every` steps, pressure per step with DALM_ADJ_KEEP_PRESSURE), allocated in ctx
+// after a check against the free device memory.
+int begin_tape(dalm_ctx* ctx, int64_t steps, int64_t every, const double* tq_w, int64_t flags, long long start) {
     // bounded so that the recording's byte counts below cannot overflow
     if (steps < 1 || every < 1 || steps > (1LL << 24)) return DALM_BAD_ARG;
     if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
     AdjointState& S = ctx->adj;
+    if (S.source) S.source->adj.sink = nullptr;
+    S.source = nullptr;
     double** big[4] = {&S.ck, &S.ptape, &S.seg, &S.fin};
     for (double** p : big)
         if (*p) {
@@ -1361,7 +1440,7 @@ int dalm_adjoint_begin(dalm_ctx* ctx, int64_t steps, int64_t every, const double
     S.steps = steps;
     S.every = every;
     S.keep_p = (flags & DALM_ADJ_KEEP_PRESSURE) != 0;
-    S.start_step = ctx->step;
+    S.start_step = start;
     S.recorded = 0;
     S.tape_bad = false;
     S.J = 0;
@@ -1381,8 +1460,41 @@ int dalm_adjoint_begin(dalm_ctx* ctx, int64_t steps, int64_t every, const double
     HIP_TRY(hipMalloc((void**)&S.seg, (size_t)(every + 1) * SEG_FIELDS * L * sizeof(double)));
     HIP_TRY(hipMalloc((void**)&S.fin, CK_FIELDS * L * sizeof(double)));
     if (S.keep_p) HIP_TRY(hipMalloc((void**)&S.ptape, (size_t)steps * L * sizeof(double)));
+    return DALM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dalm_adjoint_begin(dalm_ctx* ctx, int64_t steps, int64_t every, const double* tq_w, int64_t flags) {
+    CHECK_ADJ();
+    TRY(begin_tape(ctx, steps, every, tq_w, flags, ctx->step));
+    AdjointState& S = ctx->adj;
     TRY(save_state(ctx, S.ck));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    S.taping = true;
+    return DALM_OK;
+}
+
+int dalm_adjoint_begin_from(dalm_ctx* ctx, dalm_ctx* src, int64_t steps, int64_t every, const double* tq_w,
+                            int64_t flags) {
+    CHECK_ADJ();
+    if (!src || src == ctx || src->precision != DALM_MIXED)
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_adjoint_begin_from records a DALM_MIXED context");
+    const Dims &a = ctx->d, &b = src->d;
+    if (a.n0 != b.n0 || a.n1 != b.n1 || a.n2 != b.n2 || a.gc != b.gc || ctx->alm.wt_count != src->alm.wt_count ||
+        ctx->alm.ap_count != src->alm.ap_count)
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "the recorded context has another grid or actuator-line set-up");
+    if (!src->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called on the recorded context");
+    if (src->adj.sink) return dalm_fail_msg(ctx, DALM_BAD_ARG, "the context is already being recorded");
+    TRY(begin_tape(ctx, steps, every, tq_w, flags, src->step));
+    AdjointState& S = ctx->adj;
+    TRY(save_state_from(src, S.ck));
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    ctx->step = src->step;
+    S.source = src;
+    src->adj.sink = ctx;
     S.taping = true;
     return DALM_OK;
 }

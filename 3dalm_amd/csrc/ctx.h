@@ -144,6 +144,11 @@ struct AdjointState {
     // cannot be replayed and dalm_adjoint_run refuses it
     bool tape_bad = false;
     long long steps = 0, every = 0, recorded = 0, start_step = 0;
+    // precision-form recording (dalm_adjoint_begin_from): this FP64 context holds
+    // the tape of `source`, a DALM_MIXED context whose dalm_step calls record into
+    // it; `sink` is the reverse link, set on the source
+    dalm_ctx* source = nullptr;
+    dalm_ctx* sink = nullptr;
     bool keep_p = false;
     double* ck = nullptr;     // checkpoints [steps/every + 1][11][L]: U, JU, p, nut, f
     double* ptape = nullptr;  // pressure after each step [steps][L] (keep_p)
@@ -311,10 +316,16 @@ int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given);
 // adjoint.hip: record a forward step on the tape, free the adjoint memory
 // status: the step's dalm_status (OK, BREAKDOWN or NONFINITE)
 int adjoint_record(dalm_ctx* ctx, int status);
+int adjoint_record_from(dalm_ctx* src, int status);
 void adjoint_free(dalm_ctx* ctx);
 int ensure_adjoint_buffers(dalm_ctx* ctx);
 int sync_stream(dalm_ctx* ctx);
 int zero_device(dalm_ctx* ctx, double* a, long long n);
+// fused half-iterations of the precision form's BiCGStab (fused_mixed.hip)
+bool fused_half_applies(const dalm_ctx* ctx, int pc_kind, int pc_it);
+int fused_half_grid(const dalm_ctx* ctx, int* ni);
+int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* z_out, float* q_out);
+int launch_half_s(dalm_ctx* ctx, const float* q, float* s_out, float* z_out, float* t_out);
 void probe_begin(dalm_ctx* ctx, int kernel);
 // launches: back-to-back launches of the kernel between the two events
 void probe_end(dalm_ctx* ctx, int kernel, int launches = 1);

@@ -852,8 +852,22 @@ int dalm_step(dalm_ctx* ctx, dalm_step_stats* stats) {
     if (!ctx->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called");
     if (ctx->precision == DALM_MIXED) {
         ctx->step += 1;
-        return step_mixed(ctx, ctx->prm.dt * (double)ctx->step, stats);
+        const int rc = step_mixed(ctx, ctx->prm.dt * (double)ctx->step, stats);
+        if (dalm_ctx* sink = ctx->adj.sink) {
+            // recorded into an FP64 context (dalm_adjoint_begin_from), on the same terms
+            if (rc == DALM_OK || rc == DALM_BREAKDOWN || rc == DALM_NONFINITE) {
+                TRY(adjoint_record_from(ctx, rc));
+            } else {
+                sink->adj.tape_bad = true;
+                sink->adj.taping = false;
+                sink->adj.source = nullptr;
+                ctx->adj.sink = nullptr;
+            }
+        }
+        return rc;
     }
+    if (ctx->adj.source)
+        return dalm_fail_msg(ctx, DALM_BAD_ARG, "this context is recording a precision-form context's steps");
     const int rc = step_fp64(ctx, stats, nullptr);
     if (ctx->adj.taping) {
         // a BiCGStab breakdown ends the solve and the step goes on, as in the
@@ -895,7 +909,7 @@ int dalm_set_fp32_pass_cap(dalm_ctx* ctx, int64_t n) {
 
 int dalm_probe_kernel(dalm_ctx* ctx, int64_t kernel) {
     CHECK_CTX();
-    if (kernel < 0 || kernel > DALM_K_TIME_AVERAGE) return DALM_BAD_ARG;
+    if (kernel < 0 || kernel > DALM_K_HALF_S) return DALM_BAD_ARG;
     TRY(probe_collect(ctx));
     ctx->probe.kernel = (int)kernel;
     ctx->probe.total_ms = 0;

@@ -18,6 +18,8 @@
 // reference's bit for bit; dot products differ only in summation order.
 #include <float.h>
 
+#include <utility>
+
 #include "ctx.h"
 #include "stencil.h"
 
@@ -470,6 +472,149 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv4(Dims d, PCoef A, const doubl
     store_partials<2>(pv, partials);
 }
 
+// ---- fused update + preconditioner (rows of exactly 256 cells, Jacobi x3) -----------
+// The vector update of a BiCGStab half-iteration and the three Jacobi sweeps that
+// follow it (src/eq.h:355-380, 6-38) as one kernel: p = r + beta (p - omega q) (or
+// s = r - alpha q), z1 = v / a_c, z2 = J(z1; v), z3 = J(z2; v), writing only v and z3
+// (40 / 32 B per cell instead of 40 + 48 / 32 + 48 for the update and two sweeps
+// through HBM).  Temporal blocking along i as in the precision form's fused
+// half-iteration, two stencil levels deep: a block of 12 wavefronts (one j row
+// each, four k per lane; 12 rather than 16 so that a wavefront may hold 170 VGPRs
+// of FP64 pipeline state) owns 8 rows and computes 2 / 1 halo rows of levels 0 / 1
+// on either side; level 0 runs at plane t, the sweeps at t-1 and t-2; the new plane
+// of each sweep's input goes through LDS (read, barrier, compute, write, barrier).
+// The SpMV that ends the half-iteration stays a separate kernel (k_bi_spmv4).
+// Arithmetic per cell identical to k_bi_p4 / k_bi_s4 and k_jacobi4 (bit for bit);
+// the iterates' ghost cells are zero, as in the reference.  p and q are read at
+// halo cells owned by neighbouring blocks, so the p-half reads them from one buffer
+// of a pair and writes the new p to the other.
+constexpr int kFRows = 12;
+constexpr int kFOwn = kFRows - 4;
+constexpr int kFLag = 2;
+constexpr int kFPad = 4;
+constexpr int kFRowLen = 256 + 2 * kFPad;
+typedef double d4a __attribute__((ext_vector_type(kVecK), aligned(32)));
+
+struct FLds {
+    alignas(32) double lv[2][kFRows][kFRowLen];  // the new planes of z1 and z2
+    double at[256], ab[256], atu[256], abu[256];
+};
+
+template <int WHICH>
+__global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const double* __restrict__ r,
+                                                          const double* __restrict__ vin,
+                                                          const double* __restrict__ q, double* __restrict__ vout,
+                                                          double* __restrict__ zout, int ni, const SolveState* st) {
+    if (st->done) return;
+    __shared__ FLds sh;
+    const int lane = threadIdx.x & 63, row = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int g = d.gc;
+    const int nch = (d.nx + ni - 1) / ni;
+    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
+    const int ic = u % nch, jt = u / nch;
+    const int own0 = g + ic * ni;
+    int i1 = own0 + ni;
+    if (i1 > g + d.nx) i1 = g + d.nx;
+    const int i0 = i1 - ni;
+    const int j = g + jt * kFOwn + row - kFLag;
+    const bool jin = j >= g && j < g + d.ny;
+    const bool owned = jin && row >= kFLag && row < kFRows - kFLag;
+    const bool l1 = jin && row >= 1 && row < kFRows - 1;
+    const int k0 = g + 4 * lane, lk = 4 * lane, o = kFPad + lk;
+    const int rr = row == 0 ? 1 : (row == kFRows - 1 ? kFRows - 2 : row);  // neighbour rows stay in range
+    const double beta = st->beta, om = st->omega, alpha = st->alpha;
+    const bool first = st->it == 0;
+    const int jc = jin ? j : g;
+    const double an = A.an[jc], as = A.as[jc], anu = A.anu[jc], asu = A.asu[jc];
+    for (int t = threadIdx.x; t < 256; t += 64 * kFRows) {
+        sh.at[t] = A.at[g + t];
+        sh.ab[t] = A.ab[g + t];
+        sh.atu[t] = A.atu[g + t];
+        sh.abu[t] = A.abu[g + t];
+    }
+    for (int t = threadIdx.x; t < 2 * kFRows * kFRowLen; t += 64 * kFRows) (&sh.lv[0][0][0])[t] = 0.0;
+    __syncthreads();
+
+    const d4k zero = {0.0, 0.0, 0.0, 0.0};
+    d4k z1a = zero, z1b = zero;  // z1 at t-1, t-2
+    d4k va = zero, vb = zero;    // v at t-1, t-2
+    d4k z2a = zero, z2b = zero;  // z2 at t-2, t-3
+    struct In {
+        d4k r, v, q;
+    };
+    auto load0 = [&](int i) {
+        In x{zero, zero, zero};
+        if (jin && i >= g && i < g + d.nx) {
+            const int c = d.at(i, j, k0);
+            x.r = ld4(r, c);
+            x.q = ld4(q, c);
+            if (WHICH == 0) x.v = ld4(vin, c);
+        }
+        return x;
+    };
+    struct Nb {
+        d4k n, s;
+        double kb, kf;
+    };
+    auto nbrs = [&](int lev) {
+        Nb b;
+        b.n = *(const d4a*)&sh.lv[lev][rr + 1][o];
+        b.s = *(const d4a*)&sh.lv[lev][rr - 1][o];
+        b.kb = sh.lv[lev][rr][o - 1];
+        b.kf = sh.lv[lev][rr][o + 4];
+        return b;
+    };
+    // one Jacobi sweep at plane ip (k_jacobi4's arithmetic)
+    auto sweep = [&](int ip, const Nb& nb, const d4k& xb, const d4k& xa, const d4k& xn, const d4k& rhs) {
+        const double acj = (A.acx[ip] - anu) - asu, ae = A.ae[ip], aw = A.aw[ip];
+        d4k out;
+#pragma unroll
+        for (int m = 0; m < kVecK; ++m) {
+            const double ac = ((acj - sh.atu[lk + m]) - sh.abu[lk + m]) / A.max_diag;
+            const double tq = m + 1 < kVecK ? xa[m + 1] : nb.kf;
+            const double bq = m > 0 ? xa[m - 1] : nb.kb;
+            const double ax = ac * xa[m] + ae * xn[m] + aw * xb[m] + an * nb.n[m] + as * nb.s[m] +
+                              sh.at[lk + m] * tq + sh.ab[lk + m] * bq;
+            out[m] = xa[m] + (rhs[m] - ax) / ac;
+        }
+        return out;
+    };
+    In nx0 = load0(i0 - kFLag);
+    for (int t = i0 - kFLag; t < i1 + kFLag; ++t) {
+        // level 0 at plane t, then the next plane's inputs are requested
+        d4k z1n = zero, vn = zero;
+        if (jin && t >= g && t < g + d.nx) {
+            const double acj = (A.acx[t] - anu) - asu;
+#pragma unroll
+            for (int m = 0; m < kVecK; ++m) {
+                if (WHICH == 0)
+                    vn[m] = first ? nx0.r[m] : nx0.r[m] + beta * (nx0.v[m] - om * nx0.q[m]);
+                else
+                    vn[m] = nx0.r[m] - alpha * nx0.q[m];
+                z1n[m] = vn[m] / (((acj - sh.atu[lk + m]) - sh.abu[lk + m]) / A.max_diag);
+            }
+            if (owned && t >= own0 && t < i1) *(d4k*)(vout + d.at(t, j, k0)) = vn;
+        }
+        nx0 = load0(t + 1);
+        // neighbours of the centre planes of both sweeps (written at the previous step)
+        const Nb n1 = nbrs(0), n2 = nbrs(1);
+        __syncthreads();
+        const int p1 = t - 1, p2 = t - 2;
+        d4k z2n = zero;
+        if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 1) z2n = sweep(p1, n1, z1b, z1a, z1n, va);
+        if (owned && p2 >= own0 && p2 < i1) *(d4k*)(zout + d.at(p2, j, k0)) = sweep(p2, n2, z2b, z2a, z2n, vb);
+        *(d4a*)&sh.lv[0][row][o] = z1n;
+        *(d4a*)&sh.lv[1][row][o] = z2n;
+        __syncthreads();
+        z1b = z1a;
+        z1a = z1n;
+        vb = va;
+        va = vn;
+        z2b = z2a;
+        z2a = z2n;
+    }
+}
+
 __global__ void k_zero(long long n, double* __restrict__ a) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
          e += (long long)gridDim.x * blockDim.x)
@@ -587,6 +732,12 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     const bool vec4 = d.nz % (TK * kVecK) == 0;
     const int grid4 = d.tj * (d.nz / (TK * kVecK)) * d.nch;
     const int nb = vec4 ? grid4 : ctx->sgrid;
+    // rows of exactly 256 cells with three Jacobi sweeps: the update and the sweeps of
+    // each half-iteration are one kernel (k_half_pc4); p and q alternate between a
+    // buffer pair (the Jacobi partners, unused on that path)
+    const bool fused = jac && pc_it == 3 && d.nz == 256 && d.ny >= kFOwn;
+    const int fni = d.nx < 32 ? d.nx : 32;
+    const int fgrid = ((d.ny + kFOwn - 1) / kFOwn) * ((d.nx + fni - 1) / fni);
     // preconditioner application for one vector; returns the buffer holding z
     auto precondition = [&](double* z0, double* spare, const double* v) -> double* {
         double* cur = z0;
@@ -619,11 +770,44 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     long long launched = 0;
     int rc = read_state(ctx);
     if (rc) return rc;
+    const long long it_start = ctx->h_solve->it;
     probe_solve_begin(ctx);
     while (!ctx->h_solve->done && launched < max_it) {
         long long chunk = next_chunk(launched, ctx->prev_it[slot], max_it);
         for (long long c = 0; c < chunk; ++c) {
             probe_iteration(ctx, launched + c);
+            if (fused) {
+                // iteration k reads p, q from the pair's k-parity buffer and writes the other
+                const long long k = launched + c;
+                double* P[2] = {ctx->kp, ctx->tmp};
+                double* Q[2] = {ctx->q, ctx->tmp2};
+                probe_begin(ctx, DALM_K_HALF_P);
+                hipLaunchKernelGGL(k_half_pc4<0>, dim3(fgrid), dim3(64 * kFRows), 0, ctx->stream, d, A,
+                                   (const double*)ctx->r, (const double*)P[k & 1], (const double*)Q[k & 1],
+                                   P[(k + 1) & 1], ctx->pp, fni, (const SolveState*)ctx->solve);
+                probe_end(ctx, DALM_K_HALF_P);
+                probe_begin(ctx, DALM_K_SPMV);
+                hipLaunchKernelGGL(k_bi_spmv4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ctx->pp,
+                                   Q[(k + 1) & 1], ctx->r0, ctx->partials, ctx->solve);
+                probe_end(ctx, DALM_K_SPMV);
+                if (int rc1 = launch_finalize(ctx, FIN_ALPHA, grid4, ctx->solve, nullptr)) return rc1;
+                probe_begin(ctx, DALM_K_HALF_S);
+                hipLaunchKernelGGL(k_half_pc4<1>, dim3(fgrid), dim3(64 * kFRows), 0, ctx->stream, d, A,
+                                   (const double*)ctx->r, (const double*)nullptr, (const double*)Q[(k + 1) & 1],
+                                   ctx->s, ctx->ss, fni, (const SolveState*)ctx->solve);
+                probe_end(ctx, DALM_K_HALF_S);
+                probe_begin(ctx, DALM_K_SPMV);
+                hipLaunchKernelGGL(k_bi_spmv4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ctx->ss, ctx->t,
+                                   ctx->s, ctx->partials, ctx->solve);
+                probe_end(ctx, DALM_K_SPMV);
+                if (int rc2 = launch_finalize(ctx, FIN_OMEGA, grid4, ctx->solve, nullptr)) return rc2;
+                probe_begin(ctx, DALM_K_UPDATE);
+                hipLaunchKernelGGL(k_bi_xr4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, ctx->pp, ctx->ss,
+                                   ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->solve);
+                probe_end(ctx, DALM_K_UPDATE);
+                if (int rc3 = launch_finalize(ctx, FIN_BI_XR, grid4, ctx->solve, nullptr)) return rc3;
+                continue;
+            }
             probe_begin(ctx, DALM_K_PUPDATE);
             if (vec4)
                 hipLaunchKernelGGL(k_bi_p4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ctx->r, ctx->kp, ctx->q,
@@ -676,6 +860,11 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
         if (rc) return rc;
     }
     probe_solve_end(ctx, ctx->h_solve->it);
+    if (fused && ((ctx->h_solve->it - it_start) & 1)) {
+        // after an odd number of iterations the current p and q are the partners
+        std::swap(ctx->kp, ctx->tmp);
+        std::swap(ctx->q, ctx->tmp2);
+    }
     *it = ctx->h_solve->it;
     *err = ctx->h_solve->err;
     ctx->prev_it[slot] = *it;

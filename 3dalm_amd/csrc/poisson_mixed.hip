@@ -699,6 +699,12 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     // rows of a multiple of 256 interior cells: the four-per-lane Jacobi sweep
     const bool vec4 = d.nz % (TK * kVecK) == 0;
     const int grid4 = d.tj * (d.nz / (TK * kVecK)) * d.nch;
+    // rows of exactly 256 cells with three Jacobi sweeps: each half-iteration is one
+    // fused kernel (fused_mixed.hip); p and q alternate between a buffer pair (the
+    // Jacobi partners, which the fused path does not use)
+    const bool fused = fused_half_applies(ctx, pc_kind, pc_it);
+    int fni = 0;
+    const int fgrid = fused ? fused_half_grid(ctx, &fni) : 0;
     // FP32 preconditioner application on v32 into z0 (spare: the other Jacobi buffer)
     auto precondition = [&](float* z0, float* spare, const float* v32) -> float* {
         float* cur = z0;
@@ -788,6 +794,31 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             const long long chunk = chunk_for(launched, ctx->prev_it[1], max_it - total);
             for (long long c = 0; c < chunk; ++c) {
                 probe_iteration(ctx, launched + c);
+                if (fused) {
+                    // iteration k reads p, q from the pair's k-parity buffer and writes the other
+                    const long long k = launched + c;
+                    float* P[2] = {ctx->fp, ctx->ftmp};
+                    float* Q[2] = {ctx->fq, ctx->ftmp2};
+                    probe_begin(ctx, DALM_K_HALF_P_MIXED);
+                    if ((rc = launch_half_p(ctx, P[k & 1], Q[k & 1], P[(k + 1) & 1], ctx->fz, Q[(k + 1) & 1])))
+                        return dalm_fail_msg(ctx, rc, "fused p half-iteration launch");
+                    probe_end(ctx, DALM_K_HALF_P_MIXED);
+                    if ((rc = launch_finalize(ctx, FIN_ALPHA, fgrid, st, nullptr))) return rc;
+                    probe_begin(ctx, DALM_K_HALF_S_MIXED);
+                    if ((rc = launch_half_s(ctx, Q[(k + 1) & 1], ctx->fs, ctx->fz2, ctx->ft)))
+                        return dalm_fail_msg(ctx, rc, "fused s half-iteration launch");
+                    probe_end(ctx, DALM_K_HALF_S_MIXED);
+                    if ((rc = launch_finalize(ctx, FIN_OMEGA, fgrid, st, nullptr))) return rc;
+                    probe_begin(ctx, DALM_K_UPDATE_MIXED);
+                    hipLaunchKernelGGL(kg_xr4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fz, ctx->fz2,
+                                       ctx->fs, ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+                    probe_end(ctx, DALM_K_UPDATE_MIXED);
+                    if ((rc = launch_finalize(ctx, FIN_MX_XR, grid4, st, nullptr))) return rc;
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<false>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream,
+                                       d, A, ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
+                    if ((rc = launch_finalize(ctx, FIN_MX_RU, ctx->sgrid, st, nullptr))) return rc;
+                    continue;
+                }
                 probe_begin(ctx, DALM_K_PUPDATE_MIXED);
                 if (vec4)
                     hipLaunchKernelGGL(kg_p4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, Af, ctx->fr, ctx->fp,
@@ -866,6 +897,11 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         }
         const SolveState& h = ctx->h_solve[0];
         probe_solve_end(ctx, h.it);
+        if (fused && (h.it & 1)) {
+            // after an odd number of iterations the current p and q are the partners
+            std::swap(ctx->fp, ctx->ftmp);
+            std::swap(ctx->fq, ctx->ftmp2);
+        }
         total += h.it;
         n_ru += h.n_ru;
         status = h.status;

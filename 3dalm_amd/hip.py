@@ -141,6 +141,7 @@ def lib():
         "dalm_poisson_adj": [V, dbl, i64, PI64, P],
         "dalm_run_alm_adj": [V, dbl, P, P],
         "dalm_adjoint_begin": [V, i64, i64, P, i64],
+        "dalm_adjoint_begin_from": [V, V, i64, i64, P, i64],
         "dalm_adjoint_run": [V, dbl, P, P, P, C.POINTER(AdjointStats)],
         "dalm_adjoint_end": [V],
     }
@@ -418,6 +419,14 @@ class Context:
         tq[:len(tq_w)] = tq_w
         self.check(self.L.dalm_adjoint_begin(self.h, steps, every, ptr(tq),
                                              ADJ_KEEP_PRESSURE if keep_pressure else 0), "adjoint_begin")
+
+    def adjoint_begin_from(self, src, steps, every, tq_w, keep_pressure=True):
+        """Record the next `steps` steps of the precision-form context `src` into this
+        FP64 context's tape (dalm_adjoint_begin_from)."""
+        tq = np.zeros(MAX_TURBINES)
+        tq[:len(tq_w)] = tq_w
+        self.check(self.L.dalm_adjoint_begin_from(self.h, src.h, steps, every, ptr(tq),
+                                                  ADJ_KEEP_PRESSURE if keep_pressure else 0), "adjoint_begin_from")
 
     def adjoint_run(self, tol_rel=1e-10):
         J = dbl()

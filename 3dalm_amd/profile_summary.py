@@ -44,6 +44,8 @@ ALGO_BYTES = {
     "kf_sor": 10, "ks_resid": 24, "ks_add": 20,
     "k_jacobi4": 24, "k_bi_spmv4<0>": 24, "k_bi_spmv4<1>": 24, "k_bi_p4": 40, "k_bi_s4": 32, "k_bi_xr4": 64,
     "kf_jacobi4": 12, "kg_spmv4<0>": 12, "kg_spmv4<1>": 12, "kg_p4": 20, "kg_s4": 16, "kg_xr4": 40,
+    # fused half-iterations (rows of 256 cells, Jacobi x3)
+    "kg_half4<0>": 28, "kg_half4<1>": 20, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
 }
 # kernels that sweep every padded cell: algorithmic bytes per padded cell
 # (needs --padded-cells)

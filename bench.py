@@ -75,7 +75,10 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_alm_512x256x256.json"
 PROBE_KERNEL = {"jacobi": "k_jacobi4", "jacobi32": "kf_jacobi4", "spmv": "k_bi_spmv4<0>", "update": "k_bi_xr4",
                 "pupdate": "k_bi_p4", "supdate": "k_bi_s4", "predictor": "k_predictor",
                 "spmv_mixed": "kg_spmv4<0>", "update_mixed": "kg_xr4", "pupdate_mixed": "kg_p4",
-                "supdate_mixed": "kg_s4"}
+                "supdate_mixed": "kg_s4",
+                # rows of exactly 256 cells: the fused half-iterations
+                "half_p_mixed": "kg_half4<0>", "half_s_mixed": "kg_half4<1>",
+                "half_p": "k_half_pc4<0>", "half_s": "k_half_pc4<1>"}
 
 # algorithmic bytes per interior cell for one launch of each probed kernel (FP64):
 # compulsory reads + writes, with the operator applied matrix-free from 1-D coefficients
@@ -94,15 +97,30 @@ PROBE_BYTES = {
     "update_mixed": 40,  # read x (FP64), z, z', s, t, r0 (FP32); write x (FP64), r (FP32)
     "pupdate_mixed": 20,  # read r, p, q; write p, z (FP32)
     "supdate_mixed": 16,  # read r, q; write s, z (FP32)
+    # fused half-iterations (rows of 256 cells, Jacobi x3): every intermediate level
+    # stays on chip
+    "half_p_mixed": 28,  # read r, p, q, r0; write p, z3, q (FP32)
+    "half_s_mixed": 20,  # read r, q; write s, z3, t (FP32)
+    "half_p": 40,        # read r, p, q; write p, z3 (FP64)
+    "half_s": 32,        # read r, q; write s, z3 (FP64)
 }
 # algorithmic bytes per interior cell of one whole step: the fixed kernels (predictor,
 # flux + right-hand side, projection, post with div stored) and one BiCGStab(Jacobi 3)
 # iteration, plus the solve's start (residual and shadow residual)
 STEP_BYTES = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 280, "per_solve": 32},
               "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 148, "per_solve": 28}}
+# with the fused half-iterations (rows of exactly 256 cells, Jacobi x3): FP64 40 + 24
+# (SpMV) + 32 + 24 + 64 (x/r update); precision form 28 + 20 + 40
+STEP_BYTES_FUSED = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 184, "per_solve": 32},
+                    "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 88, "per_solve": 28}}
 PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
              "supdate": 8, "jacobi32": 9, "spmv_mixed": 10, "update_mixed": 11, "pupdate_mixed": 12,
-             "supdate_mixed": 13}
+             "supdate_mixed": 13, "half_p_mixed": 16, "half_s_mixed": 17, "half_p": 18, "half_s": 19}
+
+
+def fused_rows(mesh, case):
+    """The half-iterations run fused on rows of exactly 256 cells with BiCGStab(Jacobi x3)."""
+    return mesh.interior[2] == 256 and case.method == "BiCG" and case.pc == "Jacobi" and case.pc_it == 3
 FORMS = ("fp64", "mixed")
 
 
@@ -428,11 +446,16 @@ def main():
     results = {}
     fp64_state = None
     N = mesh = None
+    fused = False
     for form in forms:
         case = make_headline_case(a.warmup + a.steps, a.tol)
         N, mesh = case.mesh.N, case.mesh
+        fused = fused_rows(mesh, case)
         if a.probe == "auto":
-            probe = "jacobi" if form == "fp64" else "jacobi32"
+            if fused_rows(case.mesh, case):
+                probe = "half_p" if form == "fp64" else "half_p_mixed"
+            else:
+                probe = "jacobi" if form == "fp64" else "jacobi32"
         else:
             probe = a.probe if a.probe != "none" else None
         prec = hip.FP64 if form == "fp64" else hip.MIXED
@@ -509,9 +532,10 @@ def main():
         """whole-step algorithmic bytes (fixed + per iteration x iterations) over the
         time per step (the ALM projection's box and the precision form's residual
         replacements are left out; its FP64 iterations are counted at FP64 bytes)"""
-        b = STEP_BYTES[form]
+        table = STEP_BYTES_FUSED if fused else STEP_BYTES
+        b = table[form]
         if form == "mixed":
-            per_it = (b["per_it"] * (r["it_mean"] - r["it64_mean"]) + STEP_BYTES["fp64"]["per_it"] * r["it64_mean"])
+            per_it = (b["per_it"] * (r["it_mean"] - r["it64_mean"]) + table["fp64"]["per_it"] * r["it64_mean"])
         else:
             per_it = b["per_it"] * r["it_mean"]
         nbytes = (b["fixed"] + b["per_solve"] + per_it) * N

@@ -267,7 +267,13 @@ enum dalm_kernel {
     DALM_K_SUPDATE_MIXED = 13, /* s update, first PC sweep (FP32)      */
     /* output path */
     DALM_K_CALC_Q = 14,        /* Q-criterion                          */
-    DALM_K_TIME_AVERAGE = 15   /* time average of U                    */
+    DALM_K_TIME_AVERAGE = 15,  /* time average of U                    */
+    /* the precision form's fused half-iterations (rows of 256 cells, Jacobi x3) */
+    DALM_K_HALF_P_MIXED = 16,  /* p update + 3 Jacobi sweeps + A z + <r0, q> */
+    DALM_K_HALF_S_MIXED = 17,  /* s update + 3 Jacobi sweeps + A z + dots  */
+    /* the FP64 drop-in's fused update + preconditioner (the same rows)       */
+    DALM_K_HALF_P = 18,        /* p update + 3 Jacobi sweeps (FP64)        */
+    DALM_K_HALF_S = 19         /* s update + 3 Jacobi sweeps (FP64)        */
 };
 /* Time every launch of `kernel` with HIP events on the context's stream from now
  * on (DALM_K_NONE stops); resets the totals. */
@@ -340,6 +346,17 @@ typedef struct {
     double reverse_ms;     /* wall time of the reverse steps */
 } dalm_adjoint_stats;
 int dalm_adjoint_begin(dalm_ctx *ctx, int64_t steps, int64_t every, const double *tq_w, int64_t flags);
+/* The precision form's recording.  ctx: an FP64 context of the same grid, turbine
+ * set-up and parameters as src, a DALM_MIXED context; the next `steps` dalm_step
+ * calls on src record into ctx's tape as dalm_adjoint_begin would: checkpoints of
+ * src's state widened to FP64 (its FP32 U, nu_t, f exactly; JU and p as they are),
+ * the pressure of every step with DALM_ADJ_KEEP_PRESSURE, and J from src's
+ * torques.  dalm_adjoint_run(ctx, ...) then replays each segment in FP64 from
+ * those checkpoints (re-solving the pressure unless it was kept) and runs the FP64
+ * reverse steps: the discrete adjoint of the FP64 step, linearised along the
+ * precision form's trajectory.  ctx cannot step while it records. */
+int dalm_adjoint_begin_from(dalm_ctx *ctx, dalm_ctx *src, int64_t steps, int64_t every,
+                            const double *tq_w, int64_t flags);
 int dalm_adjoint_run(dalm_ctx *ctx, double tol_rel, double *J, double *ap_grad, double *wt_grad,
                      dalm_adjoint_stats *stats);
 int dalm_adjoint_end(dalm_ctx *ctx);

@@ -126,14 +126,20 @@ def test_bicgstab(mesh, ok, pc):
     assert rel_l2(c.download("p")[m], x[m]) <= 1e-10
 
 
-def test_bicgstab_rows256_first_iterations(ok):
+@pytest.mark.parametrize("shape", [(5, 6, 256), (40, 20, 256)])
+def test_bicgstab_rows256_first_iterations(ok, shape):
     """Rows of 256 interior cells run the FP64 BiCGStab kernels (p and s updates
     with the first Jacobi sweep, Jacobi sweeps, SpMV with its dot products, x/r
-    update with its norms) with four cells per lane.  Over the first two iterations
-    the Krylov path has not yet amplified the dot products' summation order, so the
-    iterate must agree with the oracle's to rounding (every stored vector is
-    bit-identical to the one-cell kernels' for the same scalars)."""
-    mesh = small_mesh(5, 6, 256, 1.2)
+    update with its norms) with four cells per lane; with at least 8 rows the
+    update and the three sweeps of each half-iteration are one kernel
+    (k_half_pc4: 40x20x256 has two plane chunks, the second overlapping the first,
+    and a last j tile of 4 of its 8 rows).  Over the first iterations the Krylov
+    path has not yet amplified the dot products' summation order (the second
+    iteration reads p and q from the partner buffers), so the iterate
+    must agree with the oracle's to rounding (every stored vector is bit-identical
+    to the one-cell kernels' for the same scalars)."""
+    print(f"[bicgstab rows256 first iterations {shape}]", flush=True)
+    mesh = small_mesh(*shape, 1.2)
     g = po.make_grid(mesh)
     A = np.zeros((mesh.L, 7))
     ok.build_A(g, A)
