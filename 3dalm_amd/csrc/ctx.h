@@ -167,6 +167,7 @@ struct dalm_ctx {
     int precision = DALM_FP64;
     hipStream_t stream = nullptr;
     int grid = 0;   // blocks for the cell kernels (tile loop)
+    int ncu = 256;  // compute units of an MI355X (sizes the one-block-per-CU fused kernels)
     int sgrid = 0;  // blocks of the column-streaming stencil kernels (= d.nunits)
 
     // mesh (device)
@@ -324,6 +325,10 @@ int zero_device(dalm_ctx* ctx, double* a, long long n);
 // fused half-iterations of the precision form's BiCGStab (fused_mixed.hip)
 bool fused_half_applies(const dalm_ctx* ctx, int pc_kind, int pc_it);
 int fused_half_grid(const dalm_ctx* ctx, int* ni);
+// planes per unit of a fused kernel that runs one block per compute unit over
+// `tiles` j tiles: as many plane chunks as fill the compute units once, chunks of at
+// least 16 planes
+int fused_planes(const dalm_ctx* ctx, int tiles);
 int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* z_out, float* q_out);
 int launch_half_s(dalm_ctx* ctx, const float* q, float* s_out, float* z_out, float* t_out);
 void probe_begin(dalm_ctx* ctx, int kernel);

@@ -30,7 +30,8 @@
 //
 // The arithmetic per cell is that of the separate kernels (kg_p4 / kg_s4, kf_jacobi4,
 // kg_spmv4): FP32 sweeps with FP32 coefficients, the SpMV in FP64 with the FP64
-// operator and its result rounded once, dot products in FP64 of the stored values.
+// operator and its result rounded once (its diagonal scaled by the reciprocal of
+// max|a_c| instead of divided by it), dot products in FP64 of the stored values.
 // Ghost cells of the preconditioner iterates are zero, as in the reference
 // (fill_array(pp, 0) before the sweeps, which write the interior only).
 //
@@ -127,6 +128,9 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
     const int k0 = g + 4 * lane;
     const double beta = st->beta, om = st->omega, alpha = st->alpha;
     const bool first = st->it == 0;
+    // the SpMV's diagonal is scaled by the reciprocal of max|a_c| (within an ulp of the
+    // division: far below the FP32 rounding of the product it feeds)
+    const double inv_md = 1.0 / A.max_diag;
 
     // coefficients: j (this row), k (this lane's four cells), FP32 and FP64
     const int jc = jin ? j : g;
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
             const int lk = 4 * lane;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double ac = ((acj - sh.atud[lk + q]) - sh.abud[lk + q]) / A.max_diag;
+                const double ac = ((acj - sh.atud[lk + q]) - sh.abud[lk + q]) * inv_md;
                 const double tq = q + 1 < 4 ? (double)z3a[q + 1] : (double)n3.kf;
                 const double bq = q > 0 ? (double)z3a[q - 1] : (double)n3.kb;
                 const double yv = ac * (double)z3a[q] + ae * (double)z3n[q] + aw * (double)z3b[q] +
@@ -280,12 +284,22 @@ bool fused_half_applies(const dalm_ctx* ctx, int pc_kind, int pc_it) {
     return ctx->d.nz == 256 && pc_kind == DALM_PC_JACOBI && pc_it == 3 && ctx->d.ny >= kOwn;
 }
 
-// planes per unit and the grid of the fused half-iterations
+int fused_planes(const dalm_ctx* ctx, int tiles) {
+    const int nx = ctx->d.nx;
+    int nch = ctx->ncu / (tiles > 0 ? tiles : 1);
+    const int most = (nx + 15) / 16;
+    if (nch > most) nch = most;
+    if (nch < 1) nch = 1;
+    return (nx + nch - 1) / nch;
+}
+
+// planes per unit and the grid of the fused half-iterations: one block per compute
+// unit (its 16 wavefronts fill the CU's VGPRs), in a single round when the units
+// allow it -- a second, partial round would leave most CUs idle
 int fused_half_grid(const dalm_ctx* ctx, int* ni) {
     const Dims& d = ctx->d;
     const int tjf = (d.ny + kOwn - 1) / kOwn;
-    int n = 32;
-    if (n > d.nx) n = d.nx;
+    const int n = fused_planes(ctx, tjf);
     *ni = n;
     return tjf * ((d.nx + n - 1) / n);
 }

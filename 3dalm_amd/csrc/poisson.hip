@@ -736,8 +736,9 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     // each half-iteration are one kernel (k_half_pc4); p and q alternate between a
     // buffer pair (the Jacobi partners, unused on that path)
     const bool fused = jac && pc_it == 3 && d.nz == 256 && d.ny >= kFOwn;
-    const int fni = d.nx < 32 ? d.nx : 32;
-    const int fgrid = ((d.ny + kFOwn - 1) / kFOwn) * ((d.nx + fni - 1) / fni);
+    const int ftiles = (d.ny + kFOwn - 1) / kFOwn;
+    const int fni = fused_planes(ctx, ftiles);
+    const int fgrid = ftiles * ((d.nx + fni - 1) / fni);
     // preconditioner application for one vector; returns the buffer holding z
     auto precondition = [&](double* z0, double* spare, const double* v) -> double* {
         double* cur = z0;

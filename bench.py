@@ -304,6 +304,49 @@ def run_adjoint(steps, every, warmup=2):
             "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]]}
 
 
+def run_precision_adjoint(case, steps, every, warmup=2):
+    """The precision form's adjoint: `warmup` steps of the precision form, then
+    `steps` recorded ones (dalm_adjoint_begin_from into an FP64 context, checkpoints
+    every `every` steps, no pressure tape: the replays re-solve in FP64) and the
+    reverse sweep.  Wall time per reverse step (FP64 segment replays plus reverse
+    steps) over wall time per recorded precision-form step."""
+    solver = importlib.import_module("3dalm_amd.solver")
+    hip = importlib.import_module("3dalm_amd.hip")
+    omega = np.array([st.rot_speed for st in case.sites])
+    log(f"[bench] precision-form adjoint: grid={case.mesh.interior} warmup={warmup} steps={steps} every={every}")
+    sm = solver.Solver(case, hip.MIXED)
+    s64 = None
+    try:
+        for _ in range(warmup):
+            sm.step()
+        s64 = solver.Solver(case)
+        s64.ctx.adjoint_begin_from(sm.ctx, steps, every, omega / steps, False)
+        its = []
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            its.append(sm.step()["it"])
+        t_rec = (time.perf_counter() - t1) / steps
+        t2 = time.perf_counter()
+        J, ap_g, wt_g, st = s64.ctx.adjoint_run(1e-8)
+        t_rev = (time.perf_counter() - t2) / steps
+        s64.ctx.adjoint_end()
+    finally:
+        sm.close()
+        if s64 is not None:
+            s64.close()
+    log(f"[bench] precision-form adjoint: recorded forward {t_rec * 1e3:.2f} ms/step, reverse {t_rev * 1e3:.2f} "
+        f"ms/step (replay {st['replay_ms'] / steps:.2f} + reverse steps {st['reverse_ms'] / steps:.2f})")
+    m = case.mesh
+    return {"ratio": t_rev / t_rec, "forward_ms": t_rec * 1e3, "reverse_ms": t_rev * 1e3,
+            "replay_ms": st["replay_ms"] / steps, "reverse_steps_ms": st["reverse_ms"] / steps,
+            "adjoint_it_per_step": st["adj_it"] / steps, "forward_it_per_step": float(np.mean(its)),
+            "steps": steps, "checkpoint_every": every, "warmup": warmup,
+            "workload": (f"{m.interior[0]}x{m.interior[1]}x{m.interior[2]}, {len(case.sites)} rotors x 3 blades x "
+                         f"{case.ap_per_blade} points, precision form recorded into an FP64 context, FP64 "
+                         "replays re-solving the pressure"),
+            "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]]}
+
+
 # output path kernels: id, algorithmic bytes per cell (FP64 form, precision form) and
 # whether the count is over interior cells (calc_q) or every padded cell (the average)
 OUTPUT_KERNELS = {"calc_q": (14, 32, 20, "interior"),        # read U(3); write q
@@ -408,6 +451,12 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--mixed-adjoint-steps", type=int, default=20,
+                    help="recorded steps of the precision form's adjoint on the 192x96x96 geometry (0: skip)")
+    ap.add_argument("--c5-adjoint-steps", type=int, default=0,
+                    help="recorded steps of the precision form's adjoint at 768x384x384 (0: skip; 100 is the "
+                         "configuration's window)")
+    ap.add_argument("--c5-warmup", type=int, default=2)
     ap.add_argument("--no-extras", action="store_true", help="skip the 256x128x128, output-path and SOR legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device and sum 4 x MIB MiB (kernels of known traffic "
@@ -512,6 +561,22 @@ def main():
             adj = run_adjoint(a.adjoint_steps, a.adjoint_every)
         except Exception as e:
             log(f"[bench] adjoint leg failed: {e}")
+    adj_mixed = adj_c5 = None
+    if not a.no_adjoint and a.mixed_adjoint_steps > 0 and have_mixed:
+        try:
+            cases = importlib.import_module("3dalm_amd.cases")
+            n = a.mixed_adjoint_steps
+            adj_mixed = run_precision_adjoint(cases.case_c4(steps=2 + n), n, max(1, n // 2))
+        except Exception as e:
+            log(f"[bench] precision-form adjoint leg failed: {e}")
+    if a.c5_adjoint_steps > 0 and have_mixed:
+        try:
+            cases = importlib.import_module("3dalm_amd.cases")
+            n = a.c5_adjoint_steps
+            adj_c5 = run_precision_adjoint(cases.case_c5(steps=a.c5_warmup + n), n, max(1, min(12, n // 2)),
+                                           warmup=a.c5_warmup)
+        except Exception as e:
+            log(f"[bench] 768x384x384 adjoint leg failed: {e}")
     c2 = output_path = sor = None
     if not a.no_extras:
         try:
@@ -574,6 +639,8 @@ def main():
         "mixed_errors_vs_fp64": errors,
         "adjoint_forward_ratio": round(adj["ratio"], 3) if adj else None,
         "adjoint": adj,
+        "adjoint_mixed": adj_mixed,
+        "adjoint_mixed_768x384x384": adj_c5,
         "roofline": roof,
         "roofline_fp64": roof_fp64,
         "cpu_baseline": cpu,
