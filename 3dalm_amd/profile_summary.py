@@ -1,8 +1,8 @@
 """Summaries of rocprofv3 output for the hot-path kernels.
 
-    python 3dalm_amd/profile_summary.py trace <run>_kernel_trace.csv [--cells N]
+    python 3dalm_amd/profile_summary.py trace <run>_kernel_trace.csv [--cells N] [--match REGEX]
     python 3dalm_amd/profile_summary.py traffic <fetch>_counter_collection.csv \\
-        <write>_counter_collection.csv --cells N [--copy-mib 256] [--json out.json]
+        <write>_counter_collection.csv --cells N --padded-cells L [--json out.json]
 
 ``trace`` tabulates every kernel of a ``rocprofv3 --kernel-trace --output-format csv``
 run.  The Krylov solvers enqueue iterations ahead of the device's stopping test,
@@ -10,13 +10,16 @@ and the launches past it return at once; they are counted apart ("idle": shorter
 than --idle-us) so that the mean duration is that of launches that did the work.
 With --cells, the algorithmic bytes per interior cell of each known kernel give
 its achieved bandwidth and fraction of the 8 TB/s HBM peak (--padded-cells for the
-kernels that sweep the whole padded box).
+kernels that sweep the whole padded box); --match keeps the kernels whose names
+match a regular expression (the shares are then of those kernels' time).
 
 ``traffic`` turns two ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs
 (kilobytes per dispatch) into bytes per interior cell per launch, scaled by what
-the same counters report for the device-to-device copy of ``--copy-mib`` MiB that
-``bench.py --copy-calibration`` issues first (a copy reads and writes exactly that
-much).  With --json the per-kernel figures are also written as JSON, which
+the same counters report for a kernel of known traffic: the library's k_copy3,
+which ``bench.py --calibration N`` runs N times after the forms and which reads and
+writes 24 B per padded cell (--padded-cells); the library's read-only reduction
+k_scalar_sum (8 B per interior cell) cross-checks the FETCH_SIZE scale.  Without
+--padded-cells the largest device-to-device copy of ``--copy-mib`` MiB calibrates.  With --json the per-kernel figures are also written as JSON, which
 bench.py reads to fill roofline.traffic.
 """
 from __future__ import annotations
@@ -58,11 +61,16 @@ def short_name(name: str) -> str:
     return m.group(1) if m else name[:40]
 
 
-def trace_table(path: str, cells: int | None, idle_us: float, padded: int | None = None) -> str:
+def trace_table(path: str, cells: int | None, idle_us: float, padded: int | None = None,
+                match: str | None = None) -> str:
     by = collections.defaultdict(list)
+    keep = re.compile(match) if match else None
     for row in csv.DictReader(open(path)):
+        name = short_name(row["Kernel_Name"])
+        if keep and not keep.search(name):
+            continue
         us = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
-        by[short_name(row["Kernel_Name"])].append(us)
+        by[name].append(us)
     total = sum(sum(v) for v in by.values())
     lines = ["| kernel | launches | idle | mean µs (working) | median µs | total ms | share | B/cell | GB/s | of 8 TB/s |",
              "|---|---|---|---|---|---|---|---|---|---|"]
@@ -93,17 +101,38 @@ def _counter(path: str, counter: str):
     return by
 
 
-def traffic_table(fetch: str, write: str, cells: int, copy_mib: float, json_out: str | None = None) -> str:
+def traffic_table(fetch: str, write: str, cells: int, padded: int | None, copy_mib: float,
+                  json_out: str | None = None) -> str:
     f, w = _counter(fetch, "FETCH_SIZE"), _counter(write, "WRITE_SIZE")
-    record = {"cells": cells, "copy_mib": copy_mib, "kernels": {}}
-    copy_kib = copy_mib * 1024
-    # the calibration copy is the largest copy dispatch of each run
-    cf = max(f.get("__amd_rocclr_copyBuffer", [0]))
-    cw = max(w.get("__amd_rocclr_copyBuffer", [0]))
-    sf = copy_kib / cf if cf else 1.0
-    sw = copy_kib / cw if cw else 1.0
-    lines = [f"FETCH_SIZE scale {sf:.3f}, WRITE_SIZE scale {sw:.3f} (a {copy_mib:g} MiB copy reads and writes "
-             f"{copy_kib:.0f} KiB; the counters report {cf:.0f} and {cw:.0f} KiB)", "",
+    record = {"cells": cells, "kernels": {}}
+    if padded and f.get("k_copy3"):
+        # the library's copy of the three velocity components over the padded
+        # array: 24 B read and 24 B written per padded cell
+        cal = "k_copy3"
+        want_kib = 24 * padded / 1024
+        what = f"k_copy3 (U to Uold, 3 x {padded} padded cells) reads and writes {want_kib:.0f} KiB"
+        record["calibration"] = {"kernel": cal, "padded_cells": padded}
+    else:
+        cal = "__amd_rocclr_copyBuffer"
+        want_kib = copy_mib * 1024
+        what = f"a {copy_mib:g} MiB copy reads and writes {want_kib:.0f} KiB"
+        record["calibration"] = {"kernel": cal, "copy_mib": copy_mib}
+    cf = float(np.median(f.get(cal, [0])))
+    cw = float(np.median(w.get(cal, [0])))
+    sf = want_kib / cf if cf else 1.0
+    sw = want_kib / cw if cw else 1.0
+    lines = [f"FETCH_SIZE scale {sf:.3f}, WRITE_SIZE scale {sw:.3f} ({what}; the counters report "
+             f"{cf:.0f} and {cw:.0f} KiB)"]
+    # cross-check of the FETCH_SIZE scale with a read-only kernel of known
+    # traffic: the library's reduction k_scalar_sum reads one FP64 field over
+    # the interior, 8 B per interior cell
+    ks = f.get("k_scalar_sum")
+    if ks:
+        got = float(np.median(ks)) * 1024 * sf
+        lines.append(f"cross-check: k_scalar_sum reads {cells * 8 / 2 ** 20:.1f} MiB (8 B x {cells} interior "
+                     f"cells); FETCH_SIZE x scale gives {got / 2 ** 20:.1f} MiB ({got / (cells * 8):.3f}x)")
+        record["cross_check_k_scalar_sum"] = round(got / (cells * 8), 4)
+    lines += ["",
              "| kernel | dispatches | read B/cell | write B/cell | total B/cell | algorithmic B/cell | ratio |",
              "|---|---|---|---|---|---|---|"]
     for name in sorted(f, key=lambda k: -np.median(f[k])):
@@ -136,17 +165,19 @@ def main():
     t.add_argument("--cells", type=int)
     t.add_argument("--idle-us", type=float, default=6.0)
     t.add_argument("--padded-cells", type=int, help="padded cell count, for the kernels that sweep it")
+    t.add_argument("--match", help="keep only the kernels whose names match this regular expression")
     r = sub.add_parser("traffic")
     r.add_argument("fetch_csv")
     r.add_argument("write_csv")
     r.add_argument("--cells", type=int, required=True)
-    r.add_argument("--copy-mib", type=float, default=256)
+    r.add_argument("--padded-cells", type=int, help="padded cell count: calibrate on the library's k_copy3")
+    r.add_argument("--copy-mib", type=float, default=256, help="else calibrate on a copy of this many MiB")
     r.add_argument("--json")
     a = ap.parse_args()
     if a.cmd == "trace":
-        print(trace_table(a.csv, a.cells, a.idle_us, a.padded_cells))
+        print(trace_table(a.csv, a.cells, a.idle_us, a.padded_cells, a.match))
     else:
-        print(traffic_table(a.fetch_csv, a.write_csv, a.cells, a.copy_mib, a.json))
+        print(traffic_table(a.fetch_csv, a.write_csv, a.cells, a.padded_cells, a.copy_mib, a.json))
 
 
 if __name__ == "__main__":

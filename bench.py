@@ -437,6 +437,26 @@ def interior(mesh, a):
     return a.reshape((nx, ny, nz) + a.shape[1:])[g:nx - g, g:ny - g, g:nz - g]
 
 
+def calibration_kernels(reps, mesh):
+    """Kernels of known HBM traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE, all of
+    them the library's own (no second HIP runtime in the process): `reps` calls each of
+    dalm_copy_u_to_uold (k_copy3 reads 24 B and writes 24 B per padded cell) and of
+    dalm_l2_norm (k_scalar_sum reads 8 B per interior cell), on a context of `mesh`."""
+    hip = importlib.import_module("3dalm_amd.hip")
+    c = hip.Context(mesh)
+    try:
+        rng = np.random.default_rng(5)
+        c.upload("U", rng.uniform(-1.0, 1.0, 3 * mesh.L))
+        c.upload("p", rng.uniform(-1.0, 1.0, mesh.L))
+        for _ in range(reps):
+            c.copy_u_to_uold()
+        for _ in range(reps):
+            c.l2_norm("p")
+    finally:
+        c.close()
+    log(f"[bench] calibration: {reps} copies of U ({mesh.L} padded cells x 3 fields) and {reps} l2 norms")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -458,9 +478,9 @@ def main():
                          "configuration's window)")
     ap.add_argument("--c5-warmup", type=int, default=2)
     ap.add_argument("--no-extras", action="store_true", help="skip the 256x128x128, output-path and SOR legs")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="first copy MIB MiB device to device and sum 4 x MIB MiB (kernels of known traffic "
-                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--calibration", type=int, default=0, metavar="N",
+                    help="after the forms, run N copies of the velocity field and N l2 norms of one field "
+                         "(kernels of known traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the whole domain lives on one MI355X and there is "
@@ -469,20 +489,6 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     hip = importlib.import_module("3dalm_amd.hip")
-    if a.copy_calibration:
-        import torch
-        n = a.copy_calibration * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        big = torch.ones(4 * n, dtype=torch.float64, device="cuda")
-        tot = float(big.sum())
-        torch.cuda.synchronize()
-        log(f"[bench] calibration: copy of {a.copy_calibration} MiB, sum over {4 * a.copy_calibration} MiB "
-            f"({tot:.0f})")
-        del src, dst, big
     forms = [f for f in a.forms.split(",") if f]
     unknown = [f for f in forms if f not in FORMS]
     if unknown:
@@ -517,6 +523,8 @@ def main():
         res["mcell_steps_s"] = N * a.steps / res["seconds"] / 1e6
         res["ms_per_step"] = 1e3 * res["seconds"] / a.steps
         results[form] = res
+    if a.calibration and mesh is not None:
+        calibration_kernels(a.calibration, mesh)
     main_form = "mixed" if "mixed" in results else next(iter(results))
     R = results[main_form]
 
