@@ -18,6 +18,8 @@
 //   -> face-flux bars -> predictor bars (U_{n-1}, JU_{n-1}, nu_t_{n-1}, f_{n-1}).
 // Compiled with -ffp-contract=off.
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <chrono>
@@ -1527,6 +1529,7 @@ int dalm_adjoint_run(dalm_ctx* ctx, double tol_rel, double* J, double* ap_grad, 
     S.adj_it = 0;
     S.replay_ms = S.reverse_ms = 0;
     const long long nseg = (S.steps + S.every - 1) / S.every;
+    static const bool trace = getenv("DALM_TRACE") != nullptr;
     for (long long sg = nseg - 1; sg >= 0; --sg) {
         const long long s0 = sg * S.every;
         const long long s1 = std::min(s0 + S.every, S.steps);
@@ -1548,6 +1551,10 @@ int dalm_adjoint_run(dalm_ctx* ctx, double tol_rel, double* J, double* ap_grad, 
         double t2 = now_ms();
         S.replay_ms += t1 - t0;
         S.reverse_ms += t2 - t1;
+        if (trace)
+            fprintf(stderr, "[adjoint] segment %lld of %lld (steps %lld-%lld): replay %.1f ms, reverse %.1f ms, "
+                            "%lld adjoint iterations so far\n",
+                    nseg - sg, nseg, s0 + 1, s1, t1 - t0, t2 - t1, S.adj_it);
     }
     // the force of the initialisation pass (t of the recording's start) drives the
     // first step; its torque is not part of J

@@ -7,6 +7,7 @@
 // a dense contraction).  Loads are 64-wide coalesced k-rows; neighbours along j
 // and i are re-reads served by L2 / the last-level cache.
 #include "ctx.h"
+#include "star_walk.h"
 
 namespace {
 
@@ -47,6 +48,41 @@ __device__ __forceinline__ FluxPair muscl_faces(double vm2, double vm1, double v
 
 // ---------------------------------------------------------------------------
 // predictor: U = Uold + dt (-conv + diff + f), src/cfd.h:175-242
+
+// the cell's metrics, face fluxes and viscosity
+struct PredCell {
+    double Je, Jw, Jn, Js, Jt, Jb;
+    double dxc, dyc, dzc, vol, hxp, hxm, hyp, hym, hzp, hzm, nu;
+};
+__device__ __forceinline__ PredCell pred_cell(const Dims& d, const M1& m, const double* __restrict__ J0,
+                                              const double* __restrict__ J1, const double* __restrict__ J2,
+                                              const double* __restrict__ nut, double inv_re, int i, int j, int k,
+                                              int c) {
+    PredCell P;
+    P.Je = J0[c], P.Jw = J0[c - d.si], P.Jn = J1[c], P.Js = J1[c - d.sj], P.Jt = J2[c], P.Jb = J2[c - 1];
+    P.dxc = m.dx[i], P.dyc = m.dy[j], P.dzc = m.dz[k];
+    P.vol = P.dxc * P.dyc * P.dzc;
+    P.hxp = m.x[i + 1] - m.x[i], P.hxm = m.x[i] - m.x[i - 1];
+    P.hyp = m.y[j + 1] - m.y[j], P.hym = m.y[j] - m.y[j - 1];
+    P.hzp = m.z[k + 1] - m.z[k], P.hzm = m.z[k] - m.z[k - 1];
+    P.nu = inv_re + nut[c];
+    return P;
+}
+// one component from its 13-point star (x*, y*, z*: the neighbours along i, j, k)
+__device__ __forceinline__ double pred_value(const PredCell& P, double vc, double xww, double xw, double xe,
+                                             double xee, double yss, double ys, double yn, double ynn, double zbb,
+                                             double zb, double zt, double ztt, double fq, double dt) {
+    FluxPair fx = muscl_faces(xww, xw, vc, xe, xee, P.Je, P.Jw);
+    FluxPair fy = muscl_faces(yss, ys, vc, yn, ynn, P.Jn, P.Js);
+    FluxPair fz = muscl_faces(zbb, zb, vc, zt, ztt, P.Jt, P.Jb);
+    double conv = (fx.p - fx.m + fy.p - fy.m + fz.p - fz.m) / P.vol;
+    double diff = P.nu * (((xe - vc) / P.hxp - (vc - xw) / P.hxm) / P.dxc +
+                          ((yn - vc) / P.hyp - (vc - ys) / P.hym) / P.dyc +
+                          ((zt - vc) / P.hzp - (vc - zb) / P.hzm) / P.dzc);
+    return vc + dt * (-conv + diff + fq);
+}
+
+// direct form: every value through the caches (grids too small for the walk)
 __global__ __launch_bounds__(BLOCK) void k_predictor(
     Dims d, M1 m, const double* __restrict__ o0, const double* __restrict__ o1,
     const double* __restrict__ o2, const double* __restrict__ J0, const double* __restrict__ J1,
@@ -60,33 +96,40 @@ __global__ __launch_bounds__(BLOCK) void k_predictor(
         if (!cc.inside) continue;
         const int i = cc.i, j = cc.j, k = cc.k;
         const int c = d.at(i, j, k);
-        const double Je = J0[c], Jw = J0[c - si], Jn = J1[c], Js = J1[c - sj], Jt = J2[c], Jb = J2[c - 1];
-        const double dxc = m.dx[i], dyc = m.dy[j], dzc = m.dz[k];
-        const double vol = dxc * dyc * dzc;
-        const double hxp = m.x[i + 1] - m.x[i], hxm = m.x[i] - m.x[i - 1];
-        const double hyp = m.y[j + 1] - m.y[j], hym = m.y[j] - m.y[j - 1];
-        const double hzp = m.z[k + 1] - m.z[k], hzm = m.z[k] - m.z[k - 1];
-        const double nu = inv_re + nut[c];
+        const PredCell P = pred_cell(d, m, J0, J1, J2, nut, inv_re, i, j, k, c);
         const double* src[3] = {o0, o1, o2};
         const double* frc[3] = {f0, f1, f2};
         double* dst[3] = {u0, u1, u2};
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const double* a = src[q];
-            const double vc = a[c];
-            const double xe = a[c + si], xee = a[c + 2 * si], xw = a[c - si], xww = a[c - 2 * si];
-            const double yn = a[c + sj], ynn = a[c + 2 * sj], ys = a[c - sj], yss = a[c - 2 * sj];
-            const double zt = a[c + 1], ztt = a[c + 2], zb = a[c - 1], zbb = a[c - 2];
-            FluxPair fx = muscl_faces(xww, xw, vc, xe, xee, Je, Jw);
-            FluxPair fy = muscl_faces(yss, ys, vc, yn, ynn, Jn, Js);
-            FluxPair fz = muscl_faces(zbb, zb, vc, zt, ztt, Jt, Jb);
-            double conv = (fx.p - fx.m + fy.p - fy.m + fz.p - fz.m) / vol;
-            double diff = nu * (((xe - vc) / hxp - (vc - xw) / hxm) / dxc +
-                                ((yn - vc) / hyp - (vc - ys) / hym) / dyc +
-                                ((zt - vc) / hzp - (vc - zb) / hzm) / dzc);
-            dst[q][c] = vc + dt * (-conv + diff + frc[q][c]);
+            dst[q][c] = pred_value(P, a[c], a[c - 2 * si], a[c - si], a[c + si], a[c + 2 * si], a[c - 2 * sj],
+                                   a[c - sj], a[c + sj], a[c + 2 * sj], a[c - 2], a[c - 1], a[c + 1], a[c + 2],
+                                   frc[q][c], dt);
         }
     }
+}
+
+// column walk with the centre plane staged through LDS (star_walk.h); same
+// arithmetic per cell
+__global__ __launch_bounds__(kSBlock) void k_predictor_star(
+    Dims d, M1 m, const double* __restrict__ o0, const double* __restrict__ o1,
+    const double* __restrict__ o2, const double* __restrict__ J0, const double* __restrict__ J1,
+    const double* __restrict__ J2, const double* __restrict__ nut, const double* __restrict__ f0,
+    const double* __restrict__ f1, const double* __restrict__ f2, double* __restrict__ u0,
+    double* __restrict__ u1, double* __restrict__ u2, double inv_re, double dt, int ni) {
+    __shared__ StarLds<double> sh;
+    star_walk(d, o0, o1, o2, ni, sh,
+              [&](int i, int j, int k, int c, const double (&x)[3][5], const StarView<double>& v) {
+                  const PredCell P = pred_cell(d, m, J0, J1, J2, nut, inv_re, i, j, k, c);
+                  const double* frc[3] = {f0, f1, f2};
+                  double* dst[3] = {u0, u1, u2};
+#pragma unroll
+                  for (int q = 0; q < 3; ++q)
+                      dst[q][c] = pred_value(P, x[q][2], x[q][0], x[q][1], x[q][3], x[q][4], v.y(q, 0), v.y(q, 1),
+                                             v.y(q, 3), v.y(q, 4), v.z(q, 0), v.z(q, 1), v.z(q, 3), v.z(q, 4),
+                                             frc[q][c], dt);
+              });
 }
 
 // Full-length copy of three arrays (cpy_array on a [L][3] field).
@@ -551,10 +594,17 @@ int grid_for(long long n) {
 
 int launch_predictor(dalm_ctx* ctx, double Re, double dt) {
     probe_begin(ctx, DALM_K_PREDICTOR);
-    hipLaunchKernelGGL(k_predictor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, mesh(ctx),
-                       ctx->Uold[0], ctx->Uold[1], ctx->Uold[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
-                       ctx->nut, ctx->f[0], ctx->f[1], ctx->f[2], ctx->U[0], ctx->U[1], ctx->U[2],
-                       1 / Re, dt);
+    const int ni = ctx->d.gc >= 2 ? star_planes(ctx->d) : 0;
+    if (ni > 0)
+        hipLaunchKernelGGL(k_predictor_star, dim3(star_grid(ctx->d, ni)), dim3(kSBlock), 0, ctx->stream, ctx->d,
+                           mesh(ctx), ctx->Uold[0], ctx->Uold[1], ctx->Uold[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
+                           ctx->nut, ctx->f[0], ctx->f[1], ctx->f[2], ctx->U[0], ctx->U[1], ctx->U[2], 1 / Re, dt,
+                           ni);
+    else
+        hipLaunchKernelGGL(k_predictor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, mesh(ctx),
+                           ctx->Uold[0], ctx->Uold[1], ctx->Uold[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
+                           ctx->nut, ctx->f[0], ctx->f[1], ctx->f[2], ctx->U[0], ctx->U[1], ctx->U[2],
+                           1 / Re, dt);
     probe_end(ctx, DALM_K_PREDICTOR);
     LAUNCH_CHECK();
     return DALM_OK;

@@ -14,6 +14,7 @@
 #include <math.h>
 
 #include "ctx.h"
+#include "star_walk.h"
 
 namespace {
 
@@ -53,6 +54,40 @@ __device__ __forceinline__ FluxPairF muscl_faces_f(float vm2, float vm1, float v
 }
 
 // predictor (src/cfd.h:175-242) in FP32 with reciprocal metrics
+
+// the cell's metrics, face fluxes (rounded to FP32) and viscosity
+struct PredCellF {
+    float Je, Jw, Jn, Js, Jt, Jb;
+    float rdx, rdy, rdz, rvol, rxp, rxm, ryp, rym, rzp, rzm, nu;
+};
+__device__ __forceinline__ PredCellF pred_cell_f(const Dims& d, const MetricF& m, const double* __restrict__ J0,
+                                                 const double* __restrict__ J1, const double* __restrict__ J2,
+                                                 const float* __restrict__ nut, float inv_re, int i, int j, int k,
+                                                 int c) {
+    PredCellF P;
+    P.Je = (float)J0[c], P.Jw = (float)J0[c - d.si], P.Jn = (float)J1[c], P.Js = (float)J1[c - d.sj];
+    P.Jt = (float)J2[c], P.Jb = (float)J2[c - 1];
+    P.rdx = m.rw[0][i], P.rdy = m.rw[1][j], P.rdz = m.rw[2][k];
+    P.rvol = P.rdx * P.rdy * P.rdz;
+    P.rxp = m.rp[0][i], P.rxm = m.rm[0][i], P.ryp = m.rp[1][j], P.rym = m.rm[1][j];
+    P.rzp = m.rp[2][k], P.rzm = m.rm[2][k];
+    P.nu = inv_re + nut[c];
+    return P;
+}
+__device__ __forceinline__ float pred_value_f(const PredCellF& P, float vc, float xww, float xw, float xe, float xee,
+                                              float yss, float ys, float yn, float ynn, float zbb, float zb, float zt,
+                                              float ztt, float fq, float dt) {
+    FluxPairF fx = muscl_faces_f(xww, xw, vc, xe, xee, P.Je, P.Jw);
+    FluxPairF fy = muscl_faces_f(yss, ys, vc, yn, ynn, P.Jn, P.Js);
+    FluxPairF fz = muscl_faces_f(zbb, zb, vc, zt, ztt, P.Jt, P.Jb);
+    float conv = ((fx.p - fx.m) + (fy.p - fy.m) + (fz.p - fz.m)) * P.rvol;
+    float diff = P.nu * (((xe - vc) * P.rxp - (vc - xw) * P.rxm) * P.rdx +
+                         ((yn - vc) * P.ryp - (vc - ys) * P.rym) * P.rdy +
+                         ((zt - vc) * P.rzp - (vc - zb) * P.rzm) * P.rdz);
+    return vc + dt * (diff - conv + fq);
+}
+
+// direct form (grids too small for the walk)
 __global__ __launch_bounds__(BLOCK) void km_predictor(
     Dims d, MetricF m, const float* __restrict__ o0, const float* __restrict__ o1,
     const float* __restrict__ o2, const double* __restrict__ J0, const double* __restrict__ J1,
@@ -66,32 +101,39 @@ __global__ __launch_bounds__(BLOCK) void km_predictor(
         if (!cc.inside) continue;
         const int i = cc.i, j = cc.j, k = cc.k;
         const int c = d.at(i, j, k);
-        const float Je = (float)J0[c], Jw = (float)J0[c - si], Jn = (float)J1[c], Js = (float)J1[c - sj];
-        const float Jt = (float)J2[c], Jb = (float)J2[c - 1];
-        const float rdx = m.rw[0][i], rdy = m.rw[1][j], rdz = m.rw[2][k];
-        const float rvol = rdx * rdy * rdz;
-        const float rxp = m.rp[0][i], rxm = m.rm[0][i], ryp = m.rp[1][j], rym = m.rm[1][j];
-        const float rzp = m.rp[2][k], rzm = m.rm[2][k];
-        const float nu = inv_re + nut[c];
+        const PredCellF P = pred_cell_f(d, m, J0, J1, J2, nut, inv_re, i, j, k, c);
         const float* src[3] = {o0, o1, o2};
         const float* frc[3] = {f0, f1, f2};
         float* dst[3] = {u0, u1, u2};
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const float* a = src[q];
-            const float vc = a[c];
-            const float xe = a[c + si], xee = a[c + 2 * si], xw = a[c - si], xww = a[c - 2 * si];
-            const float yn = a[c + sj], ynn = a[c + 2 * sj], ys = a[c - sj], yss = a[c - 2 * sj];
-            const float zt = a[c + 1], ztt = a[c + 2], zb = a[c - 1], zbb = a[c - 2];
-            FluxPairF fx = muscl_faces_f(xww, xw, vc, xe, xee, Je, Jw);
-            FluxPairF fy = muscl_faces_f(yss, ys, vc, yn, ynn, Jn, Js);
-            FluxPairF fz = muscl_faces_f(zbb, zb, vc, zt, ztt, Jt, Jb);
-            float conv = ((fx.p - fx.m) + (fy.p - fy.m) + (fz.p - fz.m)) * rvol;
-            float diff = nu * (((xe - vc) * rxp - (vc - xw) * rxm) * rdx + ((yn - vc) * ryp - (vc - ys) * rym) * rdy +
-                               ((zt - vc) * rzp - (vc - zb) * rzm) * rdz);
-            dst[q][c] = vc + dt * (diff - conv + frc[q][c]);
+            dst[q][c] = pred_value_f(P, a[c], a[c - 2 * si], a[c - si], a[c + si], a[c + 2 * si], a[c - 2 * sj],
+                                     a[c - sj], a[c + sj], a[c + 2 * sj], a[c - 2], a[c - 1], a[c + 1], a[c + 2],
+                                     frc[q][c], dt);
         }
     }
+}
+
+// column walk with the centre plane staged through LDS (star_walk.h)
+__global__ __launch_bounds__(kSBlock) void km_predictor_star(
+    Dims d, MetricF m, const float* __restrict__ o0, const float* __restrict__ o1,
+    const float* __restrict__ o2, const double* __restrict__ J0, const double* __restrict__ J1,
+    const double* __restrict__ J2, const float* __restrict__ nut, const float* __restrict__ f0,
+    const float* __restrict__ f1, const float* __restrict__ f2, float* __restrict__ u0,
+    float* __restrict__ u1, float* __restrict__ u2, float inv_re, float dt, int ni) {
+    __shared__ StarLds<float> sh;
+    star_walk(d, o0, o1, o2, ni, sh,
+              [&](int i, int j, int k, int c, const float (&x)[3][5], const StarView<float>& v) {
+                  const PredCellF P = pred_cell_f(d, m, J0, J1, J2, nut, inv_re, i, j, k, c);
+                  const float* frc[3] = {f0, f1, f2};
+                  float* dst[3] = {u0, u1, u2};
+#pragma unroll
+                  for (int q = 0; q < 3; ++q)
+                      dst[q][c] = pred_value_f(P, x[q][2], x[q][0], x[q][1], x[q][3], x[q][4], v.y(q, 0),
+                                               v.y(q, 1), v.y(q, 3), v.y(q, 4), v.z(q, 0), v.z(q, 1), v.z(q, 3),
+                                               v.z(q, 4), frc[q][c], dt);
+              });
 }
 
 // Face fluxes (FP64) with their boundary values, and -- for interior cells -- the
@@ -449,10 +491,17 @@ int launch_ghost_copy_m(dalm_ctx* ctx) {
 
 int launch_predictor_m(dalm_ctx* ctx, double Re, double dt) {
     probe_begin(ctx, DALM_K_PREDICTOR);
-    hipLaunchKernelGGL(km_predictor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->mf,
-                       ctx->Uold32[0], ctx->Uold32[1], ctx->Uold32[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
-                       ctx->nut32, ctx->f32[0], ctx->f32[1], ctx->f32[2], ctx->U32[0], ctx->U32[1], ctx->U32[2],
-                       (float)(1 / Re), (float)dt);
+    const int ni = ctx->d.gc >= 2 ? star_planes(ctx->d) : 0;
+    if (ni > 0)
+        hipLaunchKernelGGL(km_predictor_star, dim3(star_grid(ctx->d, ni)), dim3(kSBlock), 0, ctx->stream, ctx->d,
+                           ctx->mf, ctx->Uold32[0], ctx->Uold32[1], ctx->Uold32[2], ctx->JU[0], ctx->JU[1],
+                           ctx->JU[2], ctx->nut32, ctx->f32[0], ctx->f32[1], ctx->f32[2], ctx->U32[0], ctx->U32[1],
+                           ctx->U32[2], (float)(1 / Re), (float)dt, ni);
+    else
+        hipLaunchKernelGGL(km_predictor, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, ctx->mf,
+                           ctx->Uold32[0], ctx->Uold32[1], ctx->Uold32[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
+                           ctx->nut32, ctx->f32[0], ctx->f32[1], ctx->f32[2], ctx->U32[0], ctx->U32[1],
+                           ctx->U32[2], (float)(1 / Re), (float)dt);
     probe_end(ctx, DALM_K_PREDICTOR);
     LAUNCH_CHECK();
     return DALM_OK;

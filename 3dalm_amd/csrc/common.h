@@ -42,6 +42,15 @@ DALM_HD int virtual_block(int b, int nblocks) {
     return (b & 7) * (nblocks >> 3) + (b >> 3);
 }
 
+// The same for any number of blocks: XCD x receives the contiguous unit range
+// starting at x * (n / 8) + min(x, n % 8), one unit longer on the first n % 8 XCDs
+// (the fused half-iterations launch one block per compute unit, rarely a multiple
+// of eight).
+DALM_HD int xcd_unit(int b, int n) {
+    const int q = n >> 3, rem = n & 7, x = b & 7;
+    return x * q + (x < rem ? x : rem) + (b >> 3);
+}
+
 // Tile t -> base interior coordinates of this thread (may fall outside the
 // interior along j or k on ragged edges; callers test `inside`).
 struct Cell {

@@ -25,7 +25,7 @@
 // neighbours are the thread's own registers of the previous and next planes, the
 // k neighbours its own four cells or the adjacent lanes' through LDS.  Each step
 // exchanges the new plane of every stencil input through LDS (one row per
-// wavefront): read, barrier, compute, write, barrier.  A chunk starts 3 planes
+// wavefront; two copies alternate): read, compute, write, barrier.  A chunk starts 3 planes
 // early and ends 3 planes late (the pipeline's fill and drain).
 //
 // The arithmetic per cell is that of the separate kernels (kg_p4 / kg_s4, kf_jacobi4,
@@ -55,7 +55,10 @@ __device__ __forceinline__ f4 ld4(const float* p) { return *(const f4u*)p; }
 __device__ __forceinline__ void st4(float* p, f4 v) { *(f4u*)p = v; }
 
 struct Lds {
-    float lv[3][kRows][kRowLen];  // the new plane of each stencil input: z1, z2, z3
+    // the new plane of each stencil input (z1, z2, z3), in two alternating copies: a
+    // step reads the copy the previous step wrote and writes the other, so one
+    // barrier per step orders both (101 KB of the CU's 160 KB)
+    float lv[2][3][kRows][kRowLen];
     double atd[256], abd[256], atud[256], abud[256];  // FP64 k coefficients (SpMV)
     f4 atf[64], abf[64], atuf[64], abuf[64];         // FP32 k coefficients per lane (sweeps)
 };
@@ -112,10 +115,13 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
     const int lane = threadIdx.x & 63, row = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int g = d.gc;
     // unit: j tile jt (rows jt*kOwn .. +kOwn), plane chunk ic; the last chunk of a
-    // column starts inside its predecessor (it computes ni planes, owns from own0)
-    const int nch = (d.nx + ni - 1) / ni;
-    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
-    const int ic = u % nch, jt = u / nch;
+    // column starts inside its predecessor (it computes ni planes, owns from own0).
+    // Units are numbered j tile fastest, and each XCD runs a contiguous range of
+    // them: the halo rows that neighbouring tiles of a chunk both read are fetched
+    // from HBM once and served to the second from the XCD's L2.
+    const int tjf = (d.ny + kOwn - 1) / kOwn;
+    const int u = xcd_unit((int)blockIdx.x, (int)gridDim.x);
+    const int jt = u % tjf, ic = u / tjf;
     const int own0 = g + ic * ni;
     int i1 = own0 + ni;
     if (i1 > g + d.nx) i1 = g + d.nx;
@@ -146,7 +152,7 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         sh.atuf[t >> 2][t & 3] = Af.atu[g + t];
         sh.abuf[t >> 2][t & 3] = Af.abu[g + t];
     }
-    for (int t = threadIdx.x; t < 3 * kRows * kRowLen; t += 64 * kRows) (&sh.lv[0][0][0])[t] = 0.f;
+    for (int t = threadIdx.x; t < 6 * kRows * kRowLen; t += 64 * kRows) (&sh.lv[0][0][0][0])[t] = 0.f;
     __syncthreads();
 
     const f4 zero = {0.f, 0.f, 0.f, 0.f};
@@ -172,6 +178,7 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         return x;
     };
     In nx0 = load0(i0 - kLag);
+    const int rr = row == 0 ? 1 : (row == kRows - 1 ? kRows - 2 : row);  // neighbour rows stay in range
     f4 r0n = zero;  // r0 of the SpMV's plane (p-half)
 
     for (int t = i0 - kLag; t < i1 + kLag; ++t) {
@@ -182,11 +189,10 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         f4 r0c = r0n;
         if (WHICH == 0 && owned && iy + 1 >= own0 && iy + 1 < i1) r0n = ld4(r0 + d.at(iy + 1, j, k0));
         if (WHICH == 0 && !(owned && iy >= own0 && iy < i1)) r0c = zero;
-        // neighbours of the centre planes (written at the previous step)
-        const Nb n1 = nbrs(sh.lv[0], row == 0 ? 1 : (row == kRows - 1 ? kRows - 2 : row), lane);
-        const Nb n2 = nbrs(sh.lv[1], row == 0 ? 1 : (row == kRows - 1 ? kRows - 2 : row), lane);
-        const Nb n3 = nbrs(sh.lv[2], row == 0 ? 1 : (row == kRows - 1 ? kRows - 2 : row), lane);
-        __syncthreads();
+        // neighbours of the centre planes (written at the previous step, into the
+        // other copy)
+        const int cw = (t - i0 + kLag) & 1;
+        const float (*prev)[kRows][kRowLen] = sh.lv[cw ^ 1];
 
         // level 0 at plane t
         f4 z1n = zero, vn = zero;
@@ -225,10 +231,11 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         // level 1 at plane t-1, level 2 at plane t-2
         const int p1 = t - 1, p2 = t - 2;
         f4 z2n = zero, z3n = zero;
-        if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 2) z2n = sweep(p1, z1b, z1a, z1n, n1, va);
-        if (l2 && p2 >= g && p2 < g + d.nx && p2 >= i0 - 1) z3n = sweep(p2, z2b, z2a, z2n, n2, vb);
+        if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 2) z2n = sweep(p1, z1b, z1a, z1n, nbrs(prev[0], rr, lane), va);
+        if (l2 && p2 >= g && p2 < g + d.nx && p2 >= i0 - 1) z3n = sweep(p2, z2b, z2a, z2n, nbrs(prev[1], rr, lane), vb);
         // SpMV at plane t-3 (FP64 operator, FP64 arithmetic, rounded once)
         if (yplane) {
+            const Nb n3 = nbrs(prev[2], rr, lane);
             const double acj = (A.acx[iy] - anud) - asud, ae = A.ae[iy], aw = A.aw[iy];
             f4 y;
             const int lk = 4 * lane;
@@ -256,9 +263,9 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         }
         // the new planes become the neighbours of the next step
         const int o = kPad + 4 * lane;
-        *(f4*)&sh.lv[0][row][o] = z1n;
-        *(f4*)&sh.lv[1][row][o] = z2n;
-        *(f4*)&sh.lv[2][row][o] = z3n;
+        *(f4*)&sh.lv[cw][0][row][o] = z1n;
+        *(f4*)&sh.lv[cw][1][row][o] = z2n;
+        *(f4*)&sh.lv[cw][2][row][o] = z3n;
         __syncthreads();
         z1b = z1a;
         z1a = z1n;

@@ -482,7 +482,8 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv4(Dims d, PCoef A, const doubl
 // each, four k per lane; 12 rather than 16 so that a wavefront may hold 170 VGPRs
 // of FP64 pipeline state) owns 8 rows and computes 2 / 1 halo rows of levels 0 / 1
 // on either side; level 0 runs at plane t, the sweeps at t-1 and t-2; the new plane
-// of each sweep's input goes through LDS (read, barrier, compute, write, barrier).
+// of each sweep's input goes through LDS (two copies alternate: read, compute, write,
+// barrier).
 // The SpMV that ends the half-iteration stays a separate kernel (k_bi_spmv4).
 // Arithmetic per cell identical to k_bi_p4 / k_bi_s4 and k_jacobi4 (bit for bit);
 // the iterates' ghost cells are zero, as in the reference.  p and q are read at
@@ -496,7 +497,10 @@ constexpr int kFRowLen = 256 + 2 * kFPad;
 typedef double d4a __attribute__((ext_vector_type(kVecK), aligned(32)));
 
 struct FLds {
-    alignas(32) double lv[2][kFRows][kFRowLen];  // the new planes of z1 and z2
+    // the new planes of z1 and z2, in two alternating copies: a step reads the
+    // copy the previous step wrote and writes the other, so one barrier per step
+    // orders both
+    alignas(32) double lv[2][2][kFRows][kFRowLen];
     double at[256], ab[256], atu[256], abu[256];
 };
 
@@ -509,9 +513,10 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
     __shared__ FLds sh;
     const int lane = threadIdx.x & 63, row = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int g = d.gc;
-    const int nch = (d.nx + ni - 1) / ni;
-    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
-    const int ic = u % nch, jt = u / nch;
+    // j tile fastest, a contiguous range of units per XCD (as kg_half4)
+    const int tjf = (d.ny + kFOwn - 1) / kFOwn;
+    const int u = xcd_unit((int)blockIdx.x, (int)gridDim.x);
+    const int jt = u % tjf, ic = u / tjf;
     const int own0 = g + ic * ni;
     int i1 = own0 + ni;
     if (i1 > g + d.nx) i1 = g + d.nx;
@@ -532,7 +537,7 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
         sh.atu[t] = A.atu[g + t];
         sh.abu[t] = A.abu[g + t];
     }
-    for (int t = threadIdx.x; t < 2 * kFRows * kFRowLen; t += 64 * kFRows) (&sh.lv[0][0][0])[t] = 0.0;
+    for (int t = threadIdx.x; t < 4 * kFRows * kFRowLen; t += 64 * kFRows) (&sh.lv[0][0][0][0])[t] = 0.0;
     __syncthreads();
 
     const d4k zero = {0.0, 0.0, 0.0, 0.0};
@@ -556,12 +561,12 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
         d4k n, s;
         double kb, kf;
     };
-    auto nbrs = [&](int lev) {
+    auto nbrs = [&](int cp, int lev) {
         Nb b;
-        b.n = *(const d4a*)&sh.lv[lev][rr + 1][o];
-        b.s = *(const d4a*)&sh.lv[lev][rr - 1][o];
-        b.kb = sh.lv[lev][rr][o - 1];
-        b.kf = sh.lv[lev][rr][o + 4];
+        b.n = *(const d4a*)&sh.lv[cp][lev][rr + 1][o];
+        b.s = *(const d4a*)&sh.lv[cp][lev][rr - 1][o];
+        b.kb = sh.lv[cp][lev][rr][o - 1];
+        b.kf = sh.lv[cp][lev][rr][o + 4];
         return b;
     };
     // one Jacobi sweep at plane ip (k_jacobi4's arithmetic)
@@ -596,15 +601,16 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
             if (owned && t >= own0 && t < i1) *(d4k*)(vout + d.at(t, j, k0)) = vn;
         }
         nx0 = load0(t + 1);
-        // neighbours of the centre planes of both sweeps (written at the previous step)
-        const Nb n1 = nbrs(0), n2 = nbrs(1);
-        __syncthreads();
+        // neighbours of the centre planes of both sweeps (written at the previous step,
+        // into the other copy)
+        const int cw = (t - i0 + kFLag) & 1;
         const int p1 = t - 1, p2 = t - 2;
         d4k z2n = zero;
-        if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 1) z2n = sweep(p1, n1, z1b, z1a, z1n, va);
-        if (owned && p2 >= own0 && p2 < i1) *(d4k*)(zout + d.at(p2, j, k0)) = sweep(p2, n2, z2b, z2a, z2n, vb);
-        *(d4a*)&sh.lv[0][row][o] = z1n;
-        *(d4a*)&sh.lv[1][row][o] = z2n;
+        if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 1) z2n = sweep(p1, nbrs(cw ^ 1, 0), z1b, z1a, z1n, va);
+        if (owned && p2 >= own0 && p2 < i1)
+            *(d4k*)(zout + d.at(p2, j, k0)) = sweep(p2, nbrs(cw ^ 1, 1), z2b, z2a, z2n, vb);
+        *(d4a*)&sh.lv[cw][0][row][o] = z1n;
+        *(d4a*)&sh.lv[cw][1][row][o] = z2n;
         __syncthreads();
         z1b = z1a;
         z1a = z1n;

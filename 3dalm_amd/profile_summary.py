@@ -11,7 +11,9 @@ than --idle-us) so that the mean duration is that of launches that did the work.
 With --cells, the algorithmic bytes per interior cell of each known kernel give
 its achieved bandwidth and fraction of the 8 TB/s HBM peak (--padded-cells for the
 kernels that sweep the whole padded box); --match keeps the kernels whose names
-match a regular expression (the shares are then of those kernels' time).
+match a regular expression (the shares are then of those kernels' time), and --span
+keeps the dispatches from the first to the last one of the kernels matching another
+(e.g. a reverse sweep, replays included).
 
 ``traffic`` turns two ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs
 (kilobytes per dispatch) into bytes per interior cell per launch, scaled by what
@@ -62,17 +64,28 @@ def short_name(name: str) -> str:
 
 
 def trace_table(path: str, cells: int | None, idle_us: float, padded: int | None = None,
-                match: str | None = None) -> str:
+                match: str | None = None, span: str | None = None) -> str:
     by = collections.defaultdict(list)
     keep = re.compile(match) if match else None
-    for row in csv.DictReader(open(path)):
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    head = []
+    if span:
+        # the dispatches from the first to the last one whose kernel matches `span`
+        sp = re.compile(span)
+        hit = [n for n, r in enumerate(rows) if sp.search(short_name(r["Kernel_Name"]))]
+        if hit:
+            rows = rows[hit[0]:hit[-1] + 1]
+            wall = (int(rows[-1]["End_Timestamp"]) - int(rows[0]["Start_Timestamp"])) / 1e6
+            busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows) / 1e6
+            head = [f"span of /{span}/: {len(rows)} dispatches, {wall:.2f} ms wall, {busy:.2f} ms in kernels", ""]
+    for row in rows:
         name = short_name(row["Kernel_Name"])
         if keep and not keep.search(name):
             continue
         us = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
         by[name].append(us)
     total = sum(sum(v) for v in by.values())
-    lines = ["| kernel | launches | idle | mean µs (working) | median µs | total ms | share | B/cell | GB/s | of 8 TB/s |",
+    lines = head + ["| kernel | launches | idle | mean µs (working) | median µs | total ms | share | B/cell | GB/s | of 8 TB/s |",
              "|---|---|---|---|---|---|---|---|---|---|"]
     for name, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
         v = np.asarray(v)
@@ -166,6 +179,7 @@ def main():
     t.add_argument("--idle-us", type=float, default=6.0)
     t.add_argument("--padded-cells", type=int, help="padded cell count, for the kernels that sweep it")
     t.add_argument("--match", help="keep only the kernels whose names match this regular expression")
+    t.add_argument("--span", help="keep only the dispatches from the first to the last kernel matching this")
     r = sub.add_parser("traffic")
     r.add_argument("fetch_csv")
     r.add_argument("write_csv")
@@ -175,7 +189,7 @@ def main():
     r.add_argument("--json")
     a = ap.parse_args()
     if a.cmd == "trace":
-        print(trace_table(a.csv, a.cells, a.idle_us, a.padded_cells, a.match))
+        print(trace_table(a.csv, a.cells, a.idle_us, a.padded_cells, a.match, a.span))
     else:
         print(traffic_table(a.fetch_csv, a.write_csv, a.cells, a.padded_cells, a.copy_mib, a.json))
 

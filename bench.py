@@ -2,12 +2,15 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,mixed] [--tol 4e-9]
                     [--no-cpu] [--cpu-steps 1] [--probe auto]
-                    [--adjoint-steps 200] [--adjoint-every 20] [--no-adjoint] [--no-extras]
+                    [--adjoint-steps 200] [--adjoint-every 20] [--adjoint-tol 1e-6] [--no-adjoint]
+                    [--mixed-adjoint-steps 20] [--c5-adjoint-steps 0] [--adjoint-tol-study TOLS]
+                    [--no-extras] [--calibration N]
 
 Headline workload (one "step" = one Solver::main_loop_once): 512x256x256 cells on a
 16x8x8 box (y/z tanh beta=1.5), two 3-blade rotors (D = 1) 5 D apart at x = 3 and
 8 with 3 x 64 actuator points each, TSR 6, eps = 2 dx, a seeded isotropic turbulence
-of 200 random Fourier modes at 10 % intensity, dt = 0.005, Poisson by BiCGStab with
+of 200 random Fourier modes at 10 % intensity as the initial field and, frozen and
+convected at the inflow speed, as the x- inflow, dt = 0.005, Poisson by BiCGStab with
 3 Jacobi sweeps as preconditioner to 4e-9 (the tolerance that keeps avg div <= 1e-6
 at this size).  Synthetic inputs, resident in HBM before the timed region.  W
 untimed warm-up steps from the impulsive start, then steps W+1 .. W+K timed, in
@@ -25,22 +28,28 @@ mixed_vs_fp64 is their ratio and mixed_errors_vs_fp64 the relative L2 distance o
 the precision form's U, p and div from the FP64 drop-in's after the last timed step,
 with the turbine-power difference.
 
-roofline: the dominant kernel of the precision form, its FP32 Jacobi sweep
-(kf_jacobi4; roofline_fp64: the FP64 k_jacobi4): algorithmic bytes per launch
-(12 / 24 B per interior cell x N) over the mean launch duration, timed with HIP
-events on the library's stream over the timed steps (each event pair brackets the
-two back-to-back sweeps of one preconditioner application).  traffic: HBM bytes per
-launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
-(profiles/hbm_traffic_alm_512x256x256.json).
+roofline: the dominant kernel of the precision form on rows of 256 cells, its fused
+p-half-iteration kg_half4<0> (update, three FP32 Jacobi sweeps, FP64 SpMV and dot
+product in one pass; roofline_fp64: the FP64 drop-in's fused update and two sweeps,
+k_half_pc4<0>): algorithmic bytes per launch (28 / 40 B per interior cell x N) over
+the mean launch duration, timed with HIP events on the library's stream over the
+timed steps.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+runs of this workload (profiles/hbm_traffic_alm_512x256x256.json), each counter
+scaled by what it reads for the library's own copy kernel (--calibration).
 
 cpu_baseline: the oracle's OpenMP build on every host core, full steps of the
 headline workload continuing from the FP64 GPU run's final state.
 
 adjoint_forward_ratio: on the differentiation-form workload (192x96x96, three
 rotors, FP64, BiCGStab(Jacobi 3) to 1e-8), the wall time per step of the reverse
-sweep (segment replays from checkpoints plus reverse steps, adjoint solves to 1e-8
-relative) over the wall time per step of the recorded forward run of the same
-steps; 200 steps with checkpoints every 20 by default.
+sweep (segment replays from checkpoints plus reverse steps, adjoint Poisson solves
+to --adjoint-tol relative, 1e-6 by default) over the wall time per step of the
+recorded forward run of the same steps; 200 steps with checkpoints every 20 by
+default.  --adjoint-tol-study 1e-10,1e-8,1e-6 runs a short window (30 steps, then 20
+recorded with checkpoints every 10) at each tolerance and reports the cost ratio
+and the largest relative difference of the yaw and pitch gradients from those of
+the first (tightest) tolerance: at 1e-6 they agree with 1e-10's to about 1e-7,
+a thousandth of the 1e-4 finite-difference bar.
 
 c2_256x128x128: the single-rotor 256x128x128 workload (tolerance 1e-8) in both
 forms.  output_path: mean device time and achieved bandwidth of the output-path
@@ -264,7 +273,7 @@ def _cpu_baseline(case, state, steps, fp64_res):
             "seconds_per_step": t / steps, "it_per_step": float(np.mean(its))}
 
 
-def run_adjoint(steps, every, warmup=2):
+def run_adjoint(steps, every, warmup=2, tol=1e-6, grads=False):
     """Forward vs reverse-sweep cost on the differentiation-form workload."""
     cases = importlib.import_module("3dalm_amd.cases")
     solver = importlib.import_module("3dalm_amd.solver")
@@ -284,7 +293,7 @@ def run_adjoint(steps, every, warmup=2):
         sync()
         t_rec = (time.perf_counter() - t1) / steps
         t2 = time.perf_counter()
-        J, ap_g, wt_g, st = s.ctx.adjoint_run(1e-8)
+        J, ap_g, wt_g, st = s.ctx.adjoint_run(tol)
         sync()
         t_rev = (time.perf_counter() - t2) / steps
         s.ctx.adjoint_end()
@@ -298,13 +307,32 @@ def run_adjoint(steps, every, warmup=2):
             "reverse_ms": t_rev * 1e3, "replay_ms": st["replay_ms"] / steps,
             "reverse_steps_ms": st["reverse_ms"] / steps, "adjoint_it_per_step": st["adj_it"] / steps,
             "forward_it_per_step": float(np.mean(its)), "steps": steps, "checkpoint_every": every,
+            "warmup": warmup, "tol_rel": tol,
             "workload": (f"{case.mesh.interior[0]}x{case.mesh.interior[1]}x{case.mesh.interior[2]}, "
                          f"{len(case.sites)} rotors x 3 blades x {case.ap_per_blade} points, FP64, "
                          "pressure kept per step"),
-            "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]]}
+            "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]],
+            **({"grads": np.concatenate([wt_g[:, 0], wt_g[:, 1]])} if grads else {})}
 
 
-def run_precision_adjoint(case, steps, every, warmup=2):
+def adjoint_tol_study(tols, warmup=30, steps=20, every=10):
+    """The adjoint leg on a short window at each tolerance: cost ratio, adjoint
+    iterations per step and the largest relative difference of dJ/d(yaw, pitch)
+    from the first tolerance's."""
+    out, ref = [], None
+    for tol in tols:
+        r = run_adjoint(steps, every, warmup, tol, grads=True)
+        g = r.pop("grads")
+        ref = g if ref is None else ref
+        out.append({"tol_rel": tol, "ratio": round(r["ratio"], 3), "forward_ms": round(r["forward_ms"], 2),
+                    "reverse_ms": round(r["reverse_ms"], 2), "adjoint_it_per_step": r["adjoint_it_per_step"],
+                    "forward_it_per_step": r["forward_it_per_step"],
+                    "grad_max_rel_vs_first": float(np.max(np.abs(g - ref) / np.abs(ref)))})
+        log(f"[bench] adjoint tolerance {tol:g}: {out[-1]}")
+    return {"warmup": warmup, "steps": steps, "checkpoint_every": every, "runs": out}
+
+
+def run_precision_adjoint(case, steps, every, warmup=2, tol=1e-6):
     """The precision form's adjoint: `warmup` steps of the precision form, then
     `steps` recorded ones (dalm_adjoint_begin_from into an FP64 context, checkpoints
     every `every` steps, no pressure tape: the replays re-solve in FP64) and the
@@ -323,11 +351,13 @@ def run_precision_adjoint(case, steps, every, warmup=2):
         s64.ctx.adjoint_begin_from(sm.ctx, steps, every, omega / steps, False)
         its = []
         t1 = time.perf_counter()
-        for _ in range(steps):
+        for n in range(steps):
             its.append(sm.step()["it"])
+            if (n + 1) % 10 == 0:
+                log(f"[bench] precision-form adjoint: recorded {n + 1} of {steps} steps")
         t_rec = (time.perf_counter() - t1) / steps
         t2 = time.perf_counter()
-        J, ap_g, wt_g, st = s64.ctx.adjoint_run(1e-8)
+        J, ap_g, wt_g, st = s64.ctx.adjoint_run(tol)
         t_rev = (time.perf_counter() - t2) / steps
         s64.ctx.adjoint_end()
     finally:
@@ -340,7 +370,7 @@ def run_precision_adjoint(case, steps, every, warmup=2):
     return {"ratio": t_rev / t_rec, "forward_ms": t_rec * 1e3, "reverse_ms": t_rev * 1e3,
             "replay_ms": st["replay_ms"] / steps, "reverse_steps_ms": st["reverse_ms"] / steps,
             "adjoint_it_per_step": st["adj_it"] / steps, "forward_it_per_step": float(np.mean(its)),
-            "steps": steps, "checkpoint_every": every, "warmup": warmup,
+            "steps": steps, "checkpoint_every": every, "warmup": warmup, "tol_rel": tol,
             "workload": (f"{m.interior[0]}x{m.interior[1]}x{m.interior[2]}, {len(case.sites)} rotors x 3 blades x "
                          f"{case.ap_per_blade} points, precision form recorded into an FP64 context, FP64 "
                          "replays re-solving the pressure"),
@@ -471,6 +501,10 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-tol", type=float, default=1e-6,
+                    help="relative tolerance of the adjoint Poisson solves")
+    ap.add_argument("--adjoint-tol-study", default="", metavar="TOLS",
+                    help="comma-separated tolerances: the adjoint leg on a short window at each")
     ap.add_argument("--mixed-adjoint-steps", type=int, default=20,
                     help="recorded steps of the precision form's adjoint on the 192x96x96 geometry (0: skip)")
     ap.add_argument("--c5-adjoint-steps", type=int, default=0,
@@ -566,7 +600,7 @@ def main():
     adj = None
     if not a.no_adjoint:
         try:
-            adj = run_adjoint(a.adjoint_steps, a.adjoint_every)
+            adj = run_adjoint(a.adjoint_steps, a.adjoint_every, tol=a.adjoint_tol)
         except Exception as e:
             log(f"[bench] adjoint leg failed: {e}")
     adj_mixed = adj_c5 = None
@@ -574,7 +608,7 @@ def main():
         try:
             cases = importlib.import_module("3dalm_amd.cases")
             n = a.mixed_adjoint_steps
-            adj_mixed = run_precision_adjoint(cases.case_c4(steps=2 + n), n, max(1, n // 2))
+            adj_mixed = run_precision_adjoint(cases.case_c4(steps=2 + n), n, max(1, n // 2), tol=a.adjoint_tol)
         except Exception as e:
             log(f"[bench] precision-form adjoint leg failed: {e}")
     if a.c5_adjoint_steps > 0 and have_mixed:
@@ -582,9 +616,15 @@ def main():
             cases = importlib.import_module("3dalm_amd.cases")
             n = a.c5_adjoint_steps
             adj_c5 = run_precision_adjoint(cases.case_c5(steps=a.c5_warmup + n), n, max(1, min(12, n // 2)),
-                                           warmup=a.c5_warmup)
+                                           warmup=a.c5_warmup, tol=a.adjoint_tol)
         except Exception as e:
             log(f"[bench] 768x384x384 adjoint leg failed: {e}")
+    tol_study = None
+    if a.adjoint_tol_study:
+        try:
+            tol_study = adjoint_tol_study([float(v) for v in a.adjoint_tol_study.split(",") if v])
+        except Exception as e:
+            log(f"[bench] adjoint tolerance study failed: {e}")
     c2 = output_path = sor = None
     if not a.no_extras:
         try:
@@ -649,6 +689,7 @@ def main():
         "adjoint": adj,
         "adjoint_mixed": adj_mixed,
         "adjoint_mixed_768x384x384": adj_c5,
+        "adjoint_tol_study": tol_study,
         "roofline": roof,
         "roofline_fp64": roof_fp64,
         "cpu_baseline": cpu,

@@ -59,6 +59,29 @@ def test_predictor(mesh, ok):
     assert np.array_equal(got, exp), max_rel(got, exp)
 
 
+def test_predictor_column_walk(ok):
+    """The predictor's column walk with the LDS-staged plane (k_predictor_star) on a
+    mesh large enough to select it (256 units of 16 planes): ragged along i (250
+    planes), j (60 rows: a last tile of 4) and k (100 cells: a second tile of 36
+    lanes), bit-exact against the oracle."""
+    mesh = small_mesh(250, 60, 100, 1.2)
+    print(f"[predictor column walk {mesh.size}]", flush=True)
+    Uold = rand_field(mesh, 11, 3, 0.3, 1.0)
+    JU = rand_field(mesh, 12, 3, 0.05, 0.0)
+    nut = np.abs(rand_field(mesh, 13, 1, 1e-3))
+    f = rand_field(mesh, 14, 3, 0.2)
+    U0 = rand_field(mesh, 15, 3)
+    c = ctx_with(mesh, U=U0, Uold=Uold, JU=JU, nut=nut, f=f)
+    try:
+        c.predictor(1e4, 0.005)
+        got = c.download("U")
+    finally:
+        c.close()
+    exp = U0.copy()
+    ok.predictor(po.make_grid(mesh), exp, Uold, JU, nut, f, 1e4, 0.005)
+    assert np.array_equal(got, exp), max_rel(got, exp)
+
+
 def test_flux_rhs_bc(mesh, ok):
     U = rand_field(mesh, 7, 3, 0.4, 1.0)
     g = po.make_grid(mesh)
