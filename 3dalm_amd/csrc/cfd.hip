@@ -111,16 +111,17 @@ __global__ __launch_bounds__(BLOCK) void k_predictor(
 }
 
 // column walk with the centre plane staged through LDS (star_walk.h); same
-// arithmetic per cell
-__global__ __launch_bounds__(kSBlock) void k_predictor_star(
+// arithmetic per cell.  kPredRows wavefronts per block.
+constexpr int kPredRows = 4;
+__global__ __launch_bounds__(TK * kPredRows) void k_predictor_star(
     Dims d, M1 m, const double* __restrict__ o0, const double* __restrict__ o1,
     const double* __restrict__ o2, const double* __restrict__ J0, const double* __restrict__ J1,
     const double* __restrict__ J2, const double* __restrict__ nut, const double* __restrict__ f0,
     const double* __restrict__ f1, const double* __restrict__ f2, double* __restrict__ u0,
     double* __restrict__ u1, double* __restrict__ u2, double inv_re, double dt, int ni) {
-    __shared__ StarLds<double> sh;
-    star_walk(d, o0, o1, o2, ni, sh,
-              [&](int i, int j, int k, int c, const double (&x)[3][5], const StarView<double>& v) {
+    __shared__ StarLds<double, kPredRows> sh;
+    star_walk<kPredRows>(d, o0, o1, o2, ni, sh,
+              [&](int i, int j, int k, int c, const double (&x)[3][5], const StarView<double, kPredRows>& v) {
                   const PredCell P = pred_cell(d, m, J0, J1, J2, nut, inv_re, i, j, k, c);
                   const double* frc[3] = {f0, f1, f2};
                   double* dst[3] = {u0, u1, u2};
@@ -594,9 +595,10 @@ int grid_for(long long n) {
 
 int launch_predictor(dalm_ctx* ctx, double Re, double dt) {
     probe_begin(ctx, DALM_K_PREDICTOR);
-    const int ni = ctx->d.gc >= 2 ? star_planes(ctx->d) : 0;
+    const int ni = ctx->d.gc >= 2 ? star_planes<kPredRows>(ctx->d) : 0;
     if (ni > 0)
-        hipLaunchKernelGGL(k_predictor_star, dim3(star_grid(ctx->d, ni)), dim3(kSBlock), 0, ctx->stream, ctx->d,
+        hipLaunchKernelGGL(k_predictor_star, dim3(star_grid<kPredRows>(ctx->d, ni)), dim3(TK * kPredRows), 0,
+                           ctx->stream, ctx->d,
                            mesh(ctx), ctx->Uold[0], ctx->Uold[1], ctx->Uold[2], ctx->JU[0], ctx->JU[1], ctx->JU[2],
                            ctx->nut, ctx->f[0], ctx->f[1], ctx->f[2], ctx->U[0], ctx->U[1], ctx->U[2], 1 / Re, dt,
                            ni);

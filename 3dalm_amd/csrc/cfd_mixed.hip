@@ -115,16 +115,18 @@ __global__ __launch_bounds__(BLOCK) void km_predictor(
     }
 }
 
-// column walk with the centre plane staged through LDS (star_walk.h)
-__global__ __launch_bounds__(kSBlock) void km_predictor_star(
+// column walk with the centre plane staged through LDS (star_walk.h), kPredRowsF
+// wavefronts per block
+constexpr int kPredRowsF = 8;
+__global__ __launch_bounds__(TK * kPredRowsF) void km_predictor_star(
     Dims d, MetricF m, const float* __restrict__ o0, const float* __restrict__ o1,
     const float* __restrict__ o2, const double* __restrict__ J0, const double* __restrict__ J1,
     const double* __restrict__ J2, const float* __restrict__ nut, const float* __restrict__ f0,
     const float* __restrict__ f1, const float* __restrict__ f2, float* __restrict__ u0,
     float* __restrict__ u1, float* __restrict__ u2, float inv_re, float dt, int ni) {
-    __shared__ StarLds<float> sh;
-    star_walk(d, o0, o1, o2, ni, sh,
-              [&](int i, int j, int k, int c, const float (&x)[3][5], const StarView<float>& v) {
+    __shared__ StarLds<float, kPredRowsF> sh;
+    star_walk<kPredRowsF>(d, o0, o1, o2, ni, sh,
+              [&](int i, int j, int k, int c, const float (&x)[3][5], const StarView<float, kPredRowsF>& v) {
                   const PredCellF P = pred_cell_f(d, m, J0, J1, J2, nut, inv_re, i, j, k, c);
                   const float* frc[3] = {f0, f1, f2};
                   float* dst[3] = {u0, u1, u2};
@@ -491,9 +493,10 @@ int launch_ghost_copy_m(dalm_ctx* ctx) {
 
 int launch_predictor_m(dalm_ctx* ctx, double Re, double dt) {
     probe_begin(ctx, DALM_K_PREDICTOR);
-    const int ni = ctx->d.gc >= 2 ? star_planes(ctx->d) : 0;
+    const int ni = ctx->d.gc >= 2 ? star_planes<kPredRowsF>(ctx->d) : 0;
     if (ni > 0)
-        hipLaunchKernelGGL(km_predictor_star, dim3(star_grid(ctx->d, ni)), dim3(kSBlock), 0, ctx->stream, ctx->d,
+        hipLaunchKernelGGL(km_predictor_star, dim3(star_grid<kPredRowsF>(ctx->d, ni)), dim3(TK * kPredRowsF), 0,
+                           ctx->stream, ctx->d,
                            ctx->mf, ctx->Uold32[0], ctx->Uold32[1], ctx->Uold32[2], ctx->JU[0], ctx->JU[1],
                            ctx->JU[2], ctx->nut32, ctx->f32[0], ctx->f32[1], ctx->f32[2], ctx->U32[0], ctx->U32[1],
                            ctx->U32[2], (float)(1 / Re), (float)dt, ni);

@@ -31,7 +31,8 @@ struct SolveState {
     int done;       // 1 once the stopping test (or a breakdown) has ended the solve
     int status;     // DALM_OK, DALM_NONCONVERGED, DALM_BREAKDOWN, DALM_NONFINITE
     int ru;         // precision form: a residual replacement is due
-    int pad;
+    int pending;    // precision form, fused iteration: the last iteration's update of
+                    // the pressure (x += alpha z_p + omega z_s) is still to be applied
     double maxr;    // precision form: largest |r| since the last replacement
     long long n_ru; // precision form: replacements so far
 };
@@ -265,7 +266,7 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP, FIN_MX_SOR, FIN_L2 };
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP, FIN_MX_SOR, FIN_L2, FIN_MX_OMEGA };
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
 // the finalize on explicit partials (FIN_SUM / FIN_L2 only: no solve or step state)
 int launch_finalize_raw(hipStream_t stream, int mode, int nb, const double* partials, double* out);
@@ -329,8 +330,14 @@ int fused_half_grid(const dalm_ctx* ctx, int* ni);
 // `tiles` j tiles: as many plane chunks as fill the compute units once, chunks of at
 // least 16 planes
 int fused_planes(const dalm_ctx* ctx, int tiles);
-int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* z_out, float* q_out);
-int launch_half_s(dalm_ctx* ctx, const float* q, float* s_out, float* z_out, float* t_out);
+int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out);
+int launch_half_s(dalm_ctx* ctx, const float* q);
+
+// the pressure update of a precision-form iteration, x + alpha z_p + omega z_s, in one
+// expression wherever it is evaluated (the fused p-half, the replacement and the flush)
+__device__ __forceinline__ double x_update(double x, float zp, float zs, double alpha, double omega) {
+    return x + alpha * (double)zp + omega * (double)zs;
+}
 void probe_begin(dalm_ctx* ctx, int kernel);
 // launches: back-to-back launches of the kernel between the two events
 void probe_end(dalm_ctx* ctx, int kernel, int launches = 1);

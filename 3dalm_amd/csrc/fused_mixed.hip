@@ -38,6 +38,16 @@
 // p and q are read at halo cells owned by neighbouring blocks, so the p-half reads
 // the previous p and q from one buffer of a pair and writes the new ones to the
 // other (the host alternates the pair every iteration).
+//
+// The end of an iteration, x += alpha z_p + omega z_s and r = s - omega t, has no
+// pass of its own.  The s-half also returns <r0, s>, <r0, t> and |s|^2, from which
+// the finalize expands |r|^2 and <r0, r> (FIN_MX_OMEGA); both updates are then
+// pending (SolveState.pending) and the next p-half applies them: it forms r from s
+// and t where it needs r (every row it computes), stores r on its owned cells and
+// updates x there.  A residual replacement applies the pending x update first
+// (kg_apply_pending), and the host applies it after the last iteration of a pass
+// (kg_flush_pending).  Per iteration this moves 84 B per cell instead of 88, in two
+// kernels instead of three.
 #include "ctx.h"
 
 namespace {
@@ -50,9 +60,11 @@ constexpr int kRowLen = 256 + 2 * kPad;
 
 typedef float f4 __attribute__((ext_vector_type(4), aligned(16)));
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2 __attribute__((ext_vector_type(2), aligned(16)));
 
 __device__ __forceinline__ f4 ld4(const float* p) { return *(const f4u*)p; }
-__device__ __forceinline__ void st4(float* p, f4 v) { *(f4u*)p = v; }
+// nontemporal: a stored vector is next read a kernel later, past the last-level cache
+__device__ __forceinline__ void st4(float* p, f4 v) { __builtin_nontemporal_store((f4u)v, (f4u*)p); }
 
 struct Lds {
     // the new plane of each stencil input (z1, z2, z3), in two alternating copies: a
@@ -100,16 +112,32 @@ __device__ __forceinline__ void block_sum16(double (&v)[NV]) {
     }
 }
 
-// WHICH 0: the p-half (vin = previous p, qin = previous q; vout = new p,
-//          yout = new q, dot <r0, q>)
-// WHICH 1: the s-half (qin = q; vout = s, yout = t, dots <t, s>, <t, t>)
+// WHICH 0: the p-half (vin = previous p, qin = previous q; vout = new p, zout = new
+//          z_p (before that, the previous z_p of the pending update), yout = new q;
+//          r read, or formed from s_in, t_in and stored; x updated; dot <r0, q>)
+// WHICH 1: the s-half (r, qin = q; vout = s, zout = z_s, yout = t; dots <t, s>,
+//          <t, t>, <r0, s>, <r0, t>, |s|^2)
+struct HalfArgs {
+    float* r;
+    const float *s_in, *t_in, *zs_in;  // p-half: the previous iteration's s, t, z_s
+    double* x;                         // p-half: the pressure
+    const float *vin, *qin;
+    float *vout, *zout, *yout;
+    const float* r0;
+    double* partials;
+};
+
 template <int WHICH>
-__global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef A, const float* __restrict__ r,
-                                                       const float* __restrict__ vin, const float* __restrict__ qin,
-                                                       float* __restrict__ vout, float* __restrict__ zout,
-                                                       float* __restrict__ yout, const float* __restrict__ r0,
-                                                       double* partials, int ni, const SolveState* st) {
+__global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef A, HalfArgs h, int ni,
+                                                       const SolveState* st) {
     if (st->done) return;
+    float* __restrict__ r = h.r;
+    const float* __restrict__ vin = h.vin;
+    const float* __restrict__ qin = h.qin;
+    float* __restrict__ vout = h.vout;
+    float* __restrict__ zout = h.zout;
+    float* __restrict__ yout = h.yout;
+    const float* __restrict__ r0 = h.r0;
     __shared__ Lds sh;
     // the row is uniform in a wavefront: its coefficients and tests live in scalars
     const int lane = threadIdx.x & 63, row = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -134,6 +162,9 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
     const int k0 = g + 4 * lane;
     const double beta = st->beta, om = st->omega, alpha = st->alpha;
     const bool first = st->it == 0;
+    // p-half: the previous iteration's updates of x and r are still to be applied
+    // (its om and alpha are those of st: FIN_ALPHA runs after this kernel)
+    const bool pend = WHICH == 0 && st->pending;
     // the SpMV's diagonal is scaled by the reciprocal of max|a_c| (within an ulp of the
     // division: far below the FP32 rounding of the product it feeds)
     const double inv_md = 1.0 / A.max_diag;
@@ -161,17 +192,19 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
     f4 va = zero, vb = zero, vc = zero;  // v (p or s) at t-1, t-2, t-3
     f4 z2a = zero, z2b = zero;  // z2 at t-2, t-3
     f4 z3a = zero, z3b = zero;  // z3 at t-3, t-4
-    double acc[2] = {0.0, 0.0};
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
-    // level-0 inputs of plane t (prefetched one step ahead)
+    // level-0 inputs of plane t (prefetched one step ahead); with a pending update
+    // r is s - om t, formed from s (in .r) and t
     struct In {
-        f4 r, v, q;
+        f4 r, v, q, t;
     };
     auto load0 = [&](int i) {
-        In x{zero, zero, zero};
+        In x{zero, zero, zero, zero};
         if (jin && i >= g && i < g + d.nx) {
             const int c = d.at(i, j, k0);
-            x.r = ld4(r + c);
+            x.r = ld4((pend ? h.s_in : r) + c);
+            if (pend) x.t = ld4(h.t_in + c);
             x.q = ld4(qin + c);
             if (WHICH == 0) x.v = ld4(vin + c);
         }
@@ -179,7 +212,7 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
     };
     In nx0 = load0(i0 - kLag);
     const int rr = row == 0 ? 1 : (row == kRows - 1 ? kRows - 2 : row);  // neighbour rows stay in range
-    f4 r0n = zero;  // r0 of the SpMV's plane (p-half)
+    f4 r0n = zero;  // r0 of the SpMV's plane
 
     for (int t = i0 - kLag; t < i1 + kLag; ++t) {
         const In in = nx0;
@@ -187,8 +220,8 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         const int iy = t - kLag;  // SpMV plane
         const bool yplane = owned && iy >= own0 && iy < i1;
         f4 r0c = r0n;
-        if (WHICH == 0 && owned && iy + 1 >= own0 && iy + 1 < i1) r0n = ld4(r0 + d.at(iy + 1, j, k0));
-        if (WHICH == 0 && !(owned && iy >= own0 && iy < i1)) r0c = zero;
+        if (owned && iy + 1 >= own0 && iy + 1 < i1) r0n = ld4(r0 + d.at(iy + 1, j, k0));
+        if (!(owned && iy >= own0 && iy < i1)) r0c = zero;
         // neighbours of the centre planes (written at the previous step, into the
         // other copy)
         const int cw = (t - i0 + kLag) & 1;
@@ -199,18 +232,39 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         if (jin && t >= g && t < g + d.nx) {
             const float acj = (Af.acx[t] - anu) - asu;  // the diagonal's (i, j) part, in its order
             const f4 atu = sh.atuf[lane], abu = sh.abuf[lane];
+            const bool mine = owned && t >= own0 && t < i1;
+            const int c = d.at(t, j, k0);
+            f4 rv = in.r;
+            if (pend) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) rv[q] = (float)((double)in.r[q] - om * (double)in.t[q]);
+                if (mine) {
+                    // the pending pressure update on this cell (z_p is read before
+                    // this kernel writes the new z_p here, three steps later)
+                    const f4 zp = ld4(zout + c), zs = ld4(h.zs_in + c);
+                    d2* xc = (d2*)(h.x + c);  // c is even: 16-byte aligned
+                    d2 xa = xc[0], xb = xc[1];
+                    xa[0] = x_update(xa[0], zp[0], zs[0], alpha, om);
+                    xa[1] = x_update(xa[1], zp[1], zs[1], alpha, om);
+                    xb[0] = x_update(xb[0], zp[2], zs[2], alpha, om);
+                    xb[1] = x_update(xb[1], zp[3], zs[3], alpha, om);
+                    __builtin_nontemporal_store(xa, xc);
+                    __builtin_nontemporal_store(xb, xc + 1);
+                    st4(r + c, rv);
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float vf;
                 if (WHICH == 0)
-                    vf = first ? in.r[q] : (float)((double)in.r[q] + beta * ((double)in.v[q] - om * (double)in.q[q]));
+                    vf = first ? rv[q] : (float)((double)rv[q] + beta * ((double)in.v[q] - om * (double)in.q[q]));
                 else
-                    vf = (float)((double)in.r[q] - alpha * (double)in.q[q]);
+                    vf = (float)((double)rv[q] - alpha * (double)in.q[q]);
                 vn[q] = vf;
                 const float ac = (acj - atu[q]) - abu[q];
                 z1n[q] = vf * __builtin_amdgcn_rcpf(ac);
             }
-            if (owned && t >= own0 && t < i1) st4(vout + d.at(t, j, k0), vn);
+            if (mine) st4(vout + c, vn);
         }
         // a stencil sweep of level `x` (centre xa at plane ic, xb before, xn after)
         auto sweep = [&](int ip, const f4& xb, const f4& xa, const f4& xn, const Nb& nb, const f4& rhs) {
@@ -253,8 +307,12 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
                 if (WHICH == 0) {
                     acc[0] += (double)r0c[q] * ys;
                 } else {
-                    acc[0] += ys * (double)vc[q];
+                    const double sv = (double)vc[q], rz = (double)r0c[q];
+                    acc[0] += ys * sv;
                     acc[1] += ys * ys;
+                    acc[2] += rz * sv;
+                    acc[3] += rz * ys;
+                    acc[4] += sv * sv;
                 }
             }
             const int c = d.at(iy, j, k0);
@@ -277,10 +335,11 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         z3b = z3a;
         z3a = z3n;
     }
-    block_sum16<2>(acc);
+    block_sum16<5>(acc);
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = acc[0];
-        if (WHICH == 1) partials[gridDim.x + blockIdx.x] = acc[1];
+        h.partials[blockIdx.x] = acc[0];
+        if (WHICH == 1)
+            for (int q = 1; q < 5; ++q) h.partials[q * gridDim.x + blockIdx.x] = acc[q];
     }
 }
 
@@ -311,20 +370,22 @@ int fused_half_grid(const dalm_ctx* ctx, int* ni) {
     return tjf * ((d.nx + n - 1) / n);
 }
 
-int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* z_out, float* q_out) {
+int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out) {
     int ni;
     const int grid = fused_half_grid(ctx, &ni);
+    const HalfArgs h{ctx->fr, ctx->fs, ctx->ft, ctx->fz2, ctx->p, p_in, q_in, p_out, ctx->fz, q_out, ctx->fr0,
+                     ctx->partials};
     hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_half4<0>), dim3(grid), dim3(64 * kRows), 0, ctx->stream, ctx->d, ctx->pcf,
-                       ctx->coef(), (const float*)ctx->fr, p_in, q_in, p_out, z_out, q_out, (const float*)ctx->fr0,
-                       ctx->partials, ni, (const SolveState*)ctx->solve);
+                       ctx->coef(), h, ni, (const SolveState*)ctx->solve);
     return hipGetLastError() == hipSuccess ? DALM_OK : DALM_HIP_ERROR;
 }
 
-int launch_half_s(dalm_ctx* ctx, const float* q, float* s_out, float* z_out, float* t_out) {
+int launch_half_s(dalm_ctx* ctx, const float* q) {
     int ni;
     const int grid = fused_half_grid(ctx, &ni);
+    const HalfArgs h{ctx->fr, nullptr, nullptr, nullptr, nullptr, nullptr, q, ctx->fs, ctx->fz2, ctx->ft, ctx->fr0,
+                     ctx->partials};
     hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_half4<1>), dim3(grid), dim3(64 * kRows), 0, ctx->stream, ctx->d, ctx->pcf,
-                       ctx->coef(), (const float*)ctx->fr, (const float*)nullptr, q, s_out, z_out, t_out,
-                       (const float*)nullptr, ctx->partials, ni, (const SolveState*)ctx->solve);
+                       ctx->coef(), h, ni, (const SolveState*)ctx->solve);
     return hipGetLastError() == hipSuccess ? DALM_OK : DALM_HIP_ERROR;
 }

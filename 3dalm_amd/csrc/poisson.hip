@@ -155,7 +155,10 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dims d, PCoef A, const double*
 
 // k_jacobi with four consecutive k per lane: the same per-cell arithmetic (OpD's
 // diagonal and operator, in its order), so the sweep is bit-identical; each vector
-// moves with two 16-byte accesses per lane and plane.
+// moves with two 16-byte accesses per lane and plane.  The four-cell kernels store
+// nontemporally: a stored vector is next read one kernel or one iteration later,
+// after more than the 256 MB last-level cache has streamed past, so keeping it
+// there only evicts data that is still to be re-read.
 typedef double d4k __attribute__((ext_vector_type(kVecK), aligned(8)));
 
 __global__ __launch_bounds__(BLOCK) void k_jacobi4(Dims d, PCoef A, const double* __restrict__ zin,
@@ -188,7 +191,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi4(Dims d, PCoef A, const double
             const double ax = ac * x0[q] + ae * xp[q] + aw * xm[q] + an * xn[q] + as * xs[q] + at[q] * t + ab[q] * b;
             out[q] = x0[q] + (vc[q] - ax) / ac;
         }
-        *(d4k*)(zout + cc) = out;
+        __builtin_nontemporal_store(out, (d4k*)(zout + cc));
     };
     // every value of plane i+1 is requested before plane i is computed
     auto step = [&](int i, int nxt) {
@@ -341,8 +344,8 @@ __global__ __launch_bounds__(BLOCK) void k_bi_p4(Dims d, PCoef A, const double* 
                               pv[m] = first ? v.r[m] : v.r[m] + beta * (v.p[m] - om * v.q[m]);
                               zv[m] = pc_first ? pv[m] / dg.at(A, acx, m) : 0.0;
                           }
-                          *(d4k*)(p + c) = pv;
-                          *(d4k*)(z + c) = zv;
+                          __builtin_nontemporal_store(pv, (d4k*)(p + c));
+                          __builtin_nontemporal_store(zv, (d4k*)(z + c));
                       });
 }
 
@@ -366,8 +369,8 @@ __global__ __launch_bounds__(BLOCK) void k_bi_s4(Dims d, PCoef A, const double* 
                              sv[m] = v.r[m] - alpha * v.q[m];
                              zv[m] = pc_first ? sv[m] / dg.at(A, acx, m) : 0.0;
                          }
-                         *(d4k*)(s + c) = sv;
-                         *(d4k*)(z + c) = zv;
+                         __builtin_nontemporal_store(sv, (d4k*)(s + c));
+                         __builtin_nontemporal_store(zv, (d4k*)(z + c));
                      });
 }
 
@@ -395,8 +398,8 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
                                  v[0] += rv[m] * rv[m];
                                  v[1] += a.r0[m] * rv[m];
                              }
-                             *(d4k*)(x + c) = xv;
-                             *(d4k*)(r + c) = rv;
+                             __builtin_nontemporal_store(xv, (d4k*)(x + c));
+                             __builtin_nontemporal_store(rv, (d4k*)(r + c));
                          });
     }
     block_sum<2>(v);
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv4(Dims d, PCoef A, const doubl
                     v[1] += yv * yv;     // <t, t>
                 }
             }
-            *(d4k*)(y + cc) = out;
+            __builtin_nontemporal_store(out, (d4k*)(y + cc));
         };
         auto step = [&](int i, int nxt) {
             const int cn = c + nxt;
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
                     vn[m] = nx0.r[m] - alpha * nx0.q[m];
                 z1n[m] = vn[m] / (((acj - sh.atu[lk + m]) - sh.abu[lk + m]) / A.max_diag);
             }
-            if (owned && t >= own0 && t < i1) *(d4k*)(vout + d.at(t, j, k0)) = vn;
+            if (owned && t >= own0 && t < i1) __builtin_nontemporal_store(vn, (d4k*)(vout + d.at(t, j, k0)));
         }
         nx0 = load0(t + 1);
         // neighbours of the centre planes of both sweeps (written at the previous step,
@@ -608,7 +611,8 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
         d4k z2n = zero;
         if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 1) z2n = sweep(p1, nbrs(cw ^ 1, 0), z1b, z1a, z1n, va);
         if (owned && p2 >= own0 && p2 < i1)
-            *(d4k*)(zout + d.at(p2, j, k0)) = sweep(p2, nbrs(cw ^ 1, 1), z2b, z2a, z2n, vb);
+            __builtin_nontemporal_store(sweep(p2, nbrs(cw ^ 1, 1), z2b, z2a, z2n, vb),
+                                        (d4k*)(zout + d.at(p2, j, k0)));
         *(d4a*)&sh.lv[cw][0][row][o] = z1n;
         *(d4a*)&sh.lv[cw][1][row][o] = z2n;
         __syncthreads();

@@ -92,7 +92,9 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi(Dims d, PCoefF A, const float
 // row), for rows of a multiple of 256 interior cells: each lane issues 16-byte loads
 // and stores, which stream from HBM about 20 % faster than 4-byte ones at
 // 512x256x256; k-neighbours inside the four come from registers.  Identical
-// arithmetic per cell.  Launched over tj x (nz / 256) x nch units.
+// arithmetic per cell.  Launched over tj x (nz / 256) x nch units.  Stores are
+// nontemporal, as in the FP64 four-cell kernels (3 % on the precision form's
+// iteration at 512x256x256).
 typedef float f4k __attribute__((ext_vector_type(kVecK), aligned(4)));
 
 __global__ __launch_bounds__(BLOCK) void kf_jacobi4(Dims d, PCoefF A, const float* __restrict__ zin,
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi4(Dims d, PCoefF A, const floa
             const float ax = ac * x0[q] + ae * xp[q] + aw * xm[q] + an * xn[q] + as * xs[q] + at[q] * t + ab[q] * b;
             out[q] = x0[q] + (vc[q] - ax) * __builtin_amdgcn_rcpf(ac);
         }
-        *(f4k*)(zout + cc) = out;
+        __builtin_nontemporal_store(out, (f4k*)(zout + cc));
     };
     // every value of plane i+1 is requested before plane i is computed (the walk's
     // look-ahead; the last plane re-reads its own)
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(BLOCK) void kg_spmv4(Dims d, PCoef A, const float* 
                     v[1] += ys * ys;
                 }
             }
-            *(f4k*)(y + cc) = out;
+            __builtin_nontemporal_store(out, (f4k*)(y + cc));
         };
         auto step = [&](int i, int nxt) {
             const int cn = c + nxt;
@@ -271,8 +273,8 @@ __global__ __launch_bounds__(BLOCK) void kg_p4(Dims d, PCoefF Af, const float* _
                               const float ac = (((acx - anu) - asu) - atu[m]) - abu[m];
                               zv[m] = pc_first ? pf * __builtin_amdgcn_rcpf(ac) : 0.f;
                           }
-                          *(f4k*)(p + c) = pv;
-                          *(f4k*)(z + c) = zv;
+                          __builtin_nontemporal_store(pv, (f4k*)(p + c));
+                          __builtin_nontemporal_store(zv, (f4k*)(z + c));
                       });
 }
 
@@ -304,8 +306,8 @@ __global__ __launch_bounds__(BLOCK) void kg_s4(Dims d, PCoefF Af, const float* _
                              const float ac = (((acx - anu) - asu) - atu[m]) - abu[m];
                              zv[m] = pc_first ? sf * __builtin_amdgcn_rcpf(ac) : 0.f;
                          }
-                         *(f4k*)(s + c) = sv;
-                         *(f4k*)(z + c) = zv;
+                         __builtin_nontemporal_store(sv, (f4k*)(s + c));
+                         __builtin_nontemporal_store(zv, (f4k*)(z + c));
                      });
 }
 
@@ -347,9 +349,9 @@ __global__ __launch_bounds__(BLOCK) void kg_xr4(Dims d, double* __restrict__ x, 
                     v[0] += rv * rv;
                     v[1] += (double)a.r0[m] * rv;
                 }
-                *(d2k*)(x + c) = xa;
-                *(d2k*)(x + c + 2) = xb;
-                *(f4k*)(r + c) = rv4;
+                __builtin_nontemporal_store(xa, (d2k*)(x + c));
+                __builtin_nontemporal_store(xb, (d2k*)(x + c + 2));
+                __builtin_nontemporal_store(rv4, (f4k*)(r + c));
             });
     }
     block_sum<2>(v);
@@ -387,8 +389,40 @@ __global__ void kg_state_reset(SolveState* st, double tol, long long max_it, dou
         st->done = 0;
         st->status = DALM_OK;
         st->ru = 0;
+        st->pending = 0;
         st->n_ru = 0;
         st->maxr = 0;
+    }
+}
+
+// The fused iteration's pending pressure update, x += alpha z_p + omega z_s on the
+// interior, ahead of a residual replacement: only when one is due and the update is
+// pending (FIN_MX_RU then clears the flag).
+__global__ __launch_bounds__(BLOCK) void kg_apply_pending(Dims d, double* __restrict__ x,
+                                                          const float* __restrict__ zp,
+                                                          const float* __restrict__ zs, const SolveState* st) {
+    if (st->done || !st->ru || !st->pending) return;
+    const double alpha = st->alpha, om = st->omega;
+    const TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        const Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        const int c = d.at(cc.i, cc.j, cc.k);
+        x[c] = x_update(x[c], zp[c], zs[c], alpha, om);
+    }
+}
+
+// The same with the values the host read from the state: at the end of a pass
+// (out == x), or into the best pressure's copy of x (out != x).
+__global__ __launch_bounds__(BLOCK) void kg_flush_pending(Dims d, const double* x, const float* __restrict__ zp,
+                                                          const float* __restrict__ zs, double* out, double alpha,
+                                                          double om) {
+    const TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        const Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        const int c = d.at(cc.i, cc.j, cc.k);
+        out[c] = x_update(x[c], zp[c], zs[c], alpha, om);
     }
 }
 
@@ -795,25 +829,24 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             for (long long c = 0; c < chunk; ++c) {
                 probe_iteration(ctx, launched + c);
                 if (fused) {
-                    // iteration k reads p, q from the pair's k-parity buffer and writes the other
+                    // iteration k reads p, q from the pair's k-parity buffer and writes the
+                    // other; the p-half also applies the previous iteration's pending x and
+                    // r updates, the s-half's dots give omega, |r| and <r0, r> (FIN_MX_OMEGA)
                     const long long k = launched + c;
                     float* P[2] = {ctx->fp, ctx->ftmp};
                     float* Q[2] = {ctx->fq, ctx->ftmp2};
                     probe_begin(ctx, DALM_K_HALF_P_MIXED);
-                    if ((rc = launch_half_p(ctx, P[k & 1], Q[k & 1], P[(k + 1) & 1], ctx->fz, Q[(k + 1) & 1])))
+                    if ((rc = launch_half_p(ctx, P[k & 1], Q[k & 1], P[(k + 1) & 1], Q[(k + 1) & 1])))
                         return dalm_fail_msg(ctx, rc, "fused p half-iteration launch");
                     probe_end(ctx, DALM_K_HALF_P_MIXED);
                     if ((rc = launch_finalize(ctx, FIN_ALPHA, fgrid, st, nullptr))) return rc;
                     probe_begin(ctx, DALM_K_HALF_S_MIXED);
-                    if ((rc = launch_half_s(ctx, Q[(k + 1) & 1], ctx->fs, ctx->fz2, ctx->ft)))
+                    if ((rc = launch_half_s(ctx, Q[(k + 1) & 1])))
                         return dalm_fail_msg(ctx, rc, "fused s half-iteration launch");
                     probe_end(ctx, DALM_K_HALF_S_MIXED);
-                    if ((rc = launch_finalize(ctx, FIN_OMEGA, fgrid, st, nullptr))) return rc;
-                    probe_begin(ctx, DALM_K_UPDATE_MIXED);
-                    hipLaunchKernelGGL(kg_xr4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, ctx->p, ctx->fz, ctx->fz2,
-                                       ctx->fs, ctx->ft, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
-                    probe_end(ctx, DALM_K_UPDATE_MIXED);
-                    if ((rc = launch_finalize(ctx, FIN_MX_XR, grid4, st, nullptr))) return rc;
+                    if ((rc = launch_finalize(ctx, FIN_MX_OMEGA, fgrid, st, nullptr))) return rc;
+                    hipLaunchKernelGGL(kg_apply_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p,
+                                       (const float*)ctx->fz, (const float*)ctx->fz2, (const SolveState*)st);
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<false>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream,
                                        d, A, ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                     if ((rc = launch_finalize(ctx, FIN_MX_RU, ctx->sgrid, st, nullptr))) return rc;
@@ -881,6 +914,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             if (e < 0.5 * best_saved) {
                 HIP_TRY(hipMemcpyAsync(ctx->pbest, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
                                        ctx->stream));
+                if (fused && h.pending)  // the pressure of the estimate e includes the pending update
+                    hipLaunchKernelGGL(kg_flush_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d,
+                                       (const double*)ctx->p, (const float*)ctx->fz, (const float*)ctx->fz2,
+                                       ctx->pbest, h.alpha, h.omega);
                 best_saved = e;
             }
             if (e < 0.7 * anchor) {
@@ -897,6 +934,13 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         }
         const SolveState& h = ctx->h_solve[0];
         probe_solve_end(ctx, h.it);
+        if (fused && h.pending) {
+            // the pass's last x update (the next p-half would have applied it)
+            hipLaunchKernelGGL(kg_flush_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d,
+                               (const double*)ctx->p, (const float*)ctx->fz, (const float*)ctx->fz2, ctx->p, h.alpha,
+                               h.omega);
+            LAUNCH_CHECK();
+        }
         if (fused && (h.it & 1)) {
             // after an odd number of iterations the current p and q are the partners
             std::swap(ctx->fp, ctx->ftmp);

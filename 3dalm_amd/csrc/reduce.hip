@@ -23,8 +23,15 @@ struct FinArgs {
 };
 
 __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
-    double s[2];
-    reduce_partials<2>(a.partials, a.nb, s);
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (a.mode == FIN_MX_OMEGA) {
+        reduce_partials<5>(a.partials, a.nb, s);
+    } else {
+        double s2[2];
+        reduce_partials<2>(a.partials, a.nb, s2);
+        s[0] = s2[0];
+        s[1] = s2[1];
+    }
     double mx = 0;
     if (a.mode == FIN_POST) mx = reduce_partials_max(a.partials + a.nb, a.nb);
     if (threadIdx.x != 0) return;
@@ -68,7 +75,10 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         }
         break;
     case FIN_ALPHA:
-        if (!st->done) st->alpha = st->rho / s[0];
+        if (!st->done) {
+            st->alpha = st->rho / s[0];
+            st->pending = 0;  // a fused p-half has applied the previous iteration's x update
+        }
         break;
     case FIN_OMEGA:
         if (!st->done) st->omega = s[0] / s[1];
@@ -152,11 +162,50 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         }
         break;
     }
+    case FIN_MX_OMEGA: {
+        // precision form, fused iteration (the s-half's dots): s[0] = <t, s>,
+        // s[1] = <t, t>, s[2] = <r0, s>, s[3] = <r0, t>, s[4] = |s|^2.  omega, and
+        // the new residual r = s - omega t through its norm and <r0, r> expanded
+        // in these dots (no pass over r); the update of x and r is pending until
+        // the next p-half (or a replacement, or the flush at the end of the pass).
+        // Then the tests of FIN_MX_XR.
+        if (st->done) break;
+        const double om = s[0] / s[1];
+        st->omega = om;
+        st->pending = 1;
+        double r2 = s[4] - 2.0 * om * s[0] + om * om * s[1];
+        if (r2 < 0.0) r2 = 0.0;
+        const double rn = sqrt(r2);
+        const double err = rn / st->root_n;
+        const long long it = st->it + 1;
+        st->it = it;
+        st->err = err;
+        if (!(err == err) || isinf(err) || !(om == om) || isinf(om)) {
+            st->done = 1;
+            st->status = DALM_NONFINITE;
+        } else if (err <= st->tol || it >= st->max_it || rn < kMixedReliableDelta * st->maxr) {
+            st->ru = 1;
+        } else {
+            if (rn > st->maxr) st->maxr = rn;
+            const double rho_new = s[2] - om * s[3];
+            if (fabs(rho_new) < (double)FLT_MIN) {
+                st->err = rho_new;
+                st->done = 1;
+                st->status = DALM_BREAKDOWN;
+            } else {
+                st->beta = (rho_new / st->rho) * (st->alpha / om);
+                st->rho_prev = st->rho;
+                st->rho = rho_new;
+            }
+        }
+        break;
+    }
     case FIN_MX_RU: {
         // precision form, residual replacement: s[0] = <r0, r>, s[1] = |b - A x|^2
         // in FP64.  The solve ends only on the true residual.
         if (st->done || !st->ru) break;
         st->ru = 0;
+        st->pending = 0;  // the replacement applied the pending x update first
         st->n_ru += 1;
         double rn = sqrt(s[1]);
         double err = rn / st->root_n;

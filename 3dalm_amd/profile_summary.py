@@ -50,11 +50,16 @@ ALGO_BYTES = {
     "k_jacobi4": 24, "k_bi_spmv4<0>": 24, "k_bi_spmv4<1>": 24, "k_bi_p4": 40, "k_bi_s4": 32, "k_bi_xr4": 64,
     "kf_jacobi4": 12, "kg_spmv4<0>": 12, "kg_spmv4<1>": 12, "kg_p4": 20, "kg_s4": 16, "kg_xr4": 40,
     # fused half-iterations (rows of 256 cells, Jacobi x3)
-    "kg_half4<0>": 28, "kg_half4<1>": 20, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
+    "kg_half4<0>": 60, "kg_half4<1>": 24, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
+    # the predictor's column walk with the LDS-staged plane
+    "k_predictor_star": 104, "km_predictor_star": 64,
 }
 # kernels that sweep every padded cell: algorithmic bytes per padded cell
 # (needs --padded-cells)
-ALGO_BYTES_PADDED = {"k_time_average<double>": 72, "k_time_average<float>": 60}
+ALGO_BYTES_PADDED = {"k_time_average<double>": 72, "k_time_average<float>": 60,
+                     # reverse predictor: read U_b, Uold, JU (3 each), nu_t; write the bars of
+                     # Uold, JU, f (3 each) and nu_t
+                     "ka_predictor": 160}
 
 
 def short_name(name: str) -> str:
