@@ -514,25 +514,44 @@ __global__ void k_ubc(Dims d, M1 m, double* __restrict__ u0, double* __restrict_
 }
 
 // Smagorinsky nu_t at one cell (src/cfd.h:785-826)
-__device__ __forceinline__ double nut_cell(const Dims& d, const M1& m, const double* u0,
-                                           const double* u1, const double* u2, double Cs, int c,
-                                           int i, int j, int k) {
+// Smagorinsky nu_t from the central differences of U (src/cfd.h:785-826): e, w, n,
+// s, t, b hold the three components at i+1, i-1, j+1, j-1, k+1, k-1
+__device__ __forceinline__ double nut_from(const M1& m, double Cs, int i, int j, int k, const double (&e)[3],
+                                           const double (&w)[3], const double (&n)[3], const double (&s)[3],
+                                           const double (&t)[3], const double (&b)[3]) {
     double hx = m.x[i + 1] - m.x[i - 1], hy = m.y[j + 1] - m.y[j - 1], hz = m.z[k + 1] - m.z[k - 1];
     double vol = m.dx[i] * m.dy[j] * m.dz[k];
-    double dudx = (u0[c + d.si] - u0[c - d.si]) / hx;
-    double dudy = (u0[c + d.sj] - u0[c - d.sj]) / hy;
-    double dudz = (u0[c + 1] - u0[c - 1]) / hz;
-    double dvdx = (u1[c + d.si] - u1[c - d.si]) / hx;
-    double dvdy = (u1[c + d.sj] - u1[c - d.sj]) / hy;
-    double dvdz = (u1[c + 1] - u1[c - 1]) / hz;
-    double dwdx = (u2[c + d.si] - u2[c - d.si]) / hx;
-    double dwdy = (u2[c + d.sj] - u2[c - d.sj]) / hy;
-    double dwdz = (u2[c + 1] - u2[c - 1]) / hz;
+    double dudx = (e[0] - w[0]) / hx;
+    double dudy = (n[0] - s[0]) / hy;
+    double dudz = (t[0] - b[0]) / hz;
+    double dvdx = (e[1] - w[1]) / hx;
+    double dvdy = (n[1] - s[1]) / hy;
+    double dvdz = (t[1] - b[1]) / hz;
+    double dwdx = (e[2] - w[2]) / hx;
+    double dwdy = (n[2] - s[2]) / hy;
+    double dwdz = (t[2] - b[2]) / hz;
     double S = sqrt(2 * (dudx * dudx) + 2 * (dvdy * dvdy) + 2 * (dwdz * dwdz) +
                     (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx) +
                     (dvdz + dwdy) * (dvdz + dwdy));
     double l = Cs * cbrt(vol);
     return l * l * S;
+}
+__device__ __forceinline__ double nut_cell(const Dims& d, const M1& m, const double* u0,
+                                           const double* u1, const double* u2, double Cs, int c,
+                                           int i, int j, int k) {
+    const double* u[3] = {u0, u1, u2};
+    double e[3], w[3], n[3], s[3], t[3], b[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        e[q] = u[q][c + d.si], w[q] = u[q][c - d.si], n[q] = u[q][c + d.sj], s[q] = u[q][c - d.sj];
+        t[q] = u[q][c + 1], b[q] = u[q][c - 1];
+    }
+    return nut_from(m, Cs, i, j, k, e, w, n, s, t, b);
+}
+// the CFL number of a cell (src/cfd.h:1100-1141)
+__device__ __forceinline__ double cfl_cell(const M1& m, int i, int j, int k, double u0, double u1, double u2,
+                                           double dt) {
+    return fabs(u0) * dt / m.dx[i] + fabs(u1) * dt / m.dy[j] + fabs(u2) * dt / m.dz[k];
 }
 
 // Post pass fused over the interior: nu_t, divergence (stored on request),
@@ -558,9 +577,44 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dims d, M1 m, const double* __re
         double dv = flux_div(d, m, J0, J1, J2, c, i, j, k);
         if (div) div[c] = dv;
         v[0] += dv * dv;
-        double lc = fabs(u0[c]) * dt / m.dx[i] + fabs(u1[c]) * dt / m.dy[j] + fabs(u2[c]) * dt / m.dz[k];
+        double lc = cfl_cell(m, i, j, k, u0[c], u1[c], u2[c], dt);
         cfl = lc > cfl ? lc : cfl;
     }
+    block_sum<1>(v);
+    double bm = block_max(cfl);
+    double pv[2] = {v[0], bm};
+    store_partials<2>(pv, partials);
+}
+
+// the post pass as a column walk with the centre plane of U staged through LDS
+// (star_walk.h, 4-row blocks of BLOCK threads); same arithmetic per cell, the
+// reductions summed per block of the walk's grid
+__global__ __launch_bounds__(BLOCK) void k_post_star(Dims d, M1 m, const double* __restrict__ u0,
+                                                     const double* __restrict__ u1, const double* __restrict__ u2,
+                                                     const double* __restrict__ J0, const double* __restrict__ J1,
+                                                     const double* __restrict__ J2, double* __restrict__ nut,
+                                                     double* __restrict__ div, double Cs, double dt, int do_nut,
+                                                     double* partials, int ni) {
+    __shared__ StarLds<double, TJ> sh;
+    double v[1] = {0.0};
+    double cfl = 0.0;
+    star_walk<TJ>(d, u0, u1, u2, ni, sh,
+                  [&](int i, int j, int k, int c, const double (&x)[3][5], const StarView<double, TJ>& a) {
+                      if (do_nut) {
+                          double e[3], w[3], n[3], s[3], t[3], b[3];
+#pragma unroll
+                          for (int q = 0; q < 3; ++q) {
+                              e[q] = x[q][3], w[q] = x[q][1], n[q] = a.y(q, 3), s[q] = a.y(q, 1);
+                              t[q] = a.z(q, 3), b[q] = a.z(q, 1);
+                          }
+                          nut[c] = nut_from(m, Cs, i, j, k, e, w, n, s, t, b);
+                      }
+                      const double dv = flux_div(d, m, J0, J1, J2, c, i, j, k);
+                      if (div) div[c] = dv;
+                      v[0] += dv * dv;
+                      const double lc = cfl_cell(m, i, j, k, x[0][2], x[1][2], x[2][2], dt);
+                      cfl = lc > cfl ? lc : cfl;
+                  });
     block_sum<1>(v);
     double bm = block_max(cfl);
     double pv[2] = {v[0], bm};
@@ -696,7 +750,26 @@ int launch_ubc(dalm_ctx* ctx, const double Uin[3], double dt, int retracted, dou
     return DALM_OK;
 }
 
+// planes per unit of the post pass's column walk, or 0 for the direct kernel (too
+// few units, or more partials than the partials buffer holds)
+static int star_post_planes(const dalm_ctx* ctx) {
+    if (ctx->d.gc < 2) return 0;
+    const int ni = star_planes<TJ>(ctx->d);
+    if (ni == 0) return 0;
+    const long long cap = 8LL * (ctx->grid > ctx->sgrid ? ctx->grid : ctx->sgrid);
+    return 2LL * star_grid<TJ>(ctx->d, ni) <= cap ? ni : 0;
+}
+
 int launch_post(dalm_ctx* ctx, double Cs, double dt, bool store_div) {
+    const int ni = star_post_planes(ctx);
+    if (ni > 0) {
+        const int grid = star_grid<TJ>(ctx->d, ni);
+        hipLaunchKernelGGL(k_post_star, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->d, mesh(ctx), ctx->U[0],
+                           ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
+                           store_div ? ctx->div : nullptr, Cs, dt, 1, ctx->partials, ni);
+        LAUNCH_CHECK();
+        return launch_finalize(ctx, FIN_POST, grid, nullptr, nullptr);
+    }
     hipLaunchKernelGGL(k_post, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, mesh(ctx),
                        ctx->U[0], ctx->U[1], ctx->U[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut,
                        store_div ? ctx->div : nullptr, Cs, dt, 1, ctx->partials, ctx->counter, ctx->scal);

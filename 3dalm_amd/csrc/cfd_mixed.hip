@@ -379,6 +379,34 @@ __global__ __launch_bounds__(BLOCK) void km_solid(Dims d, const double* __restri
 }
 
 // nu_t (FP32), divergence (FP64, stored on request), sum div^2 and max CFL
+// nu_t in FP32 from the central differences of U (e, w, n, s, t, b: the three
+// components at i+1, i-1, j+1, j-1, k+1, k-1), and the FP64 CFL number of a cell
+__device__ __forceinline__ float nutf_from(const MetricF& mf, float Cs, int i, int j, int k, const float (&e)[3],
+                                           const float (&w)[3], const float (&n)[3], const float (&s)[3],
+                                           const float (&t)[3], const float (&b)[3]) {
+    const float rx = mf.r2[0][i], ry = mf.r2[1][j], rz = mf.r2[2][k];
+    float dudx = (e[0] - w[0]) * rx, dudy = (n[0] - s[0]) * ry;
+    float dudz = (t[0] - b[0]) * rz;
+    float dvdx = (e[1] - w[1]) * rx, dvdy = (n[1] - s[1]) * ry;
+    float dvdz = (t[1] - b[1]) * rz;
+    float dwdx = (e[2] - w[2]) * rx, dwdy = (n[2] - s[2]) * ry;
+    float dwdz = (t[2] - b[2]) * rz;
+    float a = dudy + dvdx, bb = dudz + dwdx, ee = dvdz + dwdy;
+    float S = sqrtf(2.f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + a * a + bb * bb + ee * ee);
+    float l = Cs * cbrtf(mf.w[0][i] * mf.w[1][j] * mf.w[2][k]);
+    return l * l * S;
+}
+__device__ __forceinline__ double cflm_cell(const M1& m, int i, int j, int k, float u0, float u1, float u2,
+                                            double dt) {
+    return dt * (fabs((double)u0) / m.dx[i] + fabs((double)u1) / m.dy[j] + fabs((double)u2) / m.dz[k]);
+}
+__device__ __forceinline__ double divm_cell(const Dims& d, const M1& m, const double* __restrict__ J0,
+                                            const double* __restrict__ J1, const double* __restrict__ J2, int i,
+                                            int j, int k, int c) {
+    double vol = m.dx[i] * m.dy[j] * m.dz[k];
+    return ((J0[c] - J0[c - d.si]) + (J1[c] - J1[c - d.sj]) + (J2[c] - J2[c - 1])) / vol;
+}
+
 __global__ __launch_bounds__(BLOCK) void km_post(Dims d, M1 m, MetricF mf, const float* __restrict__ u0,
                                                  const float* __restrict__ u1, const float* __restrict__ u2,
                                                  const double* __restrict__ J0, const double* __restrict__ J1,
@@ -388,30 +416,57 @@ __global__ __launch_bounds__(BLOCK) void km_post(Dims d, M1 m, MetricF mf, const
     TileRange tr = block_tiles(d);
     double v[1] = {0.0};
     double cfl = 0.0;
+    const float* u[3] = {u0, u1, u2};
     for (int t = tr.begin; t < tr.end; ++t) {
         Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         const int i = cc.i, j = cc.j, k = cc.k;
         const int c = d.at(i, j, k);
-        const float rx = mf.r2[0][i], ry = mf.r2[1][j], rz = mf.r2[2][k];
-        float dudx = (u0[c + d.si] - u0[c - d.si]) * rx, dudy = (u0[c + d.sj] - u0[c - d.sj]) * ry;
-        float dudz = (u0[c + 1] - u0[c - 1]) * rz;
-        float dvdx = (u1[c + d.si] - u1[c - d.si]) * rx, dvdy = (u1[c + d.sj] - u1[c - d.sj]) * ry;
-        float dvdz = (u1[c + 1] - u1[c - 1]) * rz;
-        float dwdx = (u2[c + d.si] - u2[c - d.si]) * rx, dwdy = (u2[c + d.sj] - u2[c - d.sj]) * ry;
-        float dwdz = (u2[c + 1] - u2[c - 1]) * rz;
-        float a = dudy + dvdx, b = dudz + dwdx, e = dvdz + dwdy;
-        float S = sqrtf(2.f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + a * a + b * b + e * e);
-        float l = Cs * cbrtf(mf.w[0][i] * mf.w[1][j] * mf.w[2][k]);
-        nut[c] = l * l * S;
-        double vol = m.dx[i] * m.dy[j] * m.dz[k];
-        double dv = ((J0[c] - J0[c - d.si]) + (J1[c] - J1[c - d.sj]) + (J2[c] - J2[c - 1])) / vol;
+        float e[3], w[3], n[3], s[3], tt[3], b[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            e[q] = u[q][c + d.si], w[q] = u[q][c - d.si], n[q] = u[q][c + d.sj], s[q] = u[q][c - d.sj];
+            tt[q] = u[q][c + 1], b[q] = u[q][c - 1];
+        }
+        nut[c] = nutf_from(mf, Cs, i, j, k, e, w, n, s, tt, b);
+        const double dv = divm_cell(d, m, J0, J1, J2, i, j, k, c);
         if (div) div[c] = dv;
         v[0] += dv * dv;
-        double lc = dt * (fabs((double)u0[c]) / m.dx[i] + fabs((double)u1[c]) / m.dy[j] +
-                          fabs((double)u2[c]) / m.dz[k]);
+        const double lc = cflm_cell(m, i, j, k, u0[c], u1[c], u2[c], dt);
         cfl = lc > cfl ? lc : cfl;
     }
+    block_sum<1>(v);
+    double bm = block_max(cfl);
+    double pv[2] = {v[0], bm};
+    store_partials<2>(pv, partials);
+}
+
+// the same as a column walk with the centre plane of U staged through LDS
+// (star_walk.h, 4-row blocks of BLOCK threads)
+__global__ __launch_bounds__(BLOCK) void km_post_star(Dims d, M1 m, MetricF mf, const float* __restrict__ u0,
+                                                      const float* __restrict__ u1, const float* __restrict__ u2,
+                                                      const double* __restrict__ J0, const double* __restrict__ J1,
+                                                      const double* __restrict__ J2, float* __restrict__ nut,
+                                                      double* __restrict__ div, float Cs, double dt, double* partials,
+                                                      int ni) {
+    __shared__ StarLds<float, TJ> sh;
+    double v[1] = {0.0};
+    double cfl = 0.0;
+    star_walk<TJ>(d, u0, u1, u2, ni, sh,
+                  [&](int i, int j, int k, int c, const float (&x)[3][5], const StarView<float, TJ>& a) {
+                      float e[3], w[3], n[3], s[3], tt[3], b[3];
+#pragma unroll
+                      for (int q = 0; q < 3; ++q) {
+                          e[q] = x[q][3], w[q] = x[q][1], n[q] = a.y(q, 3), s[q] = a.y(q, 1);
+                          tt[q] = a.z(q, 3), b[q] = a.z(q, 1);
+                      }
+                      nut[c] = nutf_from(mf, Cs, i, j, k, e, w, n, s, tt, b);
+                      const double dv = divm_cell(d, m, J0, J1, J2, i, j, k, c);
+                      if (div) div[c] = dv;
+                      v[0] += dv * dv;
+                      const double lc = cflm_cell(m, i, j, k, x[0][2], x[1][2], x[2][2], dt);
+                      cfl = lc > cfl ? lc : cfl;
+                  });
     block_sum<1>(v);
     double bm = block_max(cfl);
     double pv[2] = {v[0], bm};
@@ -546,6 +601,17 @@ int launch_solid_m(dalm_ctx* ctx) {
 }
 
 int launch_post_m(dalm_ctx* ctx, double Cs, double dt, bool store_div) {
+    int ni = ctx->d.gc >= 2 ? star_planes<TJ>(ctx->d) : 0;
+    const long long cap = 8LL * (ctx->grid > ctx->sgrid ? ctx->grid : ctx->sgrid);
+    if (ni > 0 && 2LL * star_grid<TJ>(ctx->d, ni) > cap) ni = 0;
+    if (ni > 0) {
+        const int grid = star_grid<TJ>(ctx->d, ni);
+        hipLaunchKernelGGL(km_post_star, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->d, mesh(ctx), ctx->mf,
+                           ctx->U32[0], ctx->U32[1], ctx->U32[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut32,
+                           store_div ? ctx->div : nullptr, (float)Cs, dt, ctx->partials, ni);
+        LAUNCH_CHECK();
+        return launch_finalize(ctx, FIN_POST, grid, nullptr, nullptr);
+    }
     hipLaunchKernelGGL(km_post, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, mesh(ctx), ctx->mf,
                        ctx->U32[0], ctx->U32[1], ctx->U32[2], ctx->JU[0], ctx->JU[1], ctx->JU[2], ctx->nut32,
                        store_div ? ctx->div : nullptr, (float)Cs, dt, ctx->partials, ctx->counter, ctx->scal);

@@ -52,7 +52,7 @@ ALGO_BYTES = {
     # fused half-iterations (rows of 256 cells, Jacobi x3)
     "kg_half4<0>": 60, "kg_half4<1>": 24, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
     # the predictor's column walk with the LDS-staged plane
-    "k_predictor_star": 104, "km_predictor_star": 64,
+    "k_predictor_star": 104, "km_predictor_star": 64, "k_post_star": 64, "km_post_star": 48,
 }
 # kernels that sweep every padded cell: algorithmic bytes per padded cell
 # (needs --padded-cells)

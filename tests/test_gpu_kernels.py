@@ -223,6 +223,26 @@ def test_post(mesh, ok):
     assert c.max_cfl(0.005) == ok.max_cfl(g, U, 0.005)
 
 
+def test_post_column_walk(ok):
+    """nu_t through the post pass's column walk (k_post_star: 4-row blocks of the LDS
+    walk, selected on this mesh) against the oracle, ragged along i, j and k."""
+    mesh = small_mesh(250, 60, 100, 1.2)
+    print(f"[post column walk {mesh.size}]", flush=True)
+    g = po.make_grid(mesh)
+    U = rand_field(mesh, 43, 3, 0.3, 1.0)
+    JU = rand_field(mesh, 44, 3, 0.1)
+    c = ctx_with(mesh, U=U, JU=JU)
+    try:
+        c.calc_nut(0.1)
+        got = c.download("nut")
+    finally:
+        c.close()
+    nut = np.zeros(mesh.L)
+    ok.nut(g, U, nut, 0.1)
+    m = interior_mask(mesh)
+    assert max_rel(got[m], nut[m]) <= 1e-14
+
+
 def test_alm(ok):
     case = cases.case_c2(32, 16, 16, steps=1, ap_per_blade=8)
     case.sites[0].angle = (0.0, 0.0, 7.0)   # yawed rotor
