@@ -80,7 +80,8 @@ __device__ __forceinline__ Cell tile_cell(const Dims& d, int t) {
 // Every unit walks exactly ni planes, [i0, i1): the last unit of a column starts
 // inside its predecessor's range when ni does not divide nx, and owns (writes,
 // accumulates) only the planes from own0 on.
-constexpr int kMaxPlanes = 8;  // planes per column unit at most (the unrolled walk)
+constexpr int kMaxPlanes = 8;         // planes of the unrolled walk (units of other lengths loop)
+constexpr int kMaxColumnPlanes = 32;  // planes per column unit at most
 struct Col {
     int i0, i1, own0, j, k;
     bool ok;  // (j, k) inside the interior (ragged tiles have idle lanes)

@@ -378,13 +378,15 @@ int dalm_create(dalm_ctx** out, const int64_t size[3], int64_t gc, const double*
     ctx->N = (long long)d.nx * d.ny * d.nz;
     ctx->precision = (int)precision;
     ctx->grid = cell_grid(d.ntiles);
-    // column units: about 2048 of them (more on large grids), 1 to kMaxPlanes planes
-    // each -- small grids get short columns rather than too few workgroups
+    // column units: about 2048 of them, 1 to kMaxColumnPlanes planes each -- small
+    // grids get short columns rather than too few workgroups, large ones long columns
+    // (each unit re-reads a halo plane at either end: 2 per 8 planes cost the
+    // four-cell SpMV and x/r update 4-5 % at 512x256x256 against 2 per 32)
     {
         long long per_plane = (long long)d.tj * d.tk;
         long long ni = (d.nx * per_plane + 2047) / 2048;
         if (ni < 1) ni = 1;
-        if (ni > kMaxPlanes) ni = kMaxPlanes;
+        if (ni > kMaxColumnPlanes) ni = kMaxColumnPlanes;
         if (ni > d.nx) ni = d.nx;
         d.ni = (int)ni;
         d.nch = (d.nx + d.ni - 1) / d.ni;
