@@ -314,6 +314,7 @@ def run_adjoint(steps, every, warmup=2, tol=1e-6, grads=False):
                          f"{len(case.sites)} rotors x 3 blades x {case.ap_per_blade} points, FP64, "
                          "pressure kept per step"),
             "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]],
+            "dJ_dpitch_per_turbine": [float(v) for v in wt_g[:, 1]],
             **({"grads": np.concatenate([wt_g[:, 0], wt_g[:, 1]])} if grads else {})}
 
 
@@ -376,7 +377,8 @@ def run_precision_adjoint(case, steps, every, warmup=2, tol=1e-6):
             "workload": (f"{m.interior[0]}x{m.interior[1]}x{m.interior[2]}, {len(case.sites)} rotors x 3 blades x "
                          f"{case.ap_per_blade} points, precision form recorded into an FP64 context, FP64 "
                          "replays re-solving the pressure"),
-            "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]]}
+            "J": J, "dJ_dyaw_per_turbine": [float(v) for v in wt_g[:, 0]],
+            "dJ_dpitch_per_turbine": [float(v) for v in wt_g[:, 1]]}
 
 
 # output path kernels: id, algorithmic bytes per cell (FP64 form, precision form) and
