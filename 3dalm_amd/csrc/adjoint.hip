@@ -1047,7 +1047,16 @@ int adjoint_solve(dalm_ctx* ctx, int fold_pbc, double tol_rel, long long max_it,
     }
     const double tol = tol_rel * sqrt(h) / sqrt((double)ctx->N);
     int st;
-    if (P.method == DALM_BICGSTAB)
+    if (S.from_mixed && P.method == DALM_BICGSTAB) {
+        // a precision-form recording: the precision form's solver (FP32 Krylov
+        // vectors and sweeps, ended on the true FP64 residual), on y and the
+        // right-hand side in place of the context's p and b
+        std::swap(ctx->p, S.ya);
+        std::swap(ctx->b, S.ab);
+        st = run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, max_it, tol, it_out, err_out);
+        std::swap(ctx->p, S.ya);
+        std::swap(ctx->b, S.ab);
+    } else if (P.method == DALM_BICGSTAB)
         st = run_bicgstab_on(ctx, S.ya, S.ab, 3, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, max_it, tol, it_out,
                              err_out);
     else
@@ -1432,6 +1441,7 @@ int begin_tape(dalm_ctx* ctx, int64_t steps, int64_t every, const double* tq_w, 
     AdjointState& S = ctx->adj;
     if (S.source) S.source->adj.sink = nullptr;
     S.source = nullptr;
+    S.from_mixed = false;
     double** big[4] = {&S.ck, &S.ptape, &S.seg, &S.fin};
     for (double** p : big)
         if (*p) {
@@ -1490,12 +1500,14 @@ int dalm_adjoint_begin_from(dalm_ctx* ctx, dalm_ctx* src, int64_t steps, int64_t
         return dalm_fail_msg(ctx, DALM_BAD_ARG, "the recorded context has another grid or actuator-line set-up");
     if (!src->have_prm) return dalm_fail_msg(ctx, DALM_BAD_ARG, "dalm_set_params not called on the recorded context");
     if (src->adj.sink) return dalm_fail_msg(ctx, DALM_BAD_ARG, "the context is already being recorded");
+    TRY(ensure_mixed_solver(ctx));
     TRY(begin_tape(ctx, steps, every, tq_w, flags, src->step));
     AdjointState& S = ctx->adj;
     TRY(save_state_from(src, S.ck));
     HIP_TRY(hipStreamSynchronize(src->stream));
     ctx->step = src->step;
     S.source = src;
+    S.from_mixed = true;
     src->adj.sink = ctx;
     S.taping = true;
     return DALM_OK;
@@ -1537,12 +1549,20 @@ int dalm_adjoint_run(dalm_ctx* ctx, double tol_rel, double* J, double* ap_grad, 
         TRY(load_state(ctx, S.ck + (size_t)sg * CK_FIELDS * L));
         ctx->step = S.start_step + s0;
         TRY(save_slot(ctx, 0));
+        S.replaying = true;
         for (long long n = s0 + 1; n <= s1; ++n) {
             const double* pg = S.keep_p ? S.ptape + (size_t)(n - 1) * L : nullptr;
             int rc = step_fp64(ctx, nullptr, pg);
-            if (rc && rc != DALM_NONCONVERGED && rc != DALM_BREAKDOWN) return rc;
-            TRY(save_slot(ctx, n - s0));
+            if (rc && rc != DALM_NONCONVERGED && rc != DALM_BREAKDOWN) {
+                S.replaying = false;
+                return rc;
+            }
+            if (int rc2 = save_slot(ctx, n - s0)) {
+                S.replaying = false;
+                return rc2;
+            }
         }
+        S.replaying = false;
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         double t1 = now_ms();
         for (long long n = s1; n > s0; --n)

@@ -150,6 +150,11 @@ struct AdjointState {
     // it; `sink` is the reverse link, set on the source
     dalm_ctx* source = nullptr;
     dalm_ctx* sink = nullptr;
+    // the tape came from a precision-form context (set by dalm_adjoint_begin_from
+    // until the next recording): its adjoint solves and, while `replaying`, the
+    // segment replays' pressure solves take the precision form's solver
+    bool from_mixed = false;
+    bool replaying = false;
     bool keep_p = false;
     double* ck = nullptr;     // checkpoints [steps/every + 1][11][L]: U, JU, p, nut, f
     double* ptape = nullptr;  // pressure after each step [steps][L] (keep_p)
@@ -355,6 +360,9 @@ int launch_time_average(dalm_ctx* ctx, double weight);
 
 // precision form (cfd_mixed.hip, poisson_mixed.hip, alm.hip)
 int setup_mixed(dalm_ctx* ctx);
+// an FP64 context that records a precision-form run solves its adjoint Poisson
+// equations with the precision form's solver: its FP32 tables and Krylov vectors
+int ensure_mixed_solver(dalm_ctx* ctx);
 int launch_ghost_copy_m(dalm_ctx* ctx);
 int launch_predictor_m(dalm_ctx* ctx, double Re, double dt);
 int launch_flux_rhs_m(dalm_ctx* ctx, const double Uin[3], double dt, double scale, bool rhs);

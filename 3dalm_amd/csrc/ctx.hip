@@ -286,7 +286,11 @@ int step_fp64(dalm_ctx* ctx, dalm_step_stats* stats, const double* p_given) {
         HIP_TRY(hipMemcpyAsync(ctx->p, p_given, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice, ctx->stream));
     } else {
         TRY(launch_rhs(ctx, P.dt, ctx->pc.max_diag));
-        if (P.method == DALM_BICGSTAB)
+        // the segment replays of a precision-form recording: their pressure takes the
+        // precision form's solver too
+        if (P.method == DALM_BICGSTAB && ctx->adj.replaying && ctx->adj.from_mixed)
+            status = run_poisson_mixed(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
+        else if (P.method == DALM_BICGSTAB)
             status = run_bicgstab(ctx, (int)P.pc_kind, P.pc_omega, (int)P.pc_it, P.max_it, P.tol, &it, &err);
         else
             status = run_sor(ctx, P.omega, P.max_it, P.tol, &it, &err);
@@ -326,6 +330,19 @@ Dims make_dims(const int64_t size[3], int64_t gc) {
 int cell_grid(int ntiles) {
     const int g = ntiles < 2048 ? ntiles : 2048;
     return g >= 64 ? (g / 8) * 8 : g;
+}
+
+int ensure_mixed_solver(dalm_ctx* ctx) {
+    if (ctx->precision == DALM_MIXED || ctx->fr) return DALM_OK;
+    int rc;
+    if ((rc = setup_mixed(ctx))) return rc;
+    const size_t L = (size_t)ctx->L;
+    float** scf[10] = {&ctx->fr, &ctx->fr0, &ctx->fp, &ctx->fq, &ctx->fs,
+                       &ctx->ft, &ctx->fz, &ctx->fz2, &ctx->ftmp, &ctx->ftmp2};
+    for (float** p : scf)
+        if ((rc = dev_alloc(ctx, p, L))) return rc;
+    if (!ctx->pbest && (rc = dev_alloc(ctx, &ctx->pbest, L))) return rc;
+    return DALM_OK;
 }
 
 extern "C" {
