@@ -739,8 +739,8 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     const int pc_first = (jac && pc_it >= 1) ? 1 : 0;
     // rows of a multiple of 256 interior cells: the four-per-lane kernels, whose
     // reductions leave grid4 partials
-    const bool vec4 = d.nz % (TK * kVecK) == 0;
-    const int grid4 = d.tj * (d.nz / (TK * kVecK)) * d.nch;
+    const bool vec4 = vec4_applies(d);
+    const int grid4 = d.tj * vec4_tiles(d) * d.nch;
     const int nb = vec4 ? grid4 : ctx->sgrid;
     // rows of exactly 256 cells with three Jacobi sweeps: the update and the sweeps of
     // each half-iteration are one kernel (k_half_pc4); p and q alternate between a

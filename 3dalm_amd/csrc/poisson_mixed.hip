@@ -101,14 +101,14 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi4(Dims d, PCoefF A, const floa
                                                     float* __restrict__ zout, const float* __restrict__ v,
                                                     const SolveState* st) {
     if (st->done) return;
-    const int tkv = d.nz / (TK * kVecK);
+    const int tkv = vec4_tiles(d);
     const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
     const int ic = u % d.nch, col = u / d.nch;
     const int kt = col % tkv, jt = col / tkv;
     const int lane = threadIdx.x % TK, row = threadIdx.x / TK;
     const int k0 = d.gc + (kt * TK + lane) * kVecK;
     const int j = d.gc + jt * TJ + row;
-    if (j >= d.gc + d.ny) return;
+    if (j >= d.gc + d.ny || k0 >= d.gc + d.nz) return;
     const int own0 = d.gc + ic * d.ni;
     int i1 = own0 + d.ni;
     if (i1 > d.gc + d.nx) i1 = d.gc + d.nx;
@@ -731,8 +731,8 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     const long long test_cap = ctx->fp32_pass_cap;
     const double root_n = sqrt((double)ctx->N);
     // rows of a multiple of 256 interior cells: the four-per-lane Jacobi sweep
-    const bool vec4 = d.nz % (TK * kVecK) == 0;
-    const int grid4 = d.tj * (d.nz / (TK * kVecK)) * d.nch;
+    const bool vec4 = vec4_applies(d);
+    const int grid4 = d.tj * vec4_tiles(d) * d.nch;
     // rows of exactly 256 cells with three Jacobi sweeps: each half-iteration is one
     // fused kernel (fused_mixed.hip); p and q alternate between a buffer pair (the
     // Jacobi partners, which the fused path does not use)

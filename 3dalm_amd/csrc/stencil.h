@@ -126,20 +126,26 @@ __device__ __forceinline__ void walk_points(const Dims& d, const Col& cl, Load&&
     }
 }
 
-// ---- four cells per lane (rows of a multiple of TK * kVecK interior cells) ----
+// ---- four cells per lane (rows of at least 256 interior cells, nz a multiple of 4) ----
 // A wavefront covers 256 consecutive k of one row; each vector moves with one
-// 16-byte (FP32) or two 16-byte (FP64) accesses per lane and plane.  Launch with
-// tj * (nz / (TK * kVecK)) * nch blocks.
+// 16-byte (FP32) or two 16-byte (FP64) accesses per lane and plane.  The last tile
+// of a row may be partly empty (nz = 384: 64 + 32 lanes).  Launch with
+// tj * vec4_tiles(d) * nch blocks.
 constexpr int kVecK = 4;
+DALM_HD int vec4_tiles(const Dims& d) { return (d.nz + TK * kVecK - 1) / (TK * kVecK); }
+// the four-cell kernels apply: nz a multiple of 4 and at least three quarters of the
+// tiles' lanes in use (nz = 256 k, 384, 512, ...)
+DALM_HD bool vec4_applies(const Dims& d) { return d.nz % kVecK == 0 && 4 * d.nz >= 3 * TK * kVecK * vec4_tiles(d); }
 
 // Unit of a four-per-lane kernel: a lane owns k0 .. k0+3 of row j over the unit's
-// planes [i0, i1), of which it writes [own0, i1).  ok is false on a ragged j row.
+// planes [i0, i1), of which it writes [own0, i1).  ok is false on a ragged j row and
+// on the empty lanes of a row's last, partial tile.
 struct Col4 {
     int i0, i1, own0, j, k0;
     bool ok;
 };
 __device__ __forceinline__ Col4 col_unit4(const Dims& d) {
-    const int tkv = d.nz / (TK * kVecK);
+    const int tkv = vec4_tiles(d);
     const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
     const int ic = u % d.nch, col = u / d.nch;
     const int kt = col % tkv, jt = col / tkv;
@@ -150,7 +156,7 @@ __device__ __forceinline__ Col4 col_unit4(const Dims& d) {
     r.i1 = r.own0 + d.ni;
     if (r.i1 > d.gc + d.nx) r.i1 = d.gc + d.nx;
     r.i0 = r.i1 - d.ni;
-    r.ok = r.j < d.gc + d.ny;
+    r.ok = r.j < d.gc + d.ny && r.k0 < d.gc + d.nz;
     return r;
 }
 

@@ -3,7 +3,11 @@ size (the CPU oracle would take minutes per step here):
 
   * every FP64 step ends on the reference's stopping test (err <= tol, or the
     iteration cap) and satisfies the projection identity
-    avg_div = dt * max_diag * err (the projected face fluxes are the operator's);
+    avg_div = dt * max_diag * |b - A p| / sqrt(N) (the projected face fluxes are the
+    operator's; the true residual is the oracle's, from the downloaded p and b);
+    BiCGStab's recursive residual err stays within 1 % of the true one (the two
+    drift apart in finite precision: 0.63 % measured after the 2136 iterations of
+    the 768x384x384 case's first solve, under 1e-5 on the other sizes);
   * the precision form ends every step on the same test, with avg div <= 1e-6
     or, where the tolerance allows more, dt * max_diag * tol (where the FP64 solve stops at the iteration cap, it must too, at most twice
     as far from the tolerance), turbine power within 0.1 % of the FP64 form and U
@@ -28,6 +32,7 @@ from helpers import interior_mask, rel_l2
 hip = importlib.import_module("3dalm_amd.hip")
 cases = importlib.import_module("3dalm_amd.cases")
 solver = importlib.import_module("3dalm_amd.solver")
+po = importlib.import_module("pyoracle")
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +44,16 @@ CONFIGS = {
 }
 
 
+def true_err(K, g, A, mesh, p, b):
+    """|b - A p| / sqrt(N) over the interior, with p's ghost cells zero as in the
+    solve's iterate (the pressure boundary condition fills them afterwards)."""
+    m = interior_mask(mesh)
+    x = np.where(m, p, 0.0)
+    r = np.zeros(mesh.L)
+    K.residual(g, A, x, b, r)
+    return float(np.linalg.norm(r[m]) / np.sqrt(np.count_nonzero(m)))
+
+
 @pytest.mark.parametrize("name", list(CONFIGS))
 def test_full_size_properties(name):
     builder, steps = CONFIGS[name]
@@ -47,14 +62,20 @@ def test_full_size_properties(name):
     s64 = solver.Solver(case)
     try:
         md = s64.ctx.build_A()
+        K, g = po.Kernels(), po.make_grid(case.mesh)
+        A = np.zeros(case.mesh.L * 7)
+        assert abs(K.build_A(g, A) - md) <= 1e-12 * md
         st64 = []
         for _ in range(steps):
             st = s64.step()
-            print(f"  fp64 step {st['step']} it={st['it']} err={st['err']:.3e} div={st['avg_div']:.3e}", flush=True)
+            te = true_err(K, g, A, case.mesh, s64.ctx.download("p"), s64.ctx.download("b"))
+            print(f"  fp64 step {st['step']} it={st['it']} err={st['err']:.6e} true {te:.6e} div={st['avg_div']:.6e}",
+                  flush=True)
             assert st["status"] == hip.OK or st["it"] >= case.max_it, st
             if st["status"] == hip.OK:
                 assert st["err"] <= case.tol
-            assert abs(st["avg_div"] - case.dt * md * st["err"]) <= 1e-5 * case.dt * md * st["err"], (st, md)
+            assert abs(st["avg_div"] - case.dt * md * te) <= 1e-5 * case.dt * md * te, (st, md, te)
+            assert abs(te - st["err"]) <= 1e-2 * st["err"], (st, te)
             st64.append(st)
         U64 = s64.field("U")
     finally:

@@ -149,14 +149,15 @@ def test_bicgstab(mesh, ok, pc):
     assert rel_l2(c.download("p")[m], x[m]) <= 1e-10
 
 
-@pytest.mark.parametrize("shape", [(5, 6, 256), (40, 20, 256)])
+@pytest.mark.parametrize("shape", [(5, 6, 256), (40, 20, 256), (6, 5, 384)])
 def test_bicgstab_rows256_first_iterations(ok, shape):
     """Rows of 256 interior cells run the FP64 BiCGStab kernels (p and s updates
     with the first Jacobi sweep, Jacobi sweeps, SpMV with its dot products, x/r
     update with its norms) with four cells per lane; with at least 8 rows the
     update and the three sweeps of each half-iteration are one kernel
     (k_half_pc4: 40x20x256 has two plane chunks, the second overlapping the first,
-    and a last j tile of 4 of its 8 rows).  Over the first iterations the Krylov
+    and a last j tile of 4 of its 8 rows).  Rows of 384 cells run the four-cell
+    kernels unfused, with a second, half-empty tile of lanes per row.  Over the first iterations the Krylov
     path has not yet amplified the dot products' summation order (the second
     iteration reads p and q from the partner buffers), so the iterate
     must agree with the oracle's to rounding (every stored vector is bit-identical

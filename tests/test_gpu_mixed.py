@@ -38,11 +38,22 @@ def _rows256():
                       ap_per_blade=4, width=0.25)
 
 
+def _rows384():
+    """Rows of 384 interior cells (16x8x384 on a 4x2x12 box): the four-cell kernels
+    with a half-empty second tile of lanes per row, unfused."""
+    m = cases.make_mesh(16, 8, 384, ((0.0, 4.0), (-1.0, 1.0), (0.0, 12.0)), 1.2, 1.2)
+    return cases.Case("rows384", m, method="BiCG", pc="Jacobi", pc_it=3, tol=1e-8, max_it=1000, steps=4,
+                      turbine=cases.default_turbine(R=0.5, tower=4.0), sites=[cases.TurbineSite((1.0, 0.0, 0.0), 12.0)],
+                      ap_per_blade=4, width=0.25)
+
+
 @pytest.mark.parametrize("builder,steps", [
     (lambda: cases.case_c2(32, 16, 16, steps=6, ap_per_blade=8), 6),
     (lambda: cases.case_c2(64, 32, 32, steps=4, ap_per_blade=16), 4),
-    # rows of 256 cells: the four-cells-per-lane Jacobi sweep
+    # rows of 256 cells: the fused half-iterations
     (lambda: _rows256(), 4),
+    # rows of 384 cells: the four-cell kernels with a partial tile
+    (lambda: _rows384(), 4),
 ])
 def test_mixed_against_fp64_oracle(builder, steps):
     case = builder()
