@@ -33,8 +33,12 @@ struct SolveState {
     int ru;         // precision form: a residual replacement is due
     int pending;    // precision form, fused iteration: the last iteration's update of
                     // the pressure (x += alpha z_p + omega z_s) is still to be applied
+    int xdef;       // precision form, fused iteration: the update of the iteration before
+                    // that is deferred too (its alpha_d, omega_d): the p-half applies the
+                    // pressure updates of two iterations at once, every other iteration
     double maxr;    // precision form: largest |r| since the last replacement
     long long n_ru; // precision form: replacements so far
+    double alpha_d, omega_d;
 };
 
 // Precision form: the recursive FP32 residual is replaced by the true FP64 one
@@ -208,6 +212,9 @@ struct dalm_ctx {
     float* nut32 = nullptr;
     float *fr = nullptr, *fr0 = nullptr, *fp = nullptr, *fq = nullptr, *fs = nullptr, *ft = nullptr,
           *fz = nullptr, *fz2 = nullptr, *ftmp = nullptr, *ftmp2 = nullptr;
+    // the fused iteration's second z_p / z_s buffers (iterations alternate between the
+    // pairs, so that a deferred pressure update keeps its z), allocated on first use
+    float *fzb = nullptr, *fz2b = nullptr;
     MetricF mf{};
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
@@ -335,8 +342,11 @@ int fused_half_grid(const dalm_ctx* ctx, int* ni);
 // `tiles` j tiles: as many plane chunks as fill the compute units once, chunks of at
 // least 16 planes
 int fused_planes(const dalm_ctx* ctx, int tiles);
-int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out);
-int launch_half_s(dalm_ctx* ctx, const float* q);
+// p-half of iteration k: z_p into zp_out (which holds z_p of iteration k-2), with
+// z_p of k-1 in zp1 and z_s of k-1 / k-2 in zs1 / zs2; s-half: z_s into zs_out
+int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out, float* zp_out,
+                  const float* zp1, const float* zs1, const float* zs2);
+int launch_half_s(dalm_ctx* ctx, const float* q, float* zs_out);
 
 // the pressure update of a precision-form iteration, x + alpha z_p + omega z_s, in one
 // expression wherever it is evaluated (the fused p-half, the replacement and the flush)
@@ -363,6 +373,8 @@ int setup_mixed(dalm_ctx* ctx);
 // an FP64 context that records a precision-form run solves its adjoint Poisson
 // equations with the precision form's solver: its FP32 tables and Krylov vectors
 int ensure_mixed_solver(dalm_ctx* ctx);
+// the fused iteration's second z_p / z_s buffers (fzb, fz2b), allocated on first use
+int ensure_fused_z(dalm_ctx* ctx);
 int launch_ghost_copy_m(dalm_ctx* ctx);
 int launch_predictor_m(dalm_ctx* ctx, double Re, double dt);
 int launch_flux_rhs_m(dalm_ctx* ctx, const double Uin[3], double dt, double scale, bool rhs);

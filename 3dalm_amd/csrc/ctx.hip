@@ -345,6 +345,13 @@ int ensure_mixed_solver(dalm_ctx* ctx) {
     return DALM_OK;
 }
 
+int ensure_fused_z(dalm_ctx* ctx) {
+    if (ctx->fzb && ctx->fz2b) return DALM_OK;
+    int rc;
+    if (!ctx->fzb && (rc = dev_alloc(ctx, &ctx->fzb, (size_t)ctx->L))) return rc;
+    return dev_alloc(ctx, &ctx->fz2b, (size_t)ctx->L);
+}
+
 extern "C" {
 
 int dalm_set_device(int64_t device) {
@@ -514,8 +521,8 @@ int dalm_destroy(dalm_ctx* ctx) {
         if (ctx->Uold32[q]) (void)hipFree(ctx->Uold32[q]);
         if (ctx->f32[q]) (void)hipFree(ctx->f32[q]);
     }
-    float* fptrs[] = {ctx->nut32, ctx->fr, ctx->fr0, ctx->fp,   ctx->fq,   ctx->fs,
-                      ctx->ft,    ctx->fz, ctx->fz2, ctx->ftmp, ctx->ftmp2};
+    float* fptrs[] = {ctx->nut32, ctx->fr,   ctx->fr0,  ctx->fp,  ctx->fq,   ctx->fs,  ctx->ft,
+                      ctx->fz,    ctx->fz2,  ctx->ftmp, ctx->ftmp2, ctx->fzb, ctx->fz2b};
     for (float* p : fptrs)
         if (p) (void)hipFree(p);
     for (void* p : ctx->owned_f) (void)hipFree(p);

@@ -85,6 +85,8 @@ __global__ void k_state_reset(SolveState* st, double tol, long long max_it, doub
         st->max_it = max_it;
         st->done = 0;
         st->status = DALM_OK;
+        st->pending = 0;
+        st->xdef = 0;
     }
 }
 

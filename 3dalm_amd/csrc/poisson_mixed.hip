@@ -390,39 +390,50 @@ __global__ void kg_state_reset(SolveState* st, double tol, long long max_it, dou
         st->status = DALM_OK;
         st->ru = 0;
         st->pending = 0;
+        st->xdef = 0;
         st->n_ru = 0;
         st->maxr = 0;
     }
 }
 
-// The fused iteration's pending pressure update, x += alpha z_p + omega z_s on the
-// interior, ahead of a residual replacement: only when one is due and the update is
-// pending (FIN_MX_RU then clears the flag).
+// The fused iteration's outstanding pressure updates on the interior, ahead of a
+// residual replacement: the deferred one (xdef: alpha_d, omega_d with zpd, zsd), then
+// the pending one (alpha, omega with zp, zs); only when a replacement is due
+// (FIN_MX_RU then clears both flags).
 __global__ __launch_bounds__(BLOCK) void kg_apply_pending(Dims d, double* __restrict__ x,
-                                                          const float* __restrict__ zp,
-                                                          const float* __restrict__ zs, const SolveState* st) {
-    if (st->done || !st->ru || !st->pending) return;
-    const double alpha = st->alpha, om = st->omega;
+                                                          const float* __restrict__ zp, const float* __restrict__ zs,
+                                                          const float* __restrict__ zpd, const float* __restrict__ zsd,
+                                                          const SolveState* st) {
+    if (st->done || !st->ru || !(st->pending || st->xdef)) return;
+    const bool pend = st->pending, def = st->xdef;
+    const double alpha = st->alpha, om = st->omega, alpha_d = st->alpha_d, om_d = st->omega_d;
     const TileRange tr = block_tiles(d);
     for (int t = tr.begin; t < tr.end; ++t) {
         const Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         const int c = d.at(cc.i, cc.j, cc.k);
-        x[c] = x_update(x[c], zp[c], zs[c], alpha, om);
+        double v = x[c];
+        if (def) v = x_update(v, zpd[c], zsd[c], alpha_d, om_d);
+        if (pend) v = x_update(v, zp[c], zs[c], alpha, om);
+        x[c] = v;
     }
 }
 
 // The same with the values the host read from the state: at the end of a pass
 // (out == x), or into the best pressure's copy of x (out != x).
 __global__ __launch_bounds__(BLOCK) void kg_flush_pending(Dims d, const double* x, const float* __restrict__ zp,
-                                                          const float* __restrict__ zs, double* out, double alpha,
-                                                          double om) {
+                                                          const float* __restrict__ zs, const float* __restrict__ zpd,
+                                                          const float* __restrict__ zsd, double* out, double alpha,
+                                                          double om, double alpha_d, double om_d, int pend, int def) {
     const TileRange tr = block_tiles(d);
     for (int t = tr.begin; t < tr.end; ++t) {
         const Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         const int c = d.at(cc.i, cc.j, cc.k);
-        out[c] = x_update(x[c], zp[c], zs[c], alpha, om);
+        double v = x[c];
+        if (def) v = x_update(v, zpd[c], zsd[c], alpha_d, om_d);
+        if (pend) v = x_update(v, zp[c], zs[c], alpha, om);
+        out[c] = v;
     }
 }
 
@@ -739,6 +750,21 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     const bool fused = fused_half_applies(ctx, pc_kind, pc_it);
     int fni = 0;
     const int fgrid = fused ? fused_half_grid(ctx, &fni) : 0;
+    if (fused) {
+        if (const int rz = ensure_fused_z(ctx)) return rz;
+    }
+    // the fused iteration k writes z_p / z_s into the pair of its parity
+    float* const ZP[2] = {ctx->fz, ctx->fzb};
+    float* const ZS[2] = {ctx->fz2, ctx->fz2b};
+    // the outstanding x updates after iteration `last` of a pass (pending: last,
+    // deferred: last - 1) applied to x into out
+    auto flush = [&](const SolveState& h, double* out) {
+        const long long last = h.it - 1;
+        hipLaunchKernelGGL(kg_flush_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, (const double*)ctx->p,
+                           (const float*)ZP[last & 1], (const float*)ZS[last & 1], (const float*)ZP[(last + 1) & 1],
+                           (const float*)ZS[(last + 1) & 1], out, h.alpha, h.omega, h.alpha_d, h.omega_d, h.pending,
+                           h.xdef);
+    };
     // FP32 preconditioner application on v32 into z0 (spare: the other Jacobi buffer)
     auto precondition = [&](float* z0, float* spare, const float* v32) -> float* {
         float* cur = z0;
@@ -836,17 +862,20 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                     float* P[2] = {ctx->fp, ctx->ftmp};
                     float* Q[2] = {ctx->fq, ctx->ftmp2};
                     probe_begin(ctx, DALM_K_HALF_P_MIXED);
-                    if ((rc = launch_half_p(ctx, P[k & 1], Q[k & 1], P[(k + 1) & 1], Q[(k + 1) & 1])))
+                    if ((rc = launch_half_p(ctx, P[k & 1], Q[k & 1], P[(k + 1) & 1], Q[(k + 1) & 1], ZP[k & 1],
+                                            ZP[(k + 1) & 1], ZS[(k + 1) & 1], ZS[k & 1])))
                         return dalm_fail_msg(ctx, rc, "fused p half-iteration launch");
                     probe_end(ctx, DALM_K_HALF_P_MIXED);
                     if ((rc = launch_finalize(ctx, FIN_ALPHA, fgrid, st, nullptr))) return rc;
                     probe_begin(ctx, DALM_K_HALF_S_MIXED);
-                    if ((rc = launch_half_s(ctx, Q[(k + 1) & 1])))
+                    if ((rc = launch_half_s(ctx, Q[(k + 1) & 1], ZS[k & 1])))
                         return dalm_fail_msg(ctx, rc, "fused s half-iteration launch");
                     probe_end(ctx, DALM_K_HALF_S_MIXED);
                     if ((rc = launch_finalize(ctx, FIN_MX_OMEGA, fgrid, st, nullptr))) return rc;
                     hipLaunchKernelGGL(kg_apply_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p,
-                                       (const float*)ctx->fz, (const float*)ctx->fz2, (const SolveState*)st);
+                                       (const float*)ZP[k & 1], (const float*)ZS[k & 1],
+                                       (const float*)ZP[(k + 1) & 1], (const float*)ZS[(k + 1) & 1],
+                                       (const SolveState*)st);
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<false>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream,
                                        d, A, ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                     if ((rc = launch_finalize(ctx, FIN_MX_RU, ctx->sgrid, st, nullptr))) return rc;
@@ -914,10 +943,8 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             if (e < 0.5 * best_saved) {
                 HIP_TRY(hipMemcpyAsync(ctx->pbest, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
                                        ctx->stream));
-                if (fused && h.pending)  // the pressure of the estimate e includes the pending update
-                    hipLaunchKernelGGL(kg_flush_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d,
-                                       (const double*)ctx->p, (const float*)ctx->fz, (const float*)ctx->fz2,
-                                       ctx->pbest, h.alpha, h.omega);
+                if (fused && (h.pending || h.xdef))  // the pressure of the estimate e includes them
+                    flush(h, ctx->pbest);
                 best_saved = e;
             }
             if (e < 0.7 * anchor) {
@@ -934,11 +961,9 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         }
         const SolveState& h = ctx->h_solve[0];
         probe_solve_end(ctx, h.it);
-        if (fused && h.pending) {
-            // the pass's last x update (the next p-half would have applied it)
-            hipLaunchKernelGGL(kg_flush_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d,
-                               (const double*)ctx->p, (const float*)ctx->fz, (const float*)ctx->fz2, ctx->p, h.alpha,
-                               h.omega);
+        if (fused && (h.pending || h.xdef)) {
+            // the pass's last x updates (the next p-halves would have applied them)
+            flush(h, ctx->p);
             LAUNCH_CHECK();
         }
         if (fused && (h.it & 1)) {

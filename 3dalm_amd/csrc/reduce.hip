@@ -76,8 +76,20 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         break;
     case FIN_ALPHA:
         if (!st->done) {
+            if (st->pending) {
+                // the fused p-half has formed the previous iteration's r; its x update
+                // was applied together with a deferred one, or is now deferred itself
+                // (with the alpha and omega it needs)
+                if (st->xdef) {
+                    st->xdef = 0;
+                } else {
+                    st->xdef = 1;
+                    st->alpha_d = st->alpha;
+                    st->omega_d = st->omega;
+                }
+            }
             st->alpha = st->rho / s[0];
-            st->pending = 0;  // a fused p-half has applied the previous iteration's x update
+            st->pending = 0;
         }
         break;
     case FIN_OMEGA:
@@ -205,7 +217,8 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         // in FP64.  The solve ends only on the true residual.
         if (st->done || !st->ru) break;
         st->ru = 0;
-        st->pending = 0;  // the replacement applied the pending x update first
+        st->pending = 0;  // the replacement applied the pending x updates first
+        st->xdef = 0;
         st->n_ru += 1;
         double rn = sqrt(s[1]);
         double err = rn / st->root_n;

@@ -50,7 +50,9 @@ ALGO_BYTES = {
     "k_jacobi4": 24, "k_bi_spmv4<0>": 24, "k_bi_spmv4<1>": 24, "k_bi_p4": 40, "k_bi_s4": 32, "k_bi_xr4": 64,
     "kf_jacobi4": 12, "kg_spmv4<0>": 12, "kg_spmv4<1>": 12, "kg_p4": 20, "kg_s4": 16, "kg_xr4": 40,
     # fused half-iterations (rows of 256 cells, Jacobi x3)
-    "kg_half4<0>": 60, "kg_half4<1>": 24, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
+    # (kg_half4<0>: 36 and 68 B in turn, with the x updates of two iterations every
+    # other launch: 52 on average)
+    "kg_half4<0>": 52, "kg_half4<1>": 24, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
     # the predictor's column walk with the LDS-staged plane
     "k_predictor_star": 104, "km_predictor_star": 64, "k_post_star": 64, "km_post_star": 48,
 }
