@@ -39,6 +39,7 @@ struct SolveState {
     double maxr;    // precision form: largest |r| since the last replacement
     long long n_ru; // precision form: replacements so far
     double alpha_d, omega_d;
+    double alpha_prev, omega_prev;  // FP64 BiCGStab: alpha and omega of the previous iteration
 };
 
 // Precision form: the recursive FP32 residual is replaced by the true FP64 one
@@ -194,6 +195,9 @@ struct dalm_ctx {
     // Krylov work vectors
     double *r = nullptr, *r0 = nullptr, *kp = nullptr, *pp = nullptr, *q = nullptr, *s = nullptr,
            *ss = nullptr, *t = nullptr, *tmp = nullptr, *tmp2 = nullptr;
+    // the FP64 fused iteration's second z_p / z_s buffers (iterations alternate between
+    // the pairs, so that a deferred pressure update keeps its z), allocated on first use
+    double *ppb = nullptr, *ssb = nullptr;
     // iterations of the previous solve per use: 0 SOR, 1 BiCGStab (either form),
     // 2 the FP64 finish of the precision form, 3 the adjoint solve (sizes the
     // chunks enqueued per host check)
@@ -373,8 +377,10 @@ int setup_mixed(dalm_ctx* ctx);
 // an FP64 context that records a precision-form run solves its adjoint Poisson
 // equations with the precision form's solver: its FP32 tables and Krylov vectors
 int ensure_mixed_solver(dalm_ctx* ctx);
-// the fused iteration's second z_p / z_s buffers (fzb, fz2b), allocated on first use
+// the fused iterations' second z_p / z_s buffers (fzb, fz2b; FP64: ppb, ssb),
+// allocated on first use
 int ensure_fused_z(dalm_ctx* ctx);
+int ensure_fused_z64(dalm_ctx* ctx);
 int launch_ghost_copy_m(dalm_ctx* ctx);
 int launch_predictor_m(dalm_ctx* ctx, double Re, double dt);
 int launch_flux_rhs_m(dalm_ctx* ctx, const double Uin[3], double dt, double scale, bool rhs);

@@ -352,6 +352,13 @@ int ensure_fused_z(dalm_ctx* ctx) {
     return dev_alloc(ctx, &ctx->fz2b, (size_t)ctx->L);
 }
 
+int ensure_fused_z64(dalm_ctx* ctx) {
+    if (ctx->ppb && ctx->ssb) return DALM_OK;
+    int rc;
+    if (!ctx->ppb && (rc = dev_alloc(ctx, &ctx->ppb, (size_t)ctx->L))) return rc;
+    return dev_alloc(ctx, &ctx->ssb, (size_t)ctx->L);
+}
+
 extern "C" {
 
 int dalm_set_device(int64_t device) {
@@ -504,7 +511,8 @@ int dalm_destroy(dalm_ctx* ctx) {
     for (hipEvent_t e : ctx->probe.ev) (void)hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
-                      ctx->t, ctx->tmp, ctx->tmp2, ctx->pbest, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
+                      ctx->t, ctx->tmp, ctx->tmp2, ctx->ppb, ctx->ssb, ctx->pbest, ctx->partials, ctx->pc.ae_n,
+                      ctx->pc.aw_n,
                       ctx->pc.an_n, ctx->pc.as_n, ctx->pc.at_n, ctx->pc.ab_n, ctx->pc.acx,
                       ctx->pc.an_u, ctx->pc.as_u, ctx->pc.at_u, ctx->pc.ab_u, ctx->alm.wt, ctx->alm.ap,
                       ctx->alm.cd, ctx->alm.cl, ctx->alm.atk, ctx->inflow_modes, ctx->inflow};

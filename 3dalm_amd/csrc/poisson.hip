@@ -376,37 +376,72 @@ __global__ __launch_bounds__(BLOCK) void k_bi_s4(Dims d, PCoef A, const double* 
                      });
 }
 
+// XM 2: x += alpha pp + omega ss and r = s - omega t with the dots of k_bi_xr.  The
+// fused iteration updates x every other iteration (the same two roundings per update
+// as XM 2, in the same order, so x is the same to the bit): XM 0 forms r only and
+// leaves this iteration's x update to the next one, XM 1 applies the previous
+// iteration's (ppd, ssd with its alpha_prev, omega_prev) and then this one's.
+template <int XM>
 __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x, const double* __restrict__ pp,
                                                   const double* __restrict__ ss, const double* __restrict__ s,
                                                   const double* __restrict__ t_, double* __restrict__ r,
                                                   const double* __restrict__ r0, double* partials,
+                                                  const double* __restrict__ ppd, const double* __restrict__ ssd,
                                                   SolveState* st) {
     if (st->done) return;
     const double alpha = st->alpha, om = st->omega;
+    const double alpha_d = st->alpha_prev, om_d = st->omega_prev;
     const Col4 cl = col_unit4(d);
     double v[2] = {0.0, 0.0};
     if (cl.ok) {
         struct XR {
-            d4k x, pp, ss, s, t, r0;
+            d4k x, pp, ss, ppd, ssd, s, t, r0;
         };
+        const d4k z4 = {0.0, 0.0, 0.0, 0.0};
         walk_points4<XR>(d, cl,
-                         [&](int c) { return XR{ld4(x, c), ld4(pp, c), ld4(ss, c), ld4(s, c), ld4(t_, c), ld4(r0, c)}; },
+                         [&](int c) {
+                             XR a{z4, z4, z4, z4, z4, ld4(s, c), ld4(t_, c), ld4(r0, c)};
+                             if (XM != 0) {
+                                 a.x = ld4(x, c);
+                                 a.pp = ld4(pp, c);
+                                 a.ss = ld4(ss, c);
+                             }
+                             if (XM == 1) {
+                                 a.ppd = ld4(ppd, c);
+                                 a.ssd = ld4(ssd, c);
+                             }
+                             return a;
+                         },
                          [&](int, int c, const XR& a) {
                              d4k xv, rv;
 #pragma unroll
                              for (int m = 0; m < kVecK; ++m) {
-                                 xv[m] = a.x[m] + alpha * a.pp[m] + om * a.ss[m];
+                                 double xm = a.x[m];
+                                 if (XM == 1) xm = xm + alpha_d * a.ppd[m] + om_d * a.ssd[m];
+                                 xv[m] = xm + alpha * a.pp[m] + om * a.ss[m];
                                  rv[m] = a.s[m] - om * a.t[m];
                                  v[0] += rv[m] * rv[m];
                                  v[1] += a.r0[m] * rv[m];
                              }
-                             __builtin_nontemporal_store(xv, (d4k*)(x + c));
+                             if (XM != 0) __builtin_nontemporal_store(xv, (d4k*)(x + c));
                              __builtin_nontemporal_store(rv, (d4k*)(r + c));
                          });
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
     store_partials<2>(pv, partials);
+}
+
+// x += alpha pp + omega ss on the interior: a deferred update at the end of a solve
+__global__ __launch_bounds__(BLOCK) void k_x_update(Dims d, double* __restrict__ x, const double* __restrict__ pp,
+                                                    const double* __restrict__ ss, double alpha, double om) {
+    const TileRange tr = block_tiles(d);
+    for (int t = tr.begin; t < tr.end; ++t) {
+        const Cell cc = tile_cell(d, t);
+        if (!cc.inside) continue;
+        const int c = d.at(cc.i, cc.j, cc.k);
+        x[c] = x[c] + alpha * pp[c] + om * ss[c];
+    }
 }
 
 template <int WHICH>
@@ -748,6 +783,13 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     // each half-iteration are one kernel (k_half_pc4); p and q alternate between a
     // buffer pair (the Jacobi partners, unused on that path)
     const bool fused = jac && pc_it == 3 && d.nz == 256 && d.ny >= kFOwn;
+    if (fused) {
+        if (const int rz = ensure_fused_z64(ctx)) return rz;
+    }
+    // the fused iteration k writes z_p / z_s into the pair of its parity and updates x
+    // at odd k only, with the updates of k - 1 and k
+    double* const PP[2] = {ctx->pp, ctx->ppb};
+    double* const SS[2] = {ctx->ss, ctx->ssb};
     const int ftiles = (d.ny + kFOwn - 1) / kFOwn;
     const int fni = fused_planes(ctx, ftiles);
     const int fgrid = ftiles * ((d.nx + fni - 1) / fni);
@@ -797,26 +839,34 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
                 probe_begin(ctx, DALM_K_HALF_P);
                 hipLaunchKernelGGL(k_half_pc4<0>, dim3(fgrid), dim3(64 * kFRows), 0, ctx->stream, d, A,
                                    (const double*)ctx->r, (const double*)P[k & 1], (const double*)Q[k & 1],
-                                   P[(k + 1) & 1], ctx->pp, fni, (const SolveState*)ctx->solve);
+                                   P[(k + 1) & 1], PP[k & 1], fni, (const SolveState*)ctx->solve);
                 probe_end(ctx, DALM_K_HALF_P);
                 probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(k_bi_spmv4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ctx->pp,
+                hipLaunchKernelGGL(k_bi_spmv4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, PP[k & 1],
                                    Q[(k + 1) & 1], ctx->r0, ctx->partials, ctx->solve);
                 probe_end(ctx, DALM_K_SPMV);
                 if (int rc1 = launch_finalize(ctx, FIN_ALPHA, grid4, ctx->solve, nullptr)) return rc1;
                 probe_begin(ctx, DALM_K_HALF_S);
                 hipLaunchKernelGGL(k_half_pc4<1>, dim3(fgrid), dim3(64 * kFRows), 0, ctx->stream, d, A,
                                    (const double*)ctx->r, (const double*)nullptr, (const double*)Q[(k + 1) & 1],
-                                   ctx->s, ctx->ss, fni, (const SolveState*)ctx->solve);
+                                   ctx->s, SS[k & 1], fni, (const SolveState*)ctx->solve);
                 probe_end(ctx, DALM_K_HALF_S);
                 probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(k_bi_spmv4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, ctx->ss, ctx->t,
+                hipLaunchKernelGGL(k_bi_spmv4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, SS[k & 1], ctx->t,
                                    ctx->s, ctx->partials, ctx->solve);
                 probe_end(ctx, DALM_K_SPMV);
                 if (int rc2 = launch_finalize(ctx, FIN_OMEGA, grid4, ctx->solve, nullptr)) return rc2;
                 probe_begin(ctx, DALM_K_UPDATE);
-                hipLaunchKernelGGL(k_bi_xr4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, ctx->pp, ctx->ss,
-                                   ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->solve);
+                if (k & 1)
+                    hipLaunchKernelGGL(k_bi_xr4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x,
+                                       (const double*)PP[k & 1], (const double*)SS[k & 1], (const double*)ctx->s,
+                                       (const double*)ctx->t, ctx->r, (const double*)ctx->r0, ctx->partials,
+                                       (const double*)PP[(k + 1) & 1], (const double*)SS[(k + 1) & 1], ctx->solve);
+                else
+                    hipLaunchKernelGGL(k_bi_xr4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x,
+                                       (const double*)nullptr, (const double*)nullptr, (const double*)ctx->s,
+                                       (const double*)ctx->t, ctx->r, (const double*)ctx->r0, ctx->partials,
+                                       (const double*)nullptr, (const double*)nullptr, ctx->solve);
                 probe_end(ctx, DALM_K_UPDATE);
                 if (int rc3 = launch_finalize(ctx, FIN_BI_XR, grid4, ctx->solve, nullptr)) return rc3;
                 continue;
@@ -859,8 +909,10 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             if (int rc2 = launch_finalize(ctx, FIN_OMEGA, nb, ctx->solve, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
             if (vec4)
-                hipLaunchKernelGGL(k_bi_xr4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz, ctx->s,
-                                   ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->solve);
+                hipLaunchKernelGGL(k_bi_xr4<2>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, (const double*)ppz,
+                                   (const double*)ssz, (const double*)ctx->s, (const double*)ctx->t, ctx->r,
+                                   (const double*)ctx->r0, ctx->partials, (const double*)nullptr,
+                                   (const double*)nullptr, ctx->solve);
             else
                 hipLaunchKernelGGL(k_bi_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
                                    ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
@@ -873,7 +925,16 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
         if (rc) return rc;
     }
     probe_solve_end(ctx, ctx->h_solve->it);
-    if (fused && ((ctx->h_solve->it - it_start) & 1)) {
+    const long long done_it = ctx->h_solve->it - it_start;
+    if (fused && done_it > 0 && !((done_it - 1) & 1)) {
+        // the last iteration (even k) deferred its x update to an iteration that did not run
+        const long long last = done_it - 1;
+        hipLaunchKernelGGL(k_x_update, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x,
+                           (const double*)PP[last & 1], (const double*)SS[last & 1], ctx->h_solve->alpha,
+                           ctx->h_solve->omega);
+        LAUNCH_CHECK();
+    }
+    if (fused && (done_it & 1)) {
         // after an odd number of iterations the current p and q are the partners
         std::swap(ctx->kp, ctx->tmp);
         std::swap(ctx->q, ctx->tmp2);

@@ -97,6 +97,10 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         break;
     case FIN_BI_XR: {
         if (st->done) break;
+        // the fused iteration's x update may be deferred to the next iteration's
+        // update kernel, which then needs this iteration's alpha and omega
+        st->alpha_prev = st->alpha;
+        st->omega_prev = st->omega;
         double err = sqrt(s[0]) / st->root_n;
         long long it = st->it + 1;
         st->it = it;

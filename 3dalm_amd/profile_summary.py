@@ -47,7 +47,9 @@ ALGO_BYTES = {
     "km_flux_rhs": 44,
     "k_calc_q<double>": 32, "k_calc_q<float>": 20,
     "kf_sor": 10, "ks_resid": 24, "ks_add": 20,
-    "k_jacobi4": 24, "k_bi_spmv4<0>": 24, "k_bi_spmv4<1>": 24, "k_bi_p4": 40, "k_bi_s4": 32, "k_bi_xr4": 64,
+    "k_jacobi4": 24, "k_bi_spmv4<0>": 24, "k_bi_spmv4<1>": 24, "k_bi_p4": 40, "k_bi_s4": 32,
+    # the x/r update: r only (x deferred), x of two iterations and r, x and r
+    "k_bi_xr4<0>": 32, "k_bi_xr4<1>": 80, "k_bi_xr4<2>": 64,
     "kf_jacobi4": 12, "kg_spmv4<0>": 12, "kg_spmv4<1>": 12, "kg_p4": 20, "kg_s4": 16, "kg_xr4": 40,
     # fused half-iterations (rows of 256 cells, Jacobi x3)
     # (kg_half4<0>: 36 and 68 B in turn, with the x updates of two iterations every
