@@ -790,12 +790,6 @@ int launch_divergence(dalm_ctx* ctx) {
     return launch_finalize(ctx, FIN_POST, ctx->grid, nullptr, nullptr);
 }
 
-int launch_sumsq_raw(hipStream_t stream, const Dims& d, int grid, const double* v, double* partials) {
-    hipLaunchKernelGGL(k_scalar_sum, dim3(grid), dim3(BLOCK), 0, stream, d, v, v, partials, (unsigned*)nullptr,
-                       (double*)nullptr);
-    return hipGetLastError() == hipSuccess ? DALM_OK : DALM_HIP_ERROR;
-}
-
 int launch_l2(dalm_ctx* ctx, const double* v, double* out_host) {
     double* res = &ctx->solve->aux0;
     hipLaunchKernelGGL(k_scalar_sum, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, ctx->d, v, v,

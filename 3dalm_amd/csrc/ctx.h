@@ -282,12 +282,8 @@ struct dalm_ctx {
 };
 
 // finalize modes of reduce.hip
-enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP, FIN_MX_SOR, FIN_L2, FIN_MX_OMEGA };
+enum FinMode { FIN_POST = 0, FIN_SUM, FIN_RESID, FIN_SOR, FIN_BI_INIT, FIN_ALPHA, FIN_OMEGA, FIN_BI_XR, FIN_MX_INIT, FIN_MX_XR, FIN_MX_RU, FIN_MX_UP, FIN_MX_SOR, FIN_MX_OMEGA };
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out);
-// the finalize on explicit partials (FIN_SUM / FIN_L2 only: no solve or step state)
-int launch_finalize_raw(hipStream_t stream, int mode, int nb, const double* partials, double* out);
-// partial sums of v^2 over the interior, one per block (k_scalar_sum)
-int launch_sumsq_raw(hipStream_t stream, const Dims& d, int grid, const double* v, double* partials);
 // blocks of the cell kernels (the tile loop) for a grid of ntiles tiles
 int cell_grid(int ntiles);
 // interior extents, tiles and strides of a padded box (no column units)

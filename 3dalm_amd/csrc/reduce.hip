@@ -44,9 +44,6 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
     case FIN_SUM:
         *a.out = s[0];
         break;
-    case FIN_L2:
-        *a.out = sqrt(s[0]);
-        break;
     case FIN_RESID:
         st->err = sqrt(s[0]) / st->root_n;
         st->aux0 = s[0];
@@ -309,18 +306,6 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
 }
 
 }  // namespace
-
-int launch_finalize_raw(hipStream_t stream, int mode, int nb, const double* partials, double* out) {
-    FinArgs a;
-    a.mode = mode;
-    a.nb = nb;
-    a.partials = partials;
-    a.st = nullptr;
-    a.sc = nullptr;
-    a.out = out;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, stream, a);
-    return hipGetLastError() == hipSuccess ? DALM_OK : DALM_HIP_ERROR;
-}
 
 int launch_finalize(dalm_ctx* ctx, int mode, int nb, SolveState* st, double* out) {
     FinArgs a;
