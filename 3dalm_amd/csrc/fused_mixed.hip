@@ -239,11 +239,12 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
         const int cw = (t - i0 + kLag) & 1;
         const float (*prev)[kRows][kRowLen] = sh.lv[cw ^ 1];
 
+        // this lane's k coefficients of the FP32 diagonal, read once for level 0 and both sweeps
+        const f4 atu = sh.atuf[lane], abu = sh.abuf[lane];
         // level 0 at plane t
         f4 z1n = zero, vn = zero;
         if (jin && t >= g && t < g + d.nx) {
             const float acj = (Af.acx[t] - anu) - asu;  // the diagonal's (i, j) part, in its order
-            const f4 atu = sh.atuf[lane], abu = sh.abuf[lane];
             const bool mine = owned && t >= own0 && t < i1;
             const int c = d.at(t, j, k0);
             f4 rv = in.r;
@@ -285,9 +286,9 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
             if (mine) st4(vout + c, vn);
         }
         // a stencil sweep of level `x` (centre xa at plane ic, xb before, xn after)
+        const f4 at = sh.atf[lane], ab = sh.abf[lane];
         auto sweep = [&](int ip, const f4& xb, const f4& xa, const f4& xn, const Nb& nb, const f4& rhs) {
             const float acj = (Af.acx[ip] - anu) - asu, ae = Af.ae[ip], aw = Af.aw[ip];
-            const f4 at = sh.atf[lane], ab = sh.abf[lane], atu = sh.atuf[lane], abu = sh.abuf[lane];
             f4 out;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
