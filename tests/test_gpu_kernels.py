@@ -149,20 +149,24 @@ def test_bicgstab(mesh, ok, pc):
     assert rel_l2(c.download("p")[m], x[m]) <= 1e-10
 
 
+@pytest.mark.parametrize("iters", [2, 3])
 @pytest.mark.parametrize("shape", [(5, 6, 256), (40, 20, 256), (6, 5, 384)])
-def test_bicgstab_rows256_first_iterations(ok, shape):
+def test_bicgstab_rows256_first_iterations(ok, shape, iters):
     """Rows of 256 interior cells run the FP64 BiCGStab kernels (p and s updates
     with the first Jacobi sweep, Jacobi sweeps, SpMV with its dot products, x/r
     update with its norms) with four cells per lane; with at least 8 rows the
     update and the three sweeps of each half-iteration are one kernel
     (k_half_pc4: 40x20x256 has two plane chunks, the second overlapping the first,
     and a last j tile of 4 of its 8 rows).  Rows of 384 cells run the four-cell
-    kernels unfused, with a second, half-empty tile of lanes per row.  Over the first iterations the Krylov
+    kernels unfused, with a second, half-empty tile of lanes per row.  The fused
+    iteration updates x at odd iterations with the updates of two: after 2
+    iterations both are applied in the iteration, after 3 the last one at the end of
+    the solve.  Over the first iterations the Krylov
     path has not yet amplified the dot products' summation order (the second
     iteration reads p and q from the partner buffers), so the iterate
     must agree with the oracle's to rounding (every stored vector is bit-identical
     to the one-cell kernels' for the same scalars)."""
-    print(f"[bicgstab rows256 first iterations {shape}]", flush=True)
+    print(f"[bicgstab rows256 first iterations {shape} {iters}]", flush=True)
     mesh = small_mesh(*shape, 1.2)
     g = po.make_grid(mesh)
     A = np.zeros((mesh.L, 7))
@@ -174,10 +178,10 @@ def test_bicgstab_rows256_first_iterations(ok, shape):
     work = np.zeros(9 * mesh.L)
     it, err = po.C.c_int64(), po.C.c_double()
     po.lib("oracle")[0].or_run_bicgstab(po.C.byref(g), po.ptr(A), po.ptr(x), po.ptr(b), po.ptr(work),
-                                        hip.PC_JACOBI, 1.0, 3, po.C.byref(it), 2, po.C.byref(err), 1e-30)
-    git, gerr, st = c.run_bicgstab(hip.PC_JACOBI, 1.0, 3, 2, 1e-30)
-    print(f"bicgstab rows256, 2 iterations: err {gerr:.6e} vs {err.value:.6e}", flush=True)
-    assert git == it.value == 2 and st == hip.NONCONVERGED
+                                        hip.PC_JACOBI, 1.0, 3, po.C.byref(it), iters, po.C.byref(err), 1e-30)
+    git, gerr, st = c.run_bicgstab(hip.PC_JACOBI, 1.0, 3, iters, 1e-30)
+    print(f"bicgstab rows256, {iters} iterations: err {gerr:.6e} vs {err.value:.6e}", flush=True)
+    assert git == it.value == iters and st == hip.NONCONVERGED
     assert abs(gerr - err.value) <= 1e-12 * err.value
     m = interior_mask(mesh)
     assert rel_l2(c.download("p")[m], x[m]) <= 1e-13
