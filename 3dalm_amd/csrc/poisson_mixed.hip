@@ -856,8 +856,10 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 probe_iteration(ctx, launched + c);
                 if (fused) {
                     // iteration k reads p, q from the pair's k-parity buffer and writes the
-                    // other; the p-half also applies the previous iteration's pending x and
-                    // r updates, the s-half's dots give omega, |r| and <r0, r> (FIN_MX_OMEGA)
+                    // other, and its z_p, z_s into the k-parity pair of ZP, ZS; the p-half
+                    // forms the previous iteration's r and applies the outstanding x updates
+                    // every other iteration, the s-half's dots give omega, |r| and <r0, r>
+                    // (FIN_MX_OMEGA)
                     const long long k = launched + c;
                     float* P[2] = {ctx->fp, ctx->ftmp};
                     float* Q[2] = {ctx->fq, ctx->ftmp2};
