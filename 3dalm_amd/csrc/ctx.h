@@ -19,6 +19,12 @@
         if (e_ != hipSuccess) return dalm_fail(ctx, e_, #expr, __FILE__, __LINE__); \
     } while (0)
 
+// The fused iterations (rows of 256 cells) update the FP64 pressure once per this many
+// iterations (FP64 drop-in: kXGroup; precision form: kXGroupF), with the updates of all
+// of them in turn; their z vectors rotate through as many buffers.
+constexpr int kXGroup = 4;
+constexpr int kXGroupF = 8;
+
 // Device-resident scalar state of one Poisson solve (read by every kernel of the
 // solve; written only by the last block of a reduction).
 struct SolveState {
@@ -32,14 +38,16 @@ struct SolveState {
     int status;     // DALM_OK, DALM_NONCONVERGED, DALM_BREAKDOWN, DALM_NONFINITE
     int ru;         // precision form: a residual replacement is due
     int pending;    // precision form, fused iteration: the last iteration's update of
-                    // the pressure (x += alpha z_p + omega z_s) is still to be applied
-    int xdef;       // precision form, fused iteration: the update of the iteration before
-                    // that is deferred too (its alpha_d, omega_d): the p-half applies the
-                    // pressure updates of two iterations at once, every other iteration
+                    // the residual (r = s - omega t) is still to be formed
+    int xdef;       // precision form, fused iteration: the pressure updates of the last
+                    // xdef iterations are outstanding (their alpha_d, omega_d, oldest
+                    // first); kg_apply_pending4 applies them once there are kXGroupF
     double maxr;    // precision form: largest |r| since the last replacement
     long long n_ru; // precision form: replacements so far
-    double alpha_d, omega_d;
-    double alpha_prev, omega_prev;  // FP64 BiCGStab: alpha and omega of the previous iteration
+    double alpha_d[kXGroupF], omega_d[kXGroupF];
+    // FP64 BiCGStab: alpha and omega of the last kXGroup - 1 iterations, oldest first
+    // (the fused iteration's x update applies them every kXGroup iterations)
+    double alpha_h[kXGroup - 1], omega_h[kXGroup - 1];
 };
 
 // Precision form: the recursive FP32 residual is replaced by the true FP64 one
@@ -195,9 +203,10 @@ struct dalm_ctx {
     // Krylov work vectors
     double *r = nullptr, *r0 = nullptr, *kp = nullptr, *pp = nullptr, *q = nullptr, *s = nullptr,
            *ss = nullptr, *t = nullptr, *tmp = nullptr, *tmp2 = nullptr;
-    // the FP64 fused iteration's second z_p / z_s buffers (iterations alternate between
-    // the pairs, so that a deferred pressure update keeps its z), allocated on first use
-    double *ppb = nullptr, *ssb = nullptr;
+    // the FP64 fused iteration's further z_p / z_s buffers (iteration k uses pair k mod
+    // kXGroup, so that a deferred pressure update keeps its z), allocated on first use
+    double* ppx[kXGroup - 1] = {};
+    double* ssx[kXGroup - 1] = {};
     // iterations of the previous solve per use: 0 SOR, 1 BiCGStab (either form),
     // 2 the FP64 finish of the precision form, 3 the adjoint solve (sizes the
     // chunks enqueued per host check)
@@ -216,9 +225,10 @@ struct dalm_ctx {
     float* nut32 = nullptr;
     float *fr = nullptr, *fr0 = nullptr, *fp = nullptr, *fq = nullptr, *fs = nullptr, *ft = nullptr,
           *fz = nullptr, *fz2 = nullptr, *ftmp = nullptr, *ftmp2 = nullptr;
-    // the fused iteration's second z_p / z_s buffers (iterations alternate between the
-    // pairs, so that a deferred pressure update keeps its z), allocated on first use
-    float *fzb = nullptr, *fz2b = nullptr;
+    // the fused iteration's further z_p / z_s buffers (iteration k uses pair k mod
+    // kXGroupF, so that an outstanding pressure update keeps its z), allocated on first use
+    float* fzx[kXGroupF - 1] = {};
+    float* fz2x[kXGroupF - 1] = {};
     MetricF mf{};
     PCoefF pcf{};
     std::vector<void*> owned_f;   // float 1-D arrays behind mf / pcf
@@ -342,10 +352,8 @@ int fused_half_grid(const dalm_ctx* ctx, int* ni);
 // `tiles` j tiles: as many plane chunks as fill the compute units once, chunks of at
 // least 16 planes
 int fused_planes(const dalm_ctx* ctx, int tiles);
-// p-half of iteration k: z_p into zp_out (which holds z_p of iteration k-2), with
-// z_p of k-1 in zp1 and z_s of k-1 / k-2 in zs1 / zs2; s-half: z_s into zs_out
-int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out, float* zp_out,
-                  const float* zp1, const float* zs1, const float* zs2);
+// the fused half-iterations: z_p into zp_out, z_s into zs_out
+int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out, float* zp_out);
 int launch_half_s(dalm_ctx* ctx, const float* q, float* zs_out);
 
 // the pressure update of a precision-form iteration, x + alpha z_p + omega z_s, in one
@@ -373,7 +381,7 @@ int setup_mixed(dalm_ctx* ctx);
 // an FP64 context that records a precision-form run solves its adjoint Poisson
 // equations with the precision form's solver: its FP32 tables and Krylov vectors
 int ensure_mixed_solver(dalm_ctx* ctx);
-// the fused iterations' second z_p / z_s buffers (fzb, fz2b; FP64: ppb, ssb),
+// the fused iterations' further z_p / z_s buffers (fzx, fz2x; FP64: ppx, ssx),
 // allocated on first use
 int ensure_fused_z(dalm_ctx* ctx);
 int ensure_fused_z64(dalm_ctx* ctx);

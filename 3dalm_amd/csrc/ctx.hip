@@ -346,17 +346,21 @@ int ensure_mixed_solver(dalm_ctx* ctx) {
 }
 
 int ensure_fused_z(dalm_ctx* ctx) {
-    if (ctx->fzb && ctx->fz2b) return DALM_OK;
     int rc;
-    if (!ctx->fzb && (rc = dev_alloc(ctx, &ctx->fzb, (size_t)ctx->L))) return rc;
-    return dev_alloc(ctx, &ctx->fz2b, (size_t)ctx->L);
+    for (int q = 0; q < kXGroupF - 1; ++q) {
+        if (!ctx->fzx[q] && (rc = dev_alloc(ctx, &ctx->fzx[q], (size_t)ctx->L))) return rc;
+        if (!ctx->fz2x[q] && (rc = dev_alloc(ctx, &ctx->fz2x[q], (size_t)ctx->L))) return rc;
+    }
+    return DALM_OK;
 }
 
 int ensure_fused_z64(dalm_ctx* ctx) {
-    if (ctx->ppb && ctx->ssb) return DALM_OK;
     int rc;
-    if (!ctx->ppb && (rc = dev_alloc(ctx, &ctx->ppb, (size_t)ctx->L))) return rc;
-    return dev_alloc(ctx, &ctx->ssb, (size_t)ctx->L);
+    for (int q = 0; q < kXGroup - 1; ++q) {
+        if (!ctx->ppx[q] && (rc = dev_alloc(ctx, &ctx->ppx[q], (size_t)ctx->L))) return rc;
+        if (!ctx->ssx[q] && (rc = dev_alloc(ctx, &ctx->ssx[q], (size_t)ctx->L))) return rc;
+    }
+    return DALM_OK;
 }
 
 extern "C" {
@@ -511,13 +515,20 @@ int dalm_destroy(dalm_ctx* ctx) {
     for (hipEvent_t e : ctx->probe.ev) (void)hipEventDestroy(e);
     double* ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->dx, ctx->dy, ctx->dz, ctx->p, ctx->nut, ctx->div,
                       ctx->b, ctx->solid, ctx->r, ctx->r0, ctx->kp, ctx->pp, ctx->q, ctx->s, ctx->ss,
-                      ctx->t, ctx->tmp, ctx->tmp2, ctx->ppb, ctx->ssb, ctx->pbest, ctx->partials, ctx->pc.ae_n,
-                      ctx->pc.aw_n,
+                      ctx->t, ctx->tmp, ctx->tmp2, ctx->pbest, ctx->partials, ctx->pc.ae_n, ctx->pc.aw_n,
                       ctx->pc.an_n, ctx->pc.as_n, ctx->pc.at_n, ctx->pc.ab_n, ctx->pc.acx,
                       ctx->pc.an_u, ctx->pc.as_u, ctx->pc.at_u, ctx->pc.ab_u, ctx->alm.wt, ctx->alm.ap,
                       ctx->alm.cd, ctx->alm.cl, ctx->alm.atk, ctx->inflow_modes, ctx->inflow};
     for (double* p : ptrs)
         if (p) (void)hipFree(p);
+    for (int q = 0; q < kXGroup - 1; ++q) {
+        if (ctx->ppx[q]) (void)hipFree(ctx->ppx[q]);
+        if (ctx->ssx[q]) (void)hipFree(ctx->ssx[q]);
+    }
+    for (int q = 0; q < kXGroupF - 1; ++q) {
+        if (ctx->fzx[q]) (void)hipFree(ctx->fzx[q]);
+        if (ctx->fz2x[q]) (void)hipFree(ctx->fz2x[q]);
+    }
     for (int q = 0; q < 3; ++q) {
         if (ctx->U[q]) (void)hipFree(ctx->U[q]);
         if (ctx->Uold[q]) (void)hipFree(ctx->Uold[q]);
@@ -530,7 +541,7 @@ int dalm_destroy(dalm_ctx* ctx) {
         if (ctx->f32[q]) (void)hipFree(ctx->f32[q]);
     }
     float* fptrs[] = {ctx->nut32, ctx->fr,   ctx->fr0,  ctx->fp,  ctx->fq,   ctx->fs,  ctx->ft,
-                      ctx->fz,    ctx->fz2,  ctx->ftmp, ctx->ftmp2, ctx->fzb, ctx->fz2b};
+                      ctx->fz,    ctx->fz2,  ctx->ftmp, ctx->ftmp2};
     for (float* p : fptrs)
         if (p) (void)hipFree(p);
     for (void* p : ctx->owned_f) (void)hipFree(p);

@@ -44,15 +44,15 @@
 // the finalize expands |r|^2 and <r0, r> (FIN_MX_OMEGA); both updates are then
 // pending (SolveState.pending) and the next p-half applies them: it forms r from s
 // and t where it needs r (every row it computes) and stores r on its owned cells.
-// The FP64 pressure x is updated every other iteration only, with the updates of two
-// iterations in turn (SolveState.xdef; the same two roundings as one update per
-// iteration, so x is the same to the bit): a p-half that finds an update deferred
-// applies it and the pending one, otherwise it defers the pending one.  z_p and z_s
-// alternate between two buffer pairs with the iteration's parity, so a deferred
-// update keeps its z.  A residual replacement applies the outstanding x updates first
-// (kg_apply_pending), and the host applies them after the last iteration of a pass
-// (kg_flush_pending).  Per iteration this moves 76 B per cell (the p-half 36 and 68 B
-// in turn, the s-half 24) instead of 88, in two kernels instead of three.
+// Neither half updates the FP64 pressure x: the finalize records each iteration's
+// alpha and omega (SolveState.xdef counts the outstanding updates), z_p and z_s rotate
+// through kXGroupF buffer pairs with the iteration, and kg_apply_pending4 applies the
+// outstanding updates, in turn, once kXGroupF have gathered or ahead of a residual
+// replacement; the host applies the rest after the last iteration of a pass
+// (kg_flush_pending4).  Each update is the same two roundings in the same order as one
+// update per iteration, so x is the same to the bit.  Per iteration this moves 70 B
+// per cell (the p-half 36, the s-half 24, the grouped x update (16 + 8 kXGroupF) /
+// kXGroupF) instead of 88.
 #include "ctx.h"
 
 namespace {
@@ -65,11 +65,8 @@ constexpr int kRowLen = 256 + 2 * kPad;
 
 typedef float f4 __attribute__((ext_vector_type(4), aligned(16)));
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-typedef double d2 __attribute__((ext_vector_type(2), aligned(16)));
 
 __device__ __forceinline__ f4 ld4(const float* p) { return *(const f4u*)p; }
-typedef float f2 __attribute__((ext_vector_type(2), aligned(8)));
-__device__ __forceinline__ f2 ld2(const float* p) { return *(const f2*)p; }
 // nontemporal: a stored vector is next read a kernel later, past the last-level cache
 __device__ __forceinline__ void st4(float* p, f4 v) { __builtin_nontemporal_store((f4u)v, (f4u*)p); }
 
@@ -120,16 +117,13 @@ __device__ __forceinline__ void block_sum16(double (&v)[NV]) {
 }
 
 // WHICH 0: the p-half (vin = previous p, qin = previous q; vout = new p, zout = new
-//          z_p (before that, z_p of the iteration two back), yout = new q; r read, or
-//          formed from s_in, t_in and stored; x updated every other iteration; dot
+//          z_p, yout = new q; r read, or formed from s_in, t_in and stored; dot
 //          <r0, q>)
 // WHICH 1: the s-half (r, qin = q; vout = s, zout = z_s, yout = t; dots <t, s>,
 //          <t, t>, <r0, s>, <r0, t>, |s|^2)
 struct HalfArgs {
     float* r;
-    const float *s_in, *t_in, *zs_in;  // p-half: the previous iteration's s, t, z_s
-    const float *zp1_in, *zs2_in;      // p-half: z_p of the previous iteration, z_s of the one before
-    double* x;                         // p-half: the pressure
+    const float *s_in, *t_in;  // p-half: the previous iteration's s, t
     const float *vin, *qin;
     float *vout, *zout, *yout;
     const float* r0;
@@ -171,12 +165,9 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
     const int k0 = g + 4 * lane;
     const double beta = st->beta, om = st->omega, alpha = st->alpha;
     const bool first = st->it == 0;
-    // p-half: the previous iteration's updates of x and r are still to be applied
-    // (its om and alpha are those of st: FIN_ALPHA runs after this kernel); x takes
-    // it only together with a deferred one (alpha_d, omega_d, two iterations back)
+    // p-half: the previous iteration's r is still to be formed (its omega is that of
+    // st: FIN_ALPHA runs after this kernel)
     const bool pend = WHICH == 0 && st->pending;
-    const bool xboth = pend && st->xdef;
-    const double alpha_d = st->alpha_d, om_d = st->omega_d;
     // the SpMV's diagonal is scaled by the reciprocal of max|a_c| (within an ulp of the
     // division: far below the FP32 rounding of the product it feeds)
     const double inv_md = 1.0 / A.max_diag;
@@ -251,26 +242,7 @@ __global__ __launch_bounds__(64 * kRows) void kg_half4(Dims d, PCoefF Af, PCoef 
             if (pend) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) rv[q] = (float)((double)in.r[q] - om * (double)in.t[q]);
-                if (mine) {
-                    if (xboth) {
-                        // the deferred and the pending pressure updates on this cell, in
-                        // turn (the older z_p is read before this kernel writes the new
-                        // z_p here, three steps later); two cells at a time, which keeps
-                        // the loads' registers within the block's budget
-                        d2* xc = (d2*)(h.x + c);  // c is even: 16-byte aligned
-#pragma unroll 1
-                        for (int hf = 0; hf < 2; ++hf) {
-                            const int e = c + 2 * hf;
-                            const f2 zpd = ld2(zout + e), zsd = ld2(h.zs2_in + e);
-                            const f2 zp = ld2(h.zp1_in + e), zs = ld2(h.zs_in + e);
-                            d2 xv = xc[hf];
-                            xv[0] = x_update(x_update(xv[0], zpd[0], zsd[0], alpha_d, om_d), zp[0], zs[0], alpha, om);
-                            xv[1] = x_update(x_update(xv[1], zpd[1], zsd[1], alpha_d, om_d), zp[1], zs[1], alpha, om);
-                            __builtin_nontemporal_store(xv, xc + hf);
-                        }
-                    }
-                    st4(r + c, rv);
-                }
+                if (mine) st4(r + c, rv);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -389,12 +361,20 @@ int fused_half_grid(const dalm_ctx* ctx, int* ni) {
     return tjf * ((d.nx + n - 1) / n);
 }
 
-int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out, float* zp_out,
-                  const float* zp1, const float* zs1, const float* zs2) {
+int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_out, float* q_out, float* zp_out) {
     int ni;
     const int grid = fused_half_grid(ctx, &ni);
-    const HalfArgs h{ctx->fr, ctx->fs, ctx->ft, zs1, zp1, zs2, ctx->p, p_in, q_in, p_out, zp_out, q_out, ctx->fr0,
-                     ctx->partials};
+    HalfArgs h{};
+    h.r = ctx->fr;
+    h.s_in = ctx->fs;
+    h.t_in = ctx->ft;
+    h.vin = p_in;
+    h.qin = q_in;
+    h.vout = p_out;
+    h.zout = zp_out;
+    h.yout = q_out;
+    h.r0 = ctx->fr0;
+    h.partials = ctx->partials;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_half4<0>), dim3(grid), dim3(64 * kRows), 0, ctx->stream, ctx->d, ctx->pcf,
                        ctx->coef(), h, ni, (const SolveState*)ctx->solve);
     return hipGetLastError() == hipSuccess ? DALM_OK : DALM_HIP_ERROR;
@@ -403,8 +383,14 @@ int launch_half_p(dalm_ctx* ctx, const float* p_in, const float* q_in, float* p_
 int launch_half_s(dalm_ctx* ctx, const float* q, float* zs_out) {
     int ni;
     const int grid = fused_half_grid(ctx, &ni);
-    const HalfArgs h{ctx->fr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, q, ctx->fs, zs_out, ctx->ft,
-                     ctx->fr0, ctx->partials};
+    HalfArgs h{};
+    h.r = ctx->fr;
+    h.qin = q;
+    h.vout = ctx->fs;
+    h.zout = zs_out;
+    h.yout = ctx->ft;
+    h.r0 = ctx->fr0;
+    h.partials = ctx->partials;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_half4<1>), dim3(grid), dim3(64 * kRows), 0, ctx->stream, ctx->d, ctx->pcf,
                        ctx->coef(), h, ni, (const SolveState*)ctx->solve);
     return hipGetLastError() == hipSuccess ? DALM_OK : DALM_HIP_ERROR;

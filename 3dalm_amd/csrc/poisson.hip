@@ -376,39 +376,67 @@ __global__ __launch_bounds__(BLOCK) void k_bi_s4(Dims d, PCoef A, const double* 
                      });
 }
 
-// XM 2: x += alpha pp + omega ss and r = s - omega t with the dots of k_bi_xr.  The
-// fused iteration updates x every other iteration (the same two roundings per update
-// as XM 2, in the same order, so x is the same to the bit): XM 0 forms r only and
-// leaves this iteration's x update to the next one, XM 1 applies the previous
-// iteration's (ppd, ssd with its alpha_prev, omega_prev) and then this one's.
+// z_p and z_s of kXGroup consecutive iterations, oldest first
+struct PRing {
+    const double* pp[kXGroup];
+    const double* ss[kXGroup];
+};
+
+// XM 2: x += alpha pp + omega ss and r = s - omega t with the dots of k_bi_xr (pp, ss
+// in the ring's last place).  The fused iteration updates x once per kXGroup
+// iterations (the same two roundings per update as XM 2, in the same order, so x is
+// the same to the bit): XM 0 forms r only and leaves this iteration's x update to a
+// later one; XM 1 applies the kXGroup - 1 deferred ones (the ring's z with alpha_h,
+// omega_h) and then this one's.
 template <int XM>
-__global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x, const double* __restrict__ pp,
-                                                  const double* __restrict__ ss, const double* __restrict__ s,
-                                                  const double* __restrict__ t_, double* __restrict__ r,
-                                                  const double* __restrict__ r0, double* partials,
-                                                  const double* __restrict__ ppd, const double* __restrict__ ssd,
-                                                  SolveState* st) {
+__global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x, PRing z,
+                                                  const double* __restrict__ s, const double* __restrict__ t_,
+                                                  double* __restrict__ r, const double* __restrict__ r0,
+                                                  double* partials, SolveState* st) {
     if (st->done) return;
     const double alpha = st->alpha, om = st->omega;
-    const double alpha_d = st->alpha_prev, om_d = st->omega_prev;
     const Col4 cl = col_unit4(d);
     double v[2] = {0.0, 0.0};
-    if (cl.ok) {
+    if (cl.ok && XM == 1) {
+        // every plane loads the z of kXGroup iterations: no prefetch of the next plane
+        // (its registers would halve the wavefronts per SIMD); the owned planes in order,
+        // as the walk below visits them
+        int c = d.at(cl.own0, cl.j, cl.k0);
+        for (int i = cl.own0; i < cl.i1; ++i, c += d.si) {
+            d4k xv = ld4(x, c);
+#pragma unroll
+            for (int q = 0; q < kXGroup; ++q) {
+                const d4k a = ld4(z.pp[q], c), b = ld4(z.ss[q], c);
+                const double aq = q + 1 < kXGroup ? st->alpha_h[q] : alpha;
+                const double oq = q + 1 < kXGroup ? st->omega_h[q] : om;
+#pragma unroll
+                for (int m = 0; m < kVecK; ++m) xv[m] = xv[m] + aq * a[m] + oq * b[m];
+            }
+            const d4k sv = ld4(s, c), tv = ld4(t_, c), r0v = ld4(r0, c);
+            d4k rv;
+#pragma unroll
+            for (int m = 0; m < kVecK; ++m) {
+                rv[m] = sv[m] - om * tv[m];
+                v[0] += rv[m] * rv[m];
+                v[1] += r0v[m] * rv[m];
+            }
+            __builtin_nontemporal_store(xv, (d4k*)(x + c));
+            __builtin_nontemporal_store(rv, (d4k*)(r + c));
+        }
+    } else if (cl.ok) {
+        const double* __restrict__ pp = z.pp[kXGroup - 1];
+        const double* __restrict__ ss = z.ss[kXGroup - 1];
         struct XR {
-            d4k x, pp, ss, ppd, ssd, s, t, r0;
+            d4k x, pp, ss, s, t, r0;
         };
         const d4k z4 = {0.0, 0.0, 0.0, 0.0};
         walk_points4<XR>(d, cl,
                          [&](int c) {
-                             XR a{z4, z4, z4, z4, z4, ld4(s, c), ld4(t_, c), ld4(r0, c)};
-                             if (XM != 0) {
+                             XR a{z4, z4, z4, ld4(s, c), ld4(t_, c), ld4(r0, c)};
+                             if (XM == 2) {
                                  a.x = ld4(x, c);
                                  a.pp = ld4(pp, c);
                                  a.ss = ld4(ss, c);
-                             }
-                             if (XM == 1) {
-                                 a.ppd = ld4(ppd, c);
-                                 a.ssd = ld4(ssd, c);
                              }
                              return a;
                          },
@@ -416,14 +444,12 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
                              d4k xv, rv;
 #pragma unroll
                              for (int m = 0; m < kVecK; ++m) {
-                                 double xm = a.x[m];
-                                 if (XM == 1) xm = xm + alpha_d * a.ppd[m] + om_d * a.ssd[m];
-                                 xv[m] = xm + alpha * a.pp[m] + om * a.ss[m];
+                                 xv[m] = a.x[m] + alpha * a.pp[m] + om * a.ss[m];
                                  rv[m] = a.s[m] - om * a.t[m];
                                  v[0] += rv[m] * rv[m];
                                  v[1] += a.r0[m] * rv[m];
                              }
-                             if (XM != 0) __builtin_nontemporal_store(xv, (d4k*)(x + c));
+                             if (XM == 2) __builtin_nontemporal_store(xv, (d4k*)(x + c));
                              __builtin_nontemporal_store(rv, (d4k*)(r + c));
                          });
     }
@@ -432,15 +458,23 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
     store_partials<2>(pv, partials);
 }
 
-// x += alpha pp + omega ss on the interior: a deferred update at the end of a solve
-__global__ __launch_bounds__(BLOCK) void k_x_update(Dims d, double* __restrict__ x, const double* __restrict__ pp,
-                                                    const double* __restrict__ ss, double alpha, double om) {
+// The updates deferred at the end of a solve: x += alpha pp + omega ss for the ring's
+// places [first, kXGroup), in turn, on the interior
+struct XFlush64 {
+    double alpha[kXGroup], omega[kXGroup];
+    int first;
+};
+__global__ __launch_bounds__(BLOCK) void k_x_flush(Dims d, double* __restrict__ x, PRing z, XFlush64 f) {
     const TileRange tr = block_tiles(d);
     for (int t = tr.begin; t < tr.end; ++t) {
         const Cell cc = tile_cell(d, t);
         if (!cc.inside) continue;
         const int c = d.at(cc.i, cc.j, cc.k);
-        x[c] = x[c] + alpha * pp[c] + om * ss[c];
+        double v = x[c];
+#pragma unroll
+        for (int q = 0; q < kXGroup; ++q)
+            if (q >= f.first) v = v + f.alpha[q] * z.pp[q][c] + f.omega[q] * z.ss[q][c];
+        x[c] = v;
     }
 }
 
@@ -786,10 +820,23 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     if (fused) {
         if (const int rz = ensure_fused_z64(ctx)) return rz;
     }
-    // the fused iteration k writes z_p / z_s into the pair of its parity and updates x
-    // at odd k only, with the updates of k - 1 and k
-    double* const PP[2] = {ctx->pp, ctx->ppb};
-    double* const SS[2] = {ctx->ss, ctx->ssb};
+    // the fused iteration k writes z_p / z_s into buffer pair k mod kXGroup and updates
+    // x at k = kXGroup - 1 mod kXGroup only, with the updates of the last kXGroup
+    double* PP[kXGroup] = {ctx->pp};
+    double* SS[kXGroup] = {ctx->ss};
+    for (int q = 1; q < kXGroup; ++q) {
+        PP[q] = ctx->ppx[q - 1];
+        SS[q] = ctx->ssx[q - 1];
+    }
+    // z of iterations base - kXGroup + 1 + q, q = 0 .. kXGroup - 1
+    auto pring = [&](long long base) {
+        PRing z;
+        for (int q = 0; q < kXGroup; ++q) {
+            z.pp[q] = PP[(base + 1 + q) % kXGroup];
+            z.ss[q] = SS[(base + 1 + q) % kXGroup];
+        }
+        return z;
+    };
     const int ftiles = (d.ny + kFOwn - 1) / kFOwn;
     const int fni = fused_planes(ctx, ftiles);
     const int fgrid = ftiles * ((d.nx + fni - 1) / fni);
@@ -839,34 +886,32 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
                 probe_begin(ctx, DALM_K_HALF_P);
                 hipLaunchKernelGGL(k_half_pc4<0>, dim3(fgrid), dim3(64 * kFRows), 0, ctx->stream, d, A,
                                    (const double*)ctx->r, (const double*)P[k & 1], (const double*)Q[k & 1],
-                                   P[(k + 1) & 1], PP[k & 1], fni, (const SolveState*)ctx->solve);
+                                   P[(k + 1) & 1], PP[k % kXGroup], fni, (const SolveState*)ctx->solve);
                 probe_end(ctx, DALM_K_HALF_P);
                 probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(k_bi_spmv4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, PP[k & 1],
+                hipLaunchKernelGGL(k_bi_spmv4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, PP[k % kXGroup],
                                    Q[(k + 1) & 1], ctx->r0, ctx->partials, ctx->solve);
                 probe_end(ctx, DALM_K_SPMV);
                 if (int rc1 = launch_finalize(ctx, FIN_ALPHA, grid4, ctx->solve, nullptr)) return rc1;
                 probe_begin(ctx, DALM_K_HALF_S);
                 hipLaunchKernelGGL(k_half_pc4<1>, dim3(fgrid), dim3(64 * kFRows), 0, ctx->stream, d, A,
                                    (const double*)ctx->r, (const double*)nullptr, (const double*)Q[(k + 1) & 1],
-                                   ctx->s, SS[k & 1], fni, (const SolveState*)ctx->solve);
+                                   ctx->s, SS[k % kXGroup], fni, (const SolveState*)ctx->solve);
                 probe_end(ctx, DALM_K_HALF_S);
                 probe_begin(ctx, DALM_K_SPMV);
-                hipLaunchKernelGGL(k_bi_spmv4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, SS[k & 1], ctx->t,
+                hipLaunchKernelGGL(k_bi_spmv4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, A, SS[k % kXGroup], ctx->t,
                                    ctx->s, ctx->partials, ctx->solve);
                 probe_end(ctx, DALM_K_SPMV);
                 if (int rc2 = launch_finalize(ctx, FIN_OMEGA, grid4, ctx->solve, nullptr)) return rc2;
                 probe_begin(ctx, DALM_K_UPDATE);
-                if (k & 1)
-                    hipLaunchKernelGGL(k_bi_xr4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x,
-                                       (const double*)PP[k & 1], (const double*)SS[k & 1], (const double*)ctx->s,
-                                       (const double*)ctx->t, ctx->r, (const double*)ctx->r0, ctx->partials,
-                                       (const double*)PP[(k + 1) & 1], (const double*)SS[(k + 1) & 1], ctx->solve);
+                if (k % kXGroup == kXGroup - 1)
+                    hipLaunchKernelGGL(k_bi_xr4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, pring(k),
+                                       (const double*)ctx->s, (const double*)ctx->t, ctx->r, (const double*)ctx->r0,
+                                       ctx->partials, ctx->solve);
                 else
-                    hipLaunchKernelGGL(k_bi_xr4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x,
-                                       (const double*)nullptr, (const double*)nullptr, (const double*)ctx->s,
-                                       (const double*)ctx->t, ctx->r, (const double*)ctx->r0, ctx->partials,
-                                       (const double*)nullptr, (const double*)nullptr, ctx->solve);
+                    hipLaunchKernelGGL(k_bi_xr4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, pring(k),
+                                       (const double*)ctx->s, (const double*)ctx->t, ctx->r, (const double*)ctx->r0,
+                                       ctx->partials, ctx->solve);
                 probe_end(ctx, DALM_K_UPDATE);
                 if (int rc3 = launch_finalize(ctx, FIN_BI_XR, grid4, ctx->solve, nullptr)) return rc3;
                 continue;
@@ -908,11 +953,14 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
             probe_end(ctx, DALM_K_SPMV);
             if (int rc2 = launch_finalize(ctx, FIN_OMEGA, nb, ctx->solve, nullptr)) return rc2;
             probe_begin(ctx, DALM_K_UPDATE);
-            if (vec4)
-                hipLaunchKernelGGL(k_bi_xr4<2>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, (const double*)ppz,
-                                   (const double*)ssz, (const double*)ctx->s, (const double*)ctx->t, ctx->r,
-                                   (const double*)ctx->r0, ctx->partials, (const double*)nullptr,
-                                   (const double*)nullptr, ctx->solve);
+            if (vec4) {
+                PRing z{};
+                z.pp[kXGroup - 1] = ppz;
+                z.ss[kXGroup - 1] = ssz;
+                hipLaunchKernelGGL(k_bi_xr4<2>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, z,
+                                   (const double*)ctx->s, (const double*)ctx->t, ctx->r, (const double*)ctx->r0,
+                                   ctx->partials, ctx->solve);
+            }
             else
                 hipLaunchKernelGGL(k_bi_xr, dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream, d, x, ppz, ssz,
                                    ctx->s, ctx->t, ctx->r, ctx->r0, ctx->partials, ctx->counter, ctx->solve);
@@ -926,12 +974,18 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     }
     probe_solve_end(ctx, ctx->h_solve->it);
     const long long done_it = ctx->h_solve->it - it_start;
-    if (fused && done_it > 0 && !((done_it - 1) & 1)) {
-        // the last iteration (even k) deferred its x update to an iteration that did not run
-        const long long last = done_it - 1;
-        hipLaunchKernelGGL(k_x_update, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x,
-                           (const double*)PP[last & 1], (const double*)SS[last & 1], ctx->h_solve->alpha,
-                           ctx->h_solve->omega);
+    const long long last = done_it - 1;
+    if (fused && done_it > 0 && last % kXGroup != kXGroup - 1) {
+        // the last iterations deferred their x updates to one that did not run: ring
+        // places [first, kXGroup) hold them, with alpha and omega in alpha_h, omega_h
+        // (the last place is iteration `last`, whose alpha FIN_BI_XR shifted in last)
+        XFlush64 f{};
+        f.first = kXGroup - 1 - (int)(last % kXGroup);
+        for (int q = f.first; q < kXGroup; ++q) {
+            f.alpha[q] = ctx->h_solve->alpha_h[q - 1];
+            f.omega[q] = ctx->h_solve->omega_h[q - 1];
+        }
+        hipLaunchKernelGGL(k_x_flush, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, pring(last), f);
         LAUNCH_CHECK();
     }
     if (fused && (done_it & 1)) {

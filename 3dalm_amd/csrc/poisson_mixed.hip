@@ -396,44 +396,66 @@ __global__ void kg_state_reset(SolveState* st, double tol, long long max_it, dou
     }
 }
 
-// The fused iteration's outstanding pressure updates on the interior, ahead of a
-// residual replacement: the deferred one (xdef: alpha_d, omega_d with zpd, zsd), then
-// the pending one (alpha, omega with zp, zs); only when a replacement is due
-// (FIN_MX_RU then clears both flags).
-__global__ __launch_bounds__(BLOCK) void kg_apply_pending(Dims d, double* __restrict__ x,
-                                                          const float* __restrict__ zp, const float* __restrict__ zs,
-                                                          const float* __restrict__ zpd, const float* __restrict__ zsd,
-                                                          const SolveState* st) {
-    if (st->done || !st->ru || !(st->pending || st->xdef)) return;
-    const bool pend = st->pending, def = st->xdef;
-    const double alpha = st->alpha, om = st->omega, alpha_d = st->alpha_d, om_d = st->omega_d;
-    const TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        const Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        const int c = d.at(cc.i, cc.j, cc.k);
-        double v = x[c];
-        if (def) v = x_update(v, zpd[c], zsd[c], alpha_d, om_d);
-        if (pend) v = x_update(v, zp[c], zs[c], alpha, om);
-        x[c] = v;
+// z_p and z_s of kXGroupF consecutive iterations of the fused iteration, oldest first
+struct ZRing {
+    const float* zp[kXGroupF];
+    const float* zs[kXGroupF];
+};
+
+// kg_flush_pending4: the same with the values the host read from the state, ring
+// positions [first, kXGroupF) with their alpha and omega, at the end of a pass
+// (out == x) or into the best pressure's copy of x (out != x)
+struct XFlush {
+    double alpha[kXGroupF], omega[kXGroupF];
+    int first;
+};
+
+// The fused iteration's outstanding pressure updates after iteration k (z: iterations
+// k - kXGroupF + 1 .. k), the xdef of them in turn (alpha_d, omega_d, oldest first):
+// when kXGroupF have gathered or a residual replacement is due, otherwise it returns
+// at once (FIN_MX_RU then clears the count).  Rows of 256 cells, four cells per lane
+// with 16-byte accesses: x and the pairs of z stream through once.
+__global__ __launch_bounds__(BLOCK) void kg_apply_pending4(Dims d, double* __restrict__ x, ZRing z,
+                                                           const SolveState* st) {
+    if (st->done || !(st->ru || st->xdef >= kXGroupF) || st->xdef <= 0) return;
+    const int first = kXGroupF - st->xdef;
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    int c = d.at(cl.own0, cl.j, cl.k0);
+    for (int i = cl.own0; i < cl.i1; ++i, c += d.si) {
+        d2k xa = *(const d2k*)(x + c), xb = *(const d2k*)(x + c + 2);
+#pragma unroll
+        for (int q = 0; q < kXGroupF; ++q) {
+            if (q < first) continue;
+            const f4k a = *(const f4k*)(z.zp[q] + c), b = *(const f4k*)(z.zs[q] + c);
+            const double aq = st->alpha_d[q - first], oq = st->omega_d[q - first];
+            xa[0] = x_update(xa[0], a[0], b[0], aq, oq);
+            xa[1] = x_update(xa[1], a[1], b[1], aq, oq);
+            xb[0] = x_update(xb[0], a[2], b[2], aq, oq);
+            xb[1] = x_update(xb[1], a[3], b[3], aq, oq);
+        }
+        __builtin_nontemporal_store(xa, (d2k*)(x + c));
+        __builtin_nontemporal_store(xb, (d2k*)(x + c + 2));
     }
 }
 
-// The same with the values the host read from the state: at the end of a pass
-// (out == x), or into the best pressure's copy of x (out != x).
-__global__ __launch_bounds__(BLOCK) void kg_flush_pending(Dims d, const double* x, const float* __restrict__ zp,
-                                                          const float* __restrict__ zs, const float* __restrict__ zpd,
-                                                          const float* __restrict__ zsd, double* out, double alpha,
-                                                          double om, double alpha_d, double om_d, int pend, int def) {
-    const TileRange tr = block_tiles(d);
-    for (int t = tr.begin; t < tr.end; ++t) {
-        const Cell cc = tile_cell(d, t);
-        if (!cc.inside) continue;
-        const int c = d.at(cc.i, cc.j, cc.k);
-        double v = x[c];
-        if (def) v = x_update(v, zpd[c], zsd[c], alpha_d, om_d);
-        if (pend) v = x_update(v, zp[c], zs[c], alpha, om);
-        out[c] = v;
+__global__ __launch_bounds__(BLOCK) void kg_flush_pending4(Dims d, const double* x, ZRing z, double* out, XFlush f) {
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    int c = d.at(cl.own0, cl.j, cl.k0);
+    for (int i = cl.own0; i < cl.i1; ++i, c += d.si) {
+        d2k xa = *(const d2k*)(x + c), xb = *(const d2k*)(x + c + 2);
+#pragma unroll
+        for (int q = 0; q < kXGroupF; ++q) {
+            if (q < f.first) continue;
+            const f4k a = *(const f4k*)(z.zp[q] + c), b = *(const f4k*)(z.zs[q] + c);
+            xa[0] = x_update(xa[0], a[0], b[0], f.alpha[q], f.omega[q]);
+            xa[1] = x_update(xa[1], a[1], b[1], f.alpha[q], f.omega[q]);
+            xb[0] = x_update(xb[0], a[2], b[2], f.alpha[q], f.omega[q]);
+            xb[1] = x_update(xb[1], a[3], b[3], f.alpha[q], f.omega[q]);
+        }
+        *(d2k*)(out + c) = xa;
+        *(d2k*)(out + c + 2) = xb;
     }
 }
 
@@ -753,17 +775,34 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
     if (fused) {
         if (const int rz = ensure_fused_z(ctx)) return rz;
     }
-    // the fused iteration k writes z_p / z_s into the pair of its parity
-    float* const ZP[2] = {ctx->fz, ctx->fzb};
-    float* const ZS[2] = {ctx->fz2, ctx->fz2b};
-    // the outstanding x updates after iteration `last` of a pass (pending: last,
-    // deferred: last - 1) applied to x into out
+    // the fused iteration k writes z_p / z_s into buffer pair k mod kXGroupF
+    float* ZP[kXGroupF] = {ctx->fz};
+    float* ZS[kXGroupF] = {ctx->fz2};
+    for (int q = 1; q < kXGroupF; ++q) {
+        ZP[q] = ctx->fzx[q - 1];
+        ZS[q] = ctx->fz2x[q - 1];
+    }
+    // z of iterations base - kXGroupF + 1 + q, q = 0 .. kXGroupF - 1
+    auto ring = [&](long long base) {
+        ZRing z;
+        for (int q = 0; q < kXGroupF; ++q) {
+            z.zp[q] = ZP[(base + 1 + q) % kXGroupF];
+            z.zs[q] = ZS[(base + 1 + q) % kXGroupF];
+        }
+        return z;
+    };
+    // the outstanding x updates after iteration `last` of a pass (the last xdef
+    // iterations') applied to x into out
     auto flush = [&](const SolveState& h, double* out) {
         const long long last = h.it - 1;
-        hipLaunchKernelGGL(kg_flush_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, (const double*)ctx->p,
-                           (const float*)ZP[last & 1], (const float*)ZS[last & 1], (const float*)ZP[(last + 1) & 1],
-                           (const float*)ZS[(last + 1) & 1], out, h.alpha, h.omega, h.alpha_d, h.omega_d, h.pending,
-                           h.xdef);
+        XFlush f{};
+        f.first = kXGroupF - h.xdef;
+        for (int q = f.first; q < kXGroupF; ++q) {
+            f.alpha[q] = h.alpha_d[q - f.first];
+            f.omega[q] = h.omega_d[q - f.first];
+        }
+        hipLaunchKernelGGL(kg_flush_pending4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, (const double*)ctx->p,
+                           ring(last), out, f);
     };
     // FP32 preconditioner application on v32 into z0 (spare: the other Jacobi buffer)
     auto precondition = [&](float* z0, float* spare, const float* v32) -> float* {
@@ -856,28 +895,27 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
                 probe_iteration(ctx, launched + c);
                 if (fused) {
                     // iteration k reads p, q from the pair's k-parity buffer and writes the
-                    // other, and its z_p, z_s into the k-parity pair of ZP, ZS; the p-half
-                    // forms the previous iteration's r and applies the outstanding x updates
-                    // every other iteration, the s-half's dots give omega, |r| and <r0, r>
-                    // (FIN_MX_OMEGA)
+                    // other, and its z_p, z_s into pair k mod kXGroupF of ZP, ZS; the p-half
+                    // forms the previous iteration's r, the s-half's dots give omega, |r|
+                    // and <r0, r> (FIN_MX_OMEGA), and kg_apply_pending4 applies the
+                    // outstanding x updates every kXGroupF-th iteration or ahead of a
+                    // replacement
                     const long long k = launched + c;
                     float* P[2] = {ctx->fp, ctx->ftmp};
                     float* Q[2] = {ctx->fq, ctx->ftmp2};
                     probe_begin(ctx, DALM_K_HALF_P_MIXED);
-                    if ((rc = launch_half_p(ctx, P[k & 1], Q[k & 1], P[(k + 1) & 1], Q[(k + 1) & 1], ZP[k & 1],
-                                            ZP[(k + 1) & 1], ZS[(k + 1) & 1], ZS[k & 1])))
+                    if ((rc = launch_half_p(ctx, P[k & 1], Q[k & 1], P[(k + 1) & 1], Q[(k + 1) & 1],
+                                            ZP[k % kXGroupF])))
                         return dalm_fail_msg(ctx, rc, "fused p half-iteration launch");
                     probe_end(ctx, DALM_K_HALF_P_MIXED);
                     if ((rc = launch_finalize(ctx, FIN_ALPHA, fgrid, st, nullptr))) return rc;
                     probe_begin(ctx, DALM_K_HALF_S_MIXED);
-                    if ((rc = launch_half_s(ctx, Q[(k + 1) & 1], ZS[k & 1])))
+                    if ((rc = launch_half_s(ctx, Q[(k + 1) & 1], ZS[k % kXGroupF])))
                         return dalm_fail_msg(ctx, rc, "fused s half-iteration launch");
                     probe_end(ctx, DALM_K_HALF_S_MIXED);
                     if ((rc = launch_finalize(ctx, FIN_MX_OMEGA, fgrid, st, nullptr))) return rc;
-                    hipLaunchKernelGGL(kg_apply_pending, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, ctx->p,
-                                       (const float*)ZP[k & 1], (const float*)ZS[k & 1],
-                                       (const float*)ZP[(k + 1) & 1], (const float*)ZS[(k + 1) & 1],
-                                       (const SolveState*)st);
+                    hipLaunchKernelGGL(kg_apply_pending4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, ctx->p,
+                                       ring(k), (const SolveState*)st);
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(kg_resid<false>), dim3(ctx->sgrid), dim3(BLOCK), 0, ctx->stream,
                                        d, A, ctx->p, ctx->b, ctx->fr, ctx->fr0, ctx->partials, (const SolveState*)st);
                     if ((rc = launch_finalize(ctx, FIN_MX_RU, ctx->sgrid, st, nullptr))) return rc;
@@ -945,7 +983,7 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
             if (e < 0.5 * best_saved) {
                 HIP_TRY(hipMemcpyAsync(ctx->pbest, ctx->p, sizeof(double) * ctx->L, hipMemcpyDeviceToDevice,
                                        ctx->stream));
-                if (fused && (h.pending || h.xdef))  // the pressure of the estimate e includes them
+                if (fused && h.xdef > 0)  // the pressure of the estimate e includes them
                     flush(h, ctx->pbest);
                 best_saved = e;
             }
@@ -963,8 +1001,8 @@ int run_poisson_mixed(dalm_ctx* ctx, int pc_kind, double pc_omega, int pc_it, lo
         }
         const SolveState& h = ctx->h_solve[0];
         probe_solve_end(ctx, h.it);
-        if (fused && (h.pending || h.xdef)) {
-            // the pass's last x updates (the next p-halves would have applied them)
+        if (fused && h.xdef > 0) {
+            // the pass's outstanding x updates
             flush(h, ctx->p);
             LAUNCH_CHECK();
         }

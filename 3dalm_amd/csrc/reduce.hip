@@ -73,20 +73,8 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         break;
     case FIN_ALPHA:
         if (!st->done) {
-            if (st->pending) {
-                // the fused p-half has formed the previous iteration's r; its x update
-                // was applied together with a deferred one, or is now deferred itself
-                // (with the alpha and omega it needs)
-                if (st->xdef) {
-                    st->xdef = 0;
-                } else {
-                    st->xdef = 1;
-                    st->alpha_d = st->alpha;
-                    st->omega_d = st->omega;
-                }
-            }
             st->alpha = st->rho / s[0];
-            st->pending = 0;
+            st->pending = 0;  // a fused p-half has formed the previous iteration's r
         }
         break;
     case FIN_OMEGA:
@@ -94,10 +82,14 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         break;
     case FIN_BI_XR: {
         if (st->done) break;
-        // the fused iteration's x update may be deferred to the next iteration's
-        // update kernel, which then needs this iteration's alpha and omega
-        st->alpha_prev = st->alpha;
-        st->omega_prev = st->omega;
+        // the fused iteration's x update may be deferred to a later iteration's update
+        // kernel, which then needs this iteration's alpha and omega
+        for (int q = 0; q + 1 < kXGroup - 1; ++q) {
+            st->alpha_h[q] = st->alpha_h[q + 1];
+            st->omega_h[q] = st->omega_h[q + 1];
+        }
+        st->alpha_h[kXGroup - 2] = st->alpha;
+        st->omega_h[kXGroup - 2] = st->omega;
         double err = sqrt(s[0]) / st->root_n;
         long long it = st->it + 1;
         st->it = it;
@@ -186,6 +178,17 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         const double om = s[0] / s[1];
         st->omega = om;
         st->pending = 1;
+        // this iteration's pressure update is outstanding (kg_apply_pending4 applies the
+        // outstanding ones when kXGroupF have gathered or a replacement is due, and
+        // FIN_MX_RU clears the count, so it is below kXGroupF here)
+        if (st->xdef >= kXGroupF) {
+            st->done = 1;
+            st->status = DALM_NONFINITE;
+            break;
+        }
+        st->alpha_d[st->xdef] = st->alpha;
+        st->omega_d[st->xdef] = om;
+        st->xdef += 1;
         double r2 = s[4] - 2.0 * om * s[0] + om * om * s[1];
         if (r2 < 0.0) r2 = 0.0;
         const double rn = sqrt(r2);
@@ -216,10 +219,11 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
     case FIN_MX_RU: {
         // precision form, residual replacement: s[0] = <r0, r>, s[1] = |b - A x|^2
         // in FP64.  The solve ends only on the true residual.
-        if (st->done || !st->ru) break;
+        if (st->done) break;
+        if (st->ru || st->xdef >= kXGroupF) st->xdef = 0;  // kg_apply_pending4 applied them
+        if (!st->ru) break;
         st->ru = 0;
-        st->pending = 0;  // the replacement applied the pending x updates first
-        st->xdef = 0;
+        st->pending = 0;  // the replacement replaced r
         st->n_ru += 1;
         double rn = sqrt(s[1]);
         double err = rn / st->root_n;

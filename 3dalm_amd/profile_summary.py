@@ -48,19 +48,19 @@ ALGO_BYTES = {
     "k_calc_q<double>": 32, "k_calc_q<float>": 20,
     "kf_sor": 10, "ks_resid": 24, "ks_add": 20,
     "k_jacobi4": 24, "k_bi_spmv4<0>": 24, "k_bi_spmv4<1>": 24, "k_bi_p4": 40, "k_bi_s4": 32,
-    # the x/r update: r only (x deferred), x of two iterations and r, x and r
-    "k_bi_xr4<0>": 32, "k_bi_xr4<1>": 80, "k_bi_xr4<2>": 64,
+    # the x/r update: r only (x deferred), x of four iterations and r, x and r
+    "k_bi_xr4<0>": 32, "k_bi_xr4<1>": 112, "k_bi_xr4<2>": 64,
     "kf_jacobi4": 12, "kg_spmv4<0>": 12, "kg_spmv4<1>": 12, "kg_p4": 20, "kg_s4": 16, "kg_xr4": 40,
     # fused half-iterations (rows of 256 cells, Jacobi x3)
-    # (kg_half4<0>: 36 and 68 B in turn, with the x updates of two iterations every
-    # other launch: 52 on average)
-    "kg_half4<0>": 52, "kg_half4<1>": 24, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
+    "kg_half4<0>": 36, "kg_half4<1>": 24, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,
+    # the fused iteration's x updates of eight iterations (working launches)
+    "kg_apply_pending4": 80,
     # the predictor's column walk with the LDS-staged plane
     "k_predictor_star": 104, "km_predictor_star": 64, "k_post_star": 64, "km_post_star": 48,
 }
-# kernels whose launches alternate between two algorithmic byte counts per cell (the
-# fused iteration updates x every other launch): per-launch bytes of each kind
-ALTERNATING = {"kg_half4<0>": (36, 68)}
+# kernels whose launches alternate between two algorithmic byte counts per cell:
+# per-launch bytes of each kind (none at present)
+ALTERNATING = {}
 # kernels that sweep every padded cell: algorithmic bytes per padded cell
 # (needs --padded-cells)
 ALGO_BYTES_PADDED = {"k_time_average<double>": 72, "k_time_average<float>": 60,

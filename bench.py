@@ -29,11 +29,10 @@ the precision form's U, p and div from the FP64 drop-in's after the last timed s
 with the turbine-power difference.
 
 roofline: the dominant kernel of the precision form on rows of 256 cells, its fused
-p-half-iteration kg_half4<0> (the previous iteration's r update, every other launch the
-x updates of two iterations, the p update, three FP32 Jacobi sweeps, FP64 SpMV and dot
-product in one pass; roofline_fp64: the FP64 drop-in's fused update and two sweeps,
-k_half_pc4<0>): algorithmic bytes per launch (52 on average / 40 B per interior cell
-x N) over
+p-half-iteration kg_half4<0> (the previous iteration's r update, the p update, three
+FP32 Jacobi sweeps, FP64 SpMV and dot product in one pass; roofline_fp64: the FP64
+drop-in's fused update and two sweeps, k_half_pc4<0>): algorithmic bytes per launch
+(36 / 40 B per interior cell x N) over
 the mean launch duration, timed with HIP events on the library's stream over the
 timed steps.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 runs of this workload (profiles/hbm_traffic_alm_512x256x256.json), each counter
@@ -96,10 +95,10 @@ PROBE_KERNEL = {"jacobi": "k_jacobi4", "jacobi32": "kf_jacobi4", "spmv": "k_bi_s
 PROBE_BYTES = {
     "jacobi": 24,     # read z_in, v; write z_out
     "spmv": 24,       # read z, a1 (r0 or s); write y
-    # read s, t, r0, write r; every other launch also reads pp, ss of two iterations
-    # and reads and writes x (k_bi_xr4<0> 32 B and <1> 80 B in turn; 64 B every launch
-    # on rows that do not take the fused iteration)
-    "update": 56,
+    # read s, t, r0, write r; every fourth launch also reads pp, ss of four iterations
+    # and reads and writes x (k_bi_xr4<0> 32 B three times, then <1> 112 B; 64 B every
+    # launch on rows that do not take the fused iteration)
+    "update": 52,
     "pupdate": 40,    # read r, p, q; write p, z
     "supdate": 32,    # read r, q; write s, z
     "predictor": 104,  # read Uold(3), JU(3), nut, f(3); write U(3)
@@ -113,10 +112,9 @@ PROBE_BYTES = {
     "supdate_mixed": 16,  # read r, q; write s, z (FP32)
     # fused half-iterations (rows of 256 cells, Jacobi x3): every intermediate level
     # stays on chip
-    # p-half: read s, t, p, q, r0 (FP32); write r, p, z3, q (FP32); every other launch
-    # also reads z_p, z_s of two iterations (FP32) and reads and writes x (FP64): 36
-    # and 68 B in turn, 52 on average
-    "half_p_mixed": 52,
+    # p-half: read s, t, p, q, r0; write r, p, z3, q (FP32; the pressure updates are
+    # applied by kg_apply_pending4, eight iterations' at a time)
+    "half_p_mixed": 36,
     "half_s_mixed": 24,  # read r, q, r0; write s, z3, t (FP32)
     "half_p": 40,        # read r, p, q; write p, z3 (FP64)
     "half_s": 32,        # read r, q; write s, z3 (FP64)
@@ -127,11 +125,11 @@ PROBE_BYTES = {
 STEP_BYTES = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 280, "per_solve": 32},
               "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 148, "per_solve": 28}}
 # with the fused half-iterations (rows of exactly 256 cells, Jacobi x3): FP64 40 + 24
-# (SpMV) + 32 + 24 + 56 (r update, and the x updates of two iterations every other
-# time); precision form 52 (p-half with the previous iteration's r update, and the x
-# updates of two iterations every other time) + 24
-STEP_BYTES_FUSED = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 176, "per_solve": 32},
-                    "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 76, "per_solve": 28}}
+# (SpMV) + 32 + 24 + 52 (r update, and the x updates of four iterations every fourth
+# time); precision form 36 (p-half with the previous iteration's r update) + 24 + 10
+# (the x updates of eight iterations every eighth time, 80 B)
+STEP_BYTES_FUSED = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 172, "per_solve": 32},
+                    "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 70, "per_solve": 28}}
 PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
              "supdate": 8, "jacobi32": 9, "spmv_mixed": 10, "update_mixed": 11, "pupdate_mixed": 12,
              "supdate_mixed": 13, "half_p_mixed": 16, "half_s_mixed": 17, "half_p": 18, "half_s": 19}

@@ -149,7 +149,7 @@ def test_bicgstab(mesh, ok, pc):
     assert rel_l2(c.download("p")[m], x[m]) <= 1e-10
 
 
-@pytest.mark.parametrize("iters", [2, 3])
+@pytest.mark.parametrize("iters", [2, 3, 4, 5])
 @pytest.mark.parametrize("shape", [(5, 6, 256), (40, 20, 256), (6, 5, 384)])
 def test_bicgstab_rows256_first_iterations(ok, shape, iters):
     """Rows of 256 interior cells run the FP64 BiCGStab kernels (p and s updates
@@ -159,9 +159,9 @@ def test_bicgstab_rows256_first_iterations(ok, shape, iters):
     (k_half_pc4: 40x20x256 has two plane chunks, the second overlapping the first,
     and a last j tile of 4 of its 8 rows).  Rows of 384 cells run the four-cell
     kernels unfused, with a second, half-empty tile of lanes per row.  The fused
-    iteration updates x at odd iterations with the updates of two: after 2
-    iterations both are applied in the iteration, after 3 the last one at the end of
-    the solve.  Over the first iterations the Krylov
+    iteration updates x once per four iterations with the updates of all four: after
+    4 iterations in the iteration, after 2, 3 and 5 at the end of the solve.  Over the
+    first iterations the Krylov
     path has not yet amplified the dot products' summation order (the second
     iteration reads p and q from the partner buffers), so the iterate
     must agree with the oracle's to rounding (every stored vector is bit-identical
@@ -182,9 +182,14 @@ def test_bicgstab_rows256_first_iterations(ok, shape, iters):
     git, gerr, st = c.run_bicgstab(hip.PC_JACOBI, 1.0, 3, iters, 1e-30)
     print(f"bicgstab rows256, {iters} iterations: err {gerr:.6e} vs {err.value:.6e}", flush=True)
     assert git == it.value == iters and st == hip.NONCONVERGED
-    assert abs(gerr - err.value) <= 1e-12 * err.value
+    # the summation order's rounding grows with the iterations (allowed 10x per
+    # iteration past the second); a pressure update left out would be off by far more
+    grow = 10.0 ** (iters - 2)
+    assert abs(gerr - err.value) <= 1e-12 * grow * err.value
     m = interior_mask(mesh)
-    assert rel_l2(c.download("p")[m], x[m]) <= 1e-13
+    dp = rel_l2(c.download("p")[m], x[m])
+    print(f"  pressure against the oracle's: {dp:.3e}", flush=True)
+    assert dp <= 1e-13 * grow
     c.close()
 
 

@@ -127,16 +127,17 @@ def test_mixed_sor_method(interior, tol):
         s.close()
 
 
-@pytest.mark.parametrize("builder,cap", [(lambda: cases.case_c2(48, 24, 24, steps=3, ap_per_blade=8), 40),
-                                         (lambda: _rows256(), 40), (lambda: _rows256(), 41)])
+@pytest.mark.parametrize("builder,cap", [(lambda: cases.case_c2(48, 24, 24, steps=3, ap_per_blade=8), 40)] +
+                         [(lambda: _rows256(), cap) for cap in (40, 41, 42, 43)])
 def test_mixed_passes_and_fp64_hand_over(builder, cap):
     """The precision form's recovery paths: with every FP32 pass cut short after
     40 iterations (dalm_set_fp32_pass_cap), a pass that ends far
     above the tolerance is restarted from the true residual and one that ends
     within 30x of it is handed over to FP64; every solve must still end on the
     true residual with the step matching the FP64 oracle.  Also with rows of 256
-    cells (the fused half-iterations, whose passes then end with one or two pressure
-    updates outstanding: caps of 40 and 41 iterations)."""
+    cells (the fused half-iterations, whose pressure updates are applied eight
+    iterations' at a time: caps of 40 to 43 iterations end passes with different
+    numbers of them outstanding)."""
     case = builder()
     case.steps = 3
     case.tol, case.max_it = 1e-10, 4000
