@@ -643,13 +643,19 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
         b.kf = sh.lv[cp][lev][rr][o + 4];
         return b;
     };
-    // one Jacobi sweep at plane ip (k_jacobi4's arithmetic)
-    auto sweep = [&](int ip, const Nb& nb, const d4k& xb, const d4k& xa, const d4k& xn, const d4k& rhs) {
+    // one Jacobi sweep at plane ip (k_jacobi4's arithmetic).  The s-half takes the plane's
+    // diagonal acv, computed by level 0 when it reached the plane and kept in registers
+    // since (the same value, divided by max|a_c| once per cell instead of three times:
+    // 236 against 258 us at 512x256x256); the p-half recomputes it, which measured
+    // faster there (290 against 307 us with the carried diagonal)
+    constexpr bool carry = WHICH == 1;
+    auto sweep = [&](int ip, const Nb& nb, const d4k& xb, const d4k& xa, const d4k& xn, const d4k& rhs,
+                     const d4k& acv) {
         const double acj = (A.acx[ip] - anu) - asu, ae = A.ae[ip], aw = A.aw[ip];
         d4k out;
 #pragma unroll
         for (int m = 0; m < kVecK; ++m) {
-            const double ac = ((acj - sh.atu[lk + m]) - sh.abu[lk + m]) / A.max_diag;
+            const double ac = carry ? acv[m] : ((acj - sh.atu[lk + m]) - sh.abu[lk + m]) / A.max_diag;
             const double tq = m + 1 < kVecK ? xa[m + 1] : nb.kf;
             const double bq = m > 0 ? xa[m - 1] : nb.kb;
             const double ax = ac * xa[m] + ae * xn[m] + aw * xb[m] + an * nb.n[m] + as * nb.s[m] +
@@ -659,9 +665,10 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
         return out;
     };
     In nx0 = load0(i0 - kFLag);
+    d4k aca = zero, acb = zero;  // the diagonal (OpD's, divided by max|a_c|) at t-1, t-2
     for (int t = i0 - kFLag; t < i1 + kFLag; ++t) {
         // level 0 at plane t, then the next plane's inputs are requested
-        d4k z1n = zero, vn = zero;
+        d4k z1n = zero, vn = zero, acn = zero;
         if (jin && t >= g && t < g + d.nx) {
             const double acj = (A.acx[t] - anu) - asu;
 #pragma unroll
@@ -670,7 +677,8 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
                     vn[m] = first ? nx0.r[m] : nx0.r[m] + beta * (nx0.v[m] - om * nx0.q[m]);
                 else
                     vn[m] = nx0.r[m] - alpha * nx0.q[m];
-                z1n[m] = vn[m] / (((acj - sh.atu[lk + m]) - sh.abu[lk + m]) / A.max_diag);
+                acn[m] = ((acj - sh.atu[lk + m]) - sh.abu[lk + m]) / A.max_diag;
+                z1n[m] = vn[m] / acn[m];
             }
             if (owned && t >= own0 && t < i1) __builtin_nontemporal_store(vn, (d4k*)(vout + d.at(t, j, k0)));
         }
@@ -680,9 +688,9 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
         const int cw = (t - i0 + kFLag) & 1;
         const int p1 = t - 1, p2 = t - 2;
         d4k z2n = zero;
-        if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 1) z2n = sweep(p1, nbrs(cw ^ 1, 0), z1b, z1a, z1n, va);
+        if (l1 && p1 >= g && p1 < g + d.nx && p1 >= i0 - 1) z2n = sweep(p1, nbrs(cw ^ 1, 0), z1b, z1a, z1n, va, aca);
         if (owned && p2 >= own0 && p2 < i1)
-            __builtin_nontemporal_store(sweep(p2, nbrs(cw ^ 1, 1), z2b, z2a, z2n, vb),
+            __builtin_nontemporal_store(sweep(p2, nbrs(cw ^ 1, 1), z2b, z2a, z2n, vb, acb),
                                         (d4k*)(zout + d.at(p2, j, k0)));
         *(d4a*)&sh.lv[cw][0][row][o] = z1n;
         *(d4a*)&sh.lv[cw][1][row][o] = z2n;
@@ -693,6 +701,10 @@ __global__ __launch_bounds__(64 * kFRows) void k_half_pc4(Dims d, PCoef A, const
         va = vn;
         z2b = z2a;
         z2a = z2n;
+        if (carry) {
+            acb = aca;
+            aca = acn;
+        }
     }
 }
 
