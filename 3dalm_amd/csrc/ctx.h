@@ -22,7 +22,7 @@
 // The fused iterations (rows of 256 cells) update the FP64 pressure once per this many
 // iterations (FP64 drop-in: kXGroup; precision form: kXGroupF), with the updates of all
 // of them in turn; their z vectors rotate through as many buffers.
-constexpr int kXGroup = 4;
+constexpr int kXGroup = 8;
 constexpr int kXGroupF = 8;
 
 // Device-resident scalar state of one Poisson solve (read by every kernel of the
@@ -45,9 +45,9 @@ struct SolveState {
     double maxr;    // precision form: largest |r| since the last replacement
     long long n_ru; // precision form: replacements so far
     double alpha_d[kXGroupF], omega_d[kXGroupF];
-    // FP64 BiCGStab: alpha and omega of the last kXGroup - 1 iterations, oldest first
-    // (the fused iteration's x update applies them every kXGroup iterations)
-    double alpha_h[kXGroup - 1], omega_h[kXGroup - 1];
+    // FP64 BiCGStab: alpha and omega of the last kXGroup iterations, oldest first (the
+    // fused iteration's x updates are applied kXGroup at a time)
+    double alpha_h[kXGroup], omega_h[kXGroup];
 };
 
 // Precision form: the recursive FP32 residual is replaced by the true FP64 one

@@ -382,12 +382,9 @@ struct PRing {
     const double* ss[kXGroup];
 };
 
-// XM 2: x += alpha pp + omega ss and r = s - omega t with the dots of k_bi_xr (pp, ss
-// in the ring's last place).  The fused iteration updates x once per kXGroup
-// iterations (the same two roundings per update as XM 2, in the same order, so x is
-// the same to the bit): XM 0 forms r only and leaves this iteration's x update to a
-// later one; XM 1 applies the kXGroup - 1 deferred ones (the ring's z with alpha_h,
-// omega_h) and then this one's.
+// XM 1: x += alpha pp + omega ss and r = s - omega t with the dots of k_bi_xr (pp, ss
+// in the ring's last place).  XM 0 (the fused iteration) forms r only: its x updates
+// are applied kXGroup at a time by k_x_group4.
 template <int XM>
 __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x, PRing z,
                                                   const double* __restrict__ s, const double* __restrict__ t_,
@@ -397,33 +394,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
     const double alpha = st->alpha, om = st->omega;
     const Col4 cl = col_unit4(d);
     double v[2] = {0.0, 0.0};
-    if (cl.ok && XM == 1) {
-        // every plane loads the z of kXGroup iterations: no prefetch of the next plane
-        // (its registers would halve the wavefronts per SIMD); the owned planes in order,
-        // as the walk below visits them
-        int c = d.at(cl.own0, cl.j, cl.k0);
-        for (int i = cl.own0; i < cl.i1; ++i, c += d.si) {
-            d4k xv = ld4(x, c);
-#pragma unroll
-            for (int q = 0; q < kXGroup; ++q) {
-                const d4k a = ld4(z.pp[q], c), b = ld4(z.ss[q], c);
-                const double aq = q + 1 < kXGroup ? st->alpha_h[q] : alpha;
-                const double oq = q + 1 < kXGroup ? st->omega_h[q] : om;
-#pragma unroll
-                for (int m = 0; m < kVecK; ++m) xv[m] = xv[m] + aq * a[m] + oq * b[m];
-            }
-            const d4k sv = ld4(s, c), tv = ld4(t_, c), r0v = ld4(r0, c);
-            d4k rv;
-#pragma unroll
-            for (int m = 0; m < kVecK; ++m) {
-                rv[m] = sv[m] - om * tv[m];
-                v[0] += rv[m] * rv[m];
-                v[1] += r0v[m] * rv[m];
-            }
-            __builtin_nontemporal_store(xv, (d4k*)(x + c));
-            __builtin_nontemporal_store(rv, (d4k*)(r + c));
-        }
-    } else if (cl.ok) {
+    if (cl.ok) {
         const double* __restrict__ pp = z.pp[kXGroup - 1];
         const double* __restrict__ ss = z.ss[kXGroup - 1];
         struct XR {
@@ -433,7 +404,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
         walk_points4<XR>(d, cl,
                          [&](int c) {
                              XR a{z4, z4, z4, ld4(s, c), ld4(t_, c), ld4(r0, c)};
-                             if (XM == 2) {
+                             if (XM == 1) {
                                  a.x = ld4(x, c);
                                  a.pp = ld4(pp, c);
                                  a.ss = ld4(ss, c);
@@ -449,7 +420,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
                                  v[0] += rv[m] * rv[m];
                                  v[1] += a.r0[m] * rv[m];
                              }
-                             if (XM == 2) __builtin_nontemporal_store(xv, (d4k*)(x + c));
+                             if (XM == 1) __builtin_nontemporal_store(xv, (d4k*)(x + c));
                              __builtin_nontemporal_store(rv, (d4k*)(r + c));
                          });
     }
@@ -458,8 +429,32 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
     store_partials<2>(pv, partials);
 }
 
-// The updates deferred at the end of a solve: x += alpha pp + omega ss for the ring's
-// places [first, kXGroup), in turn, on the interior
+// The fused iteration's x updates of the ring's iterations, in turn (x += alpha pp +
+// omega ss with alpha_h, omega_h: the same two roundings per update as k_bi_xr4<1>, in
+// the same order, so x is the same to the bit), after iteration `want` - 1 of the
+// solve; enqueued every kXGroup iterations, it returns at once when the solve ended
+// before (the end of the solve then applies the outstanding ones).  Rows of 256 cells,
+// four cells per lane: x and the pairs of z stream through once.
+__global__ __launch_bounds__(BLOCK) void k_x_group4(Dims d, double* __restrict__ x, PRing z, long long want,
+                                                    const SolveState* st) {
+    if (st->it != want) return;
+    const Col4 cl = col_unit4(d);
+    if (!cl.ok) return;
+    int c = d.at(cl.own0, cl.j, cl.k0);
+    for (int i = cl.own0; i < cl.i1; ++i, c += d.si) {
+        d4k xv = ld4(x, c);
+#pragma unroll
+        for (int q = 0; q < kXGroup; ++q) {
+            const d4k a = ld4(z.pp[q], c), b = ld4(z.ss[q], c);
+            const double aq = st->alpha_h[q], oq = st->omega_h[q];
+#pragma unroll
+            for (int m = 0; m < kVecK; ++m) xv[m] = xv[m] + aq * a[m] + oq * b[m];
+        }
+        __builtin_nontemporal_store(xv, (d4k*)(x + c));
+    }
+}
+// The updates outstanding at the end of a solve: x += alpha pp + omega ss for the
+// ring's places [first, kXGroup), in turn, on the interior
 struct XFlush64 {
     double alpha[kXGroup], omega[kXGroup];
     int first;
@@ -832,8 +827,8 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     if (fused) {
         if (const int rz = ensure_fused_z64(ctx)) return rz;
     }
-    // the fused iteration k writes z_p / z_s into buffer pair k mod kXGroup and updates
-    // x at k = kXGroup - 1 mod kXGroup only, with the updates of the last kXGroup
+    // the fused iteration k writes z_p / z_s into buffer pair k mod kXGroup; its x updates
+    // are applied kXGroup at a time, after iterations k = kXGroup - 1 mod kXGroup
     double* PP[kXGroup] = {ctx->pp};
     double* SS[kXGroup] = {ctx->ss};
     for (int q = 1; q < kXGroup; ++q) {
@@ -916,16 +911,14 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
                 probe_end(ctx, DALM_K_SPMV);
                 if (int rc2 = launch_finalize(ctx, FIN_OMEGA, grid4, ctx->solve, nullptr)) return rc2;
                 probe_begin(ctx, DALM_K_UPDATE);
-                if (k % kXGroup == kXGroup - 1)
-                    hipLaunchKernelGGL(k_bi_xr4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, pring(k),
-                                       (const double*)ctx->s, (const double*)ctx->t, ctx->r, (const double*)ctx->r0,
-                                       ctx->partials, ctx->solve);
-                else
-                    hipLaunchKernelGGL(k_bi_xr4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, pring(k),
-                                       (const double*)ctx->s, (const double*)ctx->t, ctx->r, (const double*)ctx->r0,
-                                       ctx->partials, ctx->solve);
+                hipLaunchKernelGGL(k_bi_xr4<0>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, pring(k),
+                                   (const double*)ctx->s, (const double*)ctx->t, ctx->r, (const double*)ctx->r0,
+                                   ctx->partials, ctx->solve);
                 probe_end(ctx, DALM_K_UPDATE);
                 if (int rc3 = launch_finalize(ctx, FIN_BI_XR, grid4, ctx->solve, nullptr)) return rc3;
+                if (k % kXGroup == kXGroup - 1)
+                    hipLaunchKernelGGL(k_x_group4, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, pring(k),
+                                       it_start + k + 1, (const SolveState*)ctx->solve);
                 continue;
             }
             probe_begin(ctx, DALM_K_PUPDATE);
@@ -969,7 +962,7 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
                 PRing z{};
                 z.pp[kXGroup - 1] = ppz;
                 z.ss[kXGroup - 1] = ssz;
-                hipLaunchKernelGGL(k_bi_xr4<2>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, z,
+                hipLaunchKernelGGL(k_bi_xr4<1>, dim3(grid4), dim3(BLOCK), 0, ctx->stream, d, x, z,
                                    (const double*)ctx->s, (const double*)ctx->t, ctx->r, (const double*)ctx->r0,
                                    ctx->partials, ctx->solve);
             }
@@ -988,14 +981,14 @@ int run_bicgstab_on(dalm_ctx* ctx, double* x, const double* b, int slot, int pc_
     const long long done_it = ctx->h_solve->it - it_start;
     const long long last = done_it - 1;
     if (fused && done_it > 0 && last % kXGroup != kXGroup - 1) {
-        // the last iterations deferred their x updates to one that did not run: ring
-        // places [first, kXGroup) hold them, with alpha and omega in alpha_h, omega_h
-        // (the last place is iteration `last`, whose alpha FIN_BI_XR shifted in last)
+        // the last iterations' x updates are outstanding: ring places [first, kXGroup)
+        // hold them, with alpha and omega in the same places of alpha_h, omega_h (the
+        // last place is iteration `last`, whose alpha FIN_BI_XR shifted in last)
         XFlush64 f{};
         f.first = kXGroup - 1 - (int)(last % kXGroup);
         for (int q = f.first; q < kXGroup; ++q) {
-            f.alpha[q] = ctx->h_solve->alpha_h[q - 1];
-            f.omega[q] = ctx->h_solve->omega_h[q - 1];
+            f.alpha[q] = ctx->h_solve->alpha_h[q];
+            f.omega[q] = ctx->h_solve->omega_h[q];
         }
         hipLaunchKernelGGL(k_x_flush, dim3(ctx->grid), dim3(BLOCK), 0, ctx->stream, d, x, pring(last), f);
         LAUNCH_CHECK();

@@ -82,14 +82,14 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
         break;
     case FIN_BI_XR: {
         if (st->done) break;
-        // the fused iteration's x update may be deferred to a later iteration's update
-        // kernel, which then needs this iteration's alpha and omega
-        for (int q = 0; q + 1 < kXGroup - 1; ++q) {
+        // the fused iteration's x updates are applied kXGroup at a time, after the last
+        // of them, with their alpha and omega
+        for (int q = 0; q + 1 < kXGroup; ++q) {
             st->alpha_h[q] = st->alpha_h[q + 1];
             st->omega_h[q] = st->omega_h[q + 1];
         }
-        st->alpha_h[kXGroup - 2] = st->alpha;
-        st->omega_h[kXGroup - 2] = st->omega;
+        st->alpha_h[kXGroup - 1] = st->alpha;
+        st->omega_h[kXGroup - 1] = st->omega;
         double err = sqrt(s[0]) / st->root_n;
         long long it = st->it + 1;
         st->it = it;

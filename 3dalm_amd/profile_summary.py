@@ -48,8 +48,9 @@ ALGO_BYTES = {
     "k_calc_q<double>": 32, "k_calc_q<float>": 20,
     "kf_sor": 10, "ks_resid": 24, "ks_add": 20,
     "k_jacobi4": 24, "k_bi_spmv4<0>": 24, "k_bi_spmv4<1>": 24, "k_bi_p4": 40, "k_bi_s4": 32,
-    # the x/r update: r only (x deferred), x of four iterations and r, x and r
-    "k_bi_xr4<0>": 32, "k_bi_xr4<1>": 112, "k_bi_xr4<2>": 64,
+    # the x/r update: r only (the fused iteration), x and r; the fused iteration's x
+    # updates of eight iterations
+    "k_bi_xr4<0>": 32, "k_bi_xr4<1>": 64, "k_x_group4": 144,
     "kf_jacobi4": 12, "kg_spmv4<0>": 12, "kg_spmv4<1>": 12, "kg_p4": 20, "kg_s4": 16, "kg_xr4": 40,
     # fused half-iterations (rows of 256 cells, Jacobi x3)
     "kg_half4<0>": 36, "kg_half4<1>": 24, "k_half_pc4<0>": 40, "k_half_pc4<1>": 32,

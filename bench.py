@@ -82,7 +82,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_alm_512x256x256.json")
 # the kernels behind each probe on rows of a multiple of 256 interior cells (the
 # headline grid), where the iteration kernels run four cells per lane
-PROBE_KERNEL = {"jacobi": "k_jacobi4", "jacobi32": "kf_jacobi4", "spmv": "k_bi_spmv4<0>", "update": "k_bi_xr4<0,1>",
+PROBE_KERNEL = {"jacobi": "k_jacobi4", "jacobi32": "kf_jacobi4", "spmv": "k_bi_spmv4<0>", "update": "k_bi_xr4<0>",
                 "pupdate": "k_bi_p4", "supdate": "k_bi_s4", "predictor": "k_predictor",
                 "spmv_mixed": "kg_spmv4<0>", "update_mixed": "kg_xr4", "pupdate_mixed": "kg_p4",
                 "supdate_mixed": "kg_s4",
@@ -95,10 +95,10 @@ PROBE_KERNEL = {"jacobi": "k_jacobi4", "jacobi32": "kf_jacobi4", "spmv": "k_bi_s
 PROBE_BYTES = {
     "jacobi": 24,     # read z_in, v; write z_out
     "spmv": 24,       # read z, a1 (r0 or s); write y
-    # read s, t, r0, write r; every fourth launch also reads pp, ss of four iterations
-    # and reads and writes x (k_bi_xr4<0> 32 B three times, then <1> 112 B; 64 B every
-    # launch on rows that do not take the fused iteration)
-    "update": 52,
+    # read s, t, r0, write r (k_bi_xr4<0>: the fused iteration's x updates are applied
+    # eight at a time by k_x_group4, 144 B; 64 B with x on rows that do not take the
+    # fused iteration)
+    "update": 32,
     "pupdate": 40,    # read r, p, q; write p, z
     "supdate": 32,    # read r, q; write s, z
     "predictor": 104,  # read Uold(3), JU(3), nut, f(3); write U(3)
@@ -125,10 +125,10 @@ PROBE_BYTES = {
 STEP_BYTES = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 280, "per_solve": 32},
               "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 148, "per_solve": 28}}
 # with the fused half-iterations (rows of exactly 256 cells, Jacobi x3): FP64 40 + 24
-# (SpMV) + 32 + 24 + 52 (r update, and the x updates of four iterations every fourth
-# time); precision form 36 (p-half with the previous iteration's r update) + 24 + 10
-# (the x updates of eight iterations every eighth time, 80 B)
-STEP_BYTES_FUSED = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 172, "per_solve": 32},
+# (SpMV) + 32 + 24 + 32 (r update) + 18 (the x updates of eight iterations every eighth
+# time, 144 B); precision form 36 (p-half with the previous iteration's r update) + 24
+# + 10 (the x updates of eight iterations every eighth time, 80 B)
+STEP_BYTES_FUSED = {"fp64": {"fixed": 104 + 48 + 32 + 104 + 64, "per_it": 170, "per_solve": 32},
                     "mixed": {"fixed": 64 + 44 + 80 + 48, "per_it": 70, "per_solve": 28}}
 PROBE_IDS = {"jacobi": 1, "spmv": 2, "update": 3, "predictor": 4, "sor": 5, "resnorm": 6, "pupdate": 7,
              "supdate": 8, "jacobi32": 9, "spmv_mixed": 10, "update_mixed": 11, "pupdate_mixed": 12,
