@@ -525,6 +525,9 @@ def main():
     ap.add_argument("--calibration", type=int, default=0, metavar="N",
                     help="after the forms, run N copies of the velocity field and N l2 norms of one field "
                          "(kernels of known traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="run only the single-rotor 256x128x128 workload in both forms (--steps, --warmup; "
+                         "then --calibration on its grid) and print its record: for profiling that size")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the whole domain lives on one MI355X and there is "
@@ -532,6 +535,12 @@ def main():
               "--gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if a.c2_only:
+        rec = run_c2_forms(a.steps, a.warmup)
+        if a.calibration:
+            calibration_kernels(a.calibration, make_case((256, 128, 128), 1).mesh)
+        print(json.dumps(rec), flush=True)
+        return
     hip = importlib.import_module("3dalm_amd.hip")
     forms = [f for f in a.forms.split(",") if f]
     unknown = [f for f in forms if f not in FORMS]
