@@ -185,6 +185,14 @@ __device__ __forceinline__ void store_partials(const double (&v)[NV], double* pa
         for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = v[q];
     }
 }
+// The same into slot `slot` of [0, gridDim.x) instead of the block's own index.
+template <int NV>
+__device__ __forceinline__ void store_partials(const double (&v)[NV], double* partials, int slot) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) partials[q * gridDim.x + slot] = v[q];
+    }
+}
 
 // Reduce partials[q*nb + b] over b in a fixed order (one block).  Result in all
 // threads' out[] after the call.  Each thread requests its values in groups of

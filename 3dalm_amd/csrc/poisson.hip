@@ -426,7 +426,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr4(Dims d, double* __restrict__ x
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // The fused iteration's x updates of the ring's iterations, in turn (x += alpha pp +
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv4(Dims d, PCoef A, const doubl
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // ---- fused update + preconditioner (rows of exactly 256 cells, Jacobi x3) -----------

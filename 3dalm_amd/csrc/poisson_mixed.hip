@@ -102,9 +102,9 @@ __global__ __launch_bounds__(BLOCK) void kf_jacobi4(Dims d, PCoefF A, const floa
                                                     const SolveState* st) {
     if (st->done) return;
     const int tkv = vec4_tiles(d);
-    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
-    const int ic = u % d.nch, col = u / d.nch;
-    const int kt = col % tkv, jt = col / tkv;
+    const int u = xcd_unit((int)blockIdx.x, (int)gridDim.x);
+    const int jt = u % d.tj, rest = u / d.tj;
+    const int kt = rest % tkv, ic = rest / tkv;
     const int lane = threadIdx.x % TK, row = threadIdx.x / TK;
     const int k0 = d.gc + (kt * TK + lane) * kVecK;
     const int j = d.gc + jt * TJ + row;
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(BLOCK) void kg_spmv4(Dims d, PCoef A, const float* 
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // kg_p and kg_s with four cells per lane (the same arithmetic per cell)
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(BLOCK) void kg_xr4(Dims d, double* __restrict__ x, 
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_sor(Dims d, PCoefF A, float* __restrict__ x,
