@@ -51,6 +51,13 @@ DALM_HD int xcd_unit(int b, int n) {
     return x * q + (x < rem ? x : rem) + (b >> 3);
 }
 
+// Inverse of virtual_block: the block that virtual_block maps to unit u.
+DALM_HD int virtual_block_index(int u, int nblocks) {
+    if (nblocks % 8) return u;
+    const int q = nblocks >> 3;
+    return (u % q) * 8 + u / q;
+}
+
 // Tile t -> base interior coordinates of this thread (may fall outside the
 // interior along j or k on ragged edges; callers test `inside`).
 struct Cell {
@@ -82,15 +89,23 @@ __device__ __forceinline__ Cell tile_cell(const Dims& d, int t) {
 // accumulates) only the planes from own0 on.
 constexpr int kMaxPlanes = 8;         // planes of the unrolled walk (units of other lengths loop)
 constexpr int kMaxColumnPlanes = 32;  // planes per column unit at most
+// Units are numbered j tile fastest, then k tile, then channel, and dealt to the
+// XCDs in contiguous ranges, so the j-neighbour tiles whose rows a unit's stencil
+// reads run on the same XCD and share its L2.  slot is the unit's index for its
+// partial results: the block that held it when units were numbered channel fastest
+// and dealt out by virtual_block, so the finalize adds the dot products in that
+// fixed order whatever the launch numbering.
 struct Col {
     int i0, i1, own0, j, k;
+    int slot;
     bool ok;  // (j, k) inside the interior (ragged tiles have idle lanes)
 };
 __device__ __forceinline__ Col col_unit(const Dims& d) {
-    const int u = virtual_block((int)blockIdx.x, (int)gridDim.x);
-    const int ic = u % d.nch, col = u / d.nch;
-    const int kt = col % d.tk, jt = col / d.tk;
+    const int u = xcd_unit((int)blockIdx.x, (int)gridDim.x);
+    const int jt = u % d.tj, rest = u / d.tj;
+    const int kt = rest % d.tk, ic = rest / d.tk;
     Col r;
+    r.slot = virtual_block_index(ic + d.nch * (kt + d.tk * jt), (int)gridDim.x);
     r.k = d.gc + kt * TK + (int)(threadIdx.x % TK);
     r.j = d.gc + jt * TJ + (int)(threadIdx.x / TK);
     r.own0 = d.gc + ic * d.ni;

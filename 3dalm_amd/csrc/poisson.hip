@@ -71,7 +71,7 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Dims d, PCoef A, const double
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
-    store_partials<1>(pv, partials);
+    store_partials<1>(pv, partials, cl.slot);
 }
 
 __global__ void k_state_reset(SolveState* st, double tol, long long max_it, double root_n) {
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_init(Dims d, PCoef A, const double
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
-    store_partials<1>(pv, partials);
+    store_partials<1>(pv, partials, cl.slot);
 }
 
 // p = r (first iteration) or r + beta (p - omega q); then the first
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_spmv(Dims d, PCoef A, const double
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // s = r - alpha q, then the first preconditioner sweep for ss (src/eq.h:373-380)
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(BLOCK) void k_bi_xr(Dims d, double* __restrict__ x,
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // ---- the BiCGStab kernels with four cells per lane (rows of 256 interior cells):

@@ -491,7 +491,7 @@ __global__ __launch_bounds__(BLOCK) void kg_resid(Dims d, PCoef A, const double*
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // p = r (first iteration) or r + beta (p - omega q), and the first preconditioner
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(BLOCK) void kg_spmv(Dims d, PCoef A, const float* _
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // x += alpha z + omega z' (FP64); r = s - omega t; partials |r|^2, <r0, r>
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(BLOCK) void kg_xr(Dims d, double* __restrict__ x, c
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // Hand-over to FP64: r = b - A x (FP64), and r0, p, q widened to FP64 (exactly);
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(BLOCK) void kg_upgrade(Dims d, PCoef A, const doubl
     }
     block_sum<2>(v);
     double pv[2] = {v[0], v[1]};
-    store_partials<2>(pv, partials);
+    store_partials<2>(pv, partials, cl.slot);
 }
 
 // ---- precision form of run_sor (src/eq.h:291-306) ------------------------------
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(BLOCK) void ks_resid(Dims d, PCoef A, const double*
     }
     block_sum<1>(v);
     double pv[1] = {v[0]};
-    store_partials<1>(pv, partials);
+    store_partials<1>(pv, partials, cl.slot);
 }
 
 // x += e (FP64)

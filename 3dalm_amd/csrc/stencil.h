@@ -162,13 +162,10 @@ __device__ __forceinline__ Col4 col_unit4(const Dims& d) {
     r.i0 = r.i1 - d.ni;
     r.ok = r.j < d.gc + d.ny && r.k0 < d.gc + d.nz;
     // The partial-result slot does not follow the launch numbering: it is the block
-    // that held this unit when units were numbered channel fastest and dealt to the
-    // XCDs round robin (virtual_block), so the finalize adds the dot products in
-    // that fixed order and the iterates are the same bit for bit as with that
-    // numbering.
-    const int uc = ic + d.nch * (kt + tkv * jt);
-    const int nb = (int)gridDim.x;
-    r.slot = nb % 8 ? uc : (uc % (nb >> 3)) * 8 + uc / (nb >> 3);
+    // that held this unit when units were numbered channel fastest and dealt out by
+    // virtual_block, so the finalize adds the dot products in that fixed order and
+    // the iterates are the same bit for bit as with that numbering.
+    r.slot = virtual_block_index(ic + d.nch * (kt + tkv * jt), (int)gridDim.x);
     return r;
 }
 
